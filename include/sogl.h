@@ -1,0 +1,207 @@
+/*
+ * sogl.h -- C ABI of the MI355X-native sparse block LU factorise-and-solve path.
+ *
+ * Plain C types only: pointers, sizes and status codes.  Two halves:
+ *
+ *   1. Host plan (CPU, no GPU needed): Matrix Market reader, GPS ordering and
+ *      the two-level BlockPlanner that emits the staged list of 64x64 unit ops.
+ *      These replace the reference's host pipeline and must match it bit for
+ *      bit (permutation, tile ids, op list, stages).
+ *   2. Device execution (HIP on one gfx950 GPU): the tile pool, the factor
+ *      executor and the triangular solve, plus mixed precision with FP64
+ *      iterative refinement.  These replace BlockPlanner::calculate() and
+ *      BlockPlanner::solve(), the two call sites in decompose_solveLU.
+ *
+ * Every function returns SOGL_OK (0) or a positive error code; the message of
+ * the last error is available from sogl_plan_log / sogl_ctx_error /
+ * sogl_last_error.
+ */
+#ifndef SOGL_H
+#define SOGL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+#define SOGL_OK 0
+#define SOGL_ERR_ARG 1           /* bad argument or input the reference cannot handle */
+#define SOGL_ERR_IO 2            /* file could not be read or written */
+#define SOGL_ERR_OOM 3           /* host/device allocation failed or admission refused */
+#define SOGL_ERR_DEVICE 4        /* HIP runtime error */
+#define SOGL_ERR_NAN 5           /* NaN found in the solution */
+#define SOGL_ERR_UNSUPPORTED 6   /* feature outside this path (e.g. symmetric LL^T) */
+#define SOGL_ERR_NOT_CONVERGED 7 /* iterative refinement missed its tolerance */
+#define SOGL_ERR_TIMEOUT 8       /* a device dependency wait hit its bound */
+#define SOGL_ERR_GROWTH 9        /* pivot-free growth above threshold in a low-precision factor */
+
+/* unit-operation codes: the same numbering as enum blockOp (operation.h:20) */
+enum {
+    SOGL_OP_INV = 0, SOGL_OP_LU = 1, SOGL_OP_LOWER_INV = 2, SOGL_OP_UPPER_INV = 3,
+    SOGL_OP_SUB = 4, SOGL_OP_ADD = 5, SOGL_OP_NEG = 6, SOGL_OP_COPY = 7,
+    SOGL_OP_MUL = 8, SOGL_OP_MULNEG = 9, SOGL_OP_LLT = 10, SOGL_OP_MULT = 11,
+    SOGL_OP_NOOP = 12
+};
+
+/* factor precisions */
+#define SOGL_FP64 0
+#define SOGL_FP32 1
+#define SOGL_BF16 2  /* experimental: bf16 MFMA operands, fp32 accumulation and storage */
+
+/* One 64x64 unit operation; mirrors struct operation (operation.h:37-52).
+ * sub: result = tile(src2) - tile(src); src == 0 means copy, src2 == 0 means negate
+ * (BlockPlanner.cpp:553-572).  mul: result += src*src2; mulneg: result -= src*src2.
+ * lu: src -> (result = L, result2 = U). */
+typedef struct sogl_op {
+    int32_t src, src2, op, result, result2, stage, seq, group;
+} sogl_op;
+
+/* ============================================================ host: reader */
+typedef struct sogl_mtx sogl_mtx;
+
+/* mtx::readMTX (mtx.cpp:44-117): coordinate file -> 0-based COO with
+ * explicit zeros and out-of-range entries dropped. */
+int sogl_mtx_read(const char* path, sogl_mtx** out);
+/* mtx::readArray (mtx.cpp:145-182) into the matrix object; missing -> 1.0 */
+int sogl_mtx_read_rhs(sogl_mtx* m, const char* path);
+int sogl_mtx_info(const sogl_mtx* m, int32_t* n, int64_t* nnz, int32_t* symmetric);
+int sogl_mtx_arrays(const sogl_mtx* m, const int32_t** row, const int32_t** col,
+                    const double** val, const double** rhs);
+void sogl_mtx_free(sogl_mtx* m);
+
+/* ============================================================ host: plan */
+typedef struct sogl_plan sogl_plan;
+
+typedef struct sogl_plan_info {
+    int32_t m_size;         /* n */
+    int32_t block_rows;     /* fine block rows (padded n = 64*block_rows) */
+    int32_t block_rows_l2;  /* coarse block rows */
+    int32_t block_size_l2;  /* coarse block edge, unknowns */
+    int32_t storage_count;  /* number of tile ids; id 0 is the null tile */
+    int32_t n_stages;       /* last stage number */
+    int32_t gps_levels, gps_bandwidth;
+    int64_t n_ops, n_coarse_ops, n_inputs, n_l_tiles, n_u_tiles;
+    int64_t n_mul, n_mulneg, n_sub, n_lu, n_lower_inv, n_upper_inv;
+    double t_reorder, t_plan; /* seconds, the reference's timer boundaries */
+} sogl_plan_info;
+
+/* solveLU's host part (solver.cpp:121-161, GPSOrder.cpp, solver.cpp:50-100):
+ * copy COO, size config, GPS + sortInBlock, coarse plan, L2 expansion, fine
+ * plan.  b may be NULL.  verbose != 0 echoes the reference's log lines. */
+int sogl_plan_create(int32_t dim, int64_t valcount, int32_t symmetric,
+                     const int32_t* index_i, const int32_t* index_j,
+                     const double* vals, const double* b, int32_t verbose,
+                     sogl_plan** out);
+void sogl_plan_destroy(sogl_plan* p);
+const char* sogl_plan_log(const sogl_plan* p);  /* reference stdout lines, or last error */
+int sogl_plan_get_info(const sogl_plan* p, sogl_plan_info* info);
+/* newOrder[new] = old and reverseOrder[old] = new, length m_size */
+int sogl_plan_get_perm(const sogl_plan* p, int32_t* new_order, int32_t* reverse_order);
+/* level 0: fine ops (n_ops), level 1: coarse ops (n_coarse_ops) */
+int sogl_plan_get_ops(const sogl_plan* p, int32_t level, sogl_op* out);
+/* per tile id (storage_count): last reading stage, and stage of last write */
+int sogl_plan_get_tile_stages(const sogl_plan* p, int32_t* laststage, int32_t* stage);
+/* input tiles in id order; tiles (n_inputs*64*64 doubles, row-major) may be NULL */
+int sogl_plan_get_inputs(const sogl_plan* p, int32_t* ids, int32_t* bi, int32_t* bj,
+                         double* tiles);
+/* which = 0: L factor tiles, 1: U factor tiles (block row, block col, id) */
+int sogl_plan_get_factor_tiles(const sogl_plan* p, int32_t which, int32_t* bi,
+                               int32_t* bj, int32_t* ids);
+/* b (n x nrhs, column stride ldb) -> permuted padded bp (64*block_rows x nrhs) */
+int sogl_plan_permute_rhs(const sogl_plan* p, const double* b, int32_t nrhs, int64_t ldb,
+                          double* bp);
+/* permuted padded x (64*block_rows x nrhs) -> original order x (n x nrhs, stride ldx);
+ * GOrder::reOrderResult (GPSOrder.cpp:41-53) */
+int sogl_plan_unpermute(const sogl_plan* p, const double* xp, int32_t nrhs, double* x,
+                        int64_t ldx);
+/* permuted padded COO of the problem (n_perm_nnz entries incl. padding diagonal) */
+int64_t sogl_plan_perm_nnz(const sogl_plan* p);
+int sogl_plan_get_perm_coo(const sogl_plan* p, int32_t* row, int32_t* col, double* val);
+
+/* ============================================================ device */
+typedef struct sogl_ctx sogl_ctx;
+
+int sogl_ctx_create(int32_t device, sogl_ctx** out);
+void sogl_ctx_destroy(sogl_ctx* ctx);
+const char* sogl_ctx_error(const sogl_ctx* ctx);
+/* free / total HBM bytes of the context's device */
+int sogl_ctx_mem_info(sogl_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
+/* synchronise the context's stream */
+int sogl_ctx_sync(sogl_ctx* ctx);
+
+/* Execution modes of sogl_factor_run */
+#define SOGL_EXEC_PERSISTENT 0 /* one persistent kernel pulling ready tasks from a device queue */
+#define SOGL_EXEC_LEVELS 1     /* one fused launch per dependency level */
+#define SOGL_EXEC_PER_OP 2     /* one launch per unit op (the unfused comparison point) */
+
+typedef struct sogl_factor_desc {
+    int32_t n_ids;          /* storage_count */
+    int32_t block_rows;     /* fine block rows */
+    const sogl_op* ops;     /* staged, sorted op list */
+    int64_t n_ops;
+    const int32_t* input_ids;
+    const double* input_tiles; /* n_inputs * 4096 doubles, row-major 64x64 */
+    int64_t n_inputs;
+    const int32_t* l_bi; const int32_t* l_bj; const int32_t* l_ids; int64_t n_l;
+    const int32_t* u_bi; const int32_t* u_bj; const int32_t* u_ids; int64_t n_u;
+    int32_t precision;      /* SOGL_FP64 / SOGL_FP32 / SOGL_BF16 */
+} sogl_factor_desc;
+
+/* Build the device task graph and tile pool for one factorisation and copy the
+ * input tiles to HBM.  The factor stays resident under *factor_id until released. */
+int sogl_factor_load(sogl_ctx* ctx, const sogl_factor_desc* desc, int32_t* factor_id);
+/* Convenience: sogl_factor_load from a host plan. */
+int sogl_factor_load_plan(sogl_ctx* ctx, const sogl_plan* plan, int32_t precision,
+                          int32_t* factor_id);
+/* Factor on the device: replaces BlockPlanner::calculate() (BlockPlanner.cpp:376-651),
+ * called from decompose_solveLU (solver.cpp:106).  Re-running refactors from the
+ * resident input tiles. */
+int sogl_factor_run(sogl_ctx* ctx, int32_t factor_id, int32_t exec_mode);
+/* Triangular solve: replaces BlockPlanner::solve() (BlockPlanner.cpp:834-862), called
+ * from decompose_solveLU (solver.cpp:115).  bp, xp: host arrays in the permuted padded
+ * order, 64*block_rows x nrhs column-major.  y (may be NULL) receives the forward
+ * substitution result, which the reference leaves in data::b. */
+int sogl_solve(sogl_ctx* ctx, int32_t factor_id, const double* bp, double* xp, double* y,
+               int32_t nrhs);
+/* Same, on device-resident arrays (hipMalloc'd by the caller) on the ctx stream. */
+int sogl_solve_device(sogl_ctx* ctx, int32_t factor_id, const double* d_bp, double* d_xp,
+                      int32_t nrhs);
+int sogl_factor_release(sogl_ctx* ctx, int32_t factor_id);
+
+typedef struct sogl_factor_stats {
+    double factor_ms;       /* last sogl_factor_run, device time */
+    double solve_ms;        /* last sogl_solve*, device time */
+    int64_t pool_bytes;     /* HBM held by the tile pool */
+    int64_t n_tasks;        /* device tasks (one per result tile) */
+    int32_t n_levels;       /* dependency levels of the task graph */
+    int32_t growth_tiles;   /* low precision only: diagonal tiles factored in FP64 */
+    double max_growth;      /* largest pivot-free growth factor seen */
+} sogl_factor_stats;
+int sogl_factor_get_stats(sogl_ctx* ctx, int32_t factor_id, sogl_factor_stats* st);
+/* Copy factor tiles back (FP64 view).  which 0 = L, 1 = U, in the order given at load. */
+int sogl_factor_get_tiles(sogl_ctx* ctx, int32_t factor_id, int32_t which, double* tiles);
+
+/* ------------------------------------------------ mixed precision + refinement */
+/* Attach the original (unpermuted) matrix for the FP64 residual r = b - A x. */
+int sogl_refine_attach(sogl_ctx* ctx, int32_t factor_id, const sogl_plan* plan);
+/* Solve with FP64 iterative refinement in the original order:
+ * x = A^-1 b to ||b-Ax||_2 / (||A||_F ||x||_2 + ||b||_2) <= tol, at most max_iter
+ * corrections.  b, x: n x nrhs column-major host arrays. */
+int sogl_solve_refined(sogl_ctx* ctx, int32_t factor_id, const double* b, double* x,
+                       int32_t nrhs, double tol, int32_t max_iter, int32_t* iters,
+                       double* resid);
+
+/* ------------------------------------------------ drop-in solveLU */
+/* Mirrors SOGLU::solveLU (solver.h:24, solver.cpp:121-184): plan on the host,
+ * factor and solve on device 0 in FP64, x (length dim) in the original order. */
+int sogl_solveLU(int32_t dim, int64_t valcount, int32_t symmetric, const int32_t* index_i,
+                 const int32_t* index_j, const double* vals, const double* b, double* x);
+const char* sogl_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SOGL_H */
