@@ -1,0 +1,292 @@
+/*
+ * Oracle: a plain-C FP64 restatement of the reference's factor executor and
+ * triangular substitution.  TEST INFRASTRUCTURE ONLY: tests, the smoke check
+ * and the CPU baseline leg of the benchmark load it as a checker; the product
+ * (build/libsogl.so) never does.
+ *
+ * What it restates (reference file:line):
+ *   oracle_gemm        C += A*B / C -= A*B, the textbook triple loop of the
+ *                      AVX kernels (MatrixStdDouble.cpp:1557-1568, 223-235)
+ *   oracle_lu          pivot-free Doolittle with long double sums and the
+ *                      |u_ii| < 1e-9 -> +-1e-9 clamp (MatrixStdDouble.cpp:2671-2709,
+ *                      clamp rule of the live kernel 2751-2761)
+ *   oracle_inv_lower   y_ii = 1/l_ii, y_ij = -(sum_k l_ik y_kj) / l_ii (2787-2802)
+ *   oracle_inv_upper   column elimination from the bottom + long double row
+ *                      scaling (the textbook inv_upper, 2804-2827)
+ *   oracle_sub         r = a - b (2938-2947), copy / negate for missing operands
+ *   oracle_factor      BlockPlanner::calculate (BlockPlanner.cpp:376-651): stages in
+ *                      order; inside a stage, ops grouped by result tile run in
+ *                      list order (the reference's result%47 streams keep that
+ *                      order for same-result ops), groups in parallel; tiles
+ *                      freed after their last reading stage
+ *   oracle_solve       BlockPlanner::solve/lowerSolver/upperSolver (669-862):
+ *                      block forward then backward substitution, dividing by
+ *                      the diagonal entries like the leaf loops do
+ *
+ * Dense 64x64 row-major tiles replace the reference's 64x72 tiles with 8x8
+ * masks; skipping a masked (zero) block is exact, so the only difference is
+ * the reference's flush of sub-blocks below 1e-18 / 1e-20 in magnitude.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define TS 64
+#define TE (TS * TS)
+
+enum { OP_LU = 1, OP_LOWER_INV = 2, OP_UPPER_INV = 3, OP_SUB = 4, OP_MUL = 8, OP_MULNEG = 9 };
+
+typedef struct {
+    int32_t src, src2, op, result, result2, stage, seq, group;
+} op_t;
+
+void oracle_gemm(const double* a, const double* b, double* c, int sign) {
+    for (int i = 0; i < TS; i++)
+        for (int k = 0; k < TS; k++) {
+            const double aik = sign * a[i * TS + k];
+            for (int j = 0; j < TS; j++) c[i * TS + j] += aik * b[k * TS + j];
+        }
+}
+
+void oracle_sub(const double* a, const double* b, double* r) {
+    for (int e = 0; e < TE; e++) r[e] = (a ? a[e] : 0.0) - (b ? b[e] : 0.0);
+}
+
+void oracle_lu(const double* a, double* l, double* u) {
+    const double TINY2 = 1e-9;
+    memset(l, 0, TE * sizeof(double));
+    memset(u, 0, TE * sizeof(double));
+    for (int i = 0; i < TS; i++) {
+        for (int j = i; j < TS; j++) {
+            long double sum = 0;
+            for (int k = 0; k < i; k++) sum += (long double)l[i * TS + k] * u[k * TS + j];
+            u[i * TS + j] = (double)(a[i * TS + j] - sum);
+        }
+        if (u[i * TS + i] < TINY2 && u[i * TS + i] > -TINY2) u[i * TS + i] = u[i * TS + i] < 0 ? -TINY2 : TINY2;
+        long double u1 = 1.0L / (long double)u[i * TS + i];
+        for (int j = i + 1; j < TS; j++) {
+            long double sum = 0;
+            for (int k = 0; k < i; k++) sum += (long double)l[j * TS + k] * u[k * TS + i];
+            l[j * TS + i] = (double)(u1 * (a[j * TS + i] - sum));
+        }
+        l[i * TS + i] = 1;
+    }
+}
+
+void oracle_inv_lower(const double* l, double* y) {
+    memset(y, 0, TE * sizeof(double));
+    for (int i = 0; i < TS; i++) {
+        const double q = 1.0 / l[i * TS + i];
+        y[i * TS + i] = q;
+        for (int j = 0; j < i; j++) {
+            double sum = 0;
+            for (int k = j; k < i; k++) sum += l[i * TS + k] * y[k * TS + j];
+            y[i * TS + j] = -sum * q;
+        }
+    }
+}
+
+void oracle_inv_upper(const double* u, double* y) {
+    memset(y, 0, TE * sizeof(double));
+    for (int i = 0; i < TS; i++) y[i * TS + i] = 1.0;
+    for (int j = TS - 1; j >= 0; j--) {
+        for (int i = j - 1; i >= 0; i--) {
+            const double scale = u[i * TS + j] / u[j * TS + j];
+            for (int k = j; k < TS; k++) y[i * TS + k] -= y[j * TS + k] * scale;
+        }
+        long double rate = 1.0L / (long double)u[j * TS + j];
+        for (int i = j; i < TS; i++) y[j * TS + i] = (double)(y[j * TS + i] * rate);
+    }
+}
+
+static double* tile_new(void) { return (double*)calloc(TE, sizeof(double)); }
+
+/*
+ * Factor: tiles[id] (array of n_ids pointers, owned by the caller afterwards)
+ * receives every tile still live at the end (the L and U outputs).
+ * Returns 0, or -1 on a malformed op list.
+ */
+int oracle_factor(int32_t n_ids, const op_t* ops, int64_t n_ops, const int32_t* input_ids,
+                  const double* input_tiles, int64_t n_inputs, const int32_t* laststage,
+                  double** tiles, int threads) {
+    for (int32_t i = 0; i < n_ids; i++) tiles[i] = NULL;
+    for (int64_t k = 0; k < n_inputs; k++) {
+        int32_t id = input_ids[k];
+        if (id <= 0 || id >= n_ids) return -1;
+        tiles[id] = (double*)malloc(TE * sizeof(double));
+        memcpy(tiles[id], input_tiles + k * TE, TE * sizeof(double));
+    }
+    /* per stage: group consecutive-by-result runs; groups of one stage are independent */
+    int64_t* gstart = (int64_t*)malloc((size_t)(n_ops + 1) * sizeof(int64_t));
+    int64_t* order = (int64_t*)malloc((size_t)(n_ops > 0 ? n_ops : 1) * sizeof(int64_t));
+    int64_t s0 = 0;
+    int rc = 0;
+    while (s0 < n_ops && rc == 0) {
+        int64_t s1 = s0;
+        while (s1 < n_ops && ops[s1].stage == ops[s0].stage) s1++;
+        /* stable order of the stage's ops by result id -> same-result runs in list order */
+        int64_t m = s1 - s0;
+        for (int64_t q = 0; q < m; q++) order[q] = s0 + q;
+        /* insertion sort by result is too slow for big stages: counting via qsort on (result, index) */
+        {
+            int64_t* keys = (int64_t*)malloc((size_t)m * 2 * sizeof(int64_t));
+            for (int64_t q = 0; q < m; q++) { keys[2 * q] = ops[s0 + q].result; keys[2 * q + 1] = s0 + q; }
+            int cmp(const void* x, const void* y);
+            qsort(keys, (size_t)m, 2 * sizeof(int64_t), cmp);
+            for (int64_t q = 0; q < m; q++) order[q] = keys[2 * q + 1];
+            free(keys);
+        }
+        int64_t ng = 0;
+        for (int64_t q = 0; q < m; q++)
+            if (q == 0 || ops[order[q]].result != ops[order[q - 1]].result) gstart[ng++] = q;
+        gstart[ng] = m;
+        /* allocate results first (serial), then compute the groups in parallel */
+        for (int64_t q = 0; q < m; q++) {
+            const op_t* o = &ops[order[q]];
+            if (o->result <= 0 || o->result >= n_ids || o->result2 >= n_ids) { rc = -1; break; }
+            if (!tiles[o->result]) tiles[o->result] = tile_new();
+            if (o->op == OP_LU && o->result2 > 0 && !tiles[o->result2]) tiles[o->result2] = tile_new();
+        }
+        if (rc) break;
+#pragma omp parallel for schedule(dynamic) num_threads(threads > 0 ? threads : 1)
+        for (int64_t g = 0; g < ng; g++) {
+            for (int64_t q = gstart[g]; q < gstart[g + 1]; q++) {
+                const op_t* o = &ops[order[q]];
+                double* r = tiles[o->result];
+                const double* a = o->src > 0 ? tiles[o->src] : NULL;
+                const double* b = o->src2 > 0 ? tiles[o->src2] : NULL;
+                switch (o->op) {
+                    case OP_MUL: if (a && b) oracle_gemm(a, b, r, 1); break;
+                    case OP_MULNEG: if (a && b) oracle_gemm(a, b, r, -1); break;
+                    case OP_SUB: oracle_sub(b, a, r); break; /* result = src2 - src */
+                    case OP_LU: if (a) oracle_lu(a, r, tiles[o->result2]); break;
+                    case OP_LOWER_INV: if (a) oracle_inv_lower(a, r); break;
+                    case OP_UPPER_INV: if (a) oracle_inv_upper(a, r); break;
+                    default: break;
+                }
+            }
+        }
+        /* free tiles whose last reading stage has passed (BlockPlanner.cpp:597-634) */
+        for (int64_t q = s0; q < s1; q++) {
+            int32_t ids[2] = {ops[q].src, ops[q].src2};
+            for (int t = 0; t < 2; t++) {
+                int32_t id = ids[t];
+                if (id > 0 && laststage[id] == ops[q].stage && tiles[id]) {
+                    free(tiles[id]);
+                    tiles[id] = NULL;
+                }
+            }
+        }
+        s0 = s1;
+    }
+    free(gstart);
+    free(order);
+    return rc;
+}
+
+int cmp(const void* x, const void* y) {
+    const int64_t* a = (const int64_t*)x;
+    const int64_t* b = (const int64_t*)y;
+    if (a[0] != b[0]) return a[0] < b[0] ? -1 : 1;
+    return a[1] < b[1] ? -1 : (a[1] > b[1]);
+}
+
+void oracle_free_tiles(double** tiles, int32_t n_ids) {
+    for (int32_t i = 0; i < n_ids; i++) {
+        free(tiles[i]);
+        tiles[i] = NULL;
+    }
+}
+
+/*
+ * Block triangular solve of the permuted padded system, nrhs columns (column-major,
+ * ld = 64*block_rows).  L / U tiles given as (bi, bj, id) triples; b is overwritten
+ * with y (like the reference leaves y in data::b), x receives the solution.
+ */
+int oracle_solve(int32_t block_rows, double** tiles, const int32_t* lbi, const int32_t* lbj,
+                 const int32_t* lid, int64_t nl, const int32_t* ubi, const int32_t* ubj,
+                 const int32_t* uid, int64_t nu, double* b, double* x, int nrhs) {
+    const int64_t N = (int64_t)block_rows * TS;
+    /* per block row lists */
+    int64_t* lp = (int64_t*)calloc((size_t)block_rows + 1, sizeof(int64_t));
+    int64_t* up = (int64_t*)calloc((size_t)block_rows + 1, sizeof(int64_t));
+    for (int64_t k = 0; k < nl; k++) lp[lbi[k] + 1]++;
+    for (int64_t k = 0; k < nu; k++) up[ubi[k] + 1]++;
+    for (int32_t i = 0; i < block_rows; i++) { lp[i + 1] += lp[i]; up[i + 1] += up[i]; }
+    int64_t* lk = (int64_t*)malloc((size_t)(nl > 0 ? nl : 1) * sizeof(int64_t));
+    int64_t* uk = (int64_t*)malloc((size_t)(nu > 0 ? nu : 1) * sizeof(int64_t));
+    int64_t* fill = (int64_t*)calloc((size_t)block_rows, sizeof(int64_t));
+    for (int64_t k = 0; k < nl; k++) lk[lp[lbi[k]] + fill[lbi[k]]++] = k;
+    memset(fill, 0, (size_t)block_rows * sizeof(int64_t));
+    for (int64_t k = 0; k < nu; k++) uk[up[ubi[k]] + fill[ubi[k]]++] = k;
+    int rc = 0;
+    for (int c = 0; c < nrhs; c++) {
+        double* bc = b + c * N;
+        double* xc = x + c * N;
+        /* forward: y_I = L_II^-1 (b_I - sum_J L_IJ y_J), y stored in b */
+        for (int32_t I = 0; I < block_rows && !rc; I++) {
+            const double* diag = NULL;
+            for (int64_t q = lp[I]; q < lp[I + 1]; q++) {
+                int64_t k = lk[q];
+                const double* t = tiles[lid[k]];
+                if (!t) { rc = -1; break; }
+                if (lbj[k] == I) { diag = t; continue; }
+                const double* yj = bc + (int64_t)lbj[k] * TS;
+                for (int i = 0; i < TS; i++) {
+                    double s = 0;
+                    for (int j = 0; j < TS; j++) s += t[i * TS + j] * yj[j];
+                    bc[(int64_t)I * TS + i] -= s;
+                }
+            }
+            if (!diag) { rc = -1; break; }
+            double* yi = bc + (int64_t)I * TS;
+            for (int i = 0; i < TS; i++) {
+                double s = 0;
+                for (int j = 0; j < i; j++) s += diag[i * TS + j] * yi[j];
+                yi[i] = (yi[i] - s) / diag[i * TS + i];
+            }
+        }
+        /* backward: x_I = U_II^-1 (y_I - sum_J U_IJ x_J) */
+        for (int64_t e = 0; e < N; e++) xc[e] = bc[e];
+        for (int32_t I = block_rows - 1; I >= 0 && !rc; I--) {
+            const double* diag = NULL;
+            for (int64_t q = up[I]; q < up[I + 1]; q++) {
+                int64_t k = uk[q];
+                const double* t = tiles[uid[k]];
+                if (!t) { rc = -1; break; }
+                if (ubj[k] == I) { diag = t; continue; }
+                const double* xj = xc + (int64_t)ubj[k] * TS;
+                for (int i = 0; i < TS; i++) {
+                    double s = 0;
+                    for (int j = 0; j < TS; j++) s += t[i * TS + j] * xj[j];
+                    xc[(int64_t)I * TS + i] -= s;
+                }
+            }
+            if (!diag) { rc = -1; break; }
+            double* xi = xc + (int64_t)I * TS;
+            for (int i = TS - 1; i >= 0; i--) {
+                double s = 0;
+                for (int j = i + 1; j < TS; j++) s += diag[i * TS + j] * xi[j];
+                xi[i] = (xi[i] - s) / diag[i * TS + i];
+            }
+        }
+    }
+    free(lp); free(up); free(lk); free(uk); free(fill);
+    return rc;
+}
+
+/* One-call form: factor + solve; tiles are freed before returning. */
+int oracle_factor_solve(int32_t n_ids, const op_t* ops, int64_t n_ops, const int32_t* input_ids,
+                        const double* input_tiles, int64_t n_inputs, const int32_t* laststage,
+                        int32_t block_rows, const int32_t* lbi, const int32_t* lbj,
+                        const int32_t* lid, int64_t nl, const int32_t* ubi, const int32_t* ubj,
+                        const int32_t* uid, int64_t nu, double* b, double* x, int nrhs,
+                        int threads) {
+    double** tiles = (double**)malloc((size_t)n_ids * sizeof(double*));
+    int rc = oracle_factor(n_ids, ops, n_ops, input_ids, input_tiles, n_inputs, laststage, tiles, threads);
+    if (rc == 0) rc = oracle_solve(block_rows, tiles, lbi, lbj, lid, nl, ubi, ubj, uid, nu, b, x, nrhs);
+    oracle_free_tiles(tiles, n_ids);
+    free(tiles);
+    return rc;
+}

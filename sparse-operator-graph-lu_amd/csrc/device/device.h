@@ -1,0 +1,95 @@
+// Internal structures shared by the device translation units (not part of
+// the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../../include/sogl.h"
+
+namespace sogl_dev {
+
+enum TaskKind : int32_t { TK_GEMM = 0, TK_SUB = 1, TK_LU = 2, TK_LINV = 3, TK_UINV = 4 };
+
+// One device task = every unit op that writes one result tile (an lu writes two).
+// Writers of one tile always share a stage in the reference's plan (the "fix"
+// pass, BlockPlanner.cpp:269-276), so a tile is produced by exactly one task.
+struct DevTask {
+    int32_t kind;
+    int32_t out0, out1;   // result slots (out1: U of an lu, else -1)
+    int32_t a, b;         // GEMM: pair begin / pair count; SUB: minuend / subtrahend slot
+                          // (-1 = zero); LU, LINV, UINV: source slot
+    int32_t dep_begin, dep_count;  // slots this task waits for (persistent executor)
+    int32_t level;        // dependency level (0-based)
+};
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Factor {
+    int32_t id = 0;
+    int32_t precision = SOGL_FP64;
+    int32_t block_rows = 0;
+    int64_t n_slots = 0;        // tiles in the pool (inputs + results + zero + diagonal inverses)
+    int32_t zero_slot = -1;
+    size_t elem = 8;            // bytes per tile element in the pool
+    // host copies
+    std::vector<DevTask> tasks;            // topological order (by level, then stage)
+    std::vector<int32_t> level_begin;      // tasks of level v: [level_begin[v], level_begin[v+1])
+    std::vector<DevTask> op_tasks;         // per-op executor: one entry per unit op, list order
+    std::vector<int32_t> input_slots;
+    int64_t n_pairs = 0, n_deps = 0;
+    // device arrays
+    DevBuf pool, tasks_d, pairs_d, deps_d, ready_d, ctrl_d, op_tasks_d, op_pairs_d, inputs_d;
+    int32_t epoch = 0;
+    // triangular solve structure (block CSR by block row, off-diagonal only)
+    std::vector<int32_t> l_diag, u_diag, linv_slot, uinv_slot;  // per block row
+    DevBuf l_rowptr, l_col, l_slot, u_rowptr, u_col, u_slot, l_diag_d, u_diag_d, linv_d, uinv_d;
+    DevBuf solve_ready, solve_ctrl, vec_b, vec_x, vec_y;
+    size_t vec_cap = 0;  // doubles per vector buffer
+    int32_t solve_epoch = 0;
+    // factor tile lists in load order
+    std::vector<int32_t> l_slots, u_slots;
+    // refinement
+    bool has_refine = false;
+    int32_t n = 0;                // unknowns of the original problem
+    int64_t nnz = 0;
+    std::vector<int32_t> new_order;  // new -> old
+    double a_fro = 0;
+    DevBuf csr_ptr, csr_col, csr_val, vec_r, vec_d, red;
+    // stats
+    double factor_ms = 0, solve_ms = 0;
+    int32_t growth_tiles = 0;
+    double max_growth = 0;
+    double growth_limit = 1e4;
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int num_cu = 0;
+    std::string err;
+    std::map<int32_t, Factor*> factors;
+    int32_t next_id = 1;
+    std::mutex mu;
+};
+
+// helpers implemented in factor.hip
+int set_error(Context* c, int code, const std::string& msg);
+int check_hip(Context* c, hipError_t e, const char* what);
+int dev_alloc(Context* c, DevBuf& b, size_t bytes, const char* what);
+void dev_free(DevBuf& b);
+Factor* find_factor(Context* c, int32_t id);
+
+}  // namespace sogl_dev
+
+struct sogl_ctx {
+    sogl_dev::Context c;
+};

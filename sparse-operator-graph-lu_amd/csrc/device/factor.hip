@@ -1,0 +1,756 @@
+// Device factor: HBM tile pool, task graph and the three executors that
+// replace BlockPlanner::calculate() (BlockPlanner.cpp:376-651).
+//
+// The reference walks stages and, inside a stage, runs 47 OpenMP "streams"
+// (ops bucketed by result % 47) with up to 6 same-result muls fused.  Here the
+// unit of work is a *task*: all ops that write one result tile (an lu writes
+// L and U).  Writers of a tile always share one stage in the reference's plan,
+// and every tile is written once (ids are never reused), so the tasks form a
+// DAG over tiles: a task depends only on the tasks producing its sources.
+//
+//   SOGL_EXEC_PERSISTENT  one kernel; workgroups take task indices from a device
+//                         counter in topological order and spin (bounded) on the
+//                         ready flags of their source tiles.  Deadlock-free: a
+//                         taken task only waits for tasks taken earlier, which
+//                         are held by running workgroups.
+//   SOGL_EXEC_LEVELS      one launch per dependency level (kernel boundaries
+//                         order the levels).
+//   SOGL_EXEC_PER_OP      one launch per unit op, in plan order (the unfused
+//                         comparison point; C += A*B reads C).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "device.h"
+#include "tile_ops.h"
+
+namespace sogl {
+void set_last_error(const std::string& s);
+bool plan_factor_desc(const sogl_plan* p, int32_t precision, sogl_factor_desc* d);
+}
+
+namespace sogl_dev {
+
+// ------------------------------------------------------------------ helpers
+int set_error(Context* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    sogl::set_last_error(msg);
+    return code;
+}
+
+int check_hip(Context* c, hipError_t e, const char* what) {
+    if (e == hipSuccess) return SOGL_OK;
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    if (e == hipErrorOutOfMemory) return set_error(c, SOGL_ERR_OOM, m);
+    return set_error(c, SOGL_ERR_DEVICE, m);
+}
+
+int dev_alloc(Context* c, DevBuf& b, size_t bytes, const char* what) {
+    if (b.p && b.bytes >= bytes) return SOGL_OK;
+    dev_free(b);
+    if (bytes == 0) return SOGL_OK;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        (void)hipGetLastError();
+        return check_hip(c, e, what);
+    }
+    b.bytes = bytes;
+    return SOGL_OK;
+}
+
+void dev_free(DevBuf& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+Factor* find_factor(Context* c, int32_t id) {
+    auto it = c->factors.find(id);
+    return it == c->factors.end() ? nullptr : it->second;
+}
+
+#define HIP_TRY(ctx, expr)                                       \
+    do {                                                         \
+        int _rc = check_hip((ctx), (expr), #expr);               \
+        if (_rc != SOGL_OK) return _rc;                          \
+    } while (0)
+
+// ------------------------------------------------------------------ control block
+// ctrl[0] task counter, ctrl[1] abort flag, ctrl[2] low-precision tiles refactored in
+// FP64, ctrl[3] max growth (float bits, non-negative so int order = float order)
+constexpr int CTRL_WORDS = 8;
+
+__global__ void k_ctrl_reset(int* ctrl) {
+    if (threadIdx.x < CTRL_WORDS) ctrl[threadIdx.x] = 0;
+}
+
+template <typename T>
+__global__ void k_fill(T* p, size_t n, T v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+constexpr int LDS = lds_bytes<double>();
+constexpr long long SPIN_LIMIT_TICKS = 20LL * 100 * 1000 * 1000;  // ~20 s of the 100 MHz wall clock
+
+struct Args {
+    void* pool;
+    const DevTask* tasks;
+    const Pair* pairs;
+    const int32_t* deps;
+    int* ready;
+    int* ctrl;
+    int32_t ntasks;
+    int32_t epoch;
+    int32_t task_offset;  // levels executor: first task of the launch
+    double growth_limit;
+};
+
+template <typename T, bool BF16, bool ACC>
+__device__ inline void run_task(const DevTask& t, const Args& a, char* lds) {
+    T* pool = reinterpret_cast<T*>(a.pool);
+    switch (t.kind) {
+        case TK_GEMM:
+            if constexpr (BF16 && !ACC) {
+                gemm_task_bf16(reinterpret_cast<float*>(a.pool), a.pairs + t.a, t.b, t.out0, lds);
+            } else {
+                gemm_task<T, ACC>(pool, a.pairs + t.a, t.b, t.out0, lds);
+            }
+            break;
+        case TK_SUB:
+            sub_task<T>(pool, t.b, t.a, t.out0);
+            break;
+        case TK_LU: {
+            double g = lu_task<T>(pool, t.a, t.out0, t.out1, lds, a.growth_limit, a.ctrl + 2);
+            if (threadIdx.x == 0 && sizeof(T) == 4) {
+                float gf = (float)g;
+                atomicMax(a.ctrl + 3, __float_as_int(gf));
+            }
+            break;
+        }
+        case TK_LINV:
+            tri_inv_task<T, true>(pool, t.a, t.out0, lds);
+            break;
+        case TK_UINV:
+            tri_inv_task<T, false>(pool, t.a, t.out0, lds);
+            break;
+        default:
+            break;
+    }
+}
+
+// Persistent executor: every workgroup loops taking the next task index.
+template <typename T, bool BF16>
+__global__ __launch_bounds__(NT) void k_factor_persistent(Args a) {
+    __shared__ __attribute__((aligned(16))) char lds[LDS];
+    __shared__ int s_task;
+    __shared__ int s_abort;
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) {
+            s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_task = s_abort ? a.ntasks : atomicAdd(a.ctrl, 1);
+        }
+        __syncthreads();
+        const int ti = s_task;
+        if (ti >= a.ntasks) break;
+        const DevTask t = a.tasks[ti];
+        // wait for the source tiles (each thread watches some of them)
+        for (int d = tid; d < t.dep_count; d += NT) {
+            int* flag = a.ready + a.deps[t.dep_begin + d];
+            if (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= a.epoch) continue;
+            const long long t0 = wall_clock64();
+            for (;;) {
+                __builtin_amdgcn_s_sleep(2);
+                if (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= a.epoch) break;
+                if (__hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                if (wall_clock64() - t0 > SPIN_LIMIT_TICKS) {
+                    __hip_atomic_store(a.ctrl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+        if (tid == 0) s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        if (s_abort) break;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        run_task<T, BF16, false>(t, a, lds);
+        // publish: every wave's stores reach device scope before the flag
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) {
+            __hip_atomic_store(a.ready + t.out0, a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (t.out1 >= 0)
+                __hip_atomic_store(a.ready + t.out1, a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// Levels / per-op executors: one workgroup per task of the launch.
+template <typename T, bool BF16, bool ACC>
+__global__ __launch_bounds__(NT) void k_factor_batch(Args a) {
+    __shared__ __attribute__((aligned(16))) char lds[LDS];
+    const DevTask t = a.tasks[a.task_offset + blockIdx.x];
+    run_task<T, BF16, ACC>(t, a, lds);
+}
+
+// Inverses of the diagonal L and U tiles, used by the triangular solve.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_diag_inverses(void* pool, const int32_t* ldiag,
+                                                      const int32_t* udiag, const int32_t* linv,
+                                                      const int32_t* uinv, int nbr) {
+    __shared__ __attribute__((aligned(16))) char lds[LDS];
+    const int b = blockIdx.x;
+    T* p = reinterpret_cast<T*>(pool);
+    if (b < nbr) tri_inv_task<T, true>(p, ldiag[b], linv[b], lds);
+    else tri_inv_task<T, false>(p, udiag[b - nbr], uinv[b - nbr], lds);
+}
+
+__global__ void k_f64_to_f32(const double* in, float* out, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) out[i] = (float)in[i];
+}
+
+__global__ void k_f32_to_f64(const float* in, double* out, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) out[i] = (double)in[i];
+}
+
+__global__ void k_gather_tiles(const char* pool, const int32_t* slots, char* out, int n,
+                               size_t tile_bytes) {
+    const int t = blockIdx.x;
+    if (t >= n) return;
+    const uint4* src = reinterpret_cast<const uint4*>(pool + (size_t)slots[t] * tile_bytes);
+    uint4* dst = reinterpret_cast<uint4*>(out + (size_t)t * tile_bytes);
+    for (size_t q = threadIdx.x; q < tile_bytes / 16; q += blockDim.x) dst[q] = src[q];
+}
+
+// ------------------------------------------------------------------ task graph
+namespace {
+
+int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
+    const int32_t S = d->n_ids;
+    std::vector<int32_t> slot(S > 0 ? (size_t)S : 1, -1);
+    int32_t next = 0;
+    for (int64_t k = 0; k < d->n_inputs; k++) {
+        int32_t id = d->input_ids[k];
+        if (id <= 0 || id >= S) return set_error(c, SOGL_ERR_ARG, "input id out of range");
+        if (slot[(size_t)id] >= 0) return set_error(c, SOGL_ERR_ARG, "duplicate input id");
+        slot[(size_t)id] = next++;
+        f->input_slots.push_back(slot[(size_t)id]);
+    }
+    for (int64_t k = 0; k < d->n_ops; k++) {
+        const sogl_op& o = d->ops[k];
+        for (int32_t r : {o.result, o.result2}) {
+            if (r <= 0) continue;
+            if (r >= S) return set_error(c, SOGL_ERR_ARG, "result id out of range");
+            if (slot[(size_t)r] < 0) slot[(size_t)r] = next++;
+        }
+        if (o.src >= S || o.src2 >= S || o.src < 0 || o.src2 < 0)
+            return set_error(c, SOGL_ERR_ARG, "source id out of range");
+    }
+    f->zero_slot = next++;
+    const int32_t R = d->block_rows;
+    f->linv_slot.resize((size_t)R);
+    f->uinv_slot.resize((size_t)R);
+    for (int32_t i = 0; i < R; i++) f->linv_slot[(size_t)i] = next++;
+    for (int32_t i = 0; i < R; i++) f->uinv_slot[(size_t)i] = next++;
+    f->n_slots = next;
+
+    // producer task of each slot (-1: input or never written)
+    std::vector<int32_t> prod((size_t)next, -1);
+    std::vector<DevTask> tasks;
+    std::vector<std::vector<Pair>> tpairs;
+    std::vector<int32_t> tstage;
+    std::vector<sogl_op> opl(d->ops, d->ops + d->n_ops);
+    std::vector<Pair> op_pairs;
+    f->op_tasks.clear();
+    auto src_slot = [&](int32_t id) -> int32_t { return id > 0 ? slot[(size_t)id] : -1; };
+    for (int64_t k = 0; k < d->n_ops; k++) {
+        const sogl_op& o = opl[(size_t)k];
+        if (o.result <= 0) continue;
+        const int32_t rs = slot[(size_t)o.result];
+        int kind;
+        switch (o.op) {
+            case SOGL_OP_MUL: case SOGL_OP_MULNEG: kind = TK_GEMM; break;
+            case SOGL_OP_SUB: kind = TK_SUB; break;
+            case SOGL_OP_LU: kind = TK_LU; break;
+            case SOGL_OP_LOWER_INV: kind = TK_LINV; break;
+            case SOGL_OP_UPPER_INV: kind = TK_UINV; break;
+            default:
+                return set_error(c, SOGL_ERR_UNSUPPORTED,
+                                 "unit op " + std::to_string(o.op) + " is outside the LU path");
+        }
+        DevTask ot{};
+        ot.kind = kind;
+        ot.out0 = rs;
+        ot.out1 = (kind == TK_LU && o.result2 > 0) ? slot[(size_t)o.result2] : -1;
+        ot.level = o.stage;
+        int32_t s1 = src_slot(o.src), s2 = src_slot(o.src2);
+        Pair pr{0, 0};
+        if (kind == TK_GEMM) {
+            if (s1 < 0) s1 = f->zero_slot;
+            if (s2 < 0) s2 = f->zero_slot;
+            pr = Pair{s1, o.op == SOGL_OP_MULNEG ? -s2 - 1 : s2};
+            ot.a = (int32_t)op_pairs.size();
+            ot.b = 1;
+            op_pairs.push_back(pr);
+        } else if (kind == TK_SUB) {
+            ot.a = s2;
+            ot.b = s1;
+        } else {
+            ot.a = s1 < 0 ? f->zero_slot : s1;
+            ot.b = 0;
+        }
+        f->op_tasks.push_back(ot);
+        int32_t t = prod[(size_t)rs];
+        if (t < 0) {
+            t = (int32_t)tasks.size();
+            DevTask nt = ot;
+            if (kind == TK_GEMM) { nt.a = 0; nt.b = 0; }
+            tasks.push_back(nt);
+            tpairs.emplace_back();
+            tstage.push_back(o.stage);
+            prod[(size_t)rs] = t;
+            if (ot.out1 >= 0) prod[(size_t)ot.out1] = t;
+        } else {
+            if (tasks[(size_t)t].kind != TK_GEMM || kind != TK_GEMM)
+                return set_error(c, SOGL_ERR_ARG, "a tile is written by more than one non-accumulating op");
+            tstage[(size_t)t] = std::max(tstage[(size_t)t], o.stage);
+        }
+        if (kind == TK_GEMM) tpairs[(size_t)t].push_back(pr);
+    }
+    // dependencies and levels
+    const size_t T = tasks.size();
+    std::vector<std::vector<int32_t>> dslots(T);
+    std::vector<int32_t> level(T, 0);
+    auto add_dep = [&](size_t t, int32_t s) {
+        if (s < 0) return;
+        int32_t p = prod[(size_t)s];
+        if (p < 0) return;
+        dslots[t].push_back(s);
+    };
+    for (size_t t = 0; t < T; t++) {
+        const DevTask& k = tasks[t];
+        if (k.kind == TK_GEMM) {
+            for (const Pair& p : tpairs[t]) {
+                add_dep(t, p.a);
+                add_dep(t, p.b < 0 ? -p.b - 1 : p.b);
+            }
+        } else if (k.kind == TK_SUB) {
+            add_dep(t, k.a);
+            add_dep(t, k.b);
+        } else {
+            add_dep(t, k.a);
+        }
+        std::sort(dslots[t].begin(), dslots[t].end());
+        dslots[t].erase(std::unique(dslots[t].begin(), dslots[t].end()), dslots[t].end());
+    }
+    // tasks were created in plan order, which is stage order, so producers precede consumers
+    for (size_t t = 0; t < T; t++) {
+        int32_t lv = 0;
+        for (int32_t s : dslots[t]) {
+            int32_t p = prod[(size_t)s];
+            if ((size_t)p >= t)
+                return set_error(c, SOGL_ERR_ARG, "op list is not in dependency order");
+            lv = std::max(lv, level[(size_t)p] + 1);
+        }
+        level[t] = lv;
+    }
+    std::vector<int32_t> order(T);
+    for (size_t t = 0; t < T; t++) order[t] = (int32_t)t;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        return level[(size_t)x] < level[(size_t)y];
+    });
+    f->tasks.clear();
+    f->tasks.reserve(T);
+    std::vector<Pair> pairs;
+    std::vector<int32_t> deps;
+    int32_t maxlv = T ? level[(size_t)order.back()] : -1;
+    f->level_begin.assign((size_t)maxlv + 2, 0);
+    for (int32_t t : order) {
+        DevTask k = tasks[(size_t)t];
+        k.level = level[(size_t)t];
+        if (k.kind == TK_GEMM) {
+            k.a = (int32_t)pairs.size();
+            k.b = (int32_t)tpairs[(size_t)t].size();
+            pairs.insert(pairs.end(), tpairs[(size_t)t].begin(), tpairs[(size_t)t].end());
+        }
+        k.dep_begin = (int32_t)deps.size();
+        k.dep_count = (int32_t)dslots[(size_t)t].size();
+        deps.insert(deps.end(), dslots[(size_t)t].begin(), dslots[(size_t)t].end());
+        f->level_begin[(size_t)k.level + 1]++;
+        f->tasks.push_back(k);
+    }
+    for (size_t v = 1; v < f->level_begin.size(); v++) f->level_begin[v] += f->level_begin[v - 1];
+    f->n_pairs = (int64_t)pairs.size();
+    f->n_deps = (int64_t)deps.size();
+
+    // triangular-solve structure
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> lrow((size_t)R), urow((size_t)R);
+    f->l_diag.assign((size_t)R, -1);
+    f->u_diag.assign((size_t)R, -1);
+    f->l_slots.clear();
+    f->u_slots.clear();
+    for (int which = 0; which < 2; which++) {
+        const int64_t cnt = which == 0 ? d->n_l : d->n_u;
+        const int32_t* bi = which == 0 ? d->l_bi : d->u_bi;
+        const int32_t* bj = which == 0 ? d->l_bj : d->u_bj;
+        const int32_t* id = which == 0 ? d->l_ids : d->u_ids;
+        for (int64_t k = 0; k < cnt; k++) {
+            if (bi[k] < 0 || bi[k] >= R || bj[k] < 0 || bj[k] >= R || id[k] <= 0 || id[k] >= S)
+                return set_error(c, SOGL_ERR_ARG, "factor tile out of range");
+            int32_t s = slot[(size_t)id[k]];
+            if (s < 0) return set_error(c, SOGL_ERR_ARG, "factor tile never produced");
+            (which == 0 ? f->l_slots : f->u_slots).push_back(s);
+            if (bi[k] == bj[k]) {
+                (which == 0 ? f->l_diag : f->u_diag)[(size_t)bi[k]] = s;
+            } else if ((which == 0) == (bj[k] < bi[k])) {
+                (which == 0 ? lrow : urow)[(size_t)bi[k]].push_back({bj[k], s});
+            } else {
+                return set_error(c, SOGL_ERR_ARG, "factor tile on the wrong side of the diagonal");
+            }
+        }
+    }
+    for (int32_t i = 0; i < R; i++)
+        if (f->l_diag[(size_t)i] < 0 || f->u_diag[(size_t)i] < 0)
+            return set_error(c, SOGL_ERR_ARG, "factor lacks a diagonal tile");
+    std::vector<int32_t> lp(1, 0), lc, ls, up(1, 0), uc, us;
+    for (int32_t i = 0; i < R; i++) {
+        auto& L = lrow[(size_t)i];
+        std::sort(L.begin(), L.end());
+        for (auto& e : L) { lc.push_back(e.first); ls.push_back(e.second); }
+        lp.push_back((int32_t)lc.size());
+        auto& U = urow[(size_t)i];
+        std::sort(U.begin(), U.end());
+        for (auto& e : U) { uc.push_back(e.first); us.push_back(e.second); }
+        up.push_back((int32_t)uc.size());
+    }
+    f->block_rows = R;
+
+    // device arrays
+    int rc;
+    const size_t tile_bytes = (size_t)TE * f->elem;
+    if ((rc = dev_alloc(c, f->pool, (size_t)f->n_slots * tile_bytes, "tile pool")) != SOGL_OK) return rc;
+    if ((rc = dev_alloc(c, f->tasks_d, std::max<size_t>(T, 1) * sizeof(DevTask), "tasks")) != SOGL_OK) return rc;
+    if ((rc = dev_alloc(c, f->pairs_d, std::max<size_t>(pairs.size(), 1) * sizeof(Pair), "pairs")) != SOGL_OK) return rc;
+    if ((rc = dev_alloc(c, f->deps_d, std::max<size_t>(deps.size(), 1) * 4, "deps")) != SOGL_OK) return rc;
+    if ((rc = dev_alloc(c, f->ready_d, (size_t)f->n_slots * 4, "ready flags")) != SOGL_OK) return rc;
+    if ((rc = dev_alloc(c, f->ctrl_d, CTRL_WORDS * 4, "control")) != SOGL_OK) return rc;
+    if ((rc = dev_alloc(c, f->op_tasks_d, std::max<size_t>(f->op_tasks.size(), 1) * sizeof(DevTask), "op tasks")) != SOGL_OK) return rc;
+    if ((rc = dev_alloc(c, f->op_pairs_d, std::max<size_t>(op_pairs.size(), 1) * sizeof(Pair), "op pairs")) != SOGL_OK) return rc;
+    auto up_i32 = [&](DevBuf& b, const std::vector<int32_t>& v, const char* what) -> int {
+        int r2 = dev_alloc(c, b, std::max<size_t>(v.size(), 1) * 4, what);
+        if (r2 != SOGL_OK) return r2;
+        if (!v.empty()) HIP_TRY(c, hipMemcpy(b.p, v.data(), v.size() * 4, hipMemcpyHostToDevice));
+        return SOGL_OK;
+    };
+    hipStream_t st = c->stream;
+    if (T) HIP_TRY(c, hipMemcpy(f->tasks_d.p, f->tasks.data(), T * sizeof(DevTask), hipMemcpyHostToDevice));
+    if (!pairs.empty()) HIP_TRY(c, hipMemcpy(f->pairs_d.p, pairs.data(), pairs.size() * sizeof(Pair), hipMemcpyHostToDevice));
+    if (!deps.empty()) HIP_TRY(c, hipMemcpy(f->deps_d.p, deps.data(), deps.size() * 4, hipMemcpyHostToDevice));
+    if (!f->op_tasks.empty()) HIP_TRY(c, hipMemcpy(f->op_tasks_d.p, f->op_tasks.data(), f->op_tasks.size() * sizeof(DevTask), hipMemcpyHostToDevice));
+    if (!op_pairs.empty()) HIP_TRY(c, hipMemcpy(f->op_pairs_d.p, op_pairs.data(), op_pairs.size() * sizeof(Pair), hipMemcpyHostToDevice));
+    if ((rc = up_i32(f->l_rowptr, lp, "L rows")) || (rc = up_i32(f->l_col, lc, "L cols")) ||
+        (rc = up_i32(f->l_slot, ls, "L slots")) || (rc = up_i32(f->u_rowptr, up, "U rows")) ||
+        (rc = up_i32(f->u_col, uc, "U cols")) || (rc = up_i32(f->u_slot, us, "U slots")) ||
+        (rc = up_i32(f->l_diag_d, f->l_diag, "L diag")) || (rc = up_i32(f->u_diag_d, f->u_diag, "U diag")) ||
+        (rc = up_i32(f->linv_d, f->linv_slot, "L inv")) || (rc = up_i32(f->uinv_d, f->uinv_slot, "U inv")) ||
+        (rc = up_i32(f->inputs_d, f->input_slots, "input slots")))
+        return rc;
+    hipLaunchKernelGGL(k_fill<int>, dim3(256), dim3(256), 0, st, (int*)f->ready_d.p, (size_t)f->n_slots, 0);
+    // the zero tile
+    if (f->elem == 8)
+        hipLaunchKernelGGL(k_fill<double>, dim3(16), dim3(256), 0, st,
+                           (double*)f->pool.p + (size_t)f->zero_slot * TE, (size_t)TE, 0.0);
+    else
+        hipLaunchKernelGGL(k_fill<float>, dim3(16), dim3(256), 0, st,
+                           (float*)f->pool.p + (size_t)f->zero_slot * TE, (size_t)TE, 0.0f);
+    HIP_TRY(c, hipGetLastError());
+    // input tiles: contiguous slots 0..n_inputs-1
+    const size_t nin = (size_t)d->n_inputs * TE;
+    if (nin) {
+        if (f->elem == 8) {
+            HIP_TRY(c, hipMemcpy(f->pool.p, d->input_tiles, nin * 8, hipMemcpyHostToDevice));
+        } else {
+            // stage FP64 in chunks and round to FP32 on the device
+            const size_t chunk = (size_t)1 << 24;
+            DevBuf tmp;
+            if ((rc = dev_alloc(c, tmp, std::min(chunk, nin) * 8, "staging")) != SOGL_OK) return rc;
+            for (size_t o = 0; o < nin; o += chunk) {
+                size_t m = std::min(chunk, nin - o);
+                hipError_t e = hipMemcpy(tmp.p, d->input_tiles + o, m * 8, hipMemcpyHostToDevice);
+                if (e != hipSuccess) { dev_free(tmp); return check_hip(c, e, "input upload"); }
+                hipLaunchKernelGGL(k_f64_to_f32, dim3(1024), dim3(256), 0, st, (const double*)tmp.p,
+                                   (float*)f->pool.p + o, m);
+                e = hipStreamSynchronize(st);
+                if (e != hipSuccess) { dev_free(tmp); return check_hip(c, e, "input convert"); }
+            }
+            dev_free(tmp);
+        }
+    }
+    HIP_TRY(c, hipStreamSynchronize(st));
+    return SOGL_OK;
+}
+
+void free_factor(Factor* f) {
+    for (DevBuf* b : {&f->pool, &f->tasks_d, &f->pairs_d, &f->deps_d, &f->ready_d, &f->ctrl_d,
+                      &f->op_tasks_d, &f->op_pairs_d, &f->inputs_d, &f->l_rowptr, &f->l_col,
+                      &f->l_slot, &f->u_rowptr, &f->u_col, &f->u_slot, &f->l_diag_d, &f->u_diag_d,
+                      &f->linv_d, &f->uinv_d, &f->solve_ready, &f->solve_ctrl, &f->vec_b, &f->vec_x,
+                      &f->vec_y, &f->csr_ptr, &f->csr_col, &f->csr_val, &f->vec_r, &f->vec_d, &f->red})
+        dev_free(*b);
+    delete f;
+}
+
+template <typename T, bool BF16>
+int launch_factor(Context* c, Factor* f, int mode) {
+    hipStream_t st = c->stream;
+    Args a{};
+    a.pool = f->pool.p;
+    a.ready = (int*)f->ready_d.p;
+    a.ctrl = (int*)f->ctrl_d.p;
+    a.epoch = f->epoch;
+    a.growth_limit = f->growth_limit;
+    a.deps = (const int32_t*)f->deps_d.p;
+    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, st, a.ctrl);
+    if (mode == SOGL_EXEC_PERSISTENT) {
+        a.tasks = (const DevTask*)f->tasks_d.p;
+        a.pairs = (const Pair*)f->pairs_d.p;
+        a.ntasks = (int32_t)f->tasks.size();
+        int per_cu = 0;
+        HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_persistent<T, BF16>, NT, 0));
+        if (per_cu < 1) per_cu = 1;
+        int grid = std::max(1, std::min<int>(a.ntasks, c->num_cu * per_cu));
+        if (a.ntasks > 0) hipLaunchKernelGGL((k_factor_persistent<T, BF16>), dim3(grid), dim3(NT), 0, st, a);
+    } else if (mode == SOGL_EXEC_LEVELS) {
+        a.tasks = (const DevTask*)f->tasks_d.p;
+        a.pairs = (const Pair*)f->pairs_d.p;
+        a.ntasks = (int32_t)f->tasks.size();
+        for (size_t v = 0; v + 1 < f->level_begin.size(); v++) {
+            int cnt = f->level_begin[v + 1] - f->level_begin[v];
+            if (cnt <= 0) continue;
+            a.task_offset = f->level_begin[v];
+            hipLaunchKernelGGL((k_factor_batch<T, BF16, false>), dim3(cnt), dim3(NT), 0, st, a);
+        }
+    } else if (mode == SOGL_EXEC_PER_OP) {
+        if (BF16) return set_error(c, SOGL_ERR_UNSUPPORTED, "per-op executor has no bf16 variant");
+        // results start at zero; every op then runs alone, C += A*B reading C
+        const size_t first_result = f->input_slots.size();
+        const size_t nres = (size_t)f->zero_slot - first_result;
+        hipLaunchKernelGGL(k_fill<T>, dim3(4096), dim3(256), 0, st,
+                           (T*)f->pool.p + first_result * TE, nres * TE, T(0));
+        a.tasks = (const DevTask*)f->op_tasks_d.p;
+        a.pairs = (const Pair*)f->op_pairs_d.p;
+        a.ntasks = (int32_t)f->op_tasks.size();
+        for (int32_t k = 0; k < a.ntasks; k++) {
+            a.task_offset = k;
+            hipLaunchKernelGGL((k_factor_batch<T, false, true>), dim3(1), dim3(NT), 0, st, a);
+        }
+    } else {
+        return set_error(c, SOGL_ERR_ARG, "unknown executor mode");
+    }
+    HIP_TRY(c, hipGetLastError());
+    const int R = f->block_rows;
+    if (R > 0)
+        hipLaunchKernelGGL(k_diag_inverses<T>, dim3(2 * R), dim3(NT), 0, st, f->pool.p,
+                           (const int32_t*)f->l_diag_d.p, (const int32_t*)f->u_diag_d.p,
+                           (const int32_t*)f->linv_d.p, (const int32_t*)f->uinv_d.p, R);
+    HIP_TRY(c, hipGetLastError());
+    return SOGL_OK;
+}
+
+}  // namespace
+
+}  // namespace sogl_dev
+
+using namespace sogl_dev;
+
+// ------------------------------------------------------------------ C ABI
+extern "C" int sogl_ctx_create(int32_t device, sogl_ctx** out) {
+    if (!out) return set_error(nullptr, SOGL_ERR_ARG, "null output");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return set_error(nullptr, SOGL_ERR_DEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= n) return set_error(nullptr, SOGL_ERR_ARG, "device index out of range");
+    sogl_ctx* x = new sogl_ctx();
+    Context* c = &x->c;
+    c->device = device;
+    int rc;
+    if ((rc = check_hip(c, hipSetDevice(device), "hipSetDevice")) ||
+        (rc = check_hip(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream")) ||
+        (rc = check_hip(c, hipEventCreate(&c->ev0), "event")) ||
+        (rc = check_hip(c, hipEventCreate(&c->ev1), "event")) ||
+        (rc = check_hip(c, hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device), "attr"))) {
+        delete x;
+        return rc;
+    }
+    *out = x;
+    return SOGL_OK;
+}
+
+extern "C" void sogl_ctx_destroy(sogl_ctx* x) {
+    if (!x) return;
+    Context* c = &x->c;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& kv : c->factors) free_factor(kv.second);
+    c->factors.clear();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete x;
+}
+
+extern "C" const char* sogl_ctx_error(const sogl_ctx* x) { return x ? x->c.err.c_str() : ""; }
+
+extern "C" int sogl_ctx_mem_info(sogl_ctx* x, size_t* free_bytes, size_t* total_bytes) {
+    if (!x) return set_error(nullptr, SOGL_ERR_ARG, "null context");
+    Context* c = &x->c;
+    HIP_TRY(c, hipSetDevice(c->device));
+    size_t f = 0, t = 0;
+    HIP_TRY(c, hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return SOGL_OK;
+}
+
+extern "C" int sogl_ctx_sync(sogl_ctx* x) {
+    if (!x) return set_error(nullptr, SOGL_ERR_ARG, "null context");
+    HIP_TRY(&x->c, hipStreamSynchronize(x->c.stream));
+    return SOGL_OK;
+}
+
+extern "C" int sogl_factor_load(sogl_ctx* x, const sogl_factor_desc* d, int32_t* factor_id) {
+    if (!x || !d || !factor_id) return set_error(x ? &x->c : nullptr, SOGL_ERR_ARG, "null argument");
+    Context* c = &x->c;
+    std::lock_guard<std::mutex> g(c->mu);
+    if (d->n_ids <= 0 || d->n_ops < 0 || d->n_inputs < 0 || d->block_rows <= 0 ||
+        (d->n_ops && !d->ops) || (d->n_inputs && (!d->input_ids || !d->input_tiles)))
+        return set_error(c, SOGL_ERR_ARG, "bad factor description");
+    if (d->precision != SOGL_FP64 && d->precision != SOGL_FP32 && d->precision != SOGL_BF16)
+        return set_error(c, SOGL_ERR_ARG, "unknown precision");
+    HIP_TRY(c, hipSetDevice(c->device));
+    Factor* f = new Factor();
+    f->precision = d->precision;
+    f->elem = d->precision == SOGL_FP64 ? 8 : 4;
+    int rc = build_graph(c, d, f);
+    if (rc != SOGL_OK) {
+        free_factor(f);
+        return rc;
+    }
+    f->id = c->next_id++;
+    c->factors[f->id] = f;
+    *factor_id = f->id;
+    return SOGL_OK;
+}
+
+extern "C" int sogl_factor_load_plan(sogl_ctx* x, const sogl_plan* plan, int32_t precision,
+                                     int32_t* factor_id) {
+    sogl_factor_desc d;
+    if (!sogl::plan_factor_desc(plan, precision, &d))
+        return set_error(x ? &x->c : nullptr, SOGL_ERR_ARG, "null plan");
+    return sogl_factor_load(x, &d, factor_id);
+}
+
+extern "C" int sogl_factor_run(sogl_ctx* x, int32_t factor_id, int32_t exec_mode) {
+    if (!x) return set_error(nullptr, SOGL_ERR_ARG, "null context");
+    Context* c = &x->c;
+    std::lock_guard<std::mutex> g(c->mu);
+    Factor* f = find_factor(c, factor_id);
+    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+    HIP_TRY(c, hipSetDevice(c->device));
+    f->epoch++;
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    int rc;
+    if (f->precision == SOGL_FP64) rc = launch_factor<double, false>(c, f, exec_mode);
+    else if (f->precision == SOGL_FP32) rc = launch_factor<float, false>(c, f, exec_mode);
+    else rc = launch_factor<float, true>(c, f, exec_mode);
+    if (rc != SOGL_OK) return rc;
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    f->factor_ms = ms;
+    int ctrl[CTRL_WORDS];
+    HIP_TRY(c, hipMemcpy(ctrl, f->ctrl_d.p, sizeof(ctrl), hipMemcpyDeviceToHost));
+    f->growth_tiles = ctrl[2];
+    float gm;
+    std::memcpy(&gm, &ctrl[3], 4);
+    f->max_growth = gm;
+    if (ctrl[1]) return set_error(c, SOGL_ERR_TIMEOUT, "factor: a dependency wait exceeded its bound");
+    return SOGL_OK;
+}
+
+extern "C" int sogl_factor_release(sogl_ctx* x, int32_t factor_id) {
+    if (!x) return set_error(nullptr, SOGL_ERR_ARG, "null context");
+    Context* c = &x->c;
+    std::lock_guard<std::mutex> g(c->mu);
+    Factor* f = find_factor(c, factor_id);
+    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_factor(f);
+    c->factors.erase(factor_id);
+    return SOGL_OK;
+}
+
+extern "C" int sogl_factor_get_stats(sogl_ctx* x, int32_t factor_id, sogl_factor_stats* st) {
+    if (!x || !st) return set_error(nullptr, SOGL_ERR_ARG, "null argument");
+    Context* c = &x->c;
+    Factor* f = find_factor(c, factor_id);
+    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+    st->factor_ms = f->factor_ms;
+    st->solve_ms = f->solve_ms;
+    st->pool_bytes = (int64_t)f->pool.bytes;
+    st->n_tasks = (int64_t)f->tasks.size();
+    st->n_levels = (int32_t)f->level_begin.size() - 1;
+    st->growth_tiles = f->growth_tiles;
+    st->max_growth = f->max_growth;
+    return SOGL_OK;
+}
+
+extern "C" int sogl_factor_get_tiles(sogl_ctx* x, int32_t factor_id, int32_t which, double* tiles) {
+    if (!x || !tiles) return set_error(nullptr, SOGL_ERR_ARG, "null argument");
+    Context* c = &x->c;
+    std::lock_guard<std::mutex> g(c->mu);
+    Factor* f = find_factor(c, factor_id);
+    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const std::vector<int32_t>& sl = which == 0 ? f->l_slots : f->u_slots;
+    if (sl.empty()) return SOGL_OK;
+    const size_t tb = (size_t)TE * f->elem;
+    DevBuf ids, out, out64;
+    int rc;
+    if ((rc = dev_alloc(c, ids, sl.size() * 4, "ids")) || (rc = dev_alloc(c, out, sl.size() * tb, "gather")))
+        return rc;
+    HIP_TRY(c, hipMemcpy(ids.p, sl.data(), sl.size() * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_gather_tiles, dim3((unsigned)sl.size()), dim3(256), 0, c->stream,
+                       (const char*)f->pool.p, (const int32_t*)ids.p, (char*)out.p, (int)sl.size(), tb);
+    if (f->elem == 4) {
+        if ((rc = dev_alloc(c, out64, sl.size() * TE * 8, "gather64"))) return rc;
+        hipLaunchKernelGGL(k_f32_to_f64, dim3(1024), dim3(256), 0, c->stream, (const float*)out.p,
+                           (double*)out64.p, sl.size() * TE);
+        HIP_TRY(c, hipMemcpyAsync(tiles, out64.p, sl.size() * TE * 8, hipMemcpyDeviceToHost, c->stream));
+    } else {
+        HIP_TRY(c, hipMemcpyAsync(tiles, out.p, sl.size() * tb, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    dev_free(ids);
+    dev_free(out);
+    dev_free(out64);
+    return SOGL_OK;
+}

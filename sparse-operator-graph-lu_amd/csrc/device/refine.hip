@@ -1,0 +1,254 @@
+// Mixed precision: an FP32 (or bf16-GEMM) factor is corrected by FP64
+// iterative refinement against the original matrix,
+//     x_0 = solve(b);  repeat  r = b - A x (FP64 CSR SpMV);  x += solve(r)
+// until ||r||_2 / (||A||_F ||x||_2 + ||b||_2) <= tol.  FP64 lives only in the
+// residual, the correction accumulation and the solve's vector sums; the
+// factor tiles stay in their low precision.  The reference has no
+// counterpart (it factors in FP64 only); its residual check is the max-abs
+// mtx::checkResult (mtx.cpp:183-215).
+//
+// Also the drop-in sogl_solveLU, which mirrors SOGLU::solveLU (solver.h:24).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+#include <vector>
+
+#include "device.h"
+#include "tile_ops.h"
+
+namespace sogl_dev {
+int solve_device(Context* c, Factor* f, const double* d_b, double* d_x, double* d_y, int nrhs);
+int ensure_vectors(Context* c, Factor* f, int nrhs);
+}  // namespace sogl_dev
+
+using namespace sogl_dev;
+
+namespace {
+
+#define HIP_TRY(ctx, expr)                                       \
+    do {                                                         \
+        int _rc = check_hip((ctx), (expr), #expr);               \
+        if (_rc != SOGL_OK) return _rc;                          \
+    } while (0)
+
+constexpr int RED_BLOCKS = 256;
+
+// r = b - A x on the permuted padded system, one row per thread, every rhs column
+__global__ void k_residual(const int64_t* ptr, const int32_t* col, const double* val,
+                           const double* b, const double* x, double* r, int64_t N, int nrhs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    for (int c = 0; c < nrhs; c++) {
+        const double* xc = x + (int64_t)c * N;
+        double s = 0;
+        for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) s = fma(val[e], xc[col[e]], s);
+        r[i + (int64_t)c * N] = b[i + (int64_t)c * N] - s;
+    }
+}
+
+__global__ void k_axpy(double* x, const double* d, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += d[i];
+}
+
+// partial sums of squares of r and x over the first n rows, per column:
+// out[(c*2 + {0,1}) * RED_BLOCKS + block]
+__global__ void k_norms(const double* r, const double* x, int64_t n, int64_t N, double* out) {
+    __shared__ double sr[256], sx[256];
+    const int c = blockIdx.y;
+    double ar = 0, ax = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double a = r[i + c * N], b = x[i + c * N];
+        ar += a * a;
+        ax += b * b;
+    }
+    sr[threadIdx.x] = ar;
+    sx[threadIdx.x] = ax;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sr[threadIdx.x] += sr[threadIdx.x + o];
+            sx[threadIdx.x] += sx[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[(c * 2 + 0) * RED_BLOCKS + blockIdx.x] = sr[0];
+        out[(c * 2 + 1) * RED_BLOCKS + blockIdx.x] = sx[0];
+    }
+}
+
+}  // namespace
+
+namespace sogl {
+struct PlanView {
+    int32_t n;
+    int64_t N;
+    const int32_t* new_order;
+    int64_t nnz;
+    const int32_t* row;
+    const int32_t* col;
+    const double* val;
+    int64_t nnz_orig;  // entries of the original matrix (the rest is padding)
+};
+bool plan_view(const sogl_plan* p, PlanView* v);
+void set_last_error(const std::string& s);
+}  // namespace sogl
+
+extern "C" int sogl_refine_attach(sogl_ctx* x, int32_t factor_id, const sogl_plan* plan) {
+    if (!x || !plan) return set_error(x ? &x->c : nullptr, SOGL_ERR_ARG, "null argument");
+    Context* c = &x->c;
+    std::lock_guard<std::mutex> g(c->mu);
+    Factor* f = find_factor(c, factor_id);
+    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+    sogl::PlanView v;
+    if (!sogl::plan_view(plan, &v)) return set_error(c, SOGL_ERR_ARG, "bad plan");
+    if (v.N != (int64_t)f->block_rows * TS) return set_error(c, SOGL_ERR_ARG, "plan does not match factor");
+    HIP_TRY(c, hipSetDevice(c->device));
+    // CSR of the permuted padded matrix
+    std::vector<int64_t> ptr((size_t)v.N + 1, 0);
+    for (int64_t k = 0; k < v.nnz; k++) ptr[(size_t)v.row[k] + 1]++;
+    for (int64_t i = 0; i < v.N; i++) ptr[(size_t)i + 1] += ptr[(size_t)i];
+    std::vector<int32_t> col((size_t)v.nnz);
+    std::vector<double> val((size_t)v.nnz);
+    std::vector<int64_t> pos(ptr.begin(), ptr.end() - 1);
+    double fro = 0;
+    for (int64_t k = 0; k < v.nnz; k++) {
+        const int64_t p = pos[(size_t)v.row[k]]++;
+        col[(size_t)p] = v.col[k];
+        val[(size_t)p] = v.val[k];
+        if (k < v.nnz_orig) fro += v.val[k] * v.val[k];
+    }
+    int rc;
+    if ((rc = dev_alloc(c, f->csr_ptr, ptr.size() * 8, "csr ptr")) ||
+        (rc = dev_alloc(c, f->csr_col, std::max<size_t>(col.size(), 1) * 4, "csr col")) ||
+        (rc = dev_alloc(c, f->csr_val, std::max<size_t>(val.size(), 1) * 8, "csr val")))
+        return rc;
+    HIP_TRY(c, hipMemcpy(f->csr_ptr.p, ptr.data(), ptr.size() * 8, hipMemcpyHostToDevice));
+    if (!col.empty()) {
+        HIP_TRY(c, hipMemcpy(f->csr_col.p, col.data(), col.size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(f->csr_val.p, val.data(), val.size() * 8, hipMemcpyHostToDevice));
+    }
+    f->n = v.n;
+    f->nnz = v.nnz;
+    f->new_order.assign(v.new_order, v.new_order + v.n);
+    f->a_fro = std::sqrt(fro);
+    f->has_refine = true;
+    return SOGL_OK;
+}
+
+extern "C" int sogl_solve_refined(sogl_ctx* x, int32_t factor_id, const double* b, double* xo,
+                                  int32_t nrhs, double tol, int32_t max_iter, int32_t* iters,
+                                  double* resid) {
+    if (!x || !b || !xo || nrhs <= 0) return set_error(x ? &x->c : nullptr, SOGL_ERR_ARG, "bad argument");
+    Context* c = &x->c;
+    std::lock_guard<std::mutex> g(c->mu);
+    Factor* f = find_factor(c, factor_id);
+    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+    if (!f->has_refine) return set_error(c, SOGL_ERR_ARG, "sogl_refine_attach was not called");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const int32_t n = f->n;
+    const int64_t N = (int64_t)f->block_rows * TS;
+    int rc;
+    if ((rc = ensure_vectors(c, f, nrhs))) return rc;
+    if ((rc = dev_alloc(c, f->vec_r, (size_t)N * nrhs * 8, "residual")) ||
+        (rc = dev_alloc(c, f->vec_d, (size_t)N * nrhs * 8, "correction")) ||
+        (rc = dev_alloc(c, f->red, (size_t)nrhs * 2 * RED_BLOCKS * 8, "norms")))
+        return rc;
+    // permute + pad on the host (padding rows are identity with b = 1)
+    std::vector<double> bp((size_t)N * nrhs, 1.0);
+    std::vector<double> bnorm((size_t)nrhs, 0.0);
+    for (int32_t r = 0; r < nrhs; r++)
+        for (int32_t i = 0; i < n; i++) {
+            const double v = b[(int64_t)r * n + f->new_order[(size_t)i]];
+            bp[(size_t)(r * N + i)] = v;
+            bnorm[(size_t)r] += v * v;
+        }
+    for (auto& v : bnorm) v = std::sqrt(v);
+    double* db = (double*)f->vec_b.p;
+    double* dx = (double*)f->vec_x.p;
+    double* dr = (double*)f->vec_r.p;
+    double* dd = (double*)f->vec_d.p;
+    HIP_TRY(c, hipMemcpyAsync(db, bp.data(), bp.size() * 8, hipMemcpyHostToDevice, c->stream));
+    if ((rc = solve_device(c, f, db, dx, (double*)f->vec_y.p, nrhs))) return rc;
+    std::vector<double> part((size_t)nrhs * 2 * RED_BLOCKS);
+    int it = 0;
+    double worst = 0;
+    const unsigned gN = (unsigned)((N + 255) / 256);
+    const unsigned gA = (unsigned)((N * nrhs + 255) / 256);
+    for (;;) {
+        hipLaunchKernelGGL(k_residual, dim3(gN), dim3(256), 0, c->stream, (const int64_t*)f->csr_ptr.p,
+                           (const int32_t*)f->csr_col.p, (const double*)f->csr_val.p, db, dx, dr, N, nrhs);
+        hipLaunchKernelGGL(k_norms, dim3(RED_BLOCKS, nrhs), dim3(256), 0, c->stream, dr, dx, (int64_t)n, N,
+                           (double*)f->red.p);
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipMemcpyAsync(part.data(), f->red.p, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        worst = 0;
+        for (int32_t r = 0; r < nrhs; r++) {
+            double sr = 0, sx = 0;
+            for (int q = 0; q < RED_BLOCKS; q++) {
+                sr += part[(size_t)((r * 2 + 0) * RED_BLOCKS + q)];
+                sx += part[(size_t)((r * 2 + 1) * RED_BLOCKS + q)];
+            }
+            const double den = f->a_fro * std::sqrt(sx) + bnorm[(size_t)r];
+            const double rel = den > 0 ? std::sqrt(sr) / den : std::sqrt(sr);
+            if (!(rel <= worst)) worst = rel;  // NaN propagates
+        }
+        if (worst <= tol || it >= max_iter || std::isnan(worst)) break;
+        if ((rc = solve_device(c, f, dr, dd, (double*)f->vec_y.p, nrhs))) return rc;
+        hipLaunchKernelGGL(k_axpy, dim3(gA), dim3(256), 0, c->stream, dx, (const double*)dd, N * nrhs);
+        HIP_TRY(c, hipGetLastError());
+        it++;
+    }
+    std::vector<double> xp((size_t)N * nrhs);
+    HIP_TRY(c, hipMemcpyAsync(xp.data(), dx, xp.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int32_t r = 0; r < nrhs; r++)
+        for (int32_t i = 0; i < n; i++) xo[(int64_t)r * n + f->new_order[(size_t)i]] = xp[(size_t)(r * N + i)];
+    if (iters) *iters = it;
+    if (resid) *resid = worst;
+    if (std::isnan(worst)) return set_error(c, SOGL_ERR_NAN, "refinement: NaN in the residual");
+    if (!(worst <= tol)) return set_error(c, SOGL_ERR_NOT_CONVERGED, "refinement did not reach the tolerance");
+    return SOGL_OK;
+}
+
+// ------------------------------------------------------------------ drop-in solveLU
+extern "C" int sogl_factor_load_plan(sogl_ctx* x, const sogl_plan* plan, int32_t precision,
+                                     int32_t* factor_id);
+
+extern "C" int sogl_solveLU(int32_t dim, int64_t valcount, int32_t symmetric, const int32_t* index_i,
+                            const int32_t* index_j, const double* vals, const double* b, double* xo) {
+    static std::mutex mu;
+    static sogl_ctx* ctx = nullptr;
+    std::lock_guard<std::mutex> g(mu);
+    if (!xo || !b) return set_error(nullptr, SOGL_ERR_ARG, "null b or x");
+    sogl_plan* plan = nullptr;
+    int rc = sogl_plan_create(dim, valcount, symmetric, index_i, index_j, vals, b, 0, &plan);
+    if (rc) return rc;
+    if (!ctx && (rc = sogl_ctx_create(0, &ctx))) {
+        sogl_plan_destroy(plan);
+        return rc;
+    }
+    int32_t fid = 0;
+    if ((rc = sogl_factor_load_plan(ctx, plan, SOGL_FP64, &fid))) {
+        sogl_plan_destroy(plan);
+        return rc;
+    }
+    sogl_plan_info info;
+    sogl_plan_get_info(plan, &info);
+    const int64_t N = (int64_t)info.block_rows * 64;
+    std::vector<double> bp((size_t)N), xp((size_t)N);
+    rc = sogl_factor_run(ctx, fid, SOGL_EXEC_PERSISTENT);
+    if (!rc) rc = sogl_plan_permute_rhs(plan, b, 1, dim, bp.data());
+    if (!rc) rc = sogl_solve(ctx, fid, bp.data(), xp.data(), nullptr, 1);
+    if (!rc) rc = sogl_plan_unpermute(plan, xp.data(), 1, xo, dim);
+    sogl_factor_release(ctx, fid);
+    sogl_plan_destroy(plan);
+    if (!rc)
+        for (int32_t i = 0; i < dim; i++)
+            if (std::isnan(xo[i])) return set_error(nullptr, SOGL_ERR_NAN, "NaN in the solution");
+    return rc;
+}

@@ -1,0 +1,271 @@
+// Block triangular solve with many right-hand sides: replaces
+// BlockPlanner::solve / lowerSolver / upperSolver / updateVector
+// (BlockPlanner.cpp:669-862), which recurse over the L and U quadtrees with
+// one right-hand side, serially.
+//
+// Here each block row is a task of a persistent kernel: forward,
+//     y_I = Linv_II (b_I - sum_{J<I} L_IJ y_J)
+// backward,
+//     x_I = Uinv_II (y_I - sum_{J>I} U_IJ x_J)
+// with the inverses of the diagonal tiles made once per factor.  Block rows
+// are taken in dependency order (ascending / descending), a row waits on the
+// ready flags of the rows it reads, so independent rows overlap and the
+// chain only serialises where the factor's block structure does.  Sums run
+// in FP64 for every tile precision.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "device.h"
+#include "tile_ops.h"
+
+namespace sogl_dev {
+
+constexpr int CW = 16;  // right-hand-side columns per pass over a row
+constexpr long long SOLVE_SPIN_LIMIT = 20LL * 100 * 1000 * 1000;
+
+struct SolveArgs {
+    const void* pool;
+    const int32_t* rowptr;
+    const int32_t* col;
+    const int32_t* slot;
+    const int32_t* dinv;  // per block row: slot of the inverted diagonal tile
+    const double* in;
+    double* out;
+    int64_t ld;           // column stride of in/out (padded n)
+    int32_t nrhs;
+    int32_t nbr;
+    int32_t forward;
+    int32_t epoch;
+    int* ready;
+    int* ctrl;            // [0] row counter, [1] abort
+};
+
+template <typename T>
+__device__ inline void stage_tile(const T* g, double* s, int tid) {
+    const int r = tid >> 2, c0 = (tid & 3) * 16;
+#pragma unroll
+    for (int q = 0; q < 16; q++) s[r * LDA + c0 + q] = (double)g[r * TS + c0 + q];
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_trsv_persistent(SolveArgs a) {
+    __shared__ double tile[TS * LDA];
+    __shared__ double vec[TS * CW];
+    __shared__ int s_row, s_abort;
+    const int tid = threadIdx.x, r = tid >> 2, kp = tid & 3;
+    const T* pool = reinterpret_cast<const T*>(a.pool);
+    for (;;) {
+        if (tid == 0) {
+            s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_row = s_abort ? a.nbr : atomicAdd(a.ctrl, 1);
+        }
+        __syncthreads();
+        const int q = s_row;
+        if (q >= a.nbr) break;
+        const int I = a.forward ? q : a.nbr - 1 - q;
+        const int e0 = a.rowptr[I], e1 = a.rowptr[I + 1];
+        // wait for every row this one reads
+        for (int e = e0 + tid; e < e1; e += NT) {
+            int* flag = a.ready + a.col[e];
+            if (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= a.epoch) continue;
+            const long long t0 = wall_clock64();
+            for (;;) {
+                __builtin_amdgcn_s_sleep(1);
+                if (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= a.epoch) break;
+                if (__hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                if (wall_clock64() - t0 > SOLVE_SPIN_LIMIT) {
+                    __hip_atomic_store(a.ctrl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+        if (tid == 0) s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        if (s_abort) break;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const int64_t rowI = (int64_t)I * TS;
+        for (int c0 = 0; c0 < a.nrhs; c0 += CW) {
+            const int cw = min(CW, a.nrhs - c0);
+            double acc[CW];
+#pragma unroll
+            for (int c = 0; c < CW; c++)
+                acc[c] = (kp == 0 && c < cw) ? a.in[rowI + r + (int64_t)(c0 + c) * a.ld] : 0.0;
+            for (int e = e0; e < e1; e++) {
+                const int J = a.col[e];
+                __syncthreads();
+                stage_tile(pool + (size_t)a.slot[e] * TE, tile, tid);
+                {   // the CW-column chunk of x_J / y_J, k-major
+                    const int k = tid >> 2, cc = (tid & 3) * 4;
+#pragma unroll
+                    for (int c = 0; c < 4; c++)
+                        vec[k * CW + cc + c] = (cc + c < cw) ? a.out[(int64_t)J * TS + k + (int64_t)(c0 + cc + c) * a.ld] : 0.0;
+                }
+                __syncthreads();
+#pragma unroll 4
+                for (int kk = 0; kk < 16; kk++) {
+                    const int k = kp * 16 + kk;
+                    const double l = tile[r * LDA + k];
+#pragma unroll
+                    for (int c = 0; c < CW; c++) acc[c] -= l * vec[k * CW + c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < CW; c++) {
+                acc[c] += __shfl_xor(acc[c], 1);
+                acc[c] += __shfl_xor(acc[c], 2);
+            }
+            __syncthreads();
+            stage_tile(pool + (size_t)a.dinv[I] * TE, tile, tid);
+            if (kp == 0) {
+#pragma unroll
+                for (int c = 0; c < CW; c++) vec[r * CW + c] = acc[c];
+            }
+            __syncthreads();
+            double o[CW];
+#pragma unroll
+            for (int c = 0; c < CW; c++) o[c] = 0.0;
+#pragma unroll 4
+            for (int kk = 0; kk < 16; kk++) {
+                const int k = kp * 16 + kk;
+                const double d = tile[r * LDA + k];
+#pragma unroll
+                for (int c = 0; c < CW; c++) o[c] += d * vec[k * CW + c];
+            }
+#pragma unroll
+            for (int c = 0; c < CW; c++) {
+                o[c] += __shfl_xor(o[c], 1);
+                o[c] += __shfl_xor(o[c], 2);
+            }
+            if (kp == 0) {
+                for (int c = 0; c < cw; c++) a.out[rowI + r + (int64_t)(c0 + c) * a.ld] = o[c];
+            }
+        }
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(a.ready + I, a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ void k_solve_ctrl_reset(int* ctrl) {
+    if (threadIdx.x < 4) ctrl[threadIdx.x] = 0;
+}
+
+__global__ void k_zero_i32(int* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+namespace {
+
+#define HIP_TRY(ctx, expr)                                       \
+    do {                                                         \
+        int _rc = check_hip((ctx), (expr), #expr);               \
+        if (_rc != SOGL_OK) return _rc;                          \
+    } while (0)
+
+template <typename T>
+int launch_pass(Context* c, Factor* f, bool forward, const double* in, double* out, int nrhs) {
+    SolveArgs a{};
+    a.pool = f->pool.p;
+    a.rowptr = (const int32_t*)(forward ? f->l_rowptr.p : f->u_rowptr.p);
+    a.col = (const int32_t*)(forward ? f->l_col.p : f->u_col.p);
+    a.slot = (const int32_t*)(forward ? f->l_slot.p : f->u_slot.p);
+    a.dinv = (const int32_t*)(forward ? f->linv_d.p : f->uinv_d.p);
+    a.in = in;
+    a.out = out;
+    a.ld = (int64_t)f->block_rows * TS;
+    a.nrhs = nrhs;
+    a.nbr = f->block_rows;
+    a.forward = forward ? 1 : 0;
+    a.epoch = ++f->solve_epoch;
+    a.ready = (int*)f->solve_ready.p;
+    a.ctrl = (int*)f->solve_ctrl.p;
+    hipLaunchKernelGGL(k_solve_ctrl_reset, dim3(1), dim3(64), 0, c->stream, a.ctrl);
+    int per_cu = 0;
+    HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trsv_persistent<T>, NT, 0));
+    if (per_cu < 1) per_cu = 1;
+    const int grid = std::max(1, std::min(a.nbr, c->num_cu * per_cu));
+    hipLaunchKernelGGL(k_trsv_persistent<T>, dim3(grid), dim3(NT), 0, c->stream, a);
+    HIP_TRY(c, hipGetLastError());
+    return SOGL_OK;
+}
+
+}  // namespace
+
+int solve_device(Context* c, Factor* f, const double* d_b, double* d_x, double* d_y, int nrhs) {
+    const int R = f->block_rows;
+    int rc;
+    if ((rc = dev_alloc(c, f->solve_ready, (size_t)R * 4, "solve flags"))) return rc;
+    if ((rc = dev_alloc(c, f->solve_ctrl, 16, "solve control"))) return rc;
+    if (f->solve_epoch == 0) {
+        hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, c->stream, (int*)f->solve_ready.p, R);
+        HIP_TRY(c, hipGetLastError());
+    }
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    if (f->elem == 8) {
+        if ((rc = launch_pass<double>(c, f, true, d_b, d_y, nrhs))) return rc;
+        if ((rc = launch_pass<double>(c, f, false, d_y, d_x, nrhs))) return rc;
+    } else {
+        if ((rc = launch_pass<float>(c, f, true, d_b, d_y, nrhs))) return rc;
+        if ((rc = launch_pass<float>(c, f, false, d_y, d_x, nrhs))) return rc;
+    }
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    f->solve_ms = ms;
+    int ctrl[4];
+    HIP_TRY(c, hipMemcpy(ctrl, f->solve_ctrl.p, sizeof(ctrl), hipMemcpyDeviceToHost));
+    if (ctrl[1]) return set_error(c, SOGL_ERR_TIMEOUT, "solve: a dependency wait exceeded its bound");
+    return SOGL_OK;
+}
+
+int ensure_vectors(Context* c, Factor* f, int nrhs) {
+    const size_t need = (size_t)f->block_rows * TS * (size_t)nrhs;
+    if (f->vec_cap >= need) return SOGL_OK;
+    int rc;
+    if ((rc = dev_alloc(c, f->vec_b, need * 8, "rhs")) || (rc = dev_alloc(c, f->vec_x, need * 8, "x")) ||
+        (rc = dev_alloc(c, f->vec_y, need * 8, "y")))
+        return rc;
+    f->vec_cap = need;
+    return SOGL_OK;
+}
+
+}  // namespace sogl_dev
+
+using namespace sogl_dev;
+
+extern "C" int sogl_solve(sogl_ctx* x, int32_t factor_id, const double* bp, double* xp, double* y,
+                          int32_t nrhs) {
+    if (!x || !bp || !xp || nrhs <= 0) return set_error(x ? &x->c : nullptr, SOGL_ERR_ARG, "bad argument");
+    Context* c = &x->c;
+    std::lock_guard<std::mutex> g(c->mu);
+    Factor* f = find_factor(c, factor_id);
+    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc;
+    if ((rc = ensure_vectors(c, f, nrhs))) return rc;
+    const size_t bytes = (size_t)f->block_rows * TS * nrhs * 8;
+    HIP_TRY(c, hipMemcpyAsync(f->vec_b.p, bp, bytes, hipMemcpyHostToDevice, c->stream));
+    if ((rc = solve_device(c, f, (const double*)f->vec_b.p, (double*)f->vec_x.p, (double*)f->vec_y.p, nrhs)))
+        return rc;
+    HIP_TRY(c, hipMemcpyAsync(xp, f->vec_x.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    if (y) HIP_TRY(c, hipMemcpyAsync(y, f->vec_y.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return SOGL_OK;
+}
+
+extern "C" int sogl_solve_device(sogl_ctx* x, int32_t factor_id, const double* d_bp, double* d_xp,
+                                 int32_t nrhs) {
+    if (!x || !d_bp || !d_xp || nrhs <= 0) return set_error(x ? &x->c : nullptr, SOGL_ERR_ARG, "bad argument");
+    Context* c = &x->c;
+    std::lock_guard<std::mutex> g(c->mu);
+    Factor* f = find_factor(c, factor_id);
+    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+    HIP_TRY(c, hipSetDevice(c->device));
+    int rc;
+    if ((rc = ensure_vectors(c, f, nrhs))) return rc;
+    return solve_device(c, f, d_bp, d_xp, (double*)f->vec_y.p, nrhs);
+}

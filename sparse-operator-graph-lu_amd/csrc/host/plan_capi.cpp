@@ -25,7 +25,57 @@ int fail(int code, const std::string& msg) {
 }
 }  // namespace
 
-extern "C" const char* sogl_host_last_error(void) { return g_host_err.c_str(); }
+namespace sogl {
+void set_last_error(const std::string& s) { g_host_err = s; }
+
+// read-only views of a plan for the device half of the library
+struct PlanView {
+    int32_t n;
+    int64_t N;
+    const int32_t* new_order;
+    int64_t nnz;
+    const int32_t* row;
+    const int32_t* col;
+    const double* val;
+    int64_t nnz_orig;
+};
+bool plan_view(const sogl_plan* p, PlanView* v) {
+    if (!p || !v) return false;
+    const Plan& P = p->plan;
+    v->n = P.cfg.mSize;
+    v->N = (int64_t)P.cfg.blockRows * P.cfg.blockSize;
+    v->new_order = P.newOrder.data();
+    v->nnz = (int64_t)P.pval.size();
+    v->row = P.prow.data();
+    v->col = P.pcol.data();
+    v->val = P.pval.data();
+    v->nnz_orig = v->nnz - (v->N - v->n);
+    return true;
+}
+bool plan_factor_desc(const sogl_plan* p, int32_t precision, sogl_factor_desc* d) {
+    if (!p || !d) return false;
+    const Plan& P = p->plan;
+    d->n_ids = P.storageCount;
+    d->block_rows = P.cfg.blockRows;
+    d->ops = P.ops.data();
+    d->n_ops = (int64_t)P.ops.size();
+    d->input_ids = P.inputIds.data();
+    d->input_tiles = P.inputTiles.data();
+    d->n_inputs = (int64_t)P.inputIds.size();
+    d->l_bi = P.lBi.data();
+    d->l_bj = P.lBj.data();
+    d->l_ids = P.lId.data();
+    d->n_l = (int64_t)P.lId.size();
+    d->u_bi = P.uBi.data();
+    d->u_bj = P.uBj.data();
+    d->u_ids = P.uId.data();
+    d->n_u = (int64_t)P.uId.size();
+    d->precision = precision;
+    return true;
+}
+}  // namespace sogl
+
+extern "C" const char* sogl_last_error(void) { return g_host_err.c_str(); }
 
 extern "C" int sogl_mtx_read(const char* path, sogl_mtx** out) {
     if (!path || !out) return fail(SOGL_ERR_ARG, "null argument");

@@ -44,6 +44,15 @@ class PlanInfo(C.Structure):
                 ("t_reorder", C.c_double), ("t_plan", C.c_double)]
 
 
+class FactorDesc(C.Structure):
+    _fields_ = [("n_ids", C.c_int32), ("block_rows", C.c_int32), ("ops", C.c_void_p),
+                ("n_ops", C.c_int64), ("input_ids", C.c_void_p), ("input_tiles", C.c_void_p),
+                ("n_inputs", C.c_int64), ("l_bi", C.c_void_p), ("l_bj", C.c_void_p),
+                ("l_ids", C.c_void_p), ("n_l", C.c_int64), ("u_bi", C.c_void_p),
+                ("u_bj", C.c_void_p), ("u_ids", C.c_void_p), ("n_u", C.c_int64),
+                ("precision", C.c_int32)]
+
+
 class FactorStats(C.Structure):
     _fields_ = [("factor_ms", C.c_double), ("solve_ms", C.c_double), ("pool_bytes", C.c_int64),
                 ("n_tasks", C.c_int64), ("n_levels", C.c_int32), ("growth_tiles", C.c_int32),
@@ -86,7 +95,7 @@ def load(name: str = "libsogl.so") -> C.CDLL:
         "sogl_mtx_info": (C.c_int, [vp, P(i32), P(i64), P(i32)]),
         "sogl_mtx_arrays": (C.c_int, [vp, P(P(i32)), P(P(i32)), P(P(dbl)), P(P(dbl))]),
         "sogl_mtx_free": (None, [vp]),
-        "sogl_host_last_error": (C.c_char_p, []),
+        "sogl_last_error": (C.c_char_p, []),
     }
     if name == "libsogl.so":
         sig.update({
@@ -96,6 +105,7 @@ def load(name: str = "libsogl.so") -> C.CDLL:
             "sogl_ctx_mem_info": (C.c_int, [vp, P(C.c_size_t), P(C.c_size_t)]),
             "sogl_ctx_sync": (C.c_int, [vp]),
             "sogl_factor_load_plan": (C.c_int, [vp, vp, i32, P(i32)]),
+            "sogl_factor_load": (C.c_int, [vp, P(FactorDesc), P(i32)]),
             "sogl_factor_run": (C.c_int, [vp, i32, i32]),
             "sogl_solve": (C.c_int, [vp, i32, P(dbl), P(dbl), P(dbl), i32]),
             "sogl_solve_device": (C.c_int, [vp, i32, vp, vp, i32]),
@@ -106,7 +116,6 @@ def load(name: str = "libsogl.so") -> C.CDLL:
             "sogl_solve_refined": (C.c_int, [vp, i32, P(dbl), P(dbl), i32, dbl, i32, P(i32),
                                              P(dbl)]),
             "sogl_solveLU": (C.c_int, [i32, i64, i32, P(i32), P(i32), P(dbl), P(dbl), P(dbl)]),
-            "sogl_last_error": (C.c_char_p, []),
         })
     for fn, (res, args) in sig.items():
         f = getattr(lib, fn)
@@ -145,7 +154,7 @@ class Plan:
                                        None if bb is None else _ptr(bb, C.c_double),
                                        1 if verbose else 0, C.byref(h))
         if rc != OK:
-            raise SoglError(rc, self.lib.sogl_host_last_error().decode())
+            raise SoglError(rc, self.lib.sogl_last_error().decode())
         self.h = h
         self.n = n
         self.info = PlanInfo()
@@ -262,6 +271,27 @@ class Device:
     def load(self, plan: Plan, precision: int = FP64) -> int:
         fid = C.c_int32()
         self._check(self.lib.sogl_factor_load_plan(self.h, plan.h, precision, C.byref(fid)))
+        return fid.value
+
+    def load_ops(self, n_ids: int, ops: np.ndarray, input_ids, input_tiles, ltiles, utiles,
+                 block_rows: int, precision: int = FP64) -> int:
+        """Load a raw op list (rows: src src2 op result result2 stage seq group).
+        ltiles / utiles: (k, 3) arrays of block row, block col, id."""
+        keep = []
+        ops = np.ascontiguousarray(ops, np.int32).reshape(-1, 8)
+        ids = np.ascontiguousarray(input_ids, np.int32)
+        tiles = np.ascontiguousarray(input_tiles, np.float64)
+        lt = np.ascontiguousarray(ltiles, np.int32).reshape(-1, 3)
+        ut = np.ascontiguousarray(utiles, np.int32).reshape(-1, 3)
+        cols = [np.ascontiguousarray(lt[:, k]) for k in range(3)] + \
+               [np.ascontiguousarray(ut[:, k]) for k in range(3)]
+        keep += [ops, ids, tiles, lt, ut] + cols
+        d = FactorDesc(n_ids, block_rows, ops.ctypes.data, ops.shape[0], ids.ctypes.data,
+                       tiles.ctypes.data, ids.size, cols[0].ctypes.data, cols[1].ctypes.data,
+                       cols[2].ctypes.data, lt.shape[0], cols[3].ctypes.data, cols[4].ctypes.data,
+                       cols[5].ctypes.data, ut.shape[0], precision)
+        fid = C.c_int32()
+        self._check(self.lib.sogl_factor_load(self.h, C.byref(d), C.byref(fid)))
         return fid.value
 
     def factor(self, fid: int, mode: int = EXEC_PERSISTENT):

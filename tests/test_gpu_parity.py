@@ -1,0 +1,227 @@
+"""GPU parity: each HIP tile kernel against the oracle through the C ABI, and the
+whole factor + solve against the reference's FP64 solutions (golden fixtures) and
+the oracle.  Tolerances: FP64 paths 1e-12 relative in x (1e-13 for single tiles),
+FP32 tiles 2e-5 relative, FP32/bf16 factors after FP64 refinement: oracle
+residual <= 1e-12."""
+import numpy as np
+import pytest
+
+import oracle
+import sogl
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+rng = np.random.default_rng(7)
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = sogl.Device(0)
+    yield d
+    d.close()
+
+
+def dd_tile(scale=1.0):
+    a = rng.standard_normal((64, 64)) * scale
+    a[np.diag_indices(64)] = np.abs(a).sum(1) + 1.0
+    return a
+
+
+def op(src, src2, code, res, res2=0, stage=2, seq=0, group=0):
+    return [src, src2, code, res, res2, stage, seq, group]
+
+
+def run_ops(dev, n_ids, ops, inputs, out_id, precision=sogl.FP64, out2=None):
+    ids = np.arange(1, len(inputs) + 1, dtype=np.int32)
+    tiles = np.stack(inputs).astype(np.float64)
+    lt = np.array([[0, 0, out_id]], np.int32)
+    ut = np.array([[0, 0, out2 if out2 else out_id]], np.int32)
+    fid = dev.load_ops(n_ids, np.array(ops, np.int32), ids, tiles, lt, ut, 1, precision)
+    dev.factor(fid, sogl.EXEC_PERSISTENT)
+    o1 = dev.factor_tiles(fid, 0, 1)[0]
+    o2 = dev.factor_tiles(fid, 1, 1)[0]
+    dev.release(fid)
+    return o1, o2
+
+
+@pytest.mark.parametrize("precision,tol", [(sogl.FP64, 1e-13), (sogl.FP32, 2e-5), (sogl.BF16, 2e-2)])
+@pytest.mark.parametrize("npairs", [1, 5])
+def test_gemm_tile(dev, precision, tol, npairs):
+    mats = [rng.standard_normal((64, 64)) for _ in range(2 * npairs)]
+    r = 2 * npairs + 1
+    ops = [op(2 * s + 1, 2 * s + 2, 8 if s % 2 == 0 else 9, r) for s in range(npairs)]
+    c, _ = run_ops(dev, r + 1, ops, mats, r, precision)
+    want = np.zeros((64, 64))
+    for s in range(npairs):
+        want = oracle.gemm(mats[2 * s], mats[2 * s + 1], want, 1 if s % 2 == 0 else -1)
+    err = np.abs(c - want).max() / np.abs(want).max()
+    assert err < tol, err
+
+
+@pytest.mark.parametrize("precision,tol", [(sogl.FP64, 1e-13), (sogl.FP32, 2e-5)])
+def test_lu_tile(dev, precision, tol):
+    a = dd_tile()
+    l, u = run_ops(dev, 4, [op(1, 0, 1, 2, 3)], [a], 2, precision, out2=3)
+    lo, uo = oracle.lu(a)
+    assert np.abs(l - lo).max() < tol * 10
+    assert np.abs(u - uo).max() / np.abs(uo).max() < tol
+    assert np.array_equal(np.diag(l), np.ones(64))
+
+
+def test_lu_tile_clamp(dev):
+    a = np.eye(64)
+    a[5, 5] = 0.0
+    a[9, 9] = -3e-10
+    l, u = run_ops(dev, 4, [op(1, 0, 1, 2, 3)], [a], 2, out2=3)
+    assert u[5, 5] == 1e-9 and u[9, 9] == -1e-9
+
+
+@pytest.mark.parametrize("lower", [True, False])
+@pytest.mark.parametrize("precision,tol", [(sogl.FP64, 1e-13), (sogl.FP32, 2e-5)])
+def test_triangular_inverse_tile(dev, lower, precision, tol):
+    t = np.tril(dd_tile()) if lower else np.triu(dd_tile())
+    y, _ = run_ops(dev, 3, [op(1, 0, 2 if lower else 3, 2)], [t], 2, precision)
+    want = oracle.inv_lower(t) if lower else oracle.inv_upper(t)
+    assert np.abs(y - want).max() / np.abs(want).max() < tol
+
+
+def test_sub_copy_neg_tiles_exact(dev):
+    a, b = rng.standard_normal((2, 64, 64))
+    r, _ = run_ops(dev, 4, [op(2, 1, 4, 3)], [a, b], 3)
+    assert np.array_equal(r, a - b)  # sub(src=b, src2=a) = a - b
+    r, _ = run_ops(dev, 4, [op(0, 1, 4, 3)], [a, b], 3)
+    assert np.array_equal(r, a)
+    r, _ = run_ops(dev, 4, [op(2, 0, 4, 3)], [a, b], 3)
+    assert np.array_equal(r, -b)
+
+
+CASES = {
+    "c1": lambda: W.config("c1"),
+    "disconnected_40": lambda: W.edge_disconnected(40),
+    "zerodiag_48": lambda: W.edge_zero_diag(48),
+    "unsym_3000": lambda: W.edge_unsym_pattern(3000),
+    "convdiff2d_128": lambda: W.convdiff2d(128),
+    "diffusion3d_20": lambda: W.diffusion_varcoef(20, 3, 5),
+    "random_band_2048": lambda: W.random_band(2048, nrhs=1),
+}
+
+
+def solve_fp64(dev, p, mode=sogl.EXEC_PERSISTENT, b=None):
+    b = (p.b if p.b.ndim == 1 else p.b[:, 0]) if b is None else b
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, b)
+    fid = dev.load(pl, sogl.FP64)
+    dev.factor(fid, mode)
+    bp = pl.permute_rhs(b)
+    xp, yp = dev.solve(fid, bp, want_y=True)
+    dev.release(fid)
+    return pl, pl.unpermute(xp), xp, yp
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_solve_vs_reference(dev, name, golden):
+    g = golden(name)
+    p = CASES[name]()
+    b = p.b if p.b.ndim == 1 else p.b[:, 0]
+    pl, x, xp, yp = solve_fp64(dev, p)
+    ref = g["xorig"]
+    err = np.linalg.norm(x - ref) / np.linalg.norm(ref)
+    assert err < 1e-12, err
+    assert np.linalg.norm(yp - g["y"]) / np.linalg.norm(g["y"]) < 1e-12
+    assert sogl.oracle_residual(p.n, p.row, p.col, p.val, x, b) < 1e-15
+    xo, _ = oracle.factor_solve(pl, pl.permute_rhs(b), threads=8)
+    assert np.linalg.norm(xp - xo) / np.linalg.norm(xo) < 1e-12
+
+
+def test_executor_modes_bit_identical(dev):
+    p = W.config("c1")
+    xs = [solve_fp64(dev, p, mode)[2] for mode in (sogl.EXEC_PERSISTENT, sogl.EXEC_LEVELS,
+                                                     sogl.EXEC_PER_OP)]
+    assert np.array_equal(xs[0], xs[1])
+    assert np.array_equal(xs[0], xs[2])
+
+
+def test_resident_factor_repeat_solves(dev):
+    p = W.convdiff2d(128)
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
+    fid = dev.load(pl, sogl.FP64)
+    dev.factor(fid)
+    x1 = dev.solve(fid, pl.permute_rhs(p.b))
+    b2 = W.uniform_pm1(99, p.n)
+    x2 = dev.solve(fid, pl.permute_rhs(b2))
+    dev.factor(fid)  # refactor from the resident inputs: same bits
+    x3 = dev.solve(fid, pl.permute_rhs(p.b))
+    dev.release(fid)
+    assert np.array_equal(x1, x3)
+    assert sogl.oracle_residual(p.n, p.row, p.col, p.val, pl.unpermute(x2), b2) < 1e-15
+
+
+@pytest.mark.parametrize("nrhs", [3, 20])
+def test_multi_rhs(dev, nrhs):
+    p = W.diffusion_varcoef(20, 3, 5)
+    B = W.uniform_pm1(5, p.n * nrhs).reshape(nrhs, p.n).T.copy()
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, B[:, 0])
+    fid = dev.load(pl, sogl.FP64)
+    dev.factor(fid)
+    Xp = dev.solve(fid, pl.permute_rhs(B))
+    x0 = dev.solve(fid, pl.permute_rhs(B[:, 0]))
+    dev.release(fid)
+    assert np.array_equal(Xp[:, 0], x0)  # columns are independent
+    X = pl.unpermute(Xp)
+    for j in range(nrhs):
+        assert sogl.oracle_residual(p.n, p.row, p.col, p.val, X[:, j], B[:, j]) < 1e-15
+
+
+@pytest.mark.parametrize("precision", [sogl.FP32, sogl.BF16])
+@pytest.mark.parametrize("name", ["c1", "convdiff2d_128", "diffusion3d_20"])
+def test_low_precision_refined(dev, precision, name, golden):
+    p = CASES[name]()
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
+    fid = dev.load(pl, precision)
+    dev.factor(fid)
+    dev.attach_refine(fid, pl)
+    x, it, res, rc = dev.solve_refined(fid, p.b, tol=1e-12, max_iter=60)
+    dev.release(fid)
+    assert rc == 0, (it, res)
+    assert sogl.oracle_residual(p.n, p.row, p.col, p.val, x, p.b) <= 1e-12
+    ref = golden(name)["xorig"]
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-9
+
+
+@pytest.mark.slow
+def test_c2_full_size(dev, golden):
+    """3D 48^3 (n=110,592, 1.08M unit ops) at its real size: FP64 drop-in vs the
+    reference x, and FP32 factor + FP64 refinement to 1e-12."""
+    g = golden("c2")
+    p = W.config("c2")
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
+    fid = dev.load(pl, sogl.FP64)
+    dev.factor(fid)
+    x = pl.unpermute(dev.solve(fid, pl.permute_rhs(p.b)))
+    dev.release(fid)
+    idx = g["xorig_idx"]
+    ref = g["xorig_sample"]
+    assert np.linalg.norm(x[idx] - ref) / np.linalg.norm(ref) < 1e-12
+    assert abs(np.linalg.norm(x) - float(g["xorig_norm"])) / float(g["xorig_norm"]) < 1e-12
+    assert sogl.oracle_residual(p.n, p.row, p.col, p.val, x, p.b) < 1e-15
+    fid = dev.load(pl, sogl.FP32)
+    dev.factor(fid)
+    dev.attach_refine(fid, pl)
+    x32, it, res, rc = dev.solve_refined(fid, p.b, tol=1e-12, max_iter=30)
+    dev.release(fid)
+    assert rc == 0 and res <= 1e-12
+    assert np.linalg.norm(x32[idx] - ref) / np.linalg.norm(ref) < 1e-10
+
+
+def test_drop_in_solveLU(dev):
+    import ctypes as C
+    p = W.config("c1")
+    lib = sogl.load()
+    x = np.empty(p.n)
+    row, col = p.row.astype(np.int32), p.col.astype(np.int32)
+    rc = lib.sogl_solveLU(p.n, p.nnz, 0, row.ctypes.data_as(C.POINTER(C.c_int32)),
+                          col.ctypes.data_as(C.POINTER(C.c_int32)),
+                          p.val.ctypes.data_as(C.POINTER(C.c_double)),
+                          p.b.ctypes.data_as(C.POINTER(C.c_double)), x.ctypes.data_as(C.POINTER(C.c_double)))
+    assert rc == 0
+    assert sogl.oracle_residual(p.n, p.row, p.col, p.val, x, p.b) < 1e-15
