@@ -158,7 +158,8 @@ __global__ __launch_bounds__(NT) void k_factor_persistent(Args a) {
             s_task = s_abort ? a.ntasks : atomicAdd(a.ctrl, 1);
         }
         __syncthreads();
-        const int ti = s_task;
+        // wave-uniform loop control: the exit branch must not depend on the lane
+        const int ti = __builtin_amdgcn_readfirstlane(s_task);
         if (ti >= a.ntasks) break;
         const DevTask t = a.tasks[ti];
         // wait for the source tiles (each thread watches some of them)
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(NT) void k_factor_persistent(Args a) {
         __syncthreads();
         if (tid == 0) s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
-        if (s_abort) break;
+        if (__builtin_amdgcn_readfirstlane(s_abort)) break;
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         run_task<T, BF16, false>(t, a, lds);
         // publish: every wave's stores reach device scope before the flag
@@ -190,6 +191,9 @@ __global__ __launch_bounds__(NT) void k_factor_persistent(Args a) {
             if (t.out1 >= 0)
                 __hip_atomic_store(a.ready + t.out1, a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
+        // keeps the publishing branch and the next fetch branch apart, so the
+        // back-edge cannot join them into a lane-dependent inner loop
+        __syncthreads();
     }
 }
 

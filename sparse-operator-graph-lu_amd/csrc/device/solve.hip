@@ -62,7 +62,7 @@ __global__ __launch_bounds__(NT) void k_trsv_persistent(SolveArgs a) {
             s_row = s_abort ? a.nbr : atomicAdd(a.ctrl, 1);
         }
         __syncthreads();
-        const int q = s_row;
+        const int q = __builtin_amdgcn_readfirstlane(s_row);  // wave-uniform loop control
         if (q >= a.nbr) break;
         const int I = a.forward ? q : a.nbr - 1 - q;
         const int e0 = a.rowptr[I], e1 = a.rowptr[I + 1];
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(NT) void k_trsv_persistent(SolveArgs a) {
         __syncthreads();
         if (tid == 0) s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
-        if (s_abort) break;
+        if (__builtin_amdgcn_readfirstlane(s_abort)) break;
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const int64_t rowI = (int64_t)I * TS;
         for (int c0 = 0; c0 < a.nrhs; c0 += CW) {
@@ -146,6 +146,7 @@ __global__ __launch_bounds__(NT) void k_trsv_persistent(SolveArgs a) {
         __threadfence();
         __syncthreads();
         if (tid == 0) __hip_atomic_store(a.ready + I, a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();  // separates the publish branch from the next fetch branch
     }
 }
 
