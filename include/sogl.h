@@ -172,7 +172,8 @@ int sogl_solve_device(sogl_ctx* ctx, int32_t factor_id, const double* d_bp, doub
 int sogl_factor_release(sogl_ctx* ctx, int32_t factor_id);
 
 typedef struct sogl_factor_stats {
-    double factor_ms;       /* last sogl_factor_run, device time */
+    double factor_ms;       /* last sogl_factor_run, device time (executor + diagonal inverses) */
+    double exec_ms;         /* last sogl_factor_run, executor kernel(s) alone */
     double solve_ms;        /* last sogl_solve*, device time */
     int64_t pool_bytes;     /* HBM held by the tile pool */
     int64_t n_tasks;        /* device tasks (one per result tile) */
@@ -193,6 +194,20 @@ int sogl_refine_attach(sogl_ctx* ctx, int32_t factor_id, const sogl_plan* plan);
 int sogl_solve_refined(sogl_ctx* ctx, int32_t factor_id, const double* b, double* x,
                        int32_t nrhs, double tol, int32_t max_iter, int32_t* iters,
                        double* resid);
+
+/* Same on device-resident arrays (original order, n x nrhs column-major, allocated
+ * by the caller, e.g. with sogl_device_alloc): the permutation, padding, solves,
+ * FP64 residuals and corrections all stay in HBM.  max_iter = 0 gives a plain
+ * solve plus its residual. */
+int sogl_solve_refined_device(sogl_ctx* ctx, int32_t factor_id, const double* d_b, double* d_x,
+                              int32_t nrhs, double tol, int32_t max_iter, int32_t* iters,
+                              double* resid);
+
+/* ------------------------------------------------ device memory helpers */
+int sogl_device_alloc(sogl_ctx* ctx, size_t bytes, void** ptr);
+int sogl_device_free(sogl_ctx* ctx, void* ptr);
+/* to_device != 0: host -> device, else device -> host; synchronous */
+int sogl_device_copy(sogl_ctx* ctx, void* dst, const void* src, size_t bytes, int32_t to_device);
 
 /* ------------------------------------------------ drop-in solveLU */
 /* Mirrors SOGLU::solveLU (solver.h:24, solver.cpp:121-184): plan on the host,

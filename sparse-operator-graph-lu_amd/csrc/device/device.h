@@ -62,9 +62,9 @@ struct Factor {
     int64_t nnz = 0;
     std::vector<int32_t> new_order;  // new -> old
     double a_fro = 0;
-    DevBuf csr_ptr, csr_col, csr_val, vec_r, vec_d, red;
+    DevBuf csr_ptr, csr_col, csr_val, vec_r, vec_d, red, perm_d, vec_io;
     // stats
-    double factor_ms = 0, solve_ms = 0;
+    double factor_ms = 0, solve_ms = 0, exec_ms = 0;
     int32_t growth_tiles = 0;
     double max_growth = 0;
     double growth_limit = 1e4;
@@ -73,7 +73,7 @@ struct Factor {
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ex0 = nullptr, ex1 = nullptr;
     int num_cu = 0;
     std::string err;
     std::map<int32_t, Factor*> factors;

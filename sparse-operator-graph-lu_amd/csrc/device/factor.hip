@@ -511,7 +511,7 @@ void free_factor(Factor* f) {
                       &f->op_tasks_d, &f->op_pairs_d, &f->inputs_d, &f->l_rowptr, &f->l_col,
                       &f->l_slot, &f->u_rowptr, &f->u_col, &f->u_slot, &f->l_diag_d, &f->u_diag_d,
                       &f->linv_d, &f->uinv_d, &f->solve_ready, &f->solve_ctrl, &f->vec_b, &f->vec_x,
-                      &f->vec_y, &f->csr_ptr, &f->csr_col, &f->csr_val, &f->vec_r, &f->vec_d, &f->red})
+                      &f->vec_y, &f->csr_ptr, &f->csr_col, &f->csr_val, &f->vec_r, &f->vec_d, &f->red, &f->perm_d, &f->vec_io})
         dev_free(*b);
     delete f;
 }
@@ -527,6 +527,7 @@ int launch_factor(Context* c, Factor* f, int mode) {
     a.growth_limit = f->growth_limit;
     a.deps = (const int32_t*)f->deps_d.p;
     hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, st, a.ctrl);
+    HIP_TRY(c, hipEventRecord(c->ex0, st));  // executor alone, for kernel-level rates
     if (mode == SOGL_EXEC_PERSISTENT) {
         a.tasks = (const DevTask*)f->tasks_d.p;
         a.pairs = (const Pair*)f->pairs_d.p;
@@ -564,6 +565,7 @@ int launch_factor(Context* c, Factor* f, int mode) {
         return set_error(c, SOGL_ERR_ARG, "unknown executor mode");
     }
     HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipEventRecord(c->ex1, st));
     const int R = f->block_rows;
     if (R > 0)
         hipLaunchKernelGGL(k_diag_inverses<T>, dim3(2 * R), dim3(NT), 0, st, f->pool.p,
@@ -598,6 +600,8 @@ extern "C" int sogl_ctx_create(int32_t device, sogl_ctx** out) {
         (rc = check_hip(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream")) ||
         (rc = check_hip(c, hipEventCreate(&c->ev0), "event")) ||
         (rc = check_hip(c, hipEventCreate(&c->ev1), "event")) ||
+        (rc = check_hip(c, hipEventCreate(&c->ex0), "event")) ||
+        (rc = check_hip(c, hipEventCreate(&c->ex1), "event")) ||
         (rc = check_hip(c, hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device), "attr"))) {
         delete x;
         return rc;
@@ -615,6 +619,8 @@ extern "C" void sogl_ctx_destroy(sogl_ctx* x) {
     c->factors.clear();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ex0) (void)hipEventDestroy(c->ex0);
+    if (c->ex1) (void)hipEventDestroy(c->ex1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete x;
 }
@@ -689,6 +695,8 @@ extern "C" int sogl_factor_run(sogl_ctx* x, int32_t factor_id, int32_t exec_mode
     float ms = 0;
     HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     f->factor_ms = ms;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->ex0, c->ex1));
+    f->exec_ms = ms;
     int ctrl[CTRL_WORDS];
     HIP_TRY(c, hipMemcpy(ctrl, f->ctrl_d.p, sizeof(ctrl), hipMemcpyDeviceToHost));
     f->growth_tiles = ctrl[2];
@@ -718,6 +726,7 @@ extern "C" int sogl_factor_get_stats(sogl_ctx* x, int32_t factor_id, sogl_factor
     Factor* f = find_factor(c, factor_id);
     if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
     st->factor_ms = f->factor_ms;
+    st->exec_ms = f->exec_ms;
     st->solve_ms = f->solve_ms;
     st->pool_bytes = (int64_t)f->pool.bytes;
     st->n_tasks = (int64_t)f->tasks.size();

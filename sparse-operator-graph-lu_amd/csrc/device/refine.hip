@@ -48,6 +48,22 @@ __global__ void k_residual(const int64_t* ptr, const int32_t* col, const double*
     }
 }
 
+// bp[i] = b[new_order[i]] for i < n, 1 on the padding rows (solver.cpp:155-161 + GPSOrder.cpp:433-447)
+__global__ void k_permute_in(const double* b, const int32_t* new_order, double* bp, int32_t n, int64_t N,
+                             int nrhs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    for (int c = 0; c < nrhs; c++) bp[i + c * N] = i < n ? b[new_order[i] + (int64_t)c * n] : 1.0;
+}
+
+// x[new_order[i]] = xp[i] (GOrder::reOrderResult, GPSOrder.cpp:41-53)
+__global__ void k_permute_out(const double* xp, const int32_t* new_order, double* x, int32_t n, int64_t N,
+                              int nrhs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int c = 0; c < nrhs; c++) x[new_order[i] + (int64_t)c * n] = xp[i + c * N];
+}
+
 __global__ void k_axpy(double* x, const double* d, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x[i] += d[i];
@@ -134,21 +150,15 @@ extern "C" int sogl_refine_attach(sogl_ctx* x, int32_t factor_id, const sogl_pla
     f->n = v.n;
     f->nnz = v.nnz;
     f->new_order.assign(v.new_order, v.new_order + v.n);
+    if ((rc = dev_alloc(c, f->perm_d, (size_t)v.n * 4, "permutation"))) return rc;
+    HIP_TRY(c, hipMemcpy(f->perm_d.p, v.new_order, (size_t)v.n * 4, hipMemcpyHostToDevice));
     f->a_fro = std::sqrt(fro);
     f->has_refine = true;
     return SOGL_OK;
 }
 
-extern "C" int sogl_solve_refined(sogl_ctx* x, int32_t factor_id, const double* b, double* xo,
-                                  int32_t nrhs, double tol, int32_t max_iter, int32_t* iters,
-                                  double* resid) {
-    if (!x || !b || !xo || nrhs <= 0) return set_error(x ? &x->c : nullptr, SOGL_ERR_ARG, "bad argument");
-    Context* c = &x->c;
-    std::lock_guard<std::mutex> g(c->mu);
-    Factor* f = find_factor(c, factor_id);
-    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
-    if (!f->has_refine) return set_error(c, SOGL_ERR_ARG, "sogl_refine_attach was not called");
-    HIP_TRY(c, hipSetDevice(c->device));
+static int refined_core(Context* c, Factor* f, const double* d_b, double* d_x, int32_t nrhs, double tol,
+                        int32_t max_iter, int32_t* iters, double* resid) {
     const int32_t n = f->n;
     const int64_t N = (int64_t)f->block_rows * TS;
     int rc;
@@ -157,27 +167,31 @@ extern "C" int sogl_solve_refined(sogl_ctx* x, int32_t factor_id, const double* 
         (rc = dev_alloc(c, f->vec_d, (size_t)N * nrhs * 8, "correction")) ||
         (rc = dev_alloc(c, f->red, (size_t)nrhs * 2 * RED_BLOCKS * 8, "norms")))
         return rc;
-    // permute + pad on the host (padding rows are identity with b = 1)
-    std::vector<double> bp((size_t)N * nrhs, 1.0);
-    std::vector<double> bnorm((size_t)nrhs, 0.0);
-    for (int32_t r = 0; r < nrhs; r++)
-        for (int32_t i = 0; i < n; i++) {
-            const double v = b[(int64_t)r * n + f->new_order[(size_t)i]];
-            bp[(size_t)(r * N + i)] = v;
-            bnorm[(size_t)r] += v * v;
-        }
-    for (auto& v : bnorm) v = std::sqrt(v);
     double* db = (double*)f->vec_b.p;
     double* dx = (double*)f->vec_x.p;
     double* dr = (double*)f->vec_r.p;
     double* dd = (double*)f->vec_d.p;
-    HIP_TRY(c, hipMemcpyAsync(db, bp.data(), bp.size() * 8, hipMemcpyHostToDevice, c->stream));
-    if ((rc = solve_device(c, f, db, dx, (double*)f->vec_y.p, nrhs))) return rc;
-    std::vector<double> part((size_t)nrhs * 2 * RED_BLOCKS);
-    int it = 0;
-    double worst = 0;
     const unsigned gN = (unsigned)((N + 255) / 256);
     const unsigned gA = (unsigned)((N * nrhs + 255) / 256);
+    hipLaunchKernelGGL(k_permute_in, dim3(gN), dim3(256), 0, c->stream, d_b, (const int32_t*)f->perm_d.p, db, n, N,
+                       nrhs);
+    HIP_TRY(c, hipGetLastError());
+    std::vector<double> part((size_t)nrhs * 2 * RED_BLOCKS);
+    // ||b|| over the real rows, once
+    hipLaunchKernelGGL(k_norms, dim3(RED_BLOCKS, nrhs), dim3(256), 0, c->stream, db, db, (int64_t)n, N,
+                       (double*)f->red.p);
+    HIP_TRY(c, hipMemcpyAsync(part.data(), f->red.p, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    std::vector<double> bnorm((size_t)nrhs, 0.0);
+    for (int32_t r = 0; r < nrhs; r++) {
+        double sb = 0;
+        for (int q = 0; q < RED_BLOCKS; q++) sb += part[(size_t)((r * 2 + 0) * RED_BLOCKS + q)];
+        bnorm[(size_t)r] = std::sqrt(sb);
+    }
+    if ((rc = solve_device(c, f, db, dx, (double*)f->vec_y.p, nrhs))) return rc;
+    double solve_ms = f->solve_ms;
+    int it = 0;
+    double worst = 0;
     for (;;) {
         hipLaunchKernelGGL(k_residual, dim3(gN), dim3(256), 0, c->stream, (const int64_t*)f->csr_ptr.p,
                            (const int32_t*)f->csr_col.p, (const double*)f->csr_val.p, db, dx, dr, N, nrhs);
@@ -199,20 +213,57 @@ extern "C" int sogl_solve_refined(sogl_ctx* x, int32_t factor_id, const double* 
         }
         if (worst <= tol || it >= max_iter || std::isnan(worst)) break;
         if ((rc = solve_device(c, f, dr, dd, (double*)f->vec_y.p, nrhs))) return rc;
+        solve_ms += f->solve_ms;
         hipLaunchKernelGGL(k_axpy, dim3(gA), dim3(256), 0, c->stream, dx, (const double*)dd, N * nrhs);
         HIP_TRY(c, hipGetLastError());
         it++;
     }
-    std::vector<double> xp((size_t)N * nrhs);
-    HIP_TRY(c, hipMemcpyAsync(xp.data(), dx, xp.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    hipLaunchKernelGGL(k_permute_out, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, (const double*)dx,
+                       (const int32_t*)f->perm_d.p, d_x, n, N, nrhs);
+    HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    for (int32_t r = 0; r < nrhs; r++)
-        for (int32_t i = 0; i < n; i++) xo[(int64_t)r * n + f->new_order[(size_t)i]] = xp[(size_t)(r * N + i)];
+    f->solve_ms = solve_ms;
     if (iters) *iters = it;
     if (resid) *resid = worst;
     if (std::isnan(worst)) return set_error(c, SOGL_ERR_NAN, "refinement: NaN in the residual");
     if (!(worst <= tol)) return set_error(c, SOGL_ERR_NOT_CONVERGED, "refinement did not reach the tolerance");
     return SOGL_OK;
+}
+
+extern "C" int sogl_solve_refined_device(sogl_ctx* x, int32_t factor_id, const double* d_b, double* d_x,
+                                         int32_t nrhs, double tol, int32_t max_iter, int32_t* iters,
+                                         double* resid) {
+    if (!x || !d_b || !d_x || nrhs <= 0) return set_error(x ? &x->c : nullptr, SOGL_ERR_ARG, "bad argument");
+    Context* c = &x->c;
+    std::lock_guard<std::mutex> g(c->mu);
+    Factor* f = find_factor(c, factor_id);
+    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+    if (!f->has_refine) return set_error(c, SOGL_ERR_ARG, "sogl_refine_attach was not called");
+    HIP_TRY(c, hipSetDevice(c->device));
+    return refined_core(c, f, d_b, d_x, nrhs, tol, max_iter, iters, resid);
+}
+
+extern "C" int sogl_solve_refined(sogl_ctx* x, int32_t factor_id, const double* b, double* xo,
+                                  int32_t nrhs, double tol, int32_t max_iter, int32_t* iters,
+                                  double* resid) {
+    if (!x || !b || !xo || nrhs <= 0) return set_error(x ? &x->c : nullptr, SOGL_ERR_ARG, "bad argument");
+    Context* c = &x->c;
+    std::lock_guard<std::mutex> g(c->mu);
+    Factor* f = find_factor(c, factor_id);
+    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+    if (!f->has_refine) return set_error(c, SOGL_ERR_ARG, "sogl_refine_attach was not called");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const size_t bytes = (size_t)f->n * nrhs * 8;
+    int rc;
+    if ((rc = dev_alloc(c, f->vec_io, 2 * bytes, "refine io"))) return rc;
+    double* db = (double*)f->vec_io.p;
+    double* dx = db + (size_t)f->n * nrhs;
+    HIP_TRY(c, hipMemcpyAsync(db, b, bytes, hipMemcpyHostToDevice, c->stream));
+    int rc2 = refined_core(c, f, db, dx, nrhs, tol, max_iter, iters, resid);
+    if (rc2 != SOGL_OK && rc2 != SOGL_ERR_NOT_CONVERGED && rc2 != SOGL_ERR_NAN) return rc2;
+    HIP_TRY(c, hipMemcpyAsync(xo, dx, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return rc2;
 }
 
 // ------------------------------------------------------------------ drop-in solveLU
@@ -251,4 +302,30 @@ extern "C" int sogl_solveLU(int32_t dim, int64_t valcount, int32_t symmetric, co
         for (int32_t i = 0; i < dim; i++)
             if (std::isnan(xo[i])) return set_error(nullptr, SOGL_ERR_NAN, "NaN in the solution");
     return rc;
+}
+
+// ------------------------------------------------------------------ device memory helpers
+extern "C" int sogl_device_alloc(sogl_ctx* x, size_t bytes, void** ptr) {
+    if (!x || !ptr) return set_error(nullptr, SOGL_ERR_ARG, "null argument");
+    Context* c = &x->c;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMalloc(ptr, bytes));
+    return SOGL_OK;
+}
+
+extern "C" int sogl_device_free(sogl_ctx* x, void* ptr) {
+    if (!x) return set_error(nullptr, SOGL_ERR_ARG, "null context");
+    Context* c = &x->c;
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (ptr) HIP_TRY(c, hipFree(ptr));
+    return SOGL_OK;
+}
+
+extern "C" int sogl_device_copy(sogl_ctx* x, void* dst, const void* src, size_t bytes, int32_t to_device) {
+    if (!x || (bytes && (!dst || !src))) return set_error(nullptr, SOGL_ERR_ARG, "null argument");
+    Context* c = &x->c;
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return SOGL_OK;
 }

@@ -72,12 +72,16 @@ __device__ inline void tile_to_lds(const T* __restrict__ g, T* __restrict__ s, i
 // ------------------------------------------------------------------ GEMM
 template <typename T>
 struct MfmaOp;
+// Output row held by register r of lane group g = lane/16 (column = lane%16).
+// Measured on gfx950: the f32 16x16x4 accumulator keeps 4 consecutive rows per
+// lane group (row 4g + r); the f64 one interleaves them (row 4r + g).
 template <>
 struct MfmaOp<double> {
     typedef v4d acc_t;
     static __device__ inline acc_t mma(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ inline int row(int g, int r) { return 4 * r + g; }
 };
 template <>
 struct MfmaOp<float> {
@@ -85,6 +89,7 @@ struct MfmaOp<float> {
     static __device__ inline acc_t mma(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ inline int row(int g, int r) { return 4 * g + r; }
 };
 
 // One (A,B) pair of a GEMM task.  b < 0 encodes a subtracted product (mulneg):
@@ -96,7 +101,7 @@ struct Pair {
 // C = sum_s sign_s A_s B_s over npairs pairs, full 64x64 output per workgroup.
 // Wave w computes the 32x32 quadrant (w/2, w%2) as 2x2 blocks of 16x16 MFMA
 // outputs; v_mfma_*_16x16x4 lane layout: A[i][k] in lane i + 16k, B[k][j] in
-// lane j + 16k, D[i][j] in lane j + 16*(i/4), register i%4.
+// lane j + 16k, D[i][j] in lane j + 16g with (g, r) given by MfmaOp<T>::row.
 template <typename T, bool ACC = false>
 __device__ void gemm_task(T* __restrict__ pool, const Pair* __restrict__ pairs, int npairs,
                           int out, char* lds_raw) {
@@ -115,7 +120,8 @@ __device__ void gemm_task(T* __restrict__ pool, const Pair* __restrict__ pairs, 
             if constexpr (ACC) {  // C += ... : start from the stored tile
                 const T* C0 = pool + (size_t)out * TE;
 #pragma unroll
-                for (int r = 0; r < 4; r++) acc[m][n][r] = C0[(wr + m * 16 + 4 * lk + r) * TS + wc + n * 16 + li];
+                for (int r = 0; r < 4; r++)
+                    acc[m][n][r] = C0[(wr + m * 16 + MfmaOp<T>::row(lk, r)) * TS + wc + n * 16 + li];
             }
         }
     for (int s = 0; s < npairs; s++) {
@@ -148,7 +154,7 @@ __device__ void gemm_task(T* __restrict__ pool, const Pair* __restrict__ pairs, 
         for (int n = 0; n < 2; n++)
 #pragma unroll
             for (int r = 0; r < 4; r++)
-                C[(wr + m * 16 + 4 * lk + r) * TS + wc + n * 16 + li] = acc[m][n][r];
+                C[(wr + m * 16 + MfmaOp<T>::row(lk, r)) * TS + wc + n * 16 + li] = acc[m][n][r];
 }
 
 // bf16 experiment: FP32 tiles, operands rounded to bf16, FP32 accumulation

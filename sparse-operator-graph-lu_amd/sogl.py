@@ -54,7 +54,8 @@ class FactorDesc(C.Structure):
 
 
 class FactorStats(C.Structure):
-    _fields_ = [("factor_ms", C.c_double), ("solve_ms", C.c_double), ("pool_bytes", C.c_int64),
+    _fields_ = [("factor_ms", C.c_double), ("exec_ms", C.c_double), ("solve_ms", C.c_double),
+                ("pool_bytes", C.c_int64),
                 ("n_tasks", C.c_int64), ("n_levels", C.c_int32), ("growth_tiles", C.c_int32),
                 ("max_growth", C.c_double)]
 
@@ -115,6 +116,10 @@ def load(name: str = "libsogl.so") -> C.CDLL:
             "sogl_refine_attach": (C.c_int, [vp, i32, vp]),
             "sogl_solve_refined": (C.c_int, [vp, i32, P(dbl), P(dbl), i32, dbl, i32, P(i32),
                                              P(dbl)]),
+            "sogl_solve_refined_device": (C.c_int, [vp, i32, vp, vp, i32, dbl, i32, P(i32), P(dbl)]),
+            "sogl_device_alloc": (C.c_int, [vp, C.c_size_t, P(vp)]),
+            "sogl_device_free": (C.c_int, [vp, vp]),
+            "sogl_device_copy": (C.c_int, [vp, vp, vp, C.c_size_t, i32]),
             "sogl_solveLU": (C.c_int, [i32, i64, i32, P(i32), P(i32), P(dbl), P(dbl), P(dbl)]),
         })
     for fn, (res, args) in sig.items():
@@ -336,6 +341,33 @@ class Device:
         if rc not in (OK, 7):
             self._check(rc)
         return (x[:, 0] if b.ndim == 1 else x), it.value, res.value, rc
+
+    def alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        self._check(self.lib.sogl_device_alloc(self.h, nbytes, C.byref(p)))
+        return p.value
+
+    def free(self, ptr: int):
+        self._check(self.lib.sogl_device_free(self.h, C.c_void_p(ptr)))
+
+    def upload(self, ptr: int, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        self._check(self.lib.sogl_device_copy(self.h, C.c_void_p(ptr), a.ctypes.data_as(C.c_void_p),
+                                              a.nbytes, 1))
+
+    def download(self, ptr: int, out: np.ndarray):
+        self._check(self.lib.sogl_device_copy(self.h, out.ctypes.data_as(C.c_void_p), C.c_void_p(ptr),
+                                              out.nbytes, 0))
+        return out
+
+    def solve_refined_device(self, fid: int, d_b: int, d_x: int, nrhs: int, tol=1e-12, max_iter=20):
+        it = C.c_int32()
+        res = C.c_double()
+        rc = self.lib.sogl_solve_refined_device(self.h, fid, C.c_void_p(d_b), C.c_void_p(d_x), nrhs, tol,
+                                                max_iter, C.byref(it), C.byref(res))
+        if rc not in (OK, 7):
+            self._check(rc)
+        return it.value, res.value, rc
 
     def release(self, fid: int):
         self._check(self.lib.sogl_factor_release(self.h, fid))

@@ -225,3 +225,21 @@ def test_drop_in_solveLU(dev):
                           p.b.ctypes.data_as(C.POINTER(C.c_double)), x.ctypes.data_as(C.POINTER(C.c_double)))
     assert rc == 0
     assert sogl.oracle_residual(p.n, p.row, p.col, p.val, x, p.b) < 1e-15
+
+
+def test_refined_device_resident_matches_host(dev):
+    p = W.convdiff2d(128)
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
+    fid = dev.load(pl, sogl.FP32)
+    dev.factor(fid)
+    dev.attach_refine(fid, pl)
+    xh, it_h, res_h, rc = dev.solve_refined(fid, p.b, tol=1e-12, max_iter=30)
+    d_b, d_x = dev.alloc(p.b.nbytes), dev.alloc(p.b.nbytes)
+    dev.upload(d_b, p.b)
+    it_d, res_d, rc_d = dev.solve_refined_device(fid, d_b, d_x, 1, 1e-12, 30)
+    xd = dev.download(d_x, np.empty(p.n))
+    dev.free(d_b)
+    dev.free(d_x)
+    dev.release(fid)
+    assert rc == 0 and rc_d == 0 and it_h == it_d
+    assert np.array_equal(xh, xd)
