@@ -162,7 +162,7 @@ def main():
                  gemm_frac_of_peak=gemm_flops / p["exec_s"] / 1e12 / PEAK[prec],
                  solve_gb_s=solve_bytes[prec] * (1 + p["ir_iters"]) / p["solve_s"] / 1e9)
         p["solve_frac_hbm"] = p["solve_gb_s"] / HBM_PEAK
-        out_paths[k] = {kk: (round(v, 6) if isinstance(v, float) else v) for kk, v in p.items()}
+        out_paths[k] = {kk: (float(f"{v:.6g}") if isinstance(v, float) else v) for kk, v in p.items()}
 
     head = out_paths["fp32_ir"]
     cpu = None

@@ -185,6 +185,14 @@ int sogl_factor_get_stats(sogl_ctx* ctx, int32_t factor_id, sogl_factor_stats* s
 /* Copy factor tiles back (FP64 view).  which 0 = L, 1 = U, in the order given at load. */
 int sogl_factor_get_tiles(sogl_ctx* ctx, int32_t factor_id, int32_t which, double* tiles);
 
+/* Diagnostics: per-task device timestamps of the persistent executor.
+ * tasks4[4t..]: kind (0 gemm, 1 sub, 2 lu, 3 lower inverse, 4 upper inverse), pair
+ * count, dependency level, dependency count; times4[4t..]: wall-clock ticks (100 MHz)
+ * when the task was taken, when its sources were ready, when it finished, and the
+ * hardware id of the compute unit.  Arrays hold n_tasks * 4 entries. */
+int sogl_factor_set_trace(sogl_ctx* ctx, int32_t factor_id, int32_t enable);
+int sogl_factor_get_trace(sogl_ctx* ctx, int32_t factor_id, int64_t* tasks4, int64_t* times4);
+
 /* ------------------------------------------------ mixed precision + refinement */
 /* Attach the original (unpermuted) matrix for the FP64 residual r = b - A x. */
 int sogl_refine_attach(sogl_ctx* ctx, int32_t factor_id, const sogl_plan* plan);

@@ -68,6 +68,8 @@ struct Factor {
     int32_t growth_tiles = 0;
     double max_growth = 0;
     double growth_limit = 1e4;
+    bool trace_on = false;
+    DevBuf trace_d;
 };
 
 struct Context {

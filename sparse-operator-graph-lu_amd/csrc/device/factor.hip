@@ -110,17 +110,49 @@ struct Args {
     int32_t epoch;
     int32_t task_offset;  // levels executor: first task of the launch
     double growth_limit;
+    long long* trace;     // optional: per task {taken, sources ready, done, hw id}
 };
 
-template <typename T, bool BF16, bool ACC>
-__device__ inline void run_task(const DevTask& t, const Args& a, char* lds) {
+// Waits (thread 0 polls, relaxed) until ready[s] >= epoch; on abort or timeout it
+// raises ctrl[1] and returns.  Relaxed polling keeps the spin loop free of cache
+// invalidations; the caller issues one acquire fence after the barrier.
+__device__ inline void spin_ready(const Args& a, int s) {
+    int* flag = a.ready + s;
+    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= a.epoch) return;
+    const long long t0 = wall_clock64();
+    for (;;) {
+        __builtin_amdgcn_s_sleep(1);
+        if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= a.epoch) return;
+        if (__hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+        if (wall_clock64() - t0 > SPIN_LIMIT_TICKS) {
+            __hip_atomic_store(a.ctrl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+    }
+}
+
+// Per-pair wait of a GEMM task inside the persistent executor
+struct PairWait {
+    const Args* a;
+    __device__ inline void operator()(int sa, int sb) const {
+        if (threadIdx.x == 0) {
+            spin_ready(*a, sa);
+            spin_ready(*a, sb);
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+};
+
+template <typename T, bool BF16, bool ACC, typename Wait = NoWait>
+__device__ inline void run_task(const DevTask& t, const Args& a, char* lds, const Wait& wait = Wait()) {
     T* pool = reinterpret_cast<T*>(a.pool);
     switch (t.kind) {
         case TK_GEMM:
             if constexpr (BF16 && !ACC) {
-                gemm_task_bf16(reinterpret_cast<float*>(a.pool), a.pairs + t.a, t.b, t.out0, lds);
+                gemm_task_bf16(reinterpret_cast<float*>(a.pool), a.pairs + t.a, t.b, t.out0, lds, wait);
             } else {
-                gemm_task<T, ACC>(pool, a.pairs + t.a, t.b, t.out0, lds);
+                gemm_task<T, ACC>(pool, a.pairs + t.a, t.b, t.out0, lds, wait);
             }
             break;
         case TK_SUB:
@@ -145,7 +177,9 @@ __device__ inline void run_task(const DevTask& t, const Args& a, char* lds) {
     }
 }
 
-// Persistent executor: every workgroup loops taking the next task index.
+// Persistent executor: every workgroup loops taking the next task index.  GEMM
+// tasks wait pair by pair (pairs are ordered by when their operands become
+// ready); other tasks wait for all their sources first.
 template <typename T, bool BF16>
 __global__ __launch_bounds__(NT) void k_factor_persistent(Args a) {
     __shared__ __attribute__((aligned(16))) char lds[LDS];
@@ -162,34 +196,33 @@ __global__ __launch_bounds__(NT) void k_factor_persistent(Args a) {
         const int ti = __builtin_amdgcn_readfirstlane(s_task);
         if (ti >= a.ntasks) break;
         const DevTask t = a.tasks[ti];
-        // wait for the source tiles (each thread watches some of them)
-        for (int d = tid; d < t.dep_count; d += NT) {
-            int* flag = a.ready + a.deps[t.dep_begin + d];
-            if (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= a.epoch) continue;
-            const long long t0 = wall_clock64();
-            for (;;) {
-                __builtin_amdgcn_s_sleep(2);
-                if (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= a.epoch) break;
-                if (__hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-                if (wall_clock64() - t0 > SPIN_LIMIT_TICKS) {
-                    __hip_atomic_store(a.ctrl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-            }
+        long long t_taken = 0, t_ready = 0;
+        if (a.trace && tid == 0) t_taken = wall_clock64();
+        if (t.kind != TK_GEMM) {
+            if (tid == 0)
+                for (int d = 0; d < t.dep_count; d++) spin_ready(a, a.deps[t.dep_begin + d]);
+            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
-        __syncthreads();
+        if (a.trace && tid == 0) t_ready = wall_clock64();
         if (tid == 0) s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         if (__builtin_amdgcn_readfirstlane(s_abort)) break;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        run_task<T, BF16, false>(t, a, lds);
+        run_task<T, BF16, false>(t, a, lds, PairWait{&a});
         // publish: every wave's stores reach device scope before the flag
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __syncthreads();
         if (tid == 0) {
             __hip_atomic_store(a.ready + t.out0, a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             if (t.out1 >= 0)
                 __hip_atomic_store(a.ready + t.out1, a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (a.trace) {
+                long long* tr = a.trace + (size_t)ti * 4;
+                tr[0] = t_taken;
+                tr[1] = t_ready;
+                tr[2] = wall_clock64();
+                tr[3] = (long long)__smid();
+            }
         }
         // keeps the publishing branch and the next fetch branch apart, so the
         // back-edge cannot join them into a lane-dependent inner loop
@@ -370,6 +403,17 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
         }
         level[t] = lv;
     }
+    // pairs of a GEMM task run in the order their operands become ready, so the task
+    // waits only for its last pair once the earlier ones are done
+    auto ready_level = [&](int32_t s) -> int32_t { return prod[(size_t)s] < 0 ? -1 : level[(size_t)prod[(size_t)s]]; };
+    for (size_t t = 0; t < T; t++) {
+        if (tasks[t].kind != TK_GEMM) continue;
+        std::stable_sort(tpairs[t].begin(), tpairs[t].end(), [&](const Pair& x, const Pair& y) {
+            const int32_t lx = std::max(ready_level(x.a), ready_level(x.b < 0 ? -x.b - 1 : x.b));
+            const int32_t ly = std::max(ready_level(y.a), ready_level(y.b < 0 ? -y.b - 1 : y.b));
+            return lx < ly;
+        });
+    }
     std::vector<int32_t> order(T);
     for (size_t t = 0; t < T; t++) order[t] = (int32_t)t;
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
@@ -472,6 +516,12 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
         (rc = up_i32(f->inputs_d, f->input_slots, "input slots")))
         return rc;
     hipLaunchKernelGGL(k_fill<int>, dim3(256), dim3(256), 0, st, (int*)f->ready_d.p, (size_t)f->n_slots, 0);
+    // input tiles (slots 0..n_inputs-1) and the zero tile never change: always ready
+    if (d->n_inputs > 0)
+        hipLaunchKernelGGL(k_fill<int>, dim3(256), dim3(256), 0, st, (int*)f->ready_d.p, (size_t)d->n_inputs,
+                           0x7fffffff);
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, st, (int*)f->ready_d.p + f->zero_slot, (size_t)1,
+                       0x7fffffff);
     // the zero tile
     if (f->elem == 8)
         hipLaunchKernelGGL(k_fill<double>, dim3(16), dim3(256), 0, st,
@@ -511,7 +561,7 @@ void free_factor(Factor* f) {
                       &f->op_tasks_d, &f->op_pairs_d, &f->inputs_d, &f->l_rowptr, &f->l_col,
                       &f->l_slot, &f->u_rowptr, &f->u_col, &f->u_slot, &f->l_diag_d, &f->u_diag_d,
                       &f->linv_d, &f->uinv_d, &f->solve_ready, &f->solve_ctrl, &f->vec_b, &f->vec_x,
-                      &f->vec_y, &f->csr_ptr, &f->csr_col, &f->csr_val, &f->vec_r, &f->vec_d, &f->red, &f->perm_d, &f->vec_io})
+                      &f->vec_y, &f->csr_ptr, &f->csr_col, &f->csr_val, &f->vec_r, &f->vec_d, &f->red, &f->perm_d, &f->vec_io, &f->trace_d})
         dev_free(*b);
     delete f;
 }
@@ -526,6 +576,7 @@ int launch_factor(Context* c, Factor* f, int mode) {
     a.epoch = f->epoch;
     a.growth_limit = f->growth_limit;
     a.deps = (const int32_t*)f->deps_d.p;
+    a.trace = (mode == SOGL_EXEC_PERSISTENT && f->trace_on) ? (long long*)f->trace_d.p : nullptr;
     hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, st, a.ctrl);
     HIP_TRY(c, hipEventRecord(c->ex0, st));  // executor alone, for kernel-level rates
     if (mode == SOGL_EXEC_PERSISTENT) {
@@ -765,5 +816,42 @@ extern "C" int sogl_factor_get_tiles(sogl_ctx* x, int32_t factor_id, int32_t whi
     dev_free(ids);
     dev_free(out);
     dev_free(out64);
+    return SOGL_OK;
+}
+
+extern "C" int sogl_factor_set_trace(sogl_ctx* x, int32_t factor_id, int32_t enable) {
+    if (!x) return set_error(nullptr, SOGL_ERR_ARG, "null context");
+    Context* c = &x->c;
+    std::lock_guard<std::mutex> g(c->mu);
+    Factor* f = find_factor(c, factor_id);
+    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (enable) {
+        int rc = dev_alloc(c, f->trace_d, std::max<size_t>(f->tasks.size(), 1) * 4 * 8, "trace");
+        if (rc) return rc;
+    }
+    f->trace_on = enable != 0;
+    return SOGL_OK;
+}
+
+extern "C" int sogl_factor_get_trace(sogl_ctx* x, int32_t factor_id, int64_t* tasks4, int64_t* times4) {
+    if (!x) return set_error(nullptr, SOGL_ERR_ARG, "null context");
+    Context* c = &x->c;
+    std::lock_guard<std::mutex> g(c->mu);
+    Factor* f = find_factor(c, factor_id);
+    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+    if (tasks4)
+        for (size_t t = 0; t < f->tasks.size(); t++) {
+            const DevTask& k = f->tasks[t];
+            tasks4[4 * t + 0] = k.kind;
+            tasks4[4 * t + 1] = k.kind == TK_GEMM ? k.b : 0;
+            tasks4[4 * t + 2] = k.level;
+            tasks4[4 * t + 3] = k.dep_count;
+        }
+    if (times4) {
+        if (!f->trace_d.p) return set_error(c, SOGL_ERR_ARG, "tracing was not enabled");
+        HIP_TRY(c, hipSetDevice(c->device));
+        HIP_TRY(c, hipMemcpy(times4, f->trace_d.p, f->tasks.size() * 4 * 8, hipMemcpyDeviceToHost));
+    }
     return SOGL_OK;
 }

@@ -22,7 +22,6 @@
 
 namespace sogl_dev {
 
-constexpr int CW = 16;  // right-hand-side columns per pass over a row
 constexpr long long SOLVE_SPIN_LIMIT = 20LL * 100 * 1000 * 1000;
 
 struct SolveArgs {
@@ -42,16 +41,36 @@ struct SolveArgs {
     int* ctrl;            // [0] row counter, [1] abort
 };
 
+// thread (r, kp) of a row task owns tile row r, columns 16kp .. 16kp+15
 template <typename T>
-__device__ inline void stage_tile(const T* g, double* s, int tid) {
-    const int r = tid >> 2, c0 = (tid & 3) * 16;
+__device__ inline void load_seg(const T* tile, int r, int kp, double* seg) {
+    const T* p = tile + r * TS + kp * 16;
 #pragma unroll
-    for (int q = 0; q < 16; q++) s[r * LDA + c0 + q] = (double)g[r * TS + c0 + q];
+    for (int k = 0; k < 16; k++) seg[k] = (double)p[k];
 }
 
-template <typename T>
+__device__ inline void spin_row(const SolveArgs& a, int J) {
+    int* flag = a.ready + J;
+    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= a.epoch) return;
+    const long long t0 = wall_clock64();
+    for (;;) {
+        __builtin_amdgcn_s_sleep(1);
+        if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= a.epoch) return;
+        if (__hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+        if (wall_clock64() - t0 > SOLVE_SPIN_LIMIT) {
+            __hip_atomic_store(a.ctrl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+    }
+}
+
+// One block row per task.  The rows a row reads are consumed oldest first (the
+// order they complete in), each after its own ready flag, with the tile row
+// already in registers; so once the newest dependency lands only one tile
+// product and the diagonal product remain on the chain.  CW right-hand sides
+// are handled per pass over the row.
+template <typename T, int CW>
 __global__ __launch_bounds__(NT) void k_trsv_persistent(SolveArgs a) {
-    __shared__ double tile[TS * LDA];
     __shared__ double vec[TS * CW];
     __shared__ int s_row, s_abort;
     const int tid = threadIdx.x, r = tid >> 2, kp = tid & 3;
@@ -65,51 +84,36 @@ __global__ __launch_bounds__(NT) void k_trsv_persistent(SolveArgs a) {
         const int q = __builtin_amdgcn_readfirstlane(s_row);  // wave-uniform loop control
         if (q >= a.nbr) break;
         const int I = a.forward ? q : a.nbr - 1 - q;
-        const int e0 = a.rowptr[I], e1 = a.rowptr[I + 1];
-        // wait for every row this one reads
-        for (int e = e0 + tid; e < e1; e += NT) {
-            int* flag = a.ready + a.col[e];
-            if (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= a.epoch) continue;
-            const long long t0 = wall_clock64();
-            for (;;) {
-                __builtin_amdgcn_s_sleep(1);
-                if (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= a.epoch) break;
-                if (__hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-                if (wall_clock64() - t0 > SOLVE_SPIN_LIMIT) {
-                    __hip_atomic_store(a.ctrl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-            }
-        }
-        __syncthreads();
-        if (tid == 0) s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        if (__builtin_amdgcn_readfirstlane(s_abort)) break;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const int e0 = a.rowptr[I], e1 = a.rowptr[I + 1], cnt = e1 - e0;
         const int64_t rowI = (int64_t)I * TS;
+        double dseg[16];
+        load_seg(pool + (size_t)a.dinv[I] * TE, r, kp, dseg);
         for (int c0 = 0; c0 < a.nrhs; c0 += CW) {
             const int cw = min(CW, a.nrhs - c0);
             double acc[CW];
 #pragma unroll
             for (int c = 0; c < CW; c++)
                 acc[c] = (kp == 0 && c < cw) ? a.in[rowI + r + (int64_t)(c0 + c) * a.ld] : 0.0;
-            for (int e = e0; e < e1; e++) {
+            for (int i = 0; i < cnt; i++) {
+                const int e = a.forward ? e0 + i : e1 - 1 - i;
                 const int J = a.col[e];
-                __syncthreads();
-                stage_tile(pool + (size_t)a.slot[e] * TE, tile, tid);
-                {   // the CW-column chunk of x_J / y_J, k-major
-                    const int k = tid >> 2, cc = (tid & 3) * 4;
-#pragma unroll
-                    for (int c = 0; c < 4; c++)
-                        vec[k * CW + cc + c] = (cc + c < cw) ? a.out[(int64_t)J * TS + k + (int64_t)(c0 + cc + c) * a.ld] : 0.0;
+                double seg[16];
+                load_seg(pool + (size_t)a.slot[e] * TE, r, kp, seg);
+                if (c0 == 0) {
+                    if (tid == 0) spin_row(a, J);
+                    __syncthreads();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 }
-                __syncthreads();
-#pragma unroll 4
-                for (int kk = 0; kk < 16; kk++) {
-                    const int k = kp * 16 + kk;
-                    const double l = tile[r * LDA + k];
+                const double* xj = a.out + (int64_t)J * TS + kp * 16;
 #pragma unroll
-                    for (int c = 0; c < CW; c++) acc[c] -= l * vec[k * CW + c];
+                for (int c = 0; c < CW; c++) {
+                    if (c < cw) {
+                        const double* xc = xj + (int64_t)(c0 + c) * a.ld;
+                        double s = 0;
+#pragma unroll
+                        for (int k = 0; k < 16; k++) s += seg[k] * xc[k];
+                        acc[c] -= s;
+                    }
                 }
             }
 #pragma unroll
@@ -117,8 +121,6 @@ __global__ __launch_bounds__(NT) void k_trsv_persistent(SolveArgs a) {
                 acc[c] += __shfl_xor(acc[c], 1);
                 acc[c] += __shfl_xor(acc[c], 2);
             }
-            __syncthreads();
-            stage_tile(pool + (size_t)a.dinv[I] * TE, tile, tid);
             if (kp == 0) {
 #pragma unroll
                 for (int c = 0; c < CW; c++) vec[r * CW + c] = acc[c];
@@ -126,13 +128,11 @@ __global__ __launch_bounds__(NT) void k_trsv_persistent(SolveArgs a) {
             __syncthreads();
             double o[CW];
 #pragma unroll
-            for (int c = 0; c < CW; c++) o[c] = 0.0;
-#pragma unroll 4
-            for (int kk = 0; kk < 16; kk++) {
-                const int k = kp * 16 + kk;
-                const double d = tile[r * LDA + k];
+            for (int c = 0; c < CW; c++) {
+                double s = 0;
 #pragma unroll
-                for (int c = 0; c < CW; c++) o[c] += d * vec[k * CW + c];
+                for (int k = 0; k < 16; k++) s += dseg[k] * vec[(kp * 16 + k) * CW + c];
+                o[c] = s;
             }
 #pragma unroll
             for (int c = 0; c < CW; c++) {
@@ -142,8 +142,9 @@ __global__ __launch_bounds__(NT) void k_trsv_persistent(SolveArgs a) {
             if (kp == 0) {
                 for (int c = 0; c < cw; c++) a.out[rowI + r + (int64_t)(c0 + c) * a.ld] = o[c];
             }
+            __syncthreads();  // vec is reused by the next chunk
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __syncthreads();
         if (tid == 0) __hip_atomic_store(a.ready + I, a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();  // separates the publish branch from the next fetch branch
@@ -185,10 +186,12 @@ int launch_pass(Context* c, Factor* f, bool forward, const double* in, double* o
     a.ctrl = (int*)f->solve_ctrl.p;
     hipLaunchKernelGGL(k_solve_ctrl_reset, dim3(1), dim3(64), 0, c->stream, a.ctrl);
     int per_cu = 0;
-    HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trsv_persistent<T>, NT, 0));
+    const void* kern = nrhs == 1 ? (const void*)k_trsv_persistent<T, 1> : (const void*)k_trsv_persistent<T, 8>;
+    HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, 0));
     if (per_cu < 1) per_cu = 1;
     const int grid = std::max(1, std::min(a.nbr, c->num_cu * per_cu));
-    hipLaunchKernelGGL(k_trsv_persistent<T>, dim3(grid), dim3(NT), 0, c->stream, a);
+    if (nrhs == 1) hipLaunchKernelGGL((k_trsv_persistent<T, 1>), dim3(grid), dim3(NT), 0, c->stream, a);
+    else hipLaunchKernelGGL((k_trsv_persistent<T, 8>), dim3(grid), dim3(NT), 0, c->stream, a);
     HIP_TRY(c, hipGetLastError());
     return SOGL_OK;
 }

@@ -102,9 +102,16 @@ struct Pair {
 // Wave w computes the 32x32 quadrant (w/2, w%2) as 2x2 blocks of 16x16 MFMA
 // outputs; v_mfma_*_16x16x4 lane layout: A[i][k] in lane i + 16k, B[k][j] in
 // lane j + 16k, D[i][j] in lane j + 16g with (g, r) given by MfmaOp<T>::row.
-template <typename T, bool ACC = false>
+// A GEMM task may start before all of its pairs' operands exist: `wait(a, b)` is
+// called by every thread before pair (a, b) is read and returns once both tiles
+// are ready (it includes the barrier that precedes the operand loads).
+struct NoWait {
+    __device__ inline void operator()(int, int) const { __syncthreads(); }
+};
+
+template <typename T, bool ACC = false, typename Wait = NoWait>
 __device__ void gemm_task(T* __restrict__ pool, const Pair* __restrict__ pairs, int npairs,
-                          int out, char* lds_raw) {
+                          int out, char* lds_raw, const Wait& wait = Wait()) {
     typedef typename MfmaOp<T>::acc_t acc_t;
     T* la = reinterpret_cast<T*>(lds_raw);
     T* lb = la + TS * LDA;
@@ -128,7 +135,7 @@ __device__ void gemm_task(T* __restrict__ pool, const Pair* __restrict__ pairs, 
         const Pair p = pairs[s];
         const bool neg = p.b < 0;
         const int sb = neg ? -p.b - 1 : p.b;
-        __syncthreads();
+        wait(p.a, sb);
         tile_to_lds(pool + (size_t)p.a * TE, la, LDA, tid);
         tile_to_lds(pool + (size_t)sb * TE, lb, LDB, tid);
         __syncthreads();
@@ -159,8 +166,9 @@ __device__ void gemm_task(T* __restrict__ pool, const Pair* __restrict__ pairs, 
 
 // bf16 experiment: FP32 tiles, operands rounded to bf16, FP32 accumulation
 // (v_mfma_f32_16x16x32_bf16: A[i][8q..8q+7] in lane i + 16q, same D layout).
+template <typename Wait = NoWait>
 __device__ inline void gemm_task_bf16(float* __restrict__ pool, const Pair* __restrict__ pairs,
-                                      int npairs, int out, char* lds_raw) {
+                                      int npairs, int out, char* lds_raw, const Wait& wait = Wait()) {
     float* la = reinterpret_cast<float*>(lds_raw);
     float* lb = la + TS * LDA;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -175,7 +183,7 @@ __device__ inline void gemm_task_bf16(float* __restrict__ pool, const Pair* __re
         const Pair p = pairs[s];
         const bool neg = p.b < 0;
         const int sb = neg ? -p.b - 1 : p.b;
-        __syncthreads();
+        wait(p.a, sb);
         tile_to_lds(pool + (size_t)p.a * TE, la, LDA, tid);
         tile_to_lds(pool + (size_t)sb * TE, lb, LDB, tid);
         __syncthreads();

@@ -120,6 +120,8 @@ def load(name: str = "libsogl.so") -> C.CDLL:
             "sogl_device_alloc": (C.c_int, [vp, C.c_size_t, P(vp)]),
             "sogl_device_free": (C.c_int, [vp, vp]),
             "sogl_device_copy": (C.c_int, [vp, vp, vp, C.c_size_t, i32]),
+            "sogl_factor_set_trace": (C.c_int, [vp, i32, i32]),
+            "sogl_factor_get_trace": (C.c_int, [vp, i32, P(i64), P(i64)]),
             "sogl_solveLU": (C.c_int, [i32, i64, i32, P(i32), P(i32), P(dbl), P(dbl), P(dbl)]),
         })
     for fn, (res, args) in sig.items():
@@ -341,6 +343,17 @@ class Device:
         if rc not in (OK, 7):
             self._check(rc)
         return (x[:, 0] if b.ndim == 1 else x), it.value, res.value, rc
+
+    def trace(self, fid: int, enable: bool = True):
+        self._check(self.lib.sogl_factor_set_trace(self.h, fid, 1 if enable else 0))
+
+    def get_trace(self, fid: int, n_tasks: int, times: bool = True):
+        """(tasks (n,4): kind, pairs, level, deps; times (n,4): taken, ready, done, hw id)"""
+        tk = np.empty((n_tasks, 4), np.int64)
+        tm = np.empty((n_tasks, 4), np.int64) if times else None
+        self._check(self.lib.sogl_factor_get_trace(self.h, fid, _ptr(tk, C.c_int64),
+                                                   None if tm is None else _ptr(tm, C.c_int64)))
+        return tk, tm
 
     def alloc(self, nbytes: int) -> int:
         p = C.c_void_p()
