@@ -25,6 +25,8 @@ struct DevTask {
                           // (-1 = zero); LU, LINV, UINV: source slot
     int32_t dep_begin, dep_count;  // slots this task waits for (persistent executor)
     int32_t level;        // dependency level (0-based)
+    int32_t out2, out3;   // lu: slots of L^-1 and U^-1 made by the same task (-1: none)
+    int32_t pad0, pad1;
 };
 
 struct DevBuf {

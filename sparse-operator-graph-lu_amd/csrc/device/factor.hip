@@ -159,7 +159,7 @@ __device__ inline void run_task(const DevTask& t, const Args& a, char* lds, cons
             sub_task<T>(pool, t.b, t.a, t.out0);
             break;
         case TK_LU: {
-            double g = lu_task<T>(pool, t.a, t.out0, t.out1, lds, a.growth_limit, a.ctrl + 2);
+            double g = lu_task<T>(pool, t.a, t.out0, t.out1, t.out2, t.out3, lds, a.growth_limit, a.ctrl + 2);
             if (threadIdx.x == 0 && sizeof(T) == 4) {
                 float gf = (float)g;
                 atomicMax(a.ctrl + 3, __float_as_int(gf));
@@ -216,6 +216,10 @@ __global__ __launch_bounds__(NT) void k_factor_persistent(Args a) {
             __hip_atomic_store(a.ready + t.out0, a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             if (t.out1 >= 0)
                 __hip_atomic_store(a.ready + t.out1, a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (t.out2 >= 0)
+                __hip_atomic_store(a.ready + t.out2, a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (t.out3 >= 0)
+                __hip_atomic_store(a.ready + t.out3, a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             if (a.trace) {
                 long long* tr = a.trace + (size_t)ti * 4;
                 tr[0] = t_taken;
@@ -236,18 +240,6 @@ __global__ __launch_bounds__(NT) void k_factor_batch(Args a) {
     __shared__ __attribute__((aligned(16))) char lds[LDS];
     const DevTask t = a.tasks[a.task_offset + blockIdx.x];
     run_task<T, BF16, ACC>(t, a, lds);
-}
-
-// Inverses of the diagonal L and U tiles, used by the triangular solve.
-template <typename T>
-__global__ __launch_bounds__(NT) void k_diag_inverses(void* pool, const int32_t* ldiag,
-                                                      const int32_t* udiag, const int32_t* linv,
-                                                      const int32_t* uinv, int nbr) {
-    __shared__ __attribute__((aligned(16))) char lds[LDS];
-    const int b = blockIdx.x;
-    T* p = reinterpret_cast<T*>(pool);
-    if (b < nbr) tri_inv_task<T, true>(p, ldiag[b], linv[b], lds);
-    else tri_inv_task<T, false>(p, udiag[b - nbr], uinv[b - nbr], lds);
 }
 
 __global__ void k_f64_to_f32(const double* in, float* out, size_t n) {
@@ -297,14 +289,27 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
     }
     f->zero_slot = next++;
     const int32_t R = d->block_rows;
-    f->linv_slot.resize((size_t)R);
-    f->uinv_slot.resize((size_t)R);
-    for (int32_t i = 0; i < R; i++) f->linv_slot[(size_t)i] = next++;
-    for (int32_t i = 0; i < R; i++) f->uinv_slot[(size_t)i] = next++;
-    f->n_slots = next;
+    // diagonal factor tiles per block row
+    f->l_diag.assign((size_t)R, -1);
+    f->u_diag.assign((size_t)R, -1);
+    for (int which = 0; which < 2; which++) {
+        const int64_t cnt = which == 0 ? d->n_l : d->n_u;
+        const int32_t* bi = which == 0 ? d->l_bi : d->u_bi;
+        const int32_t* bj = which == 0 ? d->l_bj : d->u_bj;
+        const int32_t* id = which == 0 ? d->l_ids : d->u_ids;
+        for (int64_t k = 0; k < cnt; k++) {
+            if (bi[k] < 0 || bi[k] >= R || bj[k] < 0 || bj[k] >= R || id[k] <= 0 || id[k] >= S)
+                return set_error(c, SOGL_ERR_ARG, "factor tile out of range");
+            if (bi[k] == bj[k]) (which == 0 ? f->l_diag : f->u_diag)[(size_t)bi[k]] = slot[(size_t)id[k]];
+        }
+    }
+    for (int32_t i = 0; i < R; i++)
+        if (f->l_diag[(size_t)i] < 0 || f->u_diag[(size_t)i] < 0)
+            return set_error(c, SOGL_ERR_ARG, "factor lacks a diagonal tile");
 
     // producer task of each slot (-1: input or never written)
     std::vector<int32_t> prod((size_t)next, -1);
+    std::vector<int32_t> lu_op_task;  // per task: index of its op in op_tasks (lu tasks)
     std::vector<DevTask> tasks;
     std::vector<std::vector<Pair>> tpairs;
     std::vector<int32_t> tstage;
@@ -348,7 +353,26 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
             ot.a = s1 < 0 ? f->zero_slot : s1;
             ot.b = 0;
         }
+        ot.out2 = ot.out3 = -1;
         f->op_tasks.push_back(ot);
+        // an inverse of an lu output is made by that lu's task (the lu holds the tile
+        // in registers already); the per-op executor still runs it as its own op
+        if (kind == TK_LINV || kind == TK_UINV) {
+            const int32_t pt = s1 >= 0 ? prod[(size_t)s1] : -1;
+            if (pt >= 0 && tasks[(size_t)pt].kind == TK_LU) {
+                DevTask& lt = tasks[(size_t)pt];
+                if (kind == TK_LINV && lt.out0 == s1 && lt.out2 < 0 && prod[(size_t)rs] < 0) {
+                    lt.out2 = rs;
+                    prod[(size_t)rs] = pt;
+                    continue;
+                }
+                if (kind == TK_UINV && lt.out1 == s1 && lt.out3 < 0 && prod[(size_t)rs] < 0) {
+                    lt.out3 = rs;
+                    prod[(size_t)rs] = pt;
+                    continue;
+                }
+            }
+        }
         int32_t t = prod[(size_t)rs];
         if (t < 0) {
             t = (int32_t)tasks.size();
@@ -357,6 +381,7 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
             tasks.push_back(nt);
             tpairs.emplace_back();
             tstage.push_back(o.stage);
+            lu_op_task.push_back((int32_t)f->op_tasks.size() - 1);
             prod[(size_t)rs] = t;
             if (ot.out1 >= 0) prod[(size_t)ot.out1] = t;
         } else {
@@ -366,6 +391,30 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
         }
         if (kind == TK_GEMM) tpairs[(size_t)t].push_back(pr);
     }
+    // inverses of the diagonal factor tiles for the triangular solve: made by the lu
+    // task of each diagonal tile, into the plan's inverse tile when the plan has one
+    f->linv_slot.assign((size_t)R, -1);
+    f->uinv_slot.assign((size_t)R, -1);
+    for (int32_t i = 0; i < R; i++) {
+        for (int which = 0; which < 2; which++) {
+            const int32_t ds = (which == 0 ? f->l_diag : f->u_diag)[(size_t)i];
+            const int32_t pt = ds >= 0 && ds < (int32_t)prod.size() ? prod[(size_t)ds] : -1;
+            if (pt < 0 || tasks[(size_t)pt].kind != TK_LU || (which == 0 ? tasks[(size_t)pt].out0
+                                                                          : tasks[(size_t)pt].out1) != ds)
+                return set_error(c, SOGL_ERR_ARG, "a diagonal factor tile is not the output of an lu op");
+            DevTask& lt = tasks[(size_t)pt];
+            int32_t& inv = which == 0 ? lt.out2 : lt.out3;
+            if (inv < 0) {  // the plan never inverts this tile: a solve-only slot
+                inv = next++;
+                prod.push_back(pt);
+                DevTask& ot = f->op_tasks[(size_t)lu_op_task[(size_t)pt]];
+                (which == 0 ? ot.out2 : ot.out3) = inv;
+            }
+            (which == 0 ? f->linv_slot : f->uinv_slot)[(size_t)i] = inv;
+        }
+    }
+    f->n_slots = next;
+
     // dependencies and levels
     const size_t T = tasks.size();
     std::vector<std::vector<int32_t>> dslots(T);
@@ -445,8 +494,6 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
 
     // triangular-solve structure
     std::vector<std::vector<std::pair<int32_t, int32_t>>> lrow((size_t)R), urow((size_t)R);
-    f->l_diag.assign((size_t)R, -1);
-    f->u_diag.assign((size_t)R, -1);
     f->l_slots.clear();
     f->u_slots.clear();
     for (int which = 0; which < 2; which++) {
@@ -461,7 +508,7 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
             if (s < 0) return set_error(c, SOGL_ERR_ARG, "factor tile never produced");
             (which == 0 ? f->l_slots : f->u_slots).push_back(s);
             if (bi[k] == bj[k]) {
-                (which == 0 ? f->l_diag : f->u_diag)[(size_t)bi[k]] = s;
+                continue;
             } else if ((which == 0) == (bj[k] < bi[k])) {
                 (which == 0 ? lrow : urow)[(size_t)bi[k]].push_back({bj[k], s});
             } else {
@@ -469,9 +516,6 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
             }
         }
     }
-    for (int32_t i = 0; i < R; i++)
-        if (f->l_diag[(size_t)i] < 0 || f->u_diag[(size_t)i] < 0)
-            return set_error(c, SOGL_ERR_ARG, "factor lacks a diagonal tile");
     std::vector<int32_t> lp(1, 0), lc, ls, up(1, 0), uc, us;
     for (int32_t i = 0; i < R; i++) {
         auto& L = lrow[(size_t)i];
@@ -617,11 +661,6 @@ int launch_factor(Context* c, Factor* f, int mode) {
     }
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipEventRecord(c->ex1, st));
-    const int R = f->block_rows;
-    if (R > 0)
-        hipLaunchKernelGGL(k_diag_inverses<T>, dim3(2 * R), dim3(NT), 0, st, f->pool.p,
-                           (const int32_t*)f->l_diag_d.p, (const int32_t*)f->u_diag_d.p,
-                           (const int32_t*)f->linv_d.p, (const int32_t*)f->uinv_d.p, R);
     HIP_TRY(c, hipGetLastError());
     return SOGL_OK;
 }

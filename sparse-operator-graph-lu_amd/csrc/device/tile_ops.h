@@ -38,7 +38,7 @@ template <typename T>
 constexpr int lds_bytes() {
     // gemm: A (64 x LDA) + B (64 x LDB) in the MFMA operand type; the lu / inverses use
     // 2 x 64 x LDA doubles at most (the FP64 fallback of a low-precision factor)
-    return (TS * LDA + TS * LDB) * 8 + 2 * TS * 8 + 64;
+    return (TS * LDA + TS * LDB) * 8 + 2 * TS * 8 + 128;
 }
 
 // ------------------------------------------------------------------ loads
@@ -220,68 +220,150 @@ __device__ inline void gemm_task_bf16(float* __restrict__ pool, const Pair* __re
 
 // ------------------------------------------------------------------ sub
 // sub(src, src2 -> r): r = tile(src2) - tile(src); a missing operand (slot < 0)
-// reads as zero, which gives copy and negate (BlockPlanner.cpp:553-572).
+// reads as zero, which gives copy and negate (BlockPlanner.cpp:553-572).  All
+// loads are issued before the stores (the operands may not alias the result,
+// but the compiler cannot know that).
 template <typename T>
 __device__ void sub_task(T* __restrict__ pool, int src, int src2, int out) {
     const int tid = threadIdx.x;
+    constexpr int Q = TE / NT;  // 16 elements per thread
+    T a[Q], b[Q];
     const T* A = src2 >= 0 ? pool + (size_t)src2 * TE : nullptr;
     const T* B = src >= 0 ? pool + (size_t)src * TE : nullptr;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        a[q] = A ? A[q * NT + tid] : T(0);
+        b[q] = B ? B[q * NT + tid] : T(0);
+    }
     T* R = pool + (size_t)out * TE;
 #pragma unroll
-    for (int q = 0; q < TE / NT; q++) {
-        const int e = q * NT + tid;
-        T a = A ? A[e] : T(0);
-        T b = B ? B[e] : T(0);
-        R[e] = a - b;
-    }
+    for (int q = 0; q < Q; q++) R[q * NT + tid] = a[q] - b[q];
 }
 
-// ------------------------------------------------------------------ LU
-// Pivot-free tile LU (ludcmpSimple, MatrixStdDouble.cpp:2711-2784): L unit lower,
-// U upper; a pivot with |u_kk| < 1e-9 becomes -1e-9 if negative, else +1e-9
-// (MatrixStdDouble.cpp:2751-2761).  Right-looking in LDS, compute type C.
-// Returns (through lds) max|A| and max|U| for the growth check.
+// ------------------------------------------------------------------ register-resident tile algebra
+// Layout of the LU and inverse routines: wave w owns columns 16w .. 16w+15 and lane
+// i owns row i, so a thread keeps the 16 entries (i, 16w + c) in registers.  Row k
+// of the wave's columns is read with readlane (lane k), multipliers of the LU cross
+// waves through LDS once per step.
+__device__ inline double readlane(double x, int l) {
+    const unsigned long long u = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffull), l);
+    const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+    return __longlong_as_double(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ inline float readlane(float x, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+template <typename T, typename C>
+__device__ inline void load_rowseg(const T* g, C* v, int lane, int w) {
+    const T* p = g + lane * TS + 16 * w;
+#pragma unroll
+    for (int c = 0; c < 16; c++) v[c] = C(p[c]);
+}
+
+template <typename T, typename C>
+__device__ inline void store_rowseg(T* g, const C* v, int lane, int w) {
+    T* p = g + lane * TS + 16 * w;
+#pragma unroll
+    for (int c = 0; c < 16; c++) p[c] = T(v[c]);
+}
+
+// Pivot-free right-looking LU of the tile held in v (ludcmpSimple,
+// MatrixStdDouble.cpp:2711-2784): afterwards v holds U on and above the diagonal
+// and the multipliers of L below it.  |u_kk| < 1e-9 becomes -1e-9 if u_kk < 0,
+// else +1e-9 (MatrixStdDouble.cpp:2751-2761).  One barrier per step; lcol is a
+// double buffer of 2 x 64 multipliers in LDS.
 template <typename C>
-__device__ void lu_core(C* a, C* lcol, int tid) {
+__device__ inline void lu_regs(C* v, C* lcol, int lane, int w) {
     const C TINY2 = C(1e-9);
-    const int i = tid >> 2, j0 = (tid & 3) * 16;
-    for (int k = 0; k < TS; k++) {
-        C p = a[k * LDA + k];
-        if (p < TINY2 && p > -TINY2) p = (p < 0) ? -TINY2 : TINY2;
-        const C rp = C(1) / p;
-        if (tid < TS && tid > k) lcol[tid] = a[tid * LDA + k] * rp;
-        __syncthreads();
-        if (i > k) {
-            const C l = lcol[i];
+#pragma unroll 1
+    for (int kb = 0; kb < 4; kb++) {
 #pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const int j = j0 + q;
-                if (j > k) a[i * LDA + j] -= l * a[k * LDA + j];
-                else if (j == k) a[i * LDA + j] = l;
+        for (int kc = 0; kc < 16; kc++) {
+            const int k = kb * 16 + kc;
+            C* lc = lcol + (k & 1) * TS;
+            if (w == kb) {
+                C p = readlane(v[kc], k);
+                if (p < TINY2 && p > -TINY2) p = (p < 0) ? -TINY2 : TINY2;
+                const C l = v[kc] * (C(1) / p);
+                if (lane > k) {
+                    v[kc] = l;
+                    lc[lane] = l;
+                } else if (lane == k) {
+                    v[kc] = p;
+                }
             }
-        } else if (i == k && j0 <= k && k < j0 + 16) {
-            a[k * LDA + k] = p;
+            __syncthreads();
+            const C l = lc[lane];
+#pragma unroll
+            for (int c = 0; c < 16; c++) {
+                if (16 * w + c > k) {
+                    const C u = readlane(v[c], k);
+                    if (lane > k) v[c] -= l * u;
+                }
+            }
         }
-        __syncthreads();
     }
 }
 
-template <typename T, typename C>
-__device__ void lu_load(const T* A, C* a, int tid) {
-    const int r = tid >> 2, c0 = (tid & 3) * 16;
+// y = inverse of the lower triangle in ls (LDS, row stride LDA), rows of y in
+// lanes, the wave's 16 columns in registers; right-looking over rows, no barrier
+// (inv_lower, MatrixStdDouble.cpp:2787-2802: y_ii = 1/l_ii,
+// y_ij = -(sum_k l_ik y_kj) / l_ii).  UNIT skips the divisions by l_ii = 1.
+template <typename C, bool UNIT>
+__device__ inline void inv_lower_regs(const C* ls, C* y, int lane, int w) {
 #pragma unroll
-    for (int q = 0; q < 16; q++) a[r * LDA + c0 + q] = C(A[r * TS + c0 + q]);
+    for (int c = 0; c < 16; c++) y[c] = (16 * w + c == lane) ? C(1) : C(0);
+#pragma unroll 1
+    for (int kb = 0; kb < 4; kb++) {
+#pragma unroll
+        for (int kc = 0; kc < 16; kc++) {
+            const int k = kb * 16 + kc;
+            if (!UNIT) {  // row k is final: scale by 1 / l_kk
+                const C q = C(1) / ls[k * LDA + k];
+                if (lane == k) {
+#pragma unroll
+                    for (int c = 0; c < 16; c++) y[c] *= q;
+                }
+            }
+            const C f = ls[lane * LDA + k];
+#pragma unroll
+            for (int c = 0; c < 16; c++) {
+                if (16 * w + c <= k) {
+                    const C yk = readlane(y[c], k);
+                    if (lane > k) y[c] -= f * yk;
+                }
+            }
+        }
+    }
 }
 
-template <typename T, typename C>
-__device__ void lu_store(const C* a, T* L, T* U, int tid) {
-    const int r = tid >> 2, c0 = (tid & 3) * 16;
+// y = inverse of the upper triangle in us (LDS): rows from the bottom, row k scaled
+// by 1/u_kk once final (inv_upper, MatrixStdDouble.cpp:2829-2866).
+template <typename C>
+__device__ inline void inv_upper_regs(const C* us, C* y, int lane, int w) {
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const int c = c0 + q;
-        const C v = a[r * LDA + c];
-        L[r * TS + c] = r > c ? T(v) : (r == c ? T(1) : T(0));
-        U[r * TS + c] = c >= r ? T(v) : T(0);
+    for (int c = 0; c < 16; c++) y[c] = (16 * w + c == lane) ? C(1) : C(0);
+#pragma unroll 1
+    for (int kb = 3; kb >= 0; kb--) {
+#pragma unroll
+        for (int kc = 15; kc >= 0; kc--) {
+            const int k = kb * 16 + kc;
+            const C q = C(1) / us[k * LDA + k];
+            if (lane == k) {
+#pragma unroll
+                for (int c = 0; c < 16; c++) y[c] *= q;
+            }
+            const C f = us[lane * LDA + k];
+#pragma unroll
+            for (int c = 0; c < 16; c++) {
+                if (16 * w + c >= k) {
+                    const C yk = readlane(y[c], k);
+                    if (lane < k) y[c] -= f * yk;
+                }
+            }
+        }
     }
 }
 
@@ -301,102 +383,94 @@ __device__ inline C block_absmax(C x, C* red, int tid) {
     return m;
 }
 
-// growth = max|U| / max|A|; a low-precision tile above `growth_limit` is refactored
-// in FP64 from its (low-precision) input.  Returns the growth seen.
+// LU of one tile in compute type C plus, when their slots are given (>= 0), the
+// inverses of its unit-lower L and upper U: the reference's lowerInv / upperInv
+// of the same diagonal tile, fused into the producing task.
+template <typename T, typename C>
+__device__ void lu_inv_core(T* __restrict__ pool, int src, int outl, int outu, int outli, int outui,
+                            char* lds_raw, float* amax_out, float* umax_out) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    C* ls = reinterpret_cast<C*>(lds_raw);       // L (strict lower) then U, row stride LDA
+    C* us = ls + TS * LDA;
+    C* lcol = us + TS * LDA;                      // 2 x 64 multipliers
+    C* red = lcol + 2 * TS;                       // 8 reduction slots
+    C v[16];
+    load_rowseg(pool + (size_t)src * TE, v, lane, w);
+    if (amax_out) {
+        C m = 0;
+#pragma unroll
+        for (int c = 0; c < 16; c++) m = fabs(v[c]) > m ? fabs(v[c]) : m;
+        *amax_out = (float)block_absmax(m, red, tid);
+    }
+    lu_regs(v, lcol, lane, w);
+    // L (unit lower) and U out of the same registers, to HBM and, for the inverses, to LDS
+    const bool inv = outli >= 0 || outui >= 0;
+    T* Lg = pool + (size_t)outl * TE + lane * TS + 16 * w;
+    T* Ug = pool + (size_t)outu * TE + lane * TS + 16 * w;
+    C um = 0;
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+        const int j = 16 * w + c;
+        const C l = lane > j ? v[c] : (lane == j ? C(1) : C(0));
+        const C u = j >= lane ? v[c] : C(0);
+        Lg[c] = T(l);
+        Ug[c] = T(u);
+        um = fabs(u) > um ? fabs(u) : um;
+        if (inv) {
+            ls[lane * LDA + j] = l;
+            us[lane * LDA + j] = u;
+        }
+    }
+    if (umax_out) *umax_out = (float)block_absmax(um, red + 4, tid);
+    if (!inv) return;
+    __syncthreads();
+    C y[16];
+    if (outli >= 0) {
+        inv_lower_regs<C, true>(ls, y, lane, w);
+        store_rowseg(pool + (size_t)outli * TE, y, lane, w);
+    }
+    if (outui >= 0) {
+        inv_upper_regs<C>(us, y, lane, w);
+        store_rowseg(pool + (size_t)outui * TE, y, lane, w);
+    }
+}
+
+// The lu task: FP64 tiles factor in FP64; FP32 tiles factor in FP32 and a tile whose
+// pivot-free growth max|U| / max|A| exceeds growth_limit is refactored from its input
+// in FP64 (counted in n_fp64).  Returns the growth seen.
 template <typename T>
-__device__ double lu_task(T* __restrict__ pool, int src, int outl, int outu, char* lds_raw,
-                          double growth_limit, int* n_fp64) {
-    const int tid = threadIdx.x;
-    const T* A = pool + (size_t)src * TE;
-    T* L = pool + (size_t)outl * TE;
-    T* U = pool + (size_t)outu * TE;
+__device__ double lu_task(T* __restrict__ pool, int src, int outl, int outu, int outli, int outui,
+                          char* lds_raw, double growth_limit, int* n_fp64) {
     if constexpr (sizeof(T) == 8) {
-        double* a = reinterpret_cast<double*>(lds_raw);
-        double* lcol = a + TS * LDA;
-        lu_load(A, a, tid);
-        __syncthreads();
-        lu_core(a, lcol, tid);
-        lu_store(a, L, U, tid);
+        lu_inv_core<T, double>(pool, src, outl, outu, outli, outui, lds_raw, nullptr, nullptr);
         return 1.0;
     } else {
-        float* a = reinterpret_cast<float*>(lds_raw);
-        float* lcol = a + TS * LDA;
-        float* red = lcol + TS;
-        const int r = tid >> 2, c0 = (tid & 3) * 16;
-        float mx = 0;
-        lu_load(A, a, tid);
-#pragma unroll
-        for (int q = 0; q < 16; q++) {
-            float v = a[r * LDA + c0 + q];
-            v = v < 0 ? -v : v;
-            mx = v > mx ? v : mx;
-        }
-        const float amax = block_absmax(mx, red, tid);
-        lu_core(a, lcol, tid);
-        mx = 0;
-#pragma unroll
-        for (int q = 0; q < 16; q++) {
-            if (c0 + q >= r) {
-                float v = a[r * LDA + c0 + q];
-                v = v < 0 ? -v : v;
-                mx = v > mx ? v : mx;
-            }
-        }
-        const float umax = block_absmax(mx, red + 8, tid);
+        float amax = 0, umax = 0;
+        lu_inv_core<T, float>(pool, src, outl, outu, outli, outui, lds_raw, &amax, &umax);
         const double growth = amax > 0 ? (double)umax / (double)amax : 1.0;
         if (!(growth <= growth_limit)) {
             __syncthreads();
-            double* ad = reinterpret_cast<double*>(lds_raw);
-            double* lc = ad + TS * LDA;
-            lu_load(A, ad, tid);
-            __syncthreads();
-            lu_core(ad, lc, tid);
-            lu_store(ad, L, U, tid);
-            if (tid == 0) atomicAdd(n_fp64, 1);
-        } else {
-            lu_store(a, L, U, tid);
+            lu_inv_core<T, double>(pool, src, outl, outu, outli, outui, lds_raw, nullptr, nullptr);
+            if (threadIdx.x == 0) atomicAdd(n_fp64, 1);
         }
         return growth;
     }
 }
 
-// ------------------------------------------------------------------ inverses
-// Y = L^-1 for lower L with general diagonal (inv_lower, MatrixStdDouble.cpp:2787-2802:
-// y_ii = 1/l_ii, y_ij = -(sum_k l_ik y_kj) / l_ii): right-looking over rows of Y.
+// Standalone inverse of a lower (general diagonal) or upper tile.
 template <typename T, bool LOWER>
 __device__ void tri_inv_task(T* __restrict__ pool, int src, int out, char* lds_raw) {
     typedef typename std::conditional<sizeof(T) == 8, double, float>::type C;
-    C* l = reinterpret_cast<C*>(lds_raw);
-    C* y = l + TS * LDA;
-    const int tid = threadIdx.x;
-    const T* Lg = pool + (size_t)src * TE;
-    const int i = tid >> 2, j0 = (tid & 3) * 16;
-    lu_load(Lg, l, tid);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    C* ts = reinterpret_cast<C*>(lds_raw);
+    C v[16];
+    load_rowseg(pool + (size_t)src * TE, v, lane, w);
 #pragma unroll
-    for (int q = 0; q < 16; q++) y[i * LDA + j0 + q] = (i == j0 + q) ? C(1) : C(0);
+    for (int c = 0; c < 16; c++) ts[lane * LDA + 16 * w + c] = v[c];
     __syncthreads();
-    for (int s = 0; s < TS; s++) {
-        const int k = LOWER ? s : TS - 1 - s;
-        // row k of Y is final once scaled: y_k = r_k / t_kk
-        if (tid < TS) {
-            const int j = tid;
-            if (LOWER ? (j <= k) : (j >= k)) y[k * LDA + j] = y[k * LDA + j] * (C(1) / l[k * LDA + k]);
-        }
-        __syncthreads();
-        // eliminate column k from the remaining rows: r_i -= t_ik y_k
-        if (LOWER ? (i > k) : (i < k)) {
-            const C f = l[i * LDA + k];
-#pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const int j = j0 + q;
-                if (LOWER ? (j <= k) : (j >= k)) y[i * LDA + j] -= f * y[k * LDA + j];
-            }
-        }
-        __syncthreads();
-    }
-    T* Y = pool + (size_t)out * TE;
-#pragma unroll
-    for (int q = 0; q < 16; q++) Y[i * TS + j0 + q] = T(y[i * LDA + j0 + q]);
+    if (LOWER) inv_lower_regs<C, false>(ts, v, lane, w);
+    else inv_upper_regs<C>(ts, v, lane, w);
+    store_rowseg(pool + (size_t)out * TE, v, lane, w);
 }
 
 }  // namespace sogl_dev
