@@ -399,16 +399,33 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
         for (int which = 0; which < 2; which++) {
             const int32_t ds = (which == 0 ? f->l_diag : f->u_diag)[(size_t)i];
             const int32_t pt = ds >= 0 && ds < (int32_t)prod.size() ? prod[(size_t)ds] : -1;
-            if (pt < 0 || tasks[(size_t)pt].kind != TK_LU || (which == 0 ? tasks[(size_t)pt].out0
-                                                                          : tasks[(size_t)pt].out1) != ds)
-                return set_error(c, SOGL_ERR_ARG, "a diagonal factor tile is not the output of an lu op");
-            DevTask& lt = tasks[(size_t)pt];
-            int32_t& inv = which == 0 ? lt.out2 : lt.out3;
-            if (inv < 0) {  // the plan never inverts this tile: a solve-only slot
+            const bool from_lu = pt >= 0 && tasks[(size_t)pt].kind == TK_LU &&
+                                 (which == 0 ? tasks[(size_t)pt].out0 : tasks[(size_t)pt].out1) == ds;
+            int32_t inv;
+            if (from_lu) {
+                inv = which == 0 ? tasks[(size_t)pt].out2 : tasks[(size_t)pt].out3;
+                if (inv < 0) {  // the plan never inverts this tile: a solve-only slot
+                    inv = next++;
+                    prod.push_back(pt);
+                    (which == 0 ? tasks[(size_t)pt].out2 : tasks[(size_t)pt].out3) = inv;
+                    DevTask& ot = f->op_tasks[(size_t)lu_op_task[(size_t)pt]];
+                    (which == 0 ? ot.out2 : ot.out3) = inv;
+                }
+            } else {
+                // a diagonal tile that no lu produced (hand-built descriptions): its own
+                // inverse task, general diagonal
                 inv = next++;
-                prod.push_back(pt);
-                DevTask& ot = f->op_tasks[(size_t)lu_op_task[(size_t)pt]];
-                (which == 0 ? ot.out2 : ot.out3) = inv;
+                DevTask nt{};
+                nt.kind = which == 0 ? TK_LINV : TK_UINV;
+                nt.out0 = inv;
+                nt.out1 = nt.out2 = nt.out3 = -1;
+                nt.a = ds;
+                tasks.push_back(nt);
+                tpairs.emplace_back();
+                tstage.push_back(0);
+                lu_op_task.push_back(-1);
+                prod.push_back((int32_t)tasks.size() - 1);
+                f->op_tasks.push_back(nt);
             }
             (which == 0 ? f->linv_slot : f->uinv_slot)[(size_t)i] = inv;
         }
