@@ -269,39 +269,47 @@ __device__ inline void store_rowseg(T* g, const C* v, int lane, int w) {
     for (int c = 0; c < 16; c++) p[c] = T(v[c]);
 }
 
+// v[kc] for a run-time kc, without indexing the register array (which would
+// spill it): a chain of selects
+template <typename C>
+__device__ inline C pick16(const C* v, int kc) {
+    C x = v[0];
+#pragma unroll
+    for (int c = 1; c < 16; c++) x = (kc == c) ? v[c] : x;
+    return x;
+}
+
 // Pivot-free right-looking LU of the tile held in v (ludcmpSimple,
 // MatrixStdDouble.cpp:2711-2784): afterwards v holds U on and above the diagonal
 // and the multipliers of L below it.  |u_kk| < 1e-9 becomes -1e-9 if u_kk < 0,
 // else +1e-9 (MatrixStdDouble.cpp:2751-2761).  One barrier per step; lcol is a
-// double buffer of 2 x 64 multipliers in LDS.
+// double buffer of 2 x 64 multipliers in LDS.  The step loop stays rolled so the
+// routine fits the instruction cache.
 template <typename C>
-__device__ inline void lu_regs(C* v, C* lcol, int lane, int w) {
+__device__ __noinline__ void lu_regs(C* v, C* lcol, int lane, int w) {
     const C TINY2 = C(1e-9);
 #pragma unroll 1
-    for (int kb = 0; kb < 4; kb++) {
+    for (int k = 0; k < TS; k++) {
+        const int kb = k >> 4, kc = k & 15;
+        C* lc = lcol + (k & 1) * TS;
+        if (w == kb) {  // the wave owning column k makes the multipliers
+            const C col = pick16(v, kc);
+            C p = readlane(col, k);
+            if (p < TINY2 && p > -TINY2) p = (p < 0) ? -TINY2 : TINY2;
+            const C l = col * (C(1) / p);
+            const C nv = lane > k ? l : (lane == k ? p : col);
 #pragma unroll
-        for (int kc = 0; kc < 16; kc++) {
-            const int k = kb * 16 + kc;
-            C* lc = lcol + (k & 1) * TS;
-            if (w == kb) {
-                C p = readlane(v[kc], k);
-                if (p < TINY2 && p > -TINY2) p = (p < 0) ? -TINY2 : TINY2;
-                const C l = v[kc] * (C(1) / p);
-                if (lane > k) {
-                    v[kc] = l;
-                    lc[lane] = l;
-                } else if (lane == k) {
-                    v[kc] = p;
-                }
-            }
-            __syncthreads();
-            const C l = lc[lane];
+            for (int c = 0; c < 16; c++) v[c] = (kc == c) ? nv : v[c];
+            if (lane > k) lc[lane] = l;
+        }
+        __syncthreads();
+        const C l = lc[lane];
+        const bool below = lane > k;
 #pragma unroll
-            for (int c = 0; c < 16; c++) {
-                if (16 * w + c > k) {
-                    const C u = readlane(v[c], k);
-                    if (lane > k) v[c] -= l * u;
-                }
+        for (int c = 0; c < 16; c++) {
+            if (16 * w + c > k) {
+                const C u = readlane(v[c], k);
+                if (below) v[c] -= l * u;
             }
         }
     }
@@ -312,28 +320,25 @@ __device__ inline void lu_regs(C* v, C* lcol, int lane, int w) {
 // (inv_lower, MatrixStdDouble.cpp:2787-2802: y_ii = 1/l_ii,
 // y_ij = -(sum_k l_ik y_kj) / l_ii).  UNIT skips the divisions by l_ii = 1.
 template <typename C, bool UNIT>
-__device__ inline void inv_lower_regs(const C* ls, C* y, int lane, int w) {
+__device__ __noinline__ void inv_lower_regs(const C* ls, C* y, int lane, int w) {
 #pragma unroll
     for (int c = 0; c < 16; c++) y[c] = (16 * w + c == lane) ? C(1) : C(0);
 #pragma unroll 1
-    for (int kb = 0; kb < 4; kb++) {
+    for (int k = 0; k < TS; k++) {
+        if (!UNIT) {  // row k is final: scale by 1 / l_kk
+            const C q = C(1) / ls[k * LDA + k];
+            if (lane == k) {
 #pragma unroll
-        for (int kc = 0; kc < 16; kc++) {
-            const int k = kb * 16 + kc;
-            if (!UNIT) {  // row k is final: scale by 1 / l_kk
-                const C q = C(1) / ls[k * LDA + k];
-                if (lane == k) {
-#pragma unroll
-                    for (int c = 0; c < 16; c++) y[c] *= q;
-                }
+                for (int c = 0; c < 16; c++) y[c] *= q;
             }
-            const C f = ls[lane * LDA + k];
+        }
+        const C f = ls[lane * LDA + k];
+        const bool below = lane > k;
 #pragma unroll
-            for (int c = 0; c < 16; c++) {
-                if (16 * w + c <= k) {
-                    const C yk = readlane(y[c], k);
-                    if (lane > k) y[c] -= f * yk;
-                }
+        for (int c = 0; c < 16; c++) {
+            if (16 * w + c <= k) {
+                const C yk = readlane(y[c], k);
+                if (below) y[c] -= f * yk;
             }
         }
     }
@@ -342,26 +347,23 @@ __device__ inline void inv_lower_regs(const C* ls, C* y, int lane, int w) {
 // y = inverse of the upper triangle in us (LDS): rows from the bottom, row k scaled
 // by 1/u_kk once final (inv_upper, MatrixStdDouble.cpp:2829-2866).
 template <typename C>
-__device__ inline void inv_upper_regs(const C* us, C* y, int lane, int w) {
+__device__ __noinline__ void inv_upper_regs(const C* us, C* y, int lane, int w) {
 #pragma unroll
     for (int c = 0; c < 16; c++) y[c] = (16 * w + c == lane) ? C(1) : C(0);
 #pragma unroll 1
-    for (int kb = 3; kb >= 0; kb--) {
+    for (int k = TS - 1; k >= 0; k--) {
+        const C q = C(1) / us[k * LDA + k];
+        if (lane == k) {
 #pragma unroll
-        for (int kc = 15; kc >= 0; kc--) {
-            const int k = kb * 16 + kc;
-            const C q = C(1) / us[k * LDA + k];
-            if (lane == k) {
+            for (int c = 0; c < 16; c++) y[c] *= q;
+        }
+        const C f = us[lane * LDA + k];
+        const bool above = lane < k;
 #pragma unroll
-                for (int c = 0; c < 16; c++) y[c] *= q;
-            }
-            const C f = us[lane * LDA + k];
-#pragma unroll
-            for (int c = 0; c < 16; c++) {
-                if (16 * w + c >= k) {
-                    const C yk = readlane(y[c], k);
-                    if (lane < k) y[c] -= f * yk;
-                }
+        for (int c = 0; c < 16; c++) {
+            if (16 * w + c >= k) {
+                const C yk = readlane(y[c], k);
+                if (above) y[c] -= f * yk;
             }
         }
     }
