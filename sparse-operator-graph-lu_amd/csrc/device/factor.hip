@@ -145,7 +145,7 @@ struct PairWait {
 };
 
 template <typename T, bool BF16, bool ACC, typename Wait = NoWait>
-__device__ inline void run_task(const DevTask& t, const Args& a, char* lds, const Wait& wait = Wait()) {
+__device__ __attribute__((always_inline)) void run_task(const DevTask& t, const Args& a, char* lds, const Wait& wait = Wait()) {
     T* pool = reinterpret_cast<T*>(a.pool);
     switch (t.kind) {
         case TK_GEMM:
@@ -181,7 +181,7 @@ __device__ inline void run_task(const DevTask& t, const Args& a, char* lds, cons
 // tasks wait pair by pair (pairs are ordered by when their operands become
 // ready); other tasks wait for all their sources first.
 template <typename T, bool BF16>
-__global__ __launch_bounds__(NT) void k_factor_persistent(Args a) {
+__global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
     __shared__ __attribute__((aligned(16))) char lds[LDS];
     __shared__ int s_task;
     __shared__ int s_abort;

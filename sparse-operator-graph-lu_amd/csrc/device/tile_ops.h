@@ -110,7 +110,7 @@ struct NoWait {
 };
 
 template <typename T, bool ACC = false, typename Wait = NoWait>
-__device__ void gemm_task(T* __restrict__ pool, const Pair* __restrict__ pairs, int npairs,
+__device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, const Pair* __restrict__ pairs, int npairs,
                           int out, char* lds_raw, const Wait& wait = Wait()) {
     typedef typename MfmaOp<T>::acc_t acc_t;
     T* la = reinterpret_cast<T*>(lds_raw);
@@ -167,7 +167,7 @@ __device__ void gemm_task(T* __restrict__ pool, const Pair* __restrict__ pairs, 
 // bf16 experiment: FP32 tiles, operands rounded to bf16, FP32 accumulation
 // (v_mfma_f32_16x16x32_bf16: A[i][8q..8q+7] in lane i + 16q, same D layout).
 template <typename Wait = NoWait>
-__device__ inline void gemm_task_bf16(float* __restrict__ pool, const Pair* __restrict__ pairs,
+__device__ __attribute__((always_inline)) void gemm_task_bf16(float* __restrict__ pool, const Pair* __restrict__ pairs,
                                       int npairs, int out, char* lds_raw, const Wait& wait = Wait()) {
     float* la = reinterpret_cast<float*>(lds_raw);
     float* lb = la + TS * LDA;
@@ -224,7 +224,7 @@ __device__ inline void gemm_task_bf16(float* __restrict__ pool, const Pair* __re
 // loads are issued before the stores (the operands may not alias the result,
 // but the compiler cannot know that).
 template <typename T>
-__device__ void sub_task(T* __restrict__ pool, int src, int src2, int out) {
+__device__ __attribute__((always_inline)) void sub_task(T* __restrict__ pool, int src, int src2, int out) {
     const int tid = threadIdx.x;
     constexpr int Q = TE / NT;  // 16 elements per thread
     T a[Q], b[Q];
@@ -269,47 +269,41 @@ __device__ inline void store_rowseg(T* g, const C* v, int lane, int w) {
     for (int c = 0; c < 16; c++) p[c] = T(v[c]);
 }
 
-// v[kc] for a run-time kc, without indexing the register array (which would
-// spill it): a chain of selects
-template <typename C>
-__device__ inline C pick16(const C* v, int kc) {
-    C x = v[0];
-#pragma unroll
-    for (int c = 1; c < 16; c++) x = (kc == c) ? v[c] : x;
-    return x;
-}
-
 // Pivot-free right-looking LU of the tile held in v (ludcmpSimple,
 // MatrixStdDouble.cpp:2711-2784): afterwards v holds U on and above the diagonal
 // and the multipliers of L below it.  |u_kk| < 1e-9 becomes -1e-9 if u_kk < 0,
 // else +1e-9 (MatrixStdDouble.cpp:2751-2761).  One barrier per step; lcol is a
-// double buffer of 2 x 64 multipliers in LDS.  The step loop stays rolled so the
-// routine fits the instruction cache.
+// double buffer of 2 x 64 multipliers in LDS.  The 16 steps of a column panel are
+// unrolled so the pivot column's register index is a constant; the 4 panels loop.
 template <typename C>
-__device__ __noinline__ void lu_regs(C* v, C* lcol, int lane, int w) {
+__device__ inline void lu_regs(C* v, C* lcol, int lane, int w) {
     const C TINY2 = C(1e-9);
 #pragma unroll 1
-    for (int k = 0; k < TS; k++) {
-        const int kb = k >> 4, kc = k & 15;
-        C* lc = lcol + (k & 1) * TS;
-        if (w == kb) {  // the wave owning column k makes the multipliers
-            const C col = pick16(v, kc);
-            C p = readlane(col, k);
-            if (p < TINY2 && p > -TINY2) p = (p < 0) ? -TINY2 : TINY2;
-            const C l = col * (C(1) / p);
-            const C nv = lane > k ? l : (lane == k ? p : col);
+    for (int kb = 0; kb < 4; kb++) {
 #pragma unroll
-            for (int c = 0; c < 16; c++) v[c] = (kc == c) ? nv : v[c];
-            if (lane > k) lc[lane] = l;
-        }
-        __syncthreads();
-        const C l = lc[lane];
-        const bool below = lane > k;
+        for (int kc = 0; kc < 16; kc++) {
+            const int k = kb * 16 + kc;
+            C* lc = lcol + (kc & 1) * TS;
+            if (w == kb) {  // the wave owning column k makes the multipliers
+                C p = readlane(v[kc], k);
+                if (p < TINY2 && p > -TINY2) p = (p < 0) ? -TINY2 : TINY2;
+                const C l = v[kc] * (C(1) / p);
+                if (lane > k) {
+                    v[kc] = l;
+                    lc[lane] = l;
+                } else if (lane == k) {
+                    v[kc] = p;
+                }
+            }
+            __syncthreads();
+            const C l = lc[lane];
+            const bool below = lane > k;
 #pragma unroll
-        for (int c = 0; c < 16; c++) {
-            if (16 * w + c > k) {
-                const C u = readlane(v[c], k);
-                if (below) v[c] -= l * u;
+            for (int c = 0; c < 16; c++) {
+                if (16 * w + c > k) {
+                    const C u = readlane(v[c], k);
+                    if (below) v[c] -= l * u;
+                }
             }
         }
     }
@@ -320,7 +314,7 @@ __device__ __noinline__ void lu_regs(C* v, C* lcol, int lane, int w) {
 // (inv_lower, MatrixStdDouble.cpp:2787-2802: y_ii = 1/l_ii,
 // y_ij = -(sum_k l_ik y_kj) / l_ii).  UNIT skips the divisions by l_ii = 1.
 template <typename C, bool UNIT>
-__device__ __noinline__ void inv_lower_regs(const C* ls, C* y, int lane, int w) {
+__device__ inline void inv_lower_regs(const C* ls, C* y, int lane, int w) {
 #pragma unroll
     for (int c = 0; c < 16; c++) y[c] = (16 * w + c == lane) ? C(1) : C(0);
 #pragma unroll 1
@@ -347,7 +341,7 @@ __device__ __noinline__ void inv_lower_regs(const C* ls, C* y, int lane, int w) 
 // y = inverse of the upper triangle in us (LDS): rows from the bottom, row k scaled
 // by 1/u_kk once final (inv_upper, MatrixStdDouble.cpp:2829-2866).
 template <typename C>
-__device__ __noinline__ void inv_upper_regs(const C* us, C* y, int lane, int w) {
+__device__ inline void inv_upper_regs(const C* us, C* y, int lane, int w) {
 #pragma unroll
     for (int c = 0; c < 16; c++) y[c] = (16 * w + c == lane) ? C(1) : C(0);
 #pragma unroll 1
@@ -389,7 +383,7 @@ __device__ inline C block_absmax(C x, C* red, int tid) {
 // inverses of its unit-lower L and upper U: the reference's lowerInv / upperInv
 // of the same diagonal tile, fused into the producing task.
 template <typename T, typename C>
-__device__ void lu_inv_core(T* __restrict__ pool, int src, int outl, int outu, int outli, int outui,
+__device__ __attribute__((always_inline)) void lu_inv_core(T* __restrict__ pool, int src, int outl, int outu, int outli, int outui,
                             char* lds_raw, float* amax_out, float* umax_out) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     C* ls = reinterpret_cast<C*>(lds_raw);       // L (strict lower) then U, row stride LDA
@@ -437,11 +431,67 @@ __device__ void lu_inv_core(T* __restrict__ pool, int src, int outl, int outu, i
     }
 }
 
+// FP64 refactorisation of a low-precision tile whose pivot-free growth is too large.
+// It runs rarely, so it uses a compact LDS-resident form (two barriers per step)
+// rather than a second copy of the register-resident code; L and U stay in the
+// one LDS tile, which the inverse routines read directly (strict lower part as L
+// with unit diagonal, upper part as U).
+template <typename T>
+__device__ inline void lu_inv_fp64_lds(T* __restrict__ pool, int src, int outl, int outu, int outli,
+                                       int outui, char* lds_raw) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    double* a = reinterpret_cast<double*>(lds_raw);
+    double* lcol = a + TS * LDA;
+    const T* A = pool + (size_t)src * TE;
+    const int i = tid >> 2, j0 = (tid & 3) * 16;
+#pragma unroll
+    for (int q = 0; q < 16; q++) a[i * LDA + j0 + q] = (double)A[i * TS + j0 + q];
+    __syncthreads();
+    const double TINY2 = 1e-9;
+#pragma unroll 1
+    for (int k = 0; k < TS; k++) {
+        double p = a[k * LDA + k];
+        if (p < TINY2 && p > -TINY2) p = (p < 0) ? -TINY2 : TINY2;
+        if (tid < TS && tid > k) lcol[tid] = a[tid * LDA + k] * (1.0 / p);
+        __syncthreads();
+        if (i > k) {
+            const double l = lcol[i];
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int j = j0 + q;
+                if (j > k) a[i * LDA + j] -= l * a[k * LDA + j];
+                else if (j == k) a[i * LDA + j] = l;
+            }
+        } else if (i == k && j0 <= k && k < j0 + 16) {
+            a[k * LDA + k] = p;
+        }
+        __syncthreads();
+    }
+    T* L = pool + (size_t)outl * TE;
+    T* U = pool + (size_t)outu * TE;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int j = j0 + q;
+        const double v = a[i * LDA + j];
+        L[i * TS + j] = i > j ? T(v) : (i == j ? T(1) : T(0));
+        U[i * TS + j] = j >= i ? T(v) : T(0);
+    }
+    double y[16];
+    if (outli >= 0) {
+        inv_lower_regs<double, true>(a, y, lane, w);
+        store_rowseg(pool + (size_t)outli * TE, y, lane, w);
+    }
+    if (outui >= 0) {
+        inv_upper_regs<double>(a, y, lane, w);
+        store_rowseg(pool + (size_t)outui * TE, y, lane, w);
+    }
+}
+
 // The lu task: FP64 tiles factor in FP64; FP32 tiles factor in FP32 and a tile whose
 // pivot-free growth max|U| / max|A| exceeds growth_limit is refactored from its input
 // in FP64 (counted in n_fp64).  Returns the growth seen.
 template <typename T>
-__device__ double lu_task(T* __restrict__ pool, int src, int outl, int outu, int outli, int outui,
+__device__ __attribute__((always_inline)) double lu_task(T* __restrict__ pool, int src, int outl, int outu, int outli, int outui,
                           char* lds_raw, double growth_limit, int* n_fp64) {
     if constexpr (sizeof(T) == 8) {
         lu_inv_core<T, double>(pool, src, outl, outu, outli, outui, lds_raw, nullptr, nullptr);
@@ -452,7 +502,7 @@ __device__ double lu_task(T* __restrict__ pool, int src, int outl, int outu, int
         const double growth = amax > 0 ? (double)umax / (double)amax : 1.0;
         if (!(growth <= growth_limit)) {
             __syncthreads();
-            lu_inv_core<T, double>(pool, src, outl, outu, outli, outui, lds_raw, nullptr, nullptr);
+            lu_inv_fp64_lds<T>(pool, src, outl, outu, outli, outui, lds_raw);
             if (threadIdx.x == 0) atomicAdd(n_fp64, 1);
         }
         return growth;
@@ -461,7 +511,7 @@ __device__ double lu_task(T* __restrict__ pool, int src, int outl, int outu, int
 
 // Standalone inverse of a lower (general diagonal) or upper tile.
 template <typename T, bool LOWER>
-__device__ void tri_inv_task(T* __restrict__ pool, int src, int out, char* lds_raw) {
+__device__ __attribute__((always_inline)) void tri_inv_task(T* __restrict__ pool, int src, int out, char* lds_raw) {
     typedef typename std::conditional<sizeof(T) == 8, double, float>::type C;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     C* ts = reinterpret_cast<C*>(lds_raw);
