@@ -114,6 +114,21 @@ def run_path(dev, plan, prob, precision, tol, max_iter, steps, warmup, mode):
                 growth_fp64_tiles=int(st.growth_tiles), x=x[:, 0] if nrhs == 1 else x)
 
 
+def calibrate_copy(nbytes: int = 1 << 30, reps: int = 3):
+    """A plain elementwise copy of known size: each launch reads and writes nbytes.
+
+    Its FETCH_SIZE / WRITE_SIZE per dispatch give the scale between those counters
+    and bytes moved to or from HBM."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": nbytes, "launches": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,7 +137,12 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--exec", default="persistent", choices=["persistent", "levels"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device copy kernel 3 times (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the factor on a single MI355X "
               "(multi-GPU sharding is a design note only); use --gpus 1", file=sys.stderr)

@@ -38,7 +38,7 @@ template <typename T>
 constexpr int lds_bytes() {
     // gemm: A (64 x LDA) + B (64 x LDB) in the MFMA operand type; the lu / inverses use
     // 2 x 64 x LDA doubles at most (the FP64 fallback of a low-precision factor)
-    return (TS * LDA + TS * LDB) * 8 + 2 * TS * 8 + 128;
+    return (TS * LDA + TS * LDB) * 8 + 2 * TS * 8 + 8 * 8 + 64 * 8;
 }
 
 // ------------------------------------------------------------------ loads
@@ -269,14 +269,31 @@ __device__ inline void store_rowseg(T* g, const C* v, int lane, int w) {
     for (int c = 0; c < 16; c++) p[c] = T(v[c]);
 }
 
+// Row k of the wave's 16 columns, from lane k to every lane of the wave through
+// LDS (bro: 16 values per wave).  LDS accesses of one wave complete in order, so
+// lane k's stores are seen by the loads that follow; the wave barrier only keeps
+// the compiler from reordering them.
+template <typename C>
+__device__ inline void bcast_row(const C* v, C* bro, int lane, int k, C* out) {
+    if (lane == k) {
+#pragma unroll
+        for (int c = 0; c < 16; c++) bro[c] = v[c];
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int c = 0; c < 16; c++) out[c] = bro[c];
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Pivot-free right-looking LU of the tile held in v (ludcmpSimple,
 // MatrixStdDouble.cpp:2711-2784): afterwards v holds U on and above the diagonal
 // and the multipliers of L below it.  |u_kk| < 1e-9 becomes -1e-9 if u_kk < 0,
-// else +1e-9 (MatrixStdDouble.cpp:2751-2761).  One barrier per step; lcol is a
-// double buffer of 2 x 64 multipliers in LDS.  The 16 steps of a column panel are
-// unrolled so the pivot column's register index is a constant; the 4 panels loop.
+// else +1e-9 (MatrixStdDouble.cpp:2751-2761).  One barrier per step (lcol, a
+// double buffer of 2 x 64 multipliers in LDS); the pivot row reaches the wave's
+// lanes through its bro buffer.  The 16 steps of a column panel are unrolled so
+// the pivot column's register index is a constant; the 4 panels loop.
 template <typename C>
-__device__ inline void lu_regs(C* v, C* lcol, int lane, int w) {
+__device__ inline void lu_regs(C* v, C* lcol, C* bro, int lane, int w) {
     const C TINY2 = C(1e-9);
 #pragma unroll 1
     for (int kb = 0; kb < 4; kb++) {
@@ -284,27 +301,21 @@ __device__ inline void lu_regs(C* v, C* lcol, int lane, int w) {
         for (int kc = 0; kc < 16; kc++) {
             const int k = kb * 16 + kc;
             C* lc = lcol + (kc & 1) * TS;
+            C u[16];
+            bcast_row(v, bro, lane, k, u);
             if (w == kb) {  // the wave owning column k makes the multipliers
-                C p = readlane(v[kc], k);
+                C p = u[kc];
                 if (p < TINY2 && p > -TINY2) p = (p < 0) ? -TINY2 : TINY2;
                 const C l = v[kc] * (C(1) / p);
-                if (lane > k) {
-                    v[kc] = l;
-                    lc[lane] = l;
-                } else if (lane == k) {
-                    v[kc] = p;
-                }
+                v[kc] = lane > k ? l : (lane == k ? p : v[kc]);
+                lc[lane] = l;  // read only by the lanes below the pivot
             }
             __syncthreads();
             const C l = lc[lane];
             const bool below = lane > k;
 #pragma unroll
-            for (int c = 0; c < 16; c++) {
-                if (16 * w + c > k) {
-                    const C u = readlane(v[c], k);
-                    if (below) v[c] -= l * u;
-                }
-            }
+            for (int c = 0; c < 16; c++)  // branch-free: columns <= k keep their value
+                v[c] = (below && 16 * w + c > k) ? v[c] - l * u[c] : v[c];
         }
     }
 }
@@ -314,52 +325,44 @@ __device__ inline void lu_regs(C* v, C* lcol, int lane, int w) {
 // (inv_lower, MatrixStdDouble.cpp:2787-2802: y_ii = 1/l_ii,
 // y_ij = -(sum_k l_ik y_kj) / l_ii).  UNIT skips the divisions by l_ii = 1.
 template <typename C, bool UNIT>
-__device__ inline void inv_lower_regs(const C* ls, C* y, int lane, int w) {
+__device__ inline void inv_lower_regs(const C* ls, C* y, C* bro, int lane, int w) {
 #pragma unroll
     for (int c = 0; c < 16; c++) y[c] = (16 * w + c == lane) ? C(1) : C(0);
 #pragma unroll 1
     for (int k = 0; k < TS; k++) {
         if (!UNIT) {  // row k is final: scale by 1 / l_kk
             const C q = C(1) / ls[k * LDA + k];
-            if (lane == k) {
 #pragma unroll
-                for (int c = 0; c < 16; c++) y[c] *= q;
-            }
+            for (int c = 0; c < 16; c++) y[c] = lane == k ? y[c] * q : y[c];
         }
+        C yk[16];
+        bcast_row(y, bro, lane, k, yk);
         const C f = ls[lane * LDA + k];
         const bool below = lane > k;
 #pragma unroll
-        for (int c = 0; c < 16; c++) {
-            if (16 * w + c <= k) {
-                const C yk = readlane(y[c], k);
-                if (below) y[c] -= f * yk;
-            }
-        }
+        for (int c = 0; c < 16; c++)  // branch-free: columns > k of y_k are zero
+            y[c] = (below && 16 * w + c <= k) ? y[c] - f * yk[c] : y[c];
     }
 }
 
 // y = inverse of the upper triangle in us (LDS): rows from the bottom, row k scaled
 // by 1/u_kk once final (inv_upper, MatrixStdDouble.cpp:2829-2866).
 template <typename C>
-__device__ inline void inv_upper_regs(const C* us, C* y, int lane, int w) {
+__device__ inline void inv_upper_regs(const C* us, C* y, C* bro, int lane, int w) {
 #pragma unroll
     for (int c = 0; c < 16; c++) y[c] = (16 * w + c == lane) ? C(1) : C(0);
 #pragma unroll 1
     for (int k = TS - 1; k >= 0; k--) {
         const C q = C(1) / us[k * LDA + k];
-        if (lane == k) {
 #pragma unroll
-            for (int c = 0; c < 16; c++) y[c] *= q;
-        }
+        for (int c = 0; c < 16; c++) y[c] = lane == k ? y[c] * q : y[c];
+        C yk[16];
+        bcast_row(y, bro, lane, k, yk);
         const C f = us[lane * LDA + k];
         const bool above = lane < k;
 #pragma unroll
-        for (int c = 0; c < 16; c++) {
-            if (16 * w + c >= k) {
-                const C yk = readlane(y[c], k);
-                if (above) y[c] -= f * yk;
-            }
-        }
+        for (int c = 0; c < 16; c++)  // branch-free: columns < k of y_k are zero
+            y[c] = (above && 16 * w + c >= k) ? y[c] - f * yk[c] : y[c];
     }
 }
 
@@ -385,11 +388,13 @@ __device__ inline C block_absmax(C x, C* red, int tid) {
 template <typename T, typename C>
 __device__ __attribute__((always_inline)) void lu_inv_core(T* __restrict__ pool, int src, int outl, int outu, int outli, int outui,
                             char* lds_raw, float* amax_out, float* umax_out) {
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     C* ls = reinterpret_cast<C*>(lds_raw);       // L (strict lower) then U, row stride LDA
     C* us = ls + TS * LDA;
     C* lcol = us + TS * LDA;                      // 2 x 64 multipliers
     C* red = lcol + 2 * TS;                       // 8 reduction slots
+    C* bro = red + 8 + 16 * w;                    // 16 per wave: row broadcasts
     C v[16];
     load_rowseg(pool + (size_t)src * TE, v, lane, w);
     if (amax_out) {
@@ -398,7 +403,7 @@ __device__ __attribute__((always_inline)) void lu_inv_core(T* __restrict__ pool,
         for (int c = 0; c < 16; c++) m = fabs(v[c]) > m ? fabs(v[c]) : m;
         *amax_out = (float)block_absmax(m, red, tid);
     }
-    lu_regs(v, lcol, lane, w);
+    lu_regs(v, lcol, bro, lane, w);
     // L (unit lower) and U out of the same registers, to HBM and, for the inverses, to LDS
     const bool inv = outli >= 0 || outui >= 0;
     T* Lg = pool + (size_t)outl * TE + lane * TS + 16 * w;
@@ -422,11 +427,11 @@ __device__ __attribute__((always_inline)) void lu_inv_core(T* __restrict__ pool,
     __syncthreads();
     C y[16];
     if (outli >= 0) {
-        inv_lower_regs<C, true>(ls, y, lane, w);
+        inv_lower_regs<C, true>(ls, y, bro, lane, w);
         store_rowseg(pool + (size_t)outli * TE, y, lane, w);
     }
     if (outui >= 0) {
-        inv_upper_regs<C>(us, y, lane, w);
+        inv_upper_regs<C>(us, y, bro, lane, w);
         store_rowseg(pool + (size_t)outui * TE, y, lane, w);
     }
 }
@@ -439,7 +444,8 @@ __device__ __attribute__((always_inline)) void lu_inv_core(T* __restrict__ pool,
 template <typename T>
 __device__ inline void lu_inv_fp64_lds(T* __restrict__ pool, int src, int outl, int outu, int outli,
                                        int outui, char* lds_raw) {
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     double* a = reinterpret_cast<double*>(lds_raw);
     double* lcol = a + TS * LDA;
     const T* A = pool + (size_t)src * TE;
@@ -477,12 +483,14 @@ __device__ inline void lu_inv_fp64_lds(T* __restrict__ pool, int src, int outl, 
         U[i * TS + j] = j >= i ? T(v) : T(0);
     }
     double y[16];
+    double* bro = lcol + TS + 16 * w;
+    __syncthreads();
     if (outli >= 0) {
-        inv_lower_regs<double, true>(a, y, lane, w);
+        inv_lower_regs<double, true>(a, y, bro, lane, w);
         store_rowseg(pool + (size_t)outli * TE, y, lane, w);
     }
     if (outui >= 0) {
-        inv_upper_regs<double>(a, y, lane, w);
+        inv_upper_regs<double>(a, y, bro, lane, w);
         store_rowseg(pool + (size_t)outui * TE, y, lane, w);
     }
 }
@@ -513,15 +521,17 @@ __device__ __attribute__((always_inline)) double lu_task(T* __restrict__ pool, i
 template <typename T, bool LOWER>
 __device__ __attribute__((always_inline)) void tri_inv_task(T* __restrict__ pool, int src, int out, char* lds_raw) {
     typedef typename std::conditional<sizeof(T) == 8, double, float>::type C;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     C* ts = reinterpret_cast<C*>(lds_raw);
+    C* bro = ts + TS * LDA + 16 * w;
     C v[16];
     load_rowseg(pool + (size_t)src * TE, v, lane, w);
 #pragma unroll
     for (int c = 0; c < 16; c++) ts[lane * LDA + 16 * w + c] = v[c];
     __syncthreads();
-    if (LOWER) inv_lower_regs<C, false>(ts, v, lane, w);
-    else inv_upper_regs<C>(ts, v, lane, w);
+    if (LOWER) inv_lower_regs<C, false>(ts, v, bro, lane, w);
+    else inv_upper_regs<C>(ts, v, bro, lane, w);
     store_rowseg(pool + (size_t)out * TE, v, lane, w);
 }
 
