@@ -186,7 +186,7 @@ def main():
 
     head = out_paths["fp32_ir"]
     cpu = None
-    if not args.no_cpu and args.config in ("c1", "c2"):
+    if not args.no_cpu:
         threads = min(16, physical_cores())
         ref = cpu_reference(prob, threads)
         if ref is not None:

@@ -96,7 +96,6 @@ __global__ void k_fill(T* p, size_t n, T v) {
     for (; i < n; i += stride) p[i] = v;
 }
 
-constexpr int LDS = lds_bytes<double>();
 constexpr long long SPIN_LIMIT_TICKS = 20LL * 100 * 1000 * 1000;  // ~20 s of the 100 MHz wall clock
 
 struct Args {
@@ -182,7 +181,7 @@ __device__ __attribute__((always_inline)) void run_task(const DevTask& t, const 
 // ready); other tasks wait for all their sources first.
 template <typename T, bool BF16>
 __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
-    __shared__ __attribute__((aligned(16))) char lds[LDS];
+    __shared__ __attribute__((aligned(16))) char lds[lds_bytes<T>()];
     __shared__ int s_task;
     __shared__ int s_abort;
     const int tid = threadIdx.x;
@@ -237,7 +236,7 @@ __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
 // Levels / per-op executors: one workgroup per task of the launch.
 template <typename T, bool BF16, bool ACC>
 __global__ __launch_bounds__(NT) void k_factor_batch(Args a) {
-    __shared__ __attribute__((aligned(16))) char lds[LDS];
+    __shared__ __attribute__((aligned(16))) char lds[lds_bytes<T>()];
     const DevTask t = a.tasks[a.task_offset + blockIdx.x];
     run_task<T, BF16, ACC>(t, a, lds);
 }

@@ -33,12 +33,18 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
-// LDS bytes one workgroup needs for any task kind, per storage type
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+// LDS elements of the blocked LU (defined with it below)
+constexpr int lu_lds_elems();
+
+// LDS bytes one workgroup needs for any task kind, per storage type: the GEMM's A and
+// B tiles, the blocked LU in the compute type and, for FP32 tiles, the compact FP64
+// refactorisation (tile + 2 x 64 doubles)
 template <typename T>
 constexpr int lds_bytes() {
-    // gemm: A (64 x LDA) + B (64 x LDB) in the MFMA operand type; the lu / inverses use
-    // 2 x 64 x LDA doubles at most (the FP64 fallback of a low-precision factor)
-    return (TS * LDA + TS * LDB) * 8 + 2 * TS * 8 + 8 * 8 + 64 * 8;
+    return cmax(cmax((TS * LDA + TS * LDB) * (int)sizeof(T), lu_lds_elems() * (int)sizeof(T)),
+                sizeof(T) == 8 ? 0 : (TS * LDA + 2 * TS) * 8);
 }
 
 // ------------------------------------------------------------------ loads
@@ -285,41 +291,6 @@ __device__ inline void bcast_row(const C* v, C* bro, int lane, int k, C* out) {
     __builtin_amdgcn_wave_barrier();
 }
 
-// Pivot-free right-looking LU of the tile held in v (ludcmpSimple,
-// MatrixStdDouble.cpp:2711-2784): afterwards v holds U on and above the diagonal
-// and the multipliers of L below it.  |u_kk| < 1e-9 becomes -1e-9 if u_kk < 0,
-// else +1e-9 (MatrixStdDouble.cpp:2751-2761).  One barrier per step (lcol, a
-// double buffer of 2 x 64 multipliers in LDS); the pivot row reaches the wave's
-// lanes through its bro buffer.  The 16 steps of a column panel are unrolled so
-// the pivot column's register index is a constant; the 4 panels loop.
-template <typename C>
-__device__ inline void lu_regs(C* v, C* lcol, C* bro, int lane, int w) {
-    const C TINY2 = C(1e-9);
-#pragma unroll 1
-    for (int kb = 0; kb < 4; kb++) {
-#pragma unroll
-        for (int kc = 0; kc < 16; kc++) {
-            const int k = kb * 16 + kc;
-            C* lc = lcol + (kc & 1) * TS;
-            C u[16];
-            bcast_row(v, bro, lane, k, u);
-            if (w == kb) {  // the wave owning column k makes the multipliers
-                C p = u[kc];
-                if (p < TINY2 && p > -TINY2) p = (p < 0) ? -TINY2 : TINY2;
-                const C l = v[kc] * (C(1) / p);
-                v[kc] = lane > k ? l : (lane == k ? p : v[kc]);
-                lc[lane] = l;  // read only by the lanes below the pivot
-            }
-            __syncthreads();
-            const C l = lc[lane];
-            const bool below = lane > k;
-#pragma unroll
-            for (int c = 0; c < 16; c++)  // branch-free: columns <= k keep their value
-                v[c] = (below && 16 * w + c > k) ? v[c] - l * u[c] : v[c];
-        }
-    }
-}
-
 // y = inverse of the lower triangle in ls (LDS, row stride LDA), rows of y in
 // lanes, the wave's 16 columns in registers; right-looking over rows, no barrier
 // (inv_lower, MatrixStdDouble.cpp:2787-2802: y_ii = 1/l_ii,
@@ -382,57 +353,306 @@ __device__ inline C block_absmax(C x, C* red, int tid) {
     return m;
 }
 
+// ------------------------------------------------------------------ blocked tile LU
+// The lu task in 16x16 blocks: for K = 0..3 one wave factors the diagonal block
+// (A_KK = L_KK U_KK, plus L_KK^-1 and U_KK^-1), the panel becomes U_KJ = L_KK^-1 A_KJ
+// and L_IK = A_IK U_KK^-1, and the trailing blocks take A_IJ -= L_IK U_KJ, the last two
+// on MFMA.  L^-1 and U^-1 of the whole tile are then assembled from the blocks by
+// recursive doubling (two MFMA rounds).  This is the reference's own recursive 2x2
+// block LU with explicit inverses (BlockPlanner.cpp:865-939, 1019-1092) applied one
+// level further down, inside the tile; pivots are the same Schur-complement diagonal
+// entries as in the unblocked ludcmpSimple (MatrixStdDouble.cpp:2711-2784), clamped
+// the same way.  The dependent chain is 4 x 16 short in-wave steps plus 10 barriers,
+// instead of 64 barrier-separated steps and two 64-step inverse sweeps.
+constexpr int BS = 17;        // LDS row stride of a 16x16 block
+constexpr int BE = 16 * BS;   // LDS elements of a 16x16 block
+
+// LDS elements (in the compute type) of lu_inv_blocked: the tile, 8 diagonal-block
+// inverses, 4 first-level inverse blocks, 2 temporaries per wave, 8 reduction slots
+constexpr int lu_lds_elems() { return TS * LDA + 8 * BE + 4 * BE + 8 * BE + 8; }
+
+// Lane k of each 16-lane row to every lane of that row (DPP row_newbcast).  A DPP
+// read of a lane that EXEC disables returns 0, so the broadcast must run where every
+// lane is active: the empty asm pins it there (otherwise the compiler may sink it into
+// a branch of a later select, e.g. one that masks out lane k itself).
+template <int K, typename C>
+__device__ __forceinline__ C rbcast(C x) {
+    C r = __builtin_amdgcn_update_dpp(C(0), x, 0x150 + K, 0xF, 0xF, false);
+    asm volatile("" : "+v"(r));
+    return r;
+}
+
+template <int K, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (K < N) {
+        f(std::integral_constant<int, K>());
+        static_for<K + 1, N>(f);
+    }
+}
+
+// acc += sg * A (16 x 4*KS) B (4*KS x 16), A and B row-major in LDS
+template <typename C, int KS>
+__device__ __forceinline__ void mm16(typename MfmaOp<C>::acc_t& acc, const C* pa, int lda, const C* pb,
+                                     int ldb, int li, int lk, C sg = C(1)) {
+#pragma unroll
+    for (int s = 0; s < KS; s++)
+        acc = MfmaOp<C>::mma(sg * pa[li * lda + 4 * s + lk], pb[(4 * s + lk) * ldb + li], acc);
+}
+
+template <typename C>
+__device__ __forceinline__ void st16(const typename MfmaOp<C>::acc_t& acc, C* pd, int ld, int li, int lk,
+                                     C sg = C(1)) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) pd[MfmaOp<C>::row(lk, r) * ld + li] = sg * acc[r];
+}
+
+template <typename T, typename C>
+__device__ __forceinline__ void st16_global(const typename MfmaOp<C>::acc_t& acc, T* tile, int I, int J,
+                                            int li, int lk, C sg = C(1)) {
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+        tile[(16 * I + MfmaOp<C>::row(lk, r)) * TS + 16 * J + li] = T(sg * acc[r]);
+}
+
+// 16x16 block of LDS (stride ld) to global block (I, J) of a tile; 4 elements per lane
+template <typename T, typename C>
+__device__ __forceinline__ void blk_to_global(const C* pb, int ld, T* tile, int I, int J, int lane) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int r = 4 * q + (lane >> 4), c = lane & 15;
+        tile[(16 * I + r) * TS + 16 * J + c] = T(pb[r * ld + c]);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void zero_blk_global(T* tile, int I, int J, int lane) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) tile[(16 * I + 4 * q + (lane >> 4)) * TS + 16 * J + (lane & 15)] = T(0);
+}
+
+// One wave factors the 16x16 diagonal block at (r0, r0) of the LDS tile a in place
+// (L strict lower with unit diagonal, U on and above the diagonal, |u_kk| < 1e-9
+// clamped to +-1e-9 as in MatrixStdDouble.cpp:2751-2761) and writes L^-1 to xd and
+// U^-1 to zd.  Lanes i, i+16, i+32, i+48 all hold row i, so a row_newbcast of lane k
+// hands row k to every lane.  The inverses run side by side: lanes 0-15 invert L by
+// rows, lanes 16-31 invert U^T by rows (whose rows are U^-1's columns), with the same
+// instructions (forward substitution with the diagonal scaling deferred to the end).
+template <typename C>
+__device__ inline void diag16(C* a, int r0, C* xd, C* zd, int lane) {
+    const int i = lane & 15, g = lane >> 4;
+    C v[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) v[c] = a[(r0 + i) * LDA + r0 + c];
+    const C TINY2 = C(1e-9);
+    static_for<0, 16>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        C u[16];
+#pragma unroll
+        for (int c = k; c < 16; c++) u[c] = rbcast<k>(v[c]);
+        C p = u[k];
+        if (p < TINY2 && p > -TINY2) p = (p < 0) ? -TINY2 : TINY2;
+        const C l = i > k ? v[k] * (C(1) / p) : C(0);
+        v[k] = i > k ? l : (i == k ? p : v[k]);
+#pragma unroll
+        for (int c = k + 1; c < 16; c++) v[c] -= l * u[c];  // l = 0 leaves rows <= k as they are
+    });
+    if (g == 0) {
+#pragma unroll
+        for (int c = 0; c < 16; c++) a[(r0 + i) * LDA + r0 + c] = v[c];
+    }
+    __builtin_amdgcn_wave_barrier();
+    // row i of M: L (unit lower) in lanes 0-15, U^T in the others
+    C m[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+        const C ut = c <= i ? a[(r0 + c) * LDA + r0 + i] : C(0);
+        m[c] = g == 0 ? (c < i ? v[c] : (c == i ? C(1) : C(0))) : ut;
+    }
+    const C di = C(1) / (g == 0 ? C(1) : a[(r0 + i) * LDA + r0 + i]);
+    C y[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) y[c] = c == i ? C(1) : C(0);
+    static_for<0, 16>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const C qk = rbcast<k>(di);  // outside the conditional: every lane must run it
+        const C f = i > k ? m[k] * qk : C(0);
+#pragma unroll
+        for (int c = 0; c <= k; c++) y[c] -= f * rbcast<k>(y[c]);
+    });
+    if (g == 0) {
+#pragma unroll
+        for (int c = 0; c < 16; c++) xd[i * BS + c] = y[c] * di;
+    } else if (g == 1) {
+#pragma unroll
+        for (int c = 0; c < 16; c++) zd[c * BS + i] = y[c] * di;
+    }
+}
+
 // LU of one tile in compute type C plus, when their slots are given (>= 0), the
 // inverses of its unit-lower L and upper U: the reference's lowerInv / upperInv
 // of the same diagonal tile, fused into the producing task.
 template <typename T, typename C>
-__device__ __attribute__((always_inline)) void lu_inv_core(T* __restrict__ pool, int src, int outl, int outu, int outli, int outui,
-                            char* lds_raw, float* amax_out, float* umax_out) {
+__device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ pool, int src, int outl, int outu,
+                                                              int outli, int outui, char* lds_raw,
+                                                              float* amax_out, float* umax_out) {
+    typedef typename MfmaOp<C>::acc_t acc_t;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
-    C* ls = reinterpret_cast<C*>(lds_raw);       // L (strict lower) then U, row stride LDA
-    C* us = ls + TS * LDA;
-    C* lcol = us + TS * LDA;                      // 2 x 64 multipliers
-    C* red = lcol + 2 * TS;                       // 8 reduction slots
-    C* bro = red + 8 + 16 * w;                    // 16 per wave: row broadcasts
-    C v[16];
-    load_rowseg(pool + (size_t)src * TE, v, lane, w);
-    if (amax_out) {
+    const int li = lane & 15, lk = lane >> 4;
+    C* a = reinterpret_cast<C*>(lds_raw);  // the tile, row stride LDA
+    C* xd = a + TS * LDA;                  // L_II^-1, I = 0..3
+    C* zd = xd + 4 * BE;                   // U_II^-1
+    C* x1 = zd + 4 * BE;                   // X10, X32, Z01, Z23 (first-level inverse blocks)
+    C* tw = x1 + 4 * BE + 2 * BE * w;      // 2 temporaries of this wave
+    C* red = x1 + 4 * BE + 8 * BE;         // 8 reduction slots
+    {
+        const T* A = pool + (size_t)src * TE;
+        const int r = tid >> 2, c0 = (tid & 3) * 16;
         C m = 0;
 #pragma unroll
-        for (int c = 0; c < 16; c++) m = fabs(v[c]) > m ? fabs(v[c]) : m;
-        *amax_out = (float)block_absmax(m, red, tid);
+        for (int q = 0; q < 16; q++) {
+            const C x = C(A[r * TS + c0 + q]);
+            a[r * LDA + c0 + q] = x;
+            m = fabs(x) > m ? fabs(x) : m;
+        }
+        if (amax_out) *amax_out = (float)block_absmax(m, red, tid);
+        else __syncthreads();
     }
-    lu_regs(v, lcol, bro, lane, w);
-    // L (unit lower) and U out of the same registers, to HBM and, for the inverses, to LDS
-    const bool inv = outli >= 0 || outui >= 0;
-    T* Lg = pool + (size_t)outl * TE + lane * TS + 16 * w;
-    T* Ug = pool + (size_t)outu * TE + lane * TS + 16 * w;
-    C um = 0;
+#pragma unroll 1
+    for (int K = 0; K < 4; K++) {
+        const int r0 = 16 * K;
+        if (w == 0) diag16(a, r0, xd + K * BE, zd + K * BE, lane);
+        __syncthreads();
+        if (K == 3) break;
+        const int nb = 3 - K;  // blocks right of / below the diagonal block
+        for (int p = w; p < 2 * nb; p += 4) {
+            acc_t acc = acc_t{0, 0, 0, 0};
+            if (p < nb) {  // U_KJ = L_KK^-1 A_KJ
+                C* d = a + r0 * LDA + 16 * (K + 1 + p);
+                mm16<C, 4>(acc, xd + K * BE, BS, d, LDA, li, lk);
+                st16(acc, d, LDA, li, lk);
+            } else {  // L_IK = A_IK U_KK^-1
+                C* d = a + 16 * (K + 1 + p - nb) * LDA + r0;
+                mm16<C, 4>(acc, d, LDA, zd + K * BE, BS, li, lk);
+                st16(acc, d, LDA, li, lk);
+            }
+        }
+        __syncthreads();
+        for (int q = w; q < nb * nb; q += 4) {  // A_IJ -= L_IK U_KJ
+            const int I = K + 1 + q / nb, J = K + 1 + q % nb;
+            C* d = a + 16 * I * LDA + 16 * J;
+            acc_t acc;
 #pragma unroll
-    for (int c = 0; c < 16; c++) {
-        const int j = 16 * w + c;
-        const C l = lane > j ? v[c] : (lane == j ? C(1) : C(0));
-        const C u = j >= lane ? v[c] : C(0);
-        Lg[c] = T(l);
-        Ug[c] = T(u);
-        um = fabs(u) > um ? fabs(u) : um;
-        if (inv) {
-            ls[lane * LDA + j] = l;
-            us[lane * LDA + j] = u;
+            for (int r = 0; r < 4; r++) acc[r] = d[MfmaOp<C>::row(lk, r) * LDA + li];
+            mm16<C, 4>(acc, a + 16 * I * LDA + r0, LDA, a + r0 * LDA + 16 * J, LDA, li, lk, C(-1));
+            st16(acc, d, LDA, li, lk);
+        }
+        __syncthreads();
+    }
+    // L (unit lower) and U to HBM
+    {
+        const int r = tid >> 2, c0 = (tid & 3) * 16;
+        T* Lg = pool + (size_t)outl * TE + r * TS + c0;
+        T* Ug = pool + (size_t)outu * TE + r * TS + c0;
+        C um = 0;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int j = c0 + q;
+            const C x = a[r * LDA + j];
+            Lg[q] = T(r > j ? x : (r == j ? C(1) : C(0)));
+            const C u = j >= r ? x : C(0);
+            Ug[q] = T(u);
+            um = fabs(u) > um ? fabs(u) : um;
+        }
+        if (umax_out) *umax_out = (float)block_absmax(um, red + 4, tid);
+    }
+    if (outli < 0 && outui < 0) return;
+    // X = L^-1 (lower) and Z = U^-1 (upper) from the diagonal-block inverses.
+    // First level, one chain per wave: X10 = -X11 L10 X00, X32 = -X33 L32 X22,
+    // Z01 = -Z00 U01 Z11, Z23 = -Z22 U23 Z33.
+    T* Xg = outli >= 0 ? pool + (size_t)outli * TE : nullptr;
+    T* Zg = outui >= 0 ? pool + (size_t)outui * TE : nullptr;
+    {
+        const bool lo = w < 2;  // waves 0, 1: X; waves 2, 3: Z
+        const int b = 2 * (w & 1);  // first block index of the 2x2 diagonal pair
+        acc_t acc = acc_t{0, 0, 0, 0};
+        if (lo) mm16<C, 4>(acc, a + 16 * (b + 1) * LDA + 16 * b, LDA, xd + b * BE, BS, li, lk);  // L_{b+1,b} X_bb
+        else mm16<C, 4>(acc, a + 16 * b * LDA + 16 * (b + 1), LDA, zd + (b + 1) * BE, BS, li, lk);  // U_{b,b+1} Z_{b+1,b+1}
+        st16(acc, tw, BS, li, lk);
+        __builtin_amdgcn_wave_barrier();
+        acc = acc_t{0, 0, 0, 0};
+        if (lo) mm16<C, 4>(acc, xd + (b + 1) * BE, BS, tw, BS, li, lk);
+        else mm16<C, 4>(acc, zd + b * BE, BS, tw, BS, li, lk);
+        st16(acc, x1 + w * BE, BS, li, lk, C(-1));
+        if (lo && Xg) st16_global(acc, Xg, b + 1, b, li, lk, C(-1));
+        if (!lo && Zg) st16_global(acc, Zg, b, b + 1, li, lk, C(-1));
+        // diagonal blocks and the zero half of each inverse
+        if (Xg) {
+            blk_to_global(xd + w * BE, BS, Xg, w, w, lane);
+            for (int J = w + 1; J < 4; J++) zero_blk_global(Xg, w, J, lane);
+        }
+        if (Zg) {
+            blk_to_global(zd + w * BE, BS, Zg, w, w, lane);
+            for (int I = w + 1; I < 4; I++) zero_blk_global(Zg, I, w, lane);
         }
     }
-    if (umax_out) *umax_out = (float)block_absmax(um, red + 4, tid);
-    if (!inv) return;
     __syncthreads();
-    C y[16];
-    if (outli >= 0) {
-        inv_lower_regs<C, true>(ls, y, bro, lane, w);
-        store_rowseg(pool + (size_t)outli * TE, y, lane, w);
-    }
-    if (outui >= 0) {
-        inv_upper_regs<C>(us, y, bro, lane, w);
-        store_rowseg(pool + (size_t)outui * TE, y, lane, w);
+    // Second level.  X: with T = [L20 L21; L30 L31] [X00 0; X10 X11],
+    // [X20 X21; X30 X31] = -[X22 0; X32 X33] T; wave 0 takes T's first block column,
+    // wave 1 the second.  Z: with T = [U02 U03; U12 U13] [Z22 Z23; 0 Z33],
+    // [Z02 Z03; Z12 Z13] = -[Z00 Z01; 0 Z11] T; waves 2 and 3.
+    {
+        const int col = w & 1;
+        C* t0 = tw;       // T block (0, col)
+        C* t1 = tw + BE;  // T block (1, col)
+        const C* X10 = x1;
+        const C* X32 = x1 + BE;
+        const C* Z01 = x1 + 2 * BE;
+        const C* Z23 = x1 + 3 * BE;
+        acc_t c0 = acc_t{0, 0, 0, 0}, c1 = acc_t{0, 0, 0, 0};
+        if (w < 2) {
+            if (col == 0) {  // T00 = L20 X00 + L21 X10, T10 = L30 X00 + L31 X10
+                mm16<C, 4>(c0, a + 32 * LDA, LDA, xd, BS, li, lk);
+                mm16<C, 4>(c0, a + 32 * LDA + 16, LDA, X10, BS, li, lk);
+                mm16<C, 4>(c1, a + 48 * LDA, LDA, xd, BS, li, lk);
+                mm16<C, 4>(c1, a + 48 * LDA + 16, LDA, X10, BS, li, lk);
+            } else {  // T01 = L21 X11, T11 = L31 X11
+                mm16<C, 4>(c0, a + 32 * LDA + 16, LDA, xd + BE, BS, li, lk);
+                mm16<C, 4>(c1, a + 48 * LDA + 16, LDA, xd + BE, BS, li, lk);
+            }
+        } else {
+            if (col == 0) {  // T00 = U02 Z22, T10 = U12 Z22
+                mm16<C, 4>(c0, a + 32, LDA, zd + 2 * BE, BS, li, lk);
+                mm16<C, 4>(c1, a + 16 * LDA + 32, LDA, zd + 2 * BE, BS, li, lk);
+            } else {  // T01 = U02 Z23 + U03 Z33, T11 = U12 Z23 + U13 Z33
+                mm16<C, 4>(c0, a + 32, LDA, Z23, BS, li, lk);
+                mm16<C, 4>(c0, a + 48, LDA, zd + 3 * BE, BS, li, lk);
+                mm16<C, 4>(c1, a + 16 * LDA + 32, LDA, Z23, BS, li, lk);
+                mm16<C, 4>(c1, a + 16 * LDA + 48, LDA, zd + 3 * BE, BS, li, lk);
+            }
+        }
+        st16(c0, t0, BS, li, lk);
+        st16(c1, t1, BS, li, lk);
+        __builtin_amdgcn_wave_barrier();
+        c0 = acc_t{0, 0, 0, 0};
+        c1 = acc_t{0, 0, 0, 0};
+        if (w < 2) {  // X2c = -X22 T0c, X3c = -(X32 T0c + X33 T1c)
+            mm16<C, 4>(c0, xd + 2 * BE, BS, t0, BS, li, lk);
+            mm16<C, 4>(c1, X32, BS, t0, BS, li, lk);
+            mm16<C, 4>(c1, xd + 3 * BE, BS, t1, BS, li, lk);
+            if (Xg) {
+                st16_global(c0, Xg, 2, col, li, lk, C(-1));
+                st16_global(c1, Xg, 3, col, li, lk, C(-1));
+            }
+        } else {  // Z0c' = -(Z00 T0c + Z01 T1c), Z1c' = -Z11 T1c  (c' = 2 + col)
+            mm16<C, 4>(c0, zd, BS, t0, BS, li, lk);
+            mm16<C, 4>(c0, Z01, BS, t1, BS, li, lk);
+            mm16<C, 4>(c1, zd + BE, BS, t1, BS, li, lk);
+            if (Zg) {
+                st16_global(c0, Zg, 0, 2 + col, li, lk, C(-1));
+                st16_global(c1, Zg, 1, 2 + col, li, lk, C(-1));
+            }
+        }
     }
 }
 
@@ -502,11 +722,11 @@ template <typename T>
 __device__ __attribute__((always_inline)) double lu_task(T* __restrict__ pool, int src, int outl, int outu, int outli, int outui,
                           char* lds_raw, double growth_limit, int* n_fp64) {
     if constexpr (sizeof(T) == 8) {
-        lu_inv_core<T, double>(pool, src, outl, outu, outli, outui, lds_raw, nullptr, nullptr);
+        lu_inv_blocked<T, double>(pool, src, outl, outu, outli, outui, lds_raw, nullptr, nullptr);
         return 1.0;
     } else {
         float amax = 0, umax = 0;
-        lu_inv_core<T, float>(pool, src, outl, outu, outli, outui, lds_raw, &amax, &umax);
+        lu_inv_blocked<T, float>(pool, src, outl, outu, outli, outui, lds_raw, &amax, &umax);
         const double growth = amax > 0 ? (double)umax / (double)amax : 1.0;
         if (!(growth <= growth_limit)) {
             __syncthreads();
