@@ -192,6 +192,12 @@ int sogl_factor_get_tiles(sogl_ctx* ctx, int32_t factor_id, int32_t which, doubl
  * hardware id of the compute unit.  Arrays hold n_tasks * 4 entries. */
 int sogl_factor_set_trace(sogl_ctx* ctx, int32_t factor_id, int32_t enable);
 int sogl_factor_get_trace(sogl_ctx* ctx, int32_t factor_id, int64_t* tasks4, int64_t* times4);
+/* Diagnostics: the executor's task graph.  n_deps / n_slots receive the sizes; with
+ * dep_begin (n_tasks + 1 entries) and deps (n_deps entries) given, task t waits for the
+ * pool slots deps[dep_begin[t] .. dep_begin[t+1]); slot_task (n_slots entries) receives
+ * the task that writes each slot (-1 for inputs).  Pass NULL arrays to query sizes. */
+int sogl_factor_get_graph(sogl_ctx* ctx, int32_t factor_id, int64_t* n_deps, int64_t* n_slots,
+                          int32_t* dep_begin, int32_t* deps, int32_t* slot_task);
 
 /* ------------------------------------------------ mixed precision + refinement */
 /* Attach the original (unpermuted) matrix for the FP64 residual r = b - A x. */

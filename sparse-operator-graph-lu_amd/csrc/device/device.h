@@ -47,6 +47,7 @@ struct Factor {
     std::vector<DevTask> op_tasks;         // per-op executor: one entry per unit op, list order
     std::vector<int32_t> input_slots;
     int64_t n_pairs = 0, n_deps = 0;
+    int64_t n_inv_aliased = 0;  // repeated inverses of one factor tile that share its slot
     // device arrays
     DevBuf pool, tasks_d, pairs_d, deps_d, ready_d, ctrl_d, op_tasks_d, op_pairs_d, inputs_d;
     int32_t epoch = 0;

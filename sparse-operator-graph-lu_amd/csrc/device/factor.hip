@@ -131,15 +131,23 @@ __device__ inline void spin_ready(const Args& a, int s) {
 }
 
 // Per-pair wait of a GEMM task inside the persistent executor
+// Device-scope ordering for a whole workgroup, done by one thread.  A workgroup
+// barrier orders the other waves with that thread (it is a workgroup-scope release
+// and acquire), and the device-scope fences are cumulative, so one L2 write-back
+// (release, after the barrier) or one L1/L2 invalidate (acquire, before the barrier)
+// per task serves all four waves; per-wave fences would repeat them four times.
+__device__ __forceinline__ void acquire_for_workgroup() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+__device__ __forceinline__ void release_for_workgroup() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
+
 struct PairWait {
     const Args* a;
     __device__ inline void operator()(int sa, int sb) const {
         if (threadIdx.x == 0) {
             spin_ready(*a, sa);
             spin_ready(*a, sb);
+            acquire_for_workgroup();
         }
         __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
 };
 
@@ -198,27 +206,29 @@ __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
         long long t_taken = 0, t_ready = 0;
         if (a.trace && tid == 0) t_taken = wall_clock64();
         if (t.kind != TK_GEMM) {
-            if (tid == 0)
+            if (tid == 0) {
                 for (int d = 0; d < t.dep_count; d++) spin_ready(a, a.deps[t.dep_begin + d]);
+                acquire_for_workgroup();
+            }
             __syncthreads();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         if (a.trace && tid == 0) t_ready = wall_clock64();
         if (tid == 0) s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         if (__builtin_amdgcn_readfirstlane(s_abort)) break;
         run_task<T, BF16, false>(t, a, lds, PairWait{&a});
-        // publish: every wave's stores reach device scope before the flag
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        // publish: every wave's stores complete (the barrier's workgroup release), then
+        // thread 0 makes them visible at device scope once and raises the flags
         __syncthreads();
         if (tid == 0) {
-            __hip_atomic_store(a.ready + t.out0, a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            release_for_workgroup();
+            __hip_atomic_store(a.ready + t.out0, a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (t.out1 >= 0)
-                __hip_atomic_store(a.ready + t.out1, a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(a.ready + t.out1, a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (t.out2 >= 0)
-                __hip_atomic_store(a.ready + t.out2, a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(a.ready + t.out2, a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (t.out3 >= 0)
-                __hip_atomic_store(a.ready + t.out3, a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(a.ready + t.out3, a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (a.trace) {
                 long long* tr = a.trace + (size_t)ti * 4;
                 tr[0] = t_taken;
@@ -355,21 +365,26 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
         ot.out2 = ot.out3 = -1;
         f->op_tasks.push_back(ot);
         // an inverse of an lu output is made by that lu's task (the lu holds the tile
-        // in registers already); the per-op executor still runs it as its own op
+        // already); the per-op executor still runs it as its own op.  The recursive
+        // planner inverts some diagonal factor tiles more than once (C2: 3,679
+        // lowerInv ops over 2,048 lu ops); a repeat has the same value, so its result
+        // id aliases the first inverse's slot and no task is made for it.
         if (kind == TK_LINV || kind == TK_UINV) {
             const int32_t pt = s1 >= 0 ? prod[(size_t)s1] : -1;
-            if (pt >= 0 && tasks[(size_t)pt].kind == TK_LU) {
+            const bool lower = kind == TK_LINV;
+            if (pt >= 0 && tasks[(size_t)pt].kind == TK_LU && prod[(size_t)rs] < 0 &&
+                (lower ? tasks[(size_t)pt].out0 : tasks[(size_t)pt].out1) == s1) {
                 DevTask& lt = tasks[(size_t)pt];
-                if (kind == TK_LINV && lt.out0 == s1 && lt.out2 < 0 && prod[(size_t)rs] < 0) {
-                    lt.out2 = rs;
+                int32_t& inv = lower ? lt.out2 : lt.out3;
+                if (inv < 0) {
+                    inv = rs;
                     prod[(size_t)rs] = pt;
-                    continue;
+                } else {
+                    slot[(size_t)o.result] = inv;
+                    f->op_tasks.back().out0 = inv;
+                    f->n_inv_aliased++;
                 }
-                if (kind == TK_UINV && lt.out1 == s1 && lt.out3 < 0 && prod[(size_t)rs] < 0) {
-                    lt.out3 = rs;
-                    prod[(size_t)rs] = pt;
-                    continue;
-                }
+                continue;
             }
         }
         int32_t t = prod[(size_t)rs];
@@ -886,6 +901,39 @@ extern "C" int sogl_factor_set_trace(sogl_ctx* x, int32_t factor_id, int32_t ena
         if (rc) return rc;
     }
     f->trace_on = enable != 0;
+    return SOGL_OK;
+}
+
+extern "C" int sogl_factor_get_graph(sogl_ctx* x, int32_t factor_id, int64_t* n_deps, int64_t* n_slots,
+                                     int32_t* dep_begin, int32_t* deps, int32_t* slot_task) {
+    if (!x) return set_error(nullptr, SOGL_ERR_ARG, "null context");
+    Context* c = &x->c;
+    std::lock_guard<std::mutex> g(c->mu);
+    Factor* f = find_factor(c, factor_id);
+    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+    int64_t nd = 0;
+    for (const DevTask& k : f->tasks) nd += k.dep_count;
+    if (n_deps) *n_deps = nd;
+    if (n_slots) *n_slots = f->n_slots;
+    if (dep_begin || deps) {
+        if (!dep_begin || !deps) return set_error(c, SOGL_ERR_ARG, "dep_begin and deps go together");
+        std::vector<int32_t> hd((size_t)std::max<int64_t>(nd, 1));
+        if (nd) {
+            HIP_TRY(c, hipSetDevice(c->device));
+            HIP_TRY(c, hipMemcpy(hd.data(), f->deps_d.p, (size_t)nd * 4, hipMemcpyDeviceToHost));
+        }
+        for (size_t t = 0; t < f->tasks.size(); t++) dep_begin[t] = f->tasks[t].dep_begin;
+        dep_begin[f->tasks.size()] = (int32_t)nd;
+        std::copy(hd.begin(), hd.begin() + nd, deps);
+    }
+    if (slot_task) {
+        std::fill(slot_task, slot_task + f->n_slots, -1);
+        for (size_t t = 0; t < f->tasks.size(); t++) {
+            const DevTask& k = f->tasks[t];
+            for (int32_t o : {k.out0, k.out1, k.out2, k.out3})
+                if (o >= 0 && o < f->n_slots) slot_task[o] = (int32_t)t;
+        }
+    }
     return SOGL_OK;
 }
 

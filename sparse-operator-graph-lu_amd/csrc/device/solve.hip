@@ -100,9 +100,13 @@ __global__ __launch_bounds__(NT) void k_trsv_persistent(SolveArgs a) {
                 double seg[16];
                 load_seg(pool + (size_t)a.slot[e] * TE, r, kp, seg);
                 if (c0 == 0) {
-                    if (tid == 0) spin_row(a, J);
+                    // one thread waits and acquires for the workgroup; the barrier
+                    // orders the other waves after it
+                    if (tid == 0) {
+                        spin_row(a, J);
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    }
                     __syncthreads();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 }
                 const double* xj = a.out + (int64_t)J * TS + kp * 16;
 #pragma unroll
@@ -144,9 +148,11 @@ __global__ __launch_bounds__(NT) void k_trsv_persistent(SolveArgs a) {
             }
             __syncthreads();  // vec is reused by the next chunk
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __syncthreads();
-        if (tid == 0) __hip_atomic_store(a.ready + I, a.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();  // every wave's stores are complete (workgroup release)
+        if (tid == 0) {   // one device-scope release for the workgroup, then the flag
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_store(a.ready + I, a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         __syncthreads();  // separates the publish branch from the next fetch branch
     }
 }

@@ -372,15 +372,24 @@ constexpr int BE = 16 * BS;   // LDS elements of a 16x16 block
 constexpr int lu_lds_elems() { return TS * LDA + 8 * BE + 4 * BE + 8 * BE + 8; }
 
 // Lane k of each 16-lane row to every lane of that row (DPP row_newbcast).  A DPP
-// read of a lane that EXEC disables returns 0, so the broadcast must run where every
-// lane is active: the empty asm pins it there (otherwise the compiler may sink it into
-// a branch of a later select, e.g. one that masks out lane k itself).
+// read of a lane that EXEC disables returns 0, so callers evaluate it where every
+// lane of the wave is active, never inside one arm of a conditional expression.
 template <int K, typename C>
 __device__ __forceinline__ C rbcast(C x) {
-    C r = __builtin_amdgcn_update_dpp(C(0), x, 0x150 + K, 0xF, 0xF, false);
-    asm volatile("" : "+v"(r));
-    return r;
+    return __builtin_amdgcn_update_dpp(C(0), x, 0x150 + K, 0xF, 0xF, true);
 }
+
+// 1/p for a clamped pivot (1e-9 <= |p|, finite): v_rcp plus two Newton steps for
+// FP64 (each step squares the relative error of the hardware estimate), v_rcp_f32
+// (1 ulp) for FP32.  Shorter than the IEEE division, which the pivot chain waits on.
+__device__ __forceinline__ double recip(double p) {
+    double r = __builtin_amdgcn_rcp(p);
+    double e = fma(-p, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-p, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ float recip(float p) { return __builtin_amdgcn_rcpf(p); }
 
 template <int K, int N, typename F>
 __device__ __forceinline__ void static_for(F&& f) {
@@ -433,58 +442,56 @@ __device__ __forceinline__ void zero_blk_global(T* tile, int I, int J, int lane)
 // One wave factors the 16x16 diagonal block at (r0, r0) of the LDS tile a in place
 // (L strict lower with unit diagonal, U on and above the diagonal, |u_kk| < 1e-9
 // clamped to +-1e-9 as in MatrixStdDouble.cpp:2751-2761) and writes L^-1 to xd and
-// U^-1 to zd.  Lanes i, i+16, i+32, i+48 all hold row i, so a row_newbcast of lane k
-// hands row k to every lane.  The inverses run side by side: lanes 0-15 invert L by
-// rows, lanes 16-31 invert U^T by rows (whose rows are U^-1's columns), with the same
-// instructions (forward substitution with the diagonal scaling deferred to the end).
+// U^-1 to zd.  Lanes 0-15 hold the block's rows, lanes 16-31 its columns (lanes
+// 32-63 repeat the rows); a row_newbcast of lane k hands every lane of a 16-lane
+// group row k (or column k) of the block.  In both layouts step k updates
+// x[c] -= (own_k / p) * bc[c] for lanes past k, with own_k the lane's own entry of
+// column (row) k: the row layout keeps own_k / p as the multiplier l_ik, the column
+// layout keeps own_k = u_kj.  The same factor f = own_k / p is the step-k multiplier
+// of the forward substitution for L^-1 (rows, lanes 0-15: l_ik) and for (U^T)^-1
+// (rows, lanes 16-31: u_ki / u_kk, the 1/u_ii scaling applied at the end), so both
+// inverses advance in the same 16 steps as the factorisation.
 template <typename C>
 __device__ inline void diag16(C* a, int r0, C* xd, C* zd, int lane) {
     const int i = lane & 15, g = lane >> 4;
-    C v[16];
+    const bool colw = g == 1;
+    const int rs = colw ? 1 : LDA, cs = colw ? LDA : 1;
+    C x[16], y[16];
 #pragma unroll
-    for (int c = 0; c < 16; c++) v[c] = a[(r0 + i) * LDA + r0 + c];
+    for (int c = 0; c < 16; c++) {
+        x[c] = a[r0 * LDA + r0 + i * rs + c * cs];
+        y[c] = c == i ? C(1) : C(0);
+    }
+    C dq = C(1);  // 1 / u_ii, taken at step i
     const C TINY2 = C(1e-9);
     static_for<0, 16>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        C u[16];
+        C bc[16], yb[16];
 #pragma unroll
-        for (int c = k; c < 16; c++) u[c] = rbcast<k>(v[c]);
-        C p = u[k];
+        for (int c = k; c < 16; c++) bc[c] = rbcast<k>(x[c]);
+#pragma unroll
+        for (int c = 0; c <= k; c++) yb[c] = rbcast<k>(y[c]);
+        C p = bc[k];
         if (p < TINY2 && p > -TINY2) p = (p < 0) ? -TINY2 : TINY2;
-        const C l = i > k ? v[k] * (C(1) / p) : C(0);
-        v[k] = i > k ? l : (i == k ? p : v[k]);
+        const C rp = recip(p);
+        const C own = x[k];
+        const C f = i > k ? own * rp : C(0);
+        x[k] = i > k ? (colw ? own : f) : (i == k ? p : own);
+        dq = i == k ? rp : dq;
 #pragma unroll
-        for (int c = k + 1; c < 16; c++) v[c] -= l * u[c];  // l = 0 leaves rows <= k as they are
+        for (int c = k + 1; c < 16; c++) x[c] -= f * bc[c];  // f = 0 leaves lanes <= k as they are
+#pragma unroll
+        for (int c = 0; c <= k; c++) y[c] -= f * yb[c];
     });
     if (g == 0) {
 #pragma unroll
-        for (int c = 0; c < 16; c++) a[(r0 + i) * LDA + r0 + c] = v[c];
-    }
-    __builtin_amdgcn_wave_barrier();
-    // row i of M: L (unit lower) in lanes 0-15, U^T in the others
-    C m[16];
+        for (int c = 0; c < 16; c++) {
+            a[(r0 + i) * LDA + r0 + c] = x[c];
+            xd[i * BS + c] = y[c];
+        }
+    } else if (colw) {
 #pragma unroll
-    for (int c = 0; c < 16; c++) {
-        const C ut = c <= i ? a[(r0 + c) * LDA + r0 + i] : C(0);
-        m[c] = g == 0 ? (c < i ? v[c] : (c == i ? C(1) : C(0))) : ut;
-    }
-    const C di = C(1) / (g == 0 ? C(1) : a[(r0 + i) * LDA + r0 + i]);
-    C y[16];
-#pragma unroll
-    for (int c = 0; c < 16; c++) y[c] = c == i ? C(1) : C(0);
-    static_for<0, 16>([&](auto kc) {
-        constexpr int k = decltype(kc)::value;
-        const C qk = rbcast<k>(di);  // outside the conditional: every lane must run it
-        const C f = i > k ? m[k] * qk : C(0);
-#pragma unroll
-        for (int c = 0; c <= k; c++) y[c] -= f * rbcast<k>(y[c]);
-    });
-    if (g == 0) {
-#pragma unroll
-        for (int c = 0; c < 16; c++) xd[i * BS + c] = y[c] * di;
-    } else if (g == 1) {
-#pragma unroll
-        for (int c = 0; c < 16; c++) zd[c * BS + i] = y[c] * di;
+        for (int c = 0; c < 16; c++) zd[c * BS + i] = y[c] * dq;
     }
 }
 

@@ -122,6 +122,7 @@ def load(name: str = "libsogl.so") -> C.CDLL:
             "sogl_device_copy": (C.c_int, [vp, vp, vp, C.c_size_t, i32]),
             "sogl_factor_set_trace": (C.c_int, [vp, i32, i32]),
             "sogl_factor_get_trace": (C.c_int, [vp, i32, P(i64), P(i64)]),
+            "sogl_factor_get_graph": (C.c_int, [vp, i32, P(i64), P(i64), P(i32), P(i32), P(i32)]),
             "sogl_solveLU": (C.c_int, [i32, i64, i32, P(i32), P(i32), P(dbl), P(dbl), P(dbl)]),
         })
     for fn, (res, args) in sig.items():
@@ -354,6 +355,17 @@ class Device:
         self._check(self.lib.sogl_factor_get_trace(self.h, fid, _ptr(tk, C.c_int64),
                                                    None if tm is None else _ptr(tm, C.c_int64)))
         return tk, tm
+
+    def get_graph(self, fid: int, n_tasks: int):
+        """(dep_begin (n_tasks+1,), deps (n_deps,) pool slots, slot_task (n_slots,) producer)"""
+        nd, ns = C.c_int64(), C.c_int64()
+        self._check(self.lib.sogl_factor_get_graph(self.h, fid, C.byref(nd), C.byref(ns), None, None, None))
+        db = np.empty(n_tasks + 1, np.int32)
+        dp = np.empty(max(nd.value, 1), np.int32)
+        st = np.empty(max(ns.value, 1), np.int32)
+        self._check(self.lib.sogl_factor_get_graph(self.h, fid, None, None, _ptr(db, C.c_int32),
+                                                   _ptr(dp, C.c_int32), _ptr(st, C.c_int32)))
+        return db, dp[:nd.value], st[:ns.value]
 
     def alloc(self, nbytes: int) -> int:
         p = C.c_void_p()
