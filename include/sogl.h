@@ -153,6 +153,14 @@ typedef struct sogl_factor_desc {
 /* Build the device task graph and tile pool for one factorisation and copy the
  * input tiles to HBM.  The factor stays resident under *factor_id until released. */
 int sogl_factor_load(sogl_ctx* ctx, const sogl_factor_desc* desc, int32_t* factor_id);
+/* Builds the executor's task graph of a description on the host only (no device is
+ * touched) and checks it: every slot index inside the pool, one writer per tile,
+ * every operand produced at an earlier dependency level and waited for.
+ * sogl_factor_load runs the same check before any upload.  info (may be NULL)
+ * receives 7 counts: tasks, pool tiles, dependency levels, GEMM pairs,
+ * dependencies, folded copy/negate/subtract ops, aliased repeated inverses. */
+int sogl_graph_check(const sogl_factor_desc* desc, int64_t* info);
+int sogl_graph_check_plan(const sogl_plan* plan, int32_t precision, int64_t* info);
 /* Convenience: sogl_factor_load from a host plan. */
 int sogl_factor_load_plan(sogl_ctx* ctx, const sogl_plan* plan, int32_t precision,
                           int32_t* factor_id);
@@ -188,8 +196,9 @@ int sogl_factor_get_tiles(sogl_ctx* ctx, int32_t factor_id, int32_t which, doubl
 /* Diagnostics: per-task device timestamps of the persistent executor.
  * tasks4[4t..]: kind (0 gemm, 1 sub, 2 lu, 3 lower inverse, 4 upper inverse), pair
  * count, dependency level, dependency count; times4[4t..]: wall-clock ticks (100 MHz)
- * when the task was taken, when its sources were ready, when it finished, and the
- * hardware id of the compute unit.  Arrays hold n_tasks * 4 entries. */
+ * when the task was taken, when its sources were ready (GEMM tasks: when taken; they
+ * wait pair by pair), when it finished, and when its last dependency wait ended.
+ * Arrays hold n_tasks * 4 entries. */
 int sogl_factor_set_trace(sogl_ctx* ctx, int32_t factor_id, int32_t enable);
 int sogl_factor_get_trace(sogl_ctx* ctx, int32_t factor_id, int64_t* tasks4, int64_t* times4);
 /* Diagnostics: the executor's task graph.  n_deps / n_slots receive the sizes; with

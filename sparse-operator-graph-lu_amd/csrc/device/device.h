@@ -26,7 +26,8 @@ struct DevTask {
     int32_t dep_begin, dep_count;  // slots this task waits for (persistent executor)
     int32_t level;        // dependency level (0-based)
     int32_t out2, out3;   // lu: slots of L^-1 and U^-1 made by the same task (-1: none)
-    int32_t pad0, pad1;
+    int32_t add = -1;     // GEMM: slot of a tile added to the sum (-1: none)
+    int32_t add_neg = 0;  // GEMM: the added tile is subtracted instead
 };
 
 struct DevBuf {
@@ -48,6 +49,7 @@ struct Factor {
     std::vector<int32_t> input_slots;
     int64_t n_pairs = 0, n_deps = 0;
     int64_t n_inv_aliased = 0;  // repeated inverses of one factor tile that share its slot
+    int64_t n_folded = 0;       // copies, negations and subtractions folded into other tasks
     // device arrays
     DevBuf pool, tasks_d, pairs_d, deps_d, ready_d, ctrl_d, op_tasks_d, op_pairs_d, inputs_d;
     int32_t epoch = 0;

@@ -109,7 +109,7 @@ struct Args {
     int32_t epoch;
     int32_t task_offset;  // levels executor: first task of the launch
     double growth_limit;
-    long long* trace;     // optional: per task {taken, sources ready, done, hw id}
+    long long* trace;     // optional: per task {taken, sources ready, done, last wait ended}
 };
 
 // Waits (thread 0 polls, relaxed) until ready[s] >= epoch; on abort or timeout it
@@ -141,11 +141,13 @@ __device__ __forceinline__ void release_for_workgroup() { __builtin_amdgcn_fence
 
 struct PairWait {
     const Args* a;
+    long long* t_wait;  // tracing: wall clock when the latest wait ended (thread 0)
     __device__ inline void operator()(int sa, int sb) const {
         if (threadIdx.x == 0) {
             spin_ready(*a, sa);
             spin_ready(*a, sb);
             acquire_for_workgroup();
+            if (a->trace) *t_wait = wall_clock64();
         }
         __syncthreads();
     }
@@ -157,9 +159,10 @@ __device__ __attribute__((always_inline)) void run_task(const DevTask& t, const 
     switch (t.kind) {
         case TK_GEMM:
             if constexpr (BF16 && !ACC) {
-                gemm_task_bf16(reinterpret_cast<float*>(a.pool), a.pairs + t.a, t.b, t.out0, lds, wait);
+                gemm_task_bf16(reinterpret_cast<float*>(a.pool), a.pairs + t.a, t.b, t.out0, lds, wait, t.add,
+                               t.add_neg != 0);
             } else {
-                gemm_task<T, ACC>(pool, a.pairs + t.a, t.b, t.out0, lds, wait);
+                gemm_task<T, ACC>(pool, a.pairs + t.a, t.b, t.out0, lds, wait, t.add, t.add_neg != 0);
             }
             break;
         case TK_SUB:
@@ -203,7 +206,7 @@ __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
         const int ti = __builtin_amdgcn_readfirstlane(s_task);
         if (ti >= a.ntasks) break;
         const DevTask t = a.tasks[ti];
-        long long t_taken = 0, t_ready = 0;
+        long long t_taken = 0, t_ready = 0, t_wait = 0;
         if (a.trace && tid == 0) t_taken = wall_clock64();
         if (t.kind != TK_GEMM) {
             if (tid == 0) {
@@ -212,11 +215,11 @@ __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
             }
             __syncthreads();
         }
-        if (a.trace && tid == 0) t_ready = wall_clock64();
+        if (a.trace && tid == 0) t_ready = t_wait = wall_clock64();
         if (tid == 0) s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         if (__builtin_amdgcn_readfirstlane(s_abort)) break;
-        run_task<T, BF16, false>(t, a, lds, PairWait{&a});
+        run_task<T, BF16, false>(t, a, lds, PairWait{&a, &t_wait});
         // publish: every wave's stores complete (the barrier's workgroup release), then
         // thread 0 makes them visible at device scope once and raises the flags
         __syncthreads();
@@ -234,7 +237,7 @@ __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
                 tr[0] = t_taken;
                 tr[1] = t_ready;
                 tr[2] = wall_clock64();
-                tr[3] = (long long)__smid();
+                tr[3] = t_wait;
             }
         }
         // keeps the publishing branch and the next fetch branch apart, so the
@@ -275,7 +278,14 @@ __global__ void k_gather_tiles(const char* pool, const int32_t* slots, char* out
 // ------------------------------------------------------------------ task graph
 namespace {
 
-int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
+// Host arrays of a task graph that the factor keeps only on the device.
+struct HostGraph {
+    std::vector<Pair> pairs, op_pairs;
+    std::vector<int32_t> deps, lp, lc, ls, up, uc, us;
+};
+
+// The task graph of a factor description, on the host only (no device calls).
+int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph& hg) {
     const int32_t S = d->n_ids;
     std::vector<int32_t> slot(S > 0 ? (size_t)S : 1, -1);
     int32_t next = 0;
@@ -286,21 +296,19 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
         slot[(size_t)id] = next++;
         f->input_slots.push_back(slot[(size_t)id]);
     }
+    // readers of each id (op operands; factor tiles count as read by the solve): a
+    // temporary with exactly one reader can be folded into that reader
+    std::vector<int32_t> readers((size_t)S, 0);
     for (int64_t k = 0; k < d->n_ops; k++) {
         const sogl_op& o = d->ops[k];
-        for (int32_t r : {o.result, o.result2}) {
-            if (r <= 0) continue;
+        for (int32_t r : {o.result, o.result2})
             if (r >= S) return set_error(c, SOGL_ERR_ARG, "result id out of range");
-            if (slot[(size_t)r] < 0) slot[(size_t)r] = next++;
-        }
         if (o.src >= S || o.src2 >= S || o.src < 0 || o.src2 < 0)
             return set_error(c, SOGL_ERR_ARG, "source id out of range");
+        if (o.src > 0) readers[(size_t)o.src]++;
+        if (o.src2 > 0) readers[(size_t)o.src2]++;
     }
-    f->zero_slot = next++;
     const int32_t R = d->block_rows;
-    // diagonal factor tiles per block row
-    f->l_diag.assign((size_t)R, -1);
-    f->u_diag.assign((size_t)R, -1);
     for (int which = 0; which < 2; which++) {
         const int64_t cnt = which == 0 ? d->n_l : d->n_u;
         const int32_t* bi = which == 0 ? d->l_bi : d->u_bi;
@@ -309,27 +317,38 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
         for (int64_t k = 0; k < cnt; k++) {
             if (bi[k] < 0 || bi[k] >= R || bj[k] < 0 || bj[k] >= R || id[k] <= 0 || id[k] >= S)
                 return set_error(c, SOGL_ERR_ARG, "factor tile out of range");
-            if (bi[k] == bj[k]) (which == 0 ? f->l_diag : f->u_diag)[(size_t)bi[k]] = slot[(size_t)id[k]];
+            readers[(size_t)id[k]]++;
         }
     }
-    for (int32_t i = 0; i < R; i++)
-        if (f->l_diag[(size_t)i] < 0 || f->u_diag[(size_t)i] < 0)
-            return set_error(c, SOGL_ERR_ARG, "factor lacks a diagonal tile");
+    f->zero_slot = next++;
 
-    // producer task of each slot (-1: input or never written)
+    // producer task of each slot (-1: input or never written); sealed: the slot has
+    // been read, so no later op may write it.  Result slots are made on first write.
     std::vector<int32_t> prod((size_t)next, -1);
+    std::vector<char> sealed((size_t)next, 0);
+    auto new_slot = [&]() -> int32_t {
+        prod.push_back(-1);
+        sealed.push_back(0);
+        return next++;
+    };
     std::vector<int32_t> lu_op_task;  // per task: index of its op in op_tasks (lu tasks)
     std::vector<DevTask> tasks;
     std::vector<std::vector<Pair>> tpairs;
     std::vector<int32_t> tstage;
     std::vector<sogl_op> opl(d->ops, d->ops + d->n_ops);
-    std::vector<Pair> op_pairs;
+    std::vector<Pair>& op_pairs = hg.op_pairs;
     f->op_tasks.clear();
     auto src_slot = [&](int32_t id) -> int32_t { return id > 0 ? slot[(size_t)id] : -1; };
     for (int64_t k = 0; k < d->n_ops; k++) {
         const sogl_op& o = opl[(size_t)k];
         if (o.result <= 0) continue;
-        const int32_t rs = slot[(size_t)o.result];
+        // sub and inverse results may fold into another tile's slot: theirs is made
+        // only when they do not (ensure_rs below)
+        const bool may_fold = o.op == SOGL_OP_SUB || o.op == SOGL_OP_LOWER_INV || o.op == SOGL_OP_UPPER_INV;
+        if (!may_fold && slot[(size_t)o.result] < 0) slot[(size_t)o.result] = new_slot();
+        if (o.op == SOGL_OP_LU && o.result2 > 0 && slot[(size_t)o.result2] < 0)
+            slot[(size_t)o.result2] = new_slot();
+        int32_t rs = slot[(size_t)o.result];
         int kind;
         switch (o.op) {
             case SOGL_OP_MUL: case SOGL_OP_MULNEG: kind = TK_GEMM; break;
@@ -347,6 +366,9 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
         ot.out1 = (kind == TK_LU && o.result2 > 0) ? slot[(size_t)o.result2] : -1;
         ot.level = o.stage;
         int32_t s1 = src_slot(o.src), s2 = src_slot(o.src2);
+        if ((o.src > 0 && s1 < 0) || (o.src2 > 0 && s2 < 0))
+            return set_error(c, SOGL_ERR_ARG, "op " + std::to_string(k) + " reads a tile that is neither an input "
+                                              "nor written by an earlier op");
         Pair pr{0, 0};
         if (kind == TK_GEMM) {
             if (s1 < 0) s1 = f->zero_slot;
@@ -364,6 +386,59 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
         }
         ot.out2 = ot.out3 = -1;
         f->op_tasks.push_back(ot);
+        auto seal_sources = [&]() {
+            if (s1 >= 0) sealed[(size_t)s1] = 1;
+            if (s2 >= 0) sealed[(size_t)s2] = 1;
+        };
+        // the result id reads slot `to` from now on; the per-op executor runs this op
+        // in place on that slot
+        auto alias = [&](int32_t to) {
+            slot[(size_t)o.result] = to;
+            f->op_tasks.back().out0 = to;
+            seal_sources();
+            f->n_folded++;
+        };
+        auto ensure_rs = [&]() {
+            if (rs >= 0) return;
+            rs = slot[(size_t)o.result] = new_slot();
+            ot.out0 = rs;
+            f->op_tasks.back().out0 = rs;
+        };
+        if (kind == TK_SUB && rs < 0) {
+            // copy: the result is its source (exact)
+            if (o.src == 0 && o.src2 > 0) {
+                alias(s2);
+                continue;
+            }
+            // a single-use GEMM sum M folded into this op: src2 - M, -M (src2 == 0) or
+            // M - src.  The GEMM task takes the sign and the added tile; exact up to the
+            // order of the additions.
+            auto foldable = [&](int32_t id) -> int32_t {
+                if (id <= 0 || readers[(size_t)id] != 1) return -1;
+                const int32_t pt = prod[(size_t)slot[(size_t)id]];
+                return pt >= 0 && tasks[(size_t)pt].kind == TK_GEMM ? pt : -1;
+            };
+            int32_t pt = foldable(o.src);
+            if (pt >= 0 && (o.src2 == 0 || (tasks[(size_t)pt].add < 0 && s2 != s1))) {
+                DevTask& g = tasks[(size_t)pt];
+                for (Pair& q : tpairs[(size_t)pt]) q.b = -q.b - 1;  // mul <-> mulneg
+                if (o.src2 == 0) {
+                    g.add_neg ^= 1;
+                } else {
+                    g.add = s2;
+                    g.add_neg = 0;
+                }
+                alias(s1);
+                continue;
+            }
+            pt = foldable(o.src2);
+            if (pt >= 0 && o.src > 0 && tasks[(size_t)pt].add < 0 && s1 != s2) {
+                tasks[(size_t)pt].add = s1;
+                tasks[(size_t)pt].add_neg = 1;
+                alias(s2);
+                continue;
+            }
+        }
         // an inverse of an lu output is made by that lu's task (the lu holds the tile
         // already); the per-op executor still runs it as its own op.  The recursive
         // planner inverts some diagonal factor tiles more than once (C2: 3,679
@@ -372,21 +447,25 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
         if (kind == TK_LINV || kind == TK_UINV) {
             const int32_t pt = s1 >= 0 ? prod[(size_t)s1] : -1;
             const bool lower = kind == TK_LINV;
-            if (pt >= 0 && tasks[(size_t)pt].kind == TK_LU && prod[(size_t)rs] < 0 &&
+            if (pt >= 0 && tasks[(size_t)pt].kind == TK_LU && rs < 0 &&
                 (lower ? tasks[(size_t)pt].out0 : tasks[(size_t)pt].out1) == s1) {
-                DevTask& lt = tasks[(size_t)pt];
-                int32_t& inv = lower ? lt.out2 : lt.out3;
+                int32_t inv = lower ? tasks[(size_t)pt].out2 : tasks[(size_t)pt].out3;
                 if (inv < 0) {
-                    inv = rs;
+                    ensure_rs();
+                    (lower ? tasks[(size_t)pt].out2 : tasks[(size_t)pt].out3) = rs;
                     prod[(size_t)rs] = pt;
                 } else {
                     slot[(size_t)o.result] = inv;
                     f->op_tasks.back().out0 = inv;
                     f->n_inv_aliased++;
                 }
+                seal_sources();
                 continue;
             }
         }
+        ensure_rs();
+        if (sealed[(size_t)rs])
+            return set_error(c, SOGL_ERR_ARG, "op list writes a tile after it was read");
         int32_t t = prod[(size_t)rs];
         if (t < 0) {
             t = (int32_t)tasks.size();
@@ -404,7 +483,22 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
             tstage[(size_t)t] = std::max(tstage[(size_t)t], o.stage);
         }
         if (kind == TK_GEMM) tpairs[(size_t)t].push_back(pr);
+        seal_sources();
     }
+    // diagonal factor tiles per block row (after the folds: ids may share slots)
+    f->l_diag.assign((size_t)R, -1);
+    f->u_diag.assign((size_t)R, -1);
+    for (int which = 0; which < 2; which++) {
+        const int64_t cnt = which == 0 ? d->n_l : d->n_u;
+        const int32_t* bi = which == 0 ? d->l_bi : d->u_bi;
+        const int32_t* id = which == 0 ? d->l_ids : d->u_ids;
+        for (int64_t k = 0; k < cnt; k++)
+            if (bi[k] == (which == 0 ? d->l_bj : d->u_bj)[k])
+                (which == 0 ? f->l_diag : f->u_diag)[(size_t)bi[k]] = slot[(size_t)id[k]];
+    }
+    for (int32_t i = 0; i < R; i++)
+        if (f->l_diag[(size_t)i] < 0 || f->u_diag[(size_t)i] < 0)
+            return set_error(c, SOGL_ERR_ARG, "factor lacks a diagonal tile");
     // inverses of the diagonal factor tiles for the triangular solve: made by the lu
     // task of each diagonal tile, into the plan's inverse tile when the plan has one
     f->linv_slot.assign((size_t)R, -1);
@@ -419,8 +513,8 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
             if (from_lu) {
                 inv = which == 0 ? tasks[(size_t)pt].out2 : tasks[(size_t)pt].out3;
                 if (inv < 0) {  // the plan never inverts this tile: a solve-only slot
-                    inv = next++;
-                    prod.push_back(pt);
+                    inv = new_slot();
+                    prod[(size_t)inv] = pt;
                     (which == 0 ? tasks[(size_t)pt].out2 : tasks[(size_t)pt].out3) = inv;
                     DevTask& ot = f->op_tasks[(size_t)lu_op_task[(size_t)pt]];
                     (which == 0 ? ot.out2 : ot.out3) = inv;
@@ -428,7 +522,7 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
             } else {
                 // a diagonal tile that no lu produced (hand-built descriptions): its own
                 // inverse task, general diagonal
-                inv = next++;
+                inv = new_slot();
                 DevTask nt{};
                 nt.kind = which == 0 ? TK_LINV : TK_UINV;
                 nt.out0 = inv;
@@ -438,7 +532,7 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
                 tpairs.emplace_back();
                 tstage.push_back(0);
                 lu_op_task.push_back(-1);
-                prod.push_back((int32_t)tasks.size() - 1);
+                prod[(size_t)inv] = (int32_t)tasks.size() - 1;
                 f->op_tasks.push_back(nt);
             }
             (which == 0 ? f->linv_slot : f->uinv_slot)[(size_t)i] = inv;
@@ -463,6 +557,7 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
                 add_dep(t, p.a);
                 add_dep(t, p.b < 0 ? -p.b - 1 : p.b);
             }
+            add_dep(t, k.add);
         } else if (k.kind == TK_SUB) {
             add_dep(t, k.a);
             add_dep(t, k.b);
@@ -486,24 +581,25 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
     // pairs of a GEMM task run in the order their operands become ready, so the task
     // waits only for its last pair once the earlier ones are done
     auto ready_level = [&](int32_t s) -> int32_t { return prod[(size_t)s] < 0 ? -1 : level[(size_t)prod[(size_t)s]]; };
+    auto pair_level = [&](const Pair& x) -> int32_t {
+        return std::max(ready_level(x.a), ready_level(x.b < 0 ? -x.b - 1 : x.b));
+    };
     for (size_t t = 0; t < T; t++) {
         if (tasks[t].kind != TK_GEMM) continue;
-        std::stable_sort(tpairs[t].begin(), tpairs[t].end(), [&](const Pair& x, const Pair& y) {
-            const int32_t lx = std::max(ready_level(x.a), ready_level(x.b < 0 ? -x.b - 1 : x.b));
-            const int32_t ly = std::max(ready_level(y.a), ready_level(y.b < 0 ? -y.b - 1 : y.b));
-            return lx < ly;
-        });
+        std::stable_sort(tpairs[t].begin(), tpairs[t].end(),
+                         [&](const Pair& x, const Pair& y) { return pair_level(x) < pair_level(y); });
     }
-    std::vector<int32_t> order(T);
-    for (size_t t = 0; t < T; t++) order[t] = (int32_t)t;
+    const size_t T2 = tasks.size();
+    std::vector<int32_t> order(T2);
+    for (size_t t = 0; t < T2; t++) order[t] = (int32_t)t;
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
         return level[(size_t)x] < level[(size_t)y];
     });
     f->tasks.clear();
     f->tasks.reserve(T);
-    std::vector<Pair> pairs;
-    std::vector<int32_t> deps;
-    int32_t maxlv = T ? level[(size_t)order.back()] : -1;
+    std::vector<Pair>& pairs = hg.pairs;
+    std::vector<int32_t>& deps = hg.deps;
+    int32_t maxlv = T2 ? level[(size_t)order.back()] : -1;
     f->level_begin.assign((size_t)maxlv + 2, 0);
     for (int32_t t : order) {
         DevTask k = tasks[(size_t)t];
@@ -547,7 +643,9 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
             }
         }
     }
-    std::vector<int32_t> lp(1, 0), lc, ls, up(1, 0), uc, us;
+    std::vector<int32_t>&lp = hg.lp, &lc = hg.lc, &ls = hg.ls, &up = hg.up, &uc = hg.uc, &us = hg.us;
+    lp.assign(1, 0);
+    up.assign(1, 0);
     for (int32_t i = 0; i < R; i++) {
         auto& L = lrow[(size_t)i];
         std::sort(L.begin(), L.end());
@@ -559,7 +657,117 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
         up.push_back((int32_t)uc.size());
     }
     f->block_rows = R;
+    return SOGL_OK;
+}
 
+// Checks every index of a host task graph against the pool and every dependency
+// against the task order, so that a wrong graph is an error here and never a bad
+// address or an endless wait on the device.
+int validate_graph(Context* c, const sogl_factor_desc* d, const Factor* f, const HostGraph& hg) {
+    const int64_t n = f->n_slots;
+    const size_t T = f->tasks.size();
+    auto bad = [&](const std::string& m) { return set_error(c, SOGL_ERR_ARG, "task graph check: " + m); };
+    auto ok_slot = [&](int64_t s) { return s >= 0 && s < n; };
+    auto ok_opt = [&](int64_t s) { return s == -1 || ok_slot(s); };
+    if (f->zero_slot < 0 || f->zero_slot >= n) return bad("zero slot");
+    std::vector<int32_t> writer((size_t)n, -1);
+    for (size_t t = 0; t < T; t++) {
+        const DevTask& k = f->tasks[t];
+        if (!ok_slot(k.out0) || !ok_opt(k.out1) || !ok_opt(k.out2) || !ok_opt(k.out3))
+            return bad("output slot of task " + std::to_string(t));
+        for (int32_t o : {k.out0, k.out1, k.out2, k.out3}) {
+            if (o < 0) continue;
+            if (writer[(size_t)o] >= 0) return bad("slot " + std::to_string(o) + " written by two tasks");
+            writer[(size_t)o] = (int32_t)t;
+        }
+    }
+    auto is_fixed = [&](int32_t s) { return s == f->zero_slot || (s >= 0 && s < (int64_t)d->n_inputs); };
+    if (f->level_begin.empty() || f->level_begin.front() != 0 || (size_t)f->level_begin.back() != T)
+        return bad("level ranges");
+    for (size_t v = 0; v + 1 < f->level_begin.size(); v++)
+        for (int32_t t = f->level_begin[v]; t < f->level_begin[v + 1]; t++)
+            if (t < 0 || (size_t)t >= T || f->tasks[(size_t)t].level != (int32_t)v) return bad("level order");
+    for (size_t t = 0; t < T; t++) {
+        const DevTask& k = f->tasks[t];
+        if (k.dep_begin < 0 || k.dep_count < 0 || (size_t)k.dep_begin + (size_t)k.dep_count > hg.deps.size())
+            return bad("dependency range of task " + std::to_string(t));
+        std::vector<int32_t> reads;
+        switch (k.kind) {
+            case TK_GEMM:
+                if (k.a < 0 || k.b < 1 || (size_t)k.a + (size_t)k.b > hg.pairs.size())
+                    return bad("pair range of task " + std::to_string(t));
+                for (int32_t q = k.a; q < k.a + k.b; q++) {
+                    const Pair& p = hg.pairs[(size_t)q];
+                    reads.push_back(p.a);
+                    reads.push_back(p.b < 0 ? -p.b - 1 : p.b);
+                }
+                if (!ok_opt(k.add)) return bad("added tile of task " + std::to_string(t));
+                if (k.add >= 0) reads.push_back(k.add);
+                break;
+            case TK_SUB:
+                if (!ok_opt(k.a) || !ok_opt(k.b)) return bad("operand of task " + std::to_string(t));
+                if (k.a >= 0) reads.push_back(k.a);
+                if (k.b >= 0) reads.push_back(k.b);
+                break;
+            case TK_LU: case TK_LINV: case TK_UINV:
+                reads.push_back(k.a);
+                break;
+            default:
+                return bad("kind of task " + std::to_string(t));
+        }
+        for (int32_t s2 : reads) {
+            if (!ok_slot(s2)) return bad("operand slot of task " + std::to_string(t));
+            const int32_t w = writer[(size_t)s2];
+            if (w < 0) {
+                if (!is_fixed(s2)) return bad("task " + std::to_string(t) + " reads a tile no task writes");
+                continue;
+            }
+            if (f->tasks[(size_t)w].level >= k.level)
+                return bad("task " + std::to_string(t) + " is not after the producer of its operand");
+            if (k.kind != TK_GEMM) {  // non-GEMM tasks wait for their dependency list only
+                bool found = false;
+                for (int32_t q = k.dep_begin; q < k.dep_begin + k.dep_count; q++) found |= hg.deps[(size_t)q] == s2;
+                if (!found) return bad("task " + std::to_string(t) + " does not wait for an operand");
+            }
+        }
+        for (int32_t q = k.dep_begin; q < k.dep_begin + k.dep_count; q++)
+            if (!ok_slot(hg.deps[(size_t)q])) return bad("dependency slot of task " + std::to_string(t));
+    }
+    for (const DevTask& k : f->op_tasks) {
+        if (!ok_slot(k.out0) || !ok_opt(k.out1) || !ok_opt(k.out2) || !ok_opt(k.out3)) return bad("per-op output");
+        if (k.kind == TK_GEMM) {
+            if (k.a < 0 || k.b < 1 || (size_t)k.a + (size_t)k.b > hg.op_pairs.size()) return bad("per-op pairs");
+            for (int32_t q = k.a; q < k.a + k.b; q++) {
+                const Pair& p = hg.op_pairs[(size_t)q];
+                if (!ok_slot(p.a) || !ok_slot(p.b < 0 ? -p.b - 1 : p.b)) return bad("per-op pair slot");
+            }
+        } else if (!ok_opt(k.a) || !ok_opt(k.b)) {
+            return bad("per-op operand");
+        }
+    }
+    for (const auto* v : {&f->l_diag, &f->u_diag, &f->linv_slot, &f->uinv_slot, &f->l_slots, &f->u_slots,
+                          &f->input_slots, &hg.ls, &hg.us})
+        for (int32_t s2 : *v)
+            if (!ok_slot(s2)) return bad("solve slot");
+    for (const auto* v : {&f->linv_slot, &f->uinv_slot})
+        for (int32_t s2 : *v)
+            if (writer[(size_t)s2] < 0) return bad("a diagonal inverse no task writes");
+    const int32_t R = f->block_rows;
+    if ((int32_t)hg.lp.size() != R + 1 || (int32_t)hg.up.size() != R + 1 || hg.lp.back() != (int32_t)hg.lc.size() ||
+        hg.up.back() != (int32_t)hg.uc.size())
+        return bad("solve rows");
+    for (int32_t cc : hg.lc) if (cc < 0 || cc >= R) return bad("solve column");
+    for (int32_t cc : hg.uc) if (cc < 0 || cc >= R) return bad("solve column");
+    return SOGL_OK;
+}
+
+// Device arrays of a checked graph: pool, tasks, pairs, dependencies, flags, solve
+// structure, and the input tiles copied in.
+int upload_graph(Context* c, const sogl_factor_desc* d, Factor* f, const HostGraph& hg) {
+    const std::vector<Pair>& pairs = hg.pairs;
+    const std::vector<Pair>& op_pairs = hg.op_pairs;
+    const std::vector<int32_t>& deps = hg.deps;
+    const size_t T = f->tasks.size();
     // device arrays
     int rc;
     const size_t tile_bytes = (size_t)TE * f->elem;
@@ -583,9 +791,9 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
     if (!deps.empty()) HIP_TRY(c, hipMemcpy(f->deps_d.p, deps.data(), deps.size() * 4, hipMemcpyHostToDevice));
     if (!f->op_tasks.empty()) HIP_TRY(c, hipMemcpy(f->op_tasks_d.p, f->op_tasks.data(), f->op_tasks.size() * sizeof(DevTask), hipMemcpyHostToDevice));
     if (!op_pairs.empty()) HIP_TRY(c, hipMemcpy(f->op_pairs_d.p, op_pairs.data(), op_pairs.size() * sizeof(Pair), hipMemcpyHostToDevice));
-    if ((rc = up_i32(f->l_rowptr, lp, "L rows")) || (rc = up_i32(f->l_col, lc, "L cols")) ||
-        (rc = up_i32(f->l_slot, ls, "L slots")) || (rc = up_i32(f->u_rowptr, up, "U rows")) ||
-        (rc = up_i32(f->u_col, uc, "U cols")) || (rc = up_i32(f->u_slot, us, "U slots")) ||
+    if ((rc = up_i32(f->l_rowptr, hg.lp, "L rows")) || (rc = up_i32(f->l_col, hg.lc, "L cols")) ||
+        (rc = up_i32(f->l_slot, hg.ls, "L slots")) || (rc = up_i32(f->u_rowptr, hg.up, "U rows")) ||
+        (rc = up_i32(f->u_col, hg.uc, "U cols")) || (rc = up_i32(f->u_slot, hg.us, "U slots")) ||
         (rc = up_i32(f->l_diag_d, f->l_diag, "L diag")) || (rc = up_i32(f->u_diag_d, f->u_diag, "U diag")) ||
         (rc = up_i32(f->linv_d, f->linv_slot, "L inv")) || (rc = up_i32(f->uinv_d, f->uinv_slot, "U inv")) ||
         (rc = up_i32(f->inputs_d, f->input_slots, "input slots")))
@@ -629,6 +837,14 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
     }
     HIP_TRY(c, hipStreamSynchronize(st));
     return SOGL_OK;
+}
+
+int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
+    HostGraph hg;
+    int rc = build_host_graph(c, d, f, hg);
+    if (rc == SOGL_OK) rc = validate_graph(c, d, f, hg);
+    if (rc == SOGL_OK) rc = upload_graph(c, d, f, hg);
+    return rc;
 }
 
 void free_factor(Factor* f) {
@@ -676,8 +892,8 @@ int launch_factor(Context* c, Factor* f, int mode) {
     } else if (mode == SOGL_EXEC_PER_OP) {
         if (BF16) return set_error(c, SOGL_ERR_UNSUPPORTED, "per-op executor has no bf16 variant");
         // results start at zero; every op then runs alone, C += A*B reading C
-        const size_t first_result = f->input_slots.size();
-        const size_t nres = (size_t)f->zero_slot - first_result;
+        const size_t first_result = f->input_slots.size();  // the zero slot, then results
+        const size_t nres = (size_t)f->n_slots - first_result;
         hipLaunchKernelGGL(k_fill<T>, dim3(4096), dim3(256), 0, st,
                            (T*)f->pool.p + first_result * TE, nres * TE, T(0));
         a.tasks = (const DevTask*)f->op_tasks_d.p;
@@ -787,6 +1003,34 @@ extern "C" int sogl_factor_load(sogl_ctx* x, const sogl_factor_desc* d, int32_t*
     c->factors[f->id] = f;
     *factor_id = f->id;
     return SOGL_OK;
+}
+
+extern "C" int sogl_graph_check(const sogl_factor_desc* d, int64_t* info) {
+    if (!d || d->n_ids <= 0 || d->n_ops < 0 || d->n_inputs < 0 || d->block_rows <= 0 ||
+        (d->n_ops && !d->ops) || (d->n_inputs && !d->input_ids))
+        return set_error(nullptr, SOGL_ERR_ARG, "bad factor description");
+    Factor f;
+    f.precision = d->precision;
+    f.elem = d->precision == SOGL_FP64 ? 8 : 4;
+    HostGraph hg;
+    int rc = build_host_graph(nullptr, d, &f, hg);
+    if (rc == SOGL_OK) rc = validate_graph(nullptr, d, &f, hg);
+    if (rc == SOGL_OK && info) {
+        info[0] = (int64_t)f.tasks.size();
+        info[1] = f.n_slots;
+        info[2] = (int64_t)f.level_begin.size() - 1;
+        info[3] = (int64_t)hg.pairs.size();
+        info[4] = (int64_t)hg.deps.size();
+        info[5] = f.n_folded;
+        info[6] = f.n_inv_aliased;
+    }
+    return rc;
+}
+
+extern "C" int sogl_graph_check_plan(const sogl_plan* plan, int32_t precision, int64_t* info) {
+    sogl_factor_desc d;
+    if (!sogl::plan_factor_desc(plan, precision, &d)) return set_error(nullptr, SOGL_ERR_ARG, "null plan");
+    return sogl_graph_check(&d, info);
 }
 
 extern "C" int sogl_factor_load_plan(sogl_ctx* x, const sogl_plan* plan, int32_t precision,

@@ -115,9 +115,27 @@ struct NoWait {
     __device__ inline void operator()(int, int) const { __syncthreads(); }
 };
 
+// A task made by folding a later sub/neg into the GEMM adds (or subtracts) one more
+// tile, `add`, to the sum: it is waited for after the pairs and read straight into
+// the accumulators.
+template <typename T, typename acc_t>
+__device__ __forceinline__ void add_tile(acc_t (&acc)[2][2], const T* __restrict__ X, bool neg, int wr, int wc,
+                                         int li, int lk) {
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int n = 0; n < 2; n++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const T x = X[(wr + m * 16 + MfmaOp<T>::row(lk, r)) * TS + wc + n * 16 + li];
+                acc[m][n][r] += neg ? -x : x;
+            }
+}
+
 template <typename T, bool ACC = false, typename Wait = NoWait>
 __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, const Pair* __restrict__ pairs, int npairs,
-                          int out, char* lds_raw, const Wait& wait = Wait()) {
+                          int out, char* lds_raw, const Wait& wait = Wait(), int add = -1,
+                          bool add_neg = false) {
     typedef typename MfmaOp<T>::acc_t acc_t;
     T* la = reinterpret_cast<T*>(lds_raw);
     T* lb = la + TS * LDA;
@@ -160,6 +178,10 @@ __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, c
                 for (int n = 0; n < 2; n++) acc[m][n] = MfmaOp<T>::mma(af[m], bf[n], acc[m][n]);
         }
     }
+    if (add >= 0) {
+        wait(add, add);
+        add_tile<T>(acc, pool + (size_t)add * TE, add_neg, wr, wc, li, lk);
+    }
     T* C = pool + (size_t)out * TE;
 #pragma unroll
     for (int m = 0; m < 2; m++)
@@ -174,7 +196,8 @@ __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, c
 // (v_mfma_f32_16x16x32_bf16: A[i][8q..8q+7] in lane i + 16q, same D layout).
 template <typename Wait = NoWait>
 __device__ __attribute__((always_inline)) void gemm_task_bf16(float* __restrict__ pool, const Pair* __restrict__ pairs,
-                                      int npairs, int out, char* lds_raw, const Wait& wait = Wait()) {
+                                      int npairs, int out, char* lds_raw, const Wait& wait = Wait(),
+                                      int add = -1, bool add_neg = false) {
     float* la = reinterpret_cast<float*>(lds_raw);
     float* lb = la + TS * LDA;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -213,6 +236,10 @@ __device__ __attribute__((always_inline)) void gemm_task_bf16(float* __restrict_
                 for (int n = 0; n < 2; n++)
                     acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], bf[n], acc[m][n], 0, 0, 0);
         }
+    }
+    if (add >= 0) {  // the bf16 kernel's accumulator rows follow the f32 layout
+        wait(add, add);
+        add_tile<float>(acc, pool + (size_t)add * TE, add_neg, wr, wc, li, lk);
     }
     float* C = pool + (size_t)out * TE;
 #pragma unroll
@@ -495,6 +522,58 @@ __device__ inline void diag16(C* a, int r0, C* xd, C* zd, int lane) {
     }
 }
 
+// The same outputs in two sweeps (LU, then both inverses), for FP64: holding the
+// FP64 rows of the LU and of the inverses at once would exceed the registers the
+// executor has at two workgroups per CU.  Lanes i, i+16, i+32, i+48 hold row i;
+// after the LU, lanes 0-15 invert L by rows and lanes 16-31 invert U^T by rows with
+// the same instructions (forward substitution, 1/u_ii scaling deferred to the end).
+template <typename C>
+__device__ inline void diag16_split(C* a, int r0, C* xd, C* zd, int lane) {
+    const int i = lane & 15, g = lane >> 4;
+    C v[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) v[c] = a[(r0 + i) * LDA + r0 + c];
+    const C TINY2 = C(1e-9);
+    static_for<0, 16>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        C p = rbcast<k>(v[k]);
+        if (p < TINY2 && p > -TINY2) p = (p < 0) ? -TINY2 : TINY2;
+        const C l = i > k ? v[k] * recip(p) : C(0);
+        v[k] = i > k ? l : (i == k ? p : v[k]);
+#pragma unroll
+        for (int c = k + 1; c < 16; c++) v[c] -= l * rbcast<k>(v[c]);  // lane k's row is unchanged here
+    });
+    if (g == 0) {
+#pragma unroll
+        for (int c = 0; c < 16; c++) a[(r0 + i) * LDA + r0 + c] = v[c];
+    }
+    __builtin_amdgcn_wave_barrier();
+    C m[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+        const C ut = c <= i ? a[(r0 + c) * LDA + r0 + i] : C(0);
+        m[c] = g == 0 ? (c < i ? v[c] : (c == i ? C(1) : C(0))) : ut;
+    }
+    const C di = C(1) / (g == 0 ? C(1) : a[(r0 + i) * LDA + r0 + i]);
+    C y[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) y[c] = c == i ? C(1) : C(0);
+    static_for<0, 16>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const C qk = rbcast<k>(di);  // outside the conditional: every lane must run it
+        const C f = i > k ? m[k] * qk : C(0);
+#pragma unroll
+        for (int c = 0; c <= k; c++) y[c] -= f * rbcast<k>(y[c]);
+    });
+    if (g == 0) {
+#pragma unroll
+        for (int c = 0; c < 16; c++) xd[i * BS + c] = y[c] * di;
+    } else if (g == 1) {
+#pragma unroll
+        for (int c = 0; c < 16; c++) zd[c * BS + i] = y[c] * di;
+    }
+}
+
 // LU of one tile in compute type C plus, when their slots are given (>= 0), the
 // inverses of its unit-lower L and upper U: the reference's lowerInv / upperInv
 // of the same diagonal tile, fused into the producing task.
@@ -528,7 +607,10 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
 #pragma unroll 1
     for (int K = 0; K < 4; K++) {
         const int r0 = 16 * K;
-        if (w == 0) diag16(a, r0, xd + K * BE, zd + K * BE, lane);
+        if (w == 0) {
+            if constexpr (sizeof(C) == 8) diag16_split(a, r0, xd + K * BE, zd + K * BE, lane);
+            else diag16(a, r0, xd + K * BE, zd + K * BE, lane);
+        }
         __syncthreads();
         if (K == 3) break;
         const int nb = 3 - K;  // blocks right of / below the diagonal block

@@ -123,6 +123,8 @@ def load(name: str = "libsogl.so") -> C.CDLL:
             "sogl_factor_set_trace": (C.c_int, [vp, i32, i32]),
             "sogl_factor_get_trace": (C.c_int, [vp, i32, P(i64), P(i64)]),
             "sogl_factor_get_graph": (C.c_int, [vp, i32, P(i64), P(i64), P(i32), P(i32), P(i32)]),
+            "sogl_graph_check": (C.c_int, [P(FactorDesc), P(i64)]),
+            "sogl_graph_check_plan": (C.c_int, [vp, i32, P(i64)]),
             "sogl_solveLU": (C.c_int, [i32, i64, i32, P(i32), P(i32), P(dbl), P(dbl), P(dbl)]),
         })
     for fn, (res, args) in sig.items():
@@ -242,6 +244,52 @@ class Plan:
         return r, c, v
 
 
+def make_desc(n_ids: int, ops, input_ids, input_tiles, ltiles, utiles, block_rows: int,
+              precision: int = FP64):
+    """A sogl_factor_desc over numpy copies of a raw op list (rows: src src2 op result
+    result2 stage seq group); ltiles / utiles: (k, 3) block row, block col, id.
+    Returns (desc, arrays that must stay alive while desc is used)."""
+    ops = np.ascontiguousarray(ops, np.int32).reshape(-1, 8)
+    ids = np.ascontiguousarray(input_ids, np.int32)
+    tiles = np.ascontiguousarray(input_tiles, np.float64)
+    lt = np.ascontiguousarray(ltiles, np.int32).reshape(-1, 3)
+    ut = np.ascontiguousarray(utiles, np.int32).reshape(-1, 3)
+    cols = [np.ascontiguousarray(lt[:, k]) for k in range(3)] + \
+           [np.ascontiguousarray(ut[:, k]) for k in range(3)]
+    keep = [ops, ids, tiles, lt, ut] + cols
+    d = FactorDesc(n_ids, block_rows, ops.ctypes.data, ops.shape[0], ids.ctypes.data,
+                   tiles.ctypes.data, ids.size, cols[0].ctypes.data, cols[1].ctypes.data,
+                   cols[2].ctypes.data, lt.shape[0], cols[3].ctypes.data, cols[4].ctypes.data,
+                   cols[5].ctypes.data, ut.shape[0], precision)
+    return d, keep
+
+
+GRAPH_INFO = ("tasks", "pool_tiles", "levels", "pairs", "deps", "folded", "inv_aliased")
+
+
+def graph_check_ops(n_ids: int, ops, input_ids, input_tiles, ltiles, utiles, block_rows: int,
+                    precision: int = FP64) -> dict:
+    """graph_check for a raw op list (see make_desc); raises SoglError on a bad graph."""
+    lib = load("libsogl.so")
+    d, keep = make_desc(n_ids, ops, input_ids, input_tiles, ltiles, utiles, block_rows, precision)
+    info = np.zeros(len(GRAPH_INFO), np.int64)
+    rc = lib.sogl_graph_check(C.byref(d), _ptr(info, C.c_int64))
+    del keep
+    if rc != OK:
+        raise SoglError(rc, lib.sogl_last_error().decode())
+    return dict(zip(GRAPH_INFO, (int(v) for v in info)))
+
+
+def graph_check(plan: "Plan", precision: int = FP64) -> dict:
+    """Build and check the executor's task graph on the host (no GPU); raises on error."""
+    lib = load("libsogl.so")
+    info = np.zeros(len(GRAPH_INFO), np.int64)
+    rc = lib.sogl_graph_check_plan(plan.h, precision, _ptr(info, C.c_int64))
+    if rc != OK:
+        raise SoglError(rc, lib.sogl_last_error().decode())
+    return dict(zip(GRAPH_INFO, (int(v) for v in info)))
+
+
 def canonical_ops(ops: np.ndarray) -> np.ndarray:
     """Reorder sogl_op fields into the fixture layout
     stage group seq src src2 op result result2."""
@@ -285,19 +333,7 @@ class Device:
                  block_rows: int, precision: int = FP64) -> int:
         """Load a raw op list (rows: src src2 op result result2 stage seq group).
         ltiles / utiles: (k, 3) arrays of block row, block col, id."""
-        keep = []
-        ops = np.ascontiguousarray(ops, np.int32).reshape(-1, 8)
-        ids = np.ascontiguousarray(input_ids, np.int32)
-        tiles = np.ascontiguousarray(input_tiles, np.float64)
-        lt = np.ascontiguousarray(ltiles, np.int32).reshape(-1, 3)
-        ut = np.ascontiguousarray(utiles, np.int32).reshape(-1, 3)
-        cols = [np.ascontiguousarray(lt[:, k]) for k in range(3)] + \
-               [np.ascontiguousarray(ut[:, k]) for k in range(3)]
-        keep += [ops, ids, tiles, lt, ut] + cols
-        d = FactorDesc(n_ids, block_rows, ops.ctypes.data, ops.shape[0], ids.ctypes.data,
-                       tiles.ctypes.data, ids.size, cols[0].ctypes.data, cols[1].ctypes.data,
-                       cols[2].ctypes.data, lt.shape[0], cols[3].ctypes.data, cols[4].ctypes.data,
-                       cols[5].ctypes.data, ut.shape[0], precision)
+        d, keep = make_desc(n_ids, ops, input_ids, input_tiles, ltiles, utiles, block_rows, precision)
         fid = C.c_int32()
         self._check(self.lib.sogl_factor_load(self.h, C.byref(d), C.byref(fid)))
         return fid.value
@@ -349,7 +385,7 @@ class Device:
         self._check(self.lib.sogl_factor_set_trace(self.h, fid, 1 if enable else 0))
 
     def get_trace(self, fid: int, n_tasks: int, times: bool = True):
-        """(tasks (n,4): kind, pairs, level, deps; times (n,4): taken, ready, done, hw id)"""
+        """(tasks (n,4): kind, pairs, level, deps; times (n,4): taken, ready, done, last wait end)"""
         tk = np.empty((n_tasks, 4), np.int64)
         tm = np.empty((n_tasks, 4), np.int64) if times else None
         self._check(self.lib.sogl_factor_get_trace(self.h, fid, _ptr(tk, C.c_int64),

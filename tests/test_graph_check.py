@@ -1,0 +1,82 @@
+"""Host-side checks of the executor's task graph (no GPU needed).
+
+Every plan's graph must pass sogl_graph_check (slot indices inside the pool, one
+writer per tile, every operand made at an earlier dependency level and waited for),
+and the graph transforms must keep the op counts they are built from: one GEMM pair
+per mul/mulneg op, one folded op per copy/neg/sub that became part of another task.
+Hand-built op lists with a planted mistake must be rejected with an error.
+"""
+import numpy as np
+import pytest
+
+import sogl
+import workloads as W
+
+CASES = {
+    "c1": lambda: W.config("c1"),
+    "disconnected_40": lambda: W.edge_disconnected(40),
+    "zerodiag_48": lambda: W.edge_zero_diag(48),
+    "unsym_3000": lambda: W.edge_unsym_pattern(3000),
+    "convdiff2d_128": lambda: W.convdiff2d(128),
+    "diffusion3d_20": lambda: W.diffusion_varcoef(20, 3, 5),
+    "random_band_2048": lambda: W.random_band(2048),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_plan_graphs_pass(name):
+    p = CASES[name]()
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
+    info = pl.info
+    for prec in (sogl.FP64, sogl.FP32):
+        g = sogl.graph_check(pl, prec)
+        assert g["pairs"] == info.n_mul + info.n_mulneg
+        # folded ops are sub ops (copy / neg / sub), never more than the plan has
+        assert 0 <= g["folded"] <= info.n_sub
+        assert g["inv_aliased"] <= info.n_lower_inv + info.n_upper_inv
+        # every unit op is a task, one more pair of a GEMM task, folded or aliased
+        assert g["tasks"] <= info.n_ops - g["folded"] - g["inv_aliased"]
+        assert g["levels"] > 0 and g["pool_tiles"] <= info.storage_count + 2 * info.block_rows + 1
+
+
+def test_c2_graph_passes():
+    p = W.config("c2")
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
+    g = sogl.graph_check(pl, sogl.FP32)
+    assert g["pairs"] == 1_000_386
+    assert g["levels"] < pl.info.n_stages  # fused tasks: fewer dependency levels than stages
+
+
+# ---- planted mistakes in hand-built op lists (ids 1, 2 are inputs)
+MUL, SUB, LU = 8, 4, 1
+
+
+def _check(ops, n_ids=8, block_rows=1, lt=None, ut=None):
+    tiles = np.zeros((2, 64, 64))
+    ids = np.array([1, 2], np.int32)
+    lt = np.array(lt if lt is not None else [[0, 0, 3]], np.int32)
+    ut = np.array(ut if ut is not None else [[0, 0, 4]], np.int32)
+    return sogl.graph_check_ops(n_ids, np.array(ops, np.int32), ids, tiles, lt, ut, block_rows)
+
+
+def test_valid_hand_built_graph():
+    g = _check([[1, 0, LU, 3, 4, 2, 0, 0]])
+    assert g["tasks"] >= 1
+
+
+def test_rejects_read_before_write():
+    # op 0 reads tile 5, which only op 1 writes
+    with pytest.raises(sogl.SoglError, match="neither an input"):
+        _check([[5, 2, MUL, 6, 0, 2, 0, 0], [1, 2, MUL, 5, 0, 3, 1, 0], [6, 0, LU, 3, 4, 4, 2, 0]])
+
+
+def test_rejects_write_after_read():
+    # tile 5 is read by op 1 and then accumulated into again by op 2
+    with pytest.raises(sogl.SoglError, match="after it was read"):
+        _check([[1, 2, MUL, 5, 0, 2, 0, 0], [5, 1, MUL, 6, 0, 3, 1, 0], [1, 1, MUL, 5, 0, 4, 2, 0],
+                [6, 0, LU, 3, 4, 5, 3, 0]])
+
+
+def test_rejects_two_writers():
+    with pytest.raises(sogl.SoglError, match="more than one non-accumulating"):
+        _check([[1, 0, LU, 3, 4, 2, 0, 0], [2, 0, LU, 3, 7, 3, 1, 0]])
