@@ -19,6 +19,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2]
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import re
@@ -45,7 +46,36 @@ WORKLOADS = {
            "1.05*sum|off|, nonsymmetric, 1 RHS U[-1,1] seed 1001", 1e-12),
     "c5": ("2D convection-diffusion 1000x1000 (n=1,000,000), 5-point upwind, v=(1,0.5), cell "
            "Peclet 10, 1 RHS U[-1,1] seed 4", 1e-10),
+    "c3": ("random nonsymmetric n=16,384 (the largest size whose fill fits the planner; n=262,144 "
+           "fills to ~550 GB), band half-width 32 + 4 random entries per row seed 2, N(0,1) values, "
+           "diagonal sum|a_ij|+1, 128 RHS N(0,1) seed 1002", 1e-12),
 }
+
+
+def compare_executors(dev, plan, per_op: bool):
+    """FP64 factor time of every executor on one resident factor (ms, best of 2), the
+    autotuner's choice, and the per-op executor (one launch per unit op) once."""
+    fid = dev.load(plan, sogl.FP64)
+    out = {}
+    dev.factor(fid, sogl.EXEC_GRAPH)  # capture outside the timed runs
+    for name, mode in (("persistent", sogl.EXEC_PERSISTENT), ("levels", sogl.EXEC_LEVELS),
+                       ("level_graph", sogl.EXEC_GRAPH)):
+        ts = []
+        for _ in range(2):
+            dev.factor(fid, mode)
+            ts.append(dev.stats(fid).exec_ms)
+        out[name + "_ms"] = round(min(ts), 3)
+    best, times = dev.autotune(fid, 2)
+    out["autotune_ms"] = {k: round(float(t), 3) for k, t in zip(sogl.TUNE_CANDIDATES, times)}
+    out["autotune_choice"] = sogl.TUNE_CANDIDATES[best]
+    if per_op:
+        t0 = time.perf_counter()
+        dev.factor(fid, sogl.EXEC_PER_OP)
+        out["per_op_ms"] = round(dev.stats(fid).exec_ms, 3)
+        out["per_op_wall_s"] = round(time.perf_counter() - t0, 3)
+        out["per_op_launches"] = int(plan.info.n_ops)
+    dev.release(fid)
+    return out
 
 
 def cpu_reference(prob, threads: int, timeout: float = 600.0):
@@ -129,14 +159,43 @@ def calibrate_copy(nbytes: int = 1 << 30, reps: int = 3):
     print(json.dumps({"copy_bytes_per_launch": nbytes, "launches": reps}))
 
 
+def run_serving(args):
+    """The serving stream (2D/3D variable-coefficient diffusion, 1-16 RHS, 30% reuse)
+    on one GPU: p50/p99 request latency and solves/s."""
+    import serve
+    reqs = serve.make_stream(args.requests, args.rate)
+    m, reqs = serve.serve(reqs, args.slots, sogl.FP64, progress=25)
+    worst = 0.0
+    for r in reqs:
+        prob = serve.matrix_problem(r.matrix)
+        worst = max(worst, sogl.oracle_residual(prob.n, prob.row, prob.col, prob.val, r.x, r.b))
+    line = {
+        "metric": "serving: p50/p99 request latency (ms) and solves/s",
+        "value": round(m["solves_per_s"], 3), "unit": "solves/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"c4: {args.requests} requests (sample of the 10k stream), Poisson arrivals "
+                               f"{args.rate}/s seed 3, 2D side U{{32..512}} / 3D side U{{16..64}} "
+                               "variable-coefficient diffusion (log-normal sigma 1), 1-16 RHS, 30% reuse",
+                   "slots": args.slots},
+        "serving": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in m.items()},
+        "max_oracle_residual": worst,
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS) + ["c4"])
+    ap.add_argument("--requests", type=int, default=200, help="c4: requests in the serving stream")
+    ap.add_argument("--rate", type=float, default=20.0, help="c4: mean arrivals per second")
+    ap.add_argument("--slots", type=int, default=4, help="c4: concurrent requests on the GPU")
     ap.add_argument("--exec", default="persistent", choices=["persistent", "levels"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--compare-executors", action="store_true",
+                    help="also time every executor (FP64) incl. one launch per unit op (default for c3)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy kernel 3 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -147,20 +206,27 @@ def main():
         print("bench.py measures one GPU: this build runs the factor on a single MI355X "
               "(multi-GPU sharding is a design note only); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.config == "c4":
+        run_serving(args)
+        return
     mode = sogl.EXEC_PERSISTENT if args.exec == "persistent" else sogl.EXEC_LEVELS
     desc, tol = WORKLOADS[args.config]
     prob = W.config(args.config)
     t0 = time.perf_counter()
-    plan = sogl.Plan(prob.n, prob.row, prob.col, prob.val, prob.b)
+    plan = sogl.Plan(prob.n, prob.row, prob.col, prob.val, prob.b if prob.b.ndim == 1 else prob.b[:, 0])
     t_plan = time.perf_counter() - t0
     info = plan.info
     gemm_flops = (info.n_mul + info.n_mulneg) * 2.0 * 64 ** 3
-    solve_bytes = {k: (info.n_l_tiles + info.n_u_tiles) * 64 * 64 * s + 2 * plan.N * 8
+    nrhs = 1 if prob.b.ndim == 1 else prob.b.shape[1]
+    solve_bytes = {k: (info.n_l_tiles + info.n_u_tiles) * 64 * 64 * s + 2 * plan.N * 8 * nrhs
                    for k, s in (("fp64", 8), ("fp32", 4))}
     dev = sogl.Device(0)
     paths = {}
     paths["fp32_ir"] = run_path(dev, plan, prob, sogl.FP32, tol, 30, args.steps, args.warmup, mode)
     paths["fp64"] = run_path(dev, plan, prob, sogl.FP64, tol, 0, args.steps, args.warmup, mode)
+    executors = None
+    if args.compare_executors or args.config == "c3":
+        executors = compare_executors(dev, plan, per_op=True)
     dev.close()
 
     # parity: vs the reference's FP64 x (golden fixture) and the FP64 oracle residual
@@ -188,13 +254,14 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = min(16, physical_cores())
-        ref = cpu_reference(prob, threads)
+        ref = cpu_reference(prob if nrhs == 1 else dataclasses.replace(prob, b=prob.b[:, 0].copy()), threads)
         if ref is not None:
             cpu = {"value": round(ref["factor_s"] + ref["solve_s"], 4), "unit": "s", "cores": threads,
                    "kind": "reference",
                    "sample": f"the full {args.config} workload, one factor+solve of the reference CLI "
                              f"(built from its sources) at OMP_NUM_THREADS={threads}; value = its "
-                             "'kernel time' + 'solve triangled'",
+                             "'kernel time' + 'solve triangled'" +
+                             ("" if nrhs == 1 else f" for the first of the {nrhs} RHS (the CLI solves one)"),
                    "factor_s": ref["factor_s"], "solve_s": ref["solve_s"],
                    "reorder_s": ref["reorder_s"], "plan_s": ref["plan_s"],
                    "max_rhs_error": ref["max_rhs_error"]}
@@ -224,6 +291,8 @@ def main():
         "fp32_ir": out_paths["fp32_ir"],
         "fp64": out_paths["fp64"],
     }
+    if executors is not None:
+        line["executors_fp64"] = executors
     print(json.dumps(line))
 
 

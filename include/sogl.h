@@ -136,6 +136,8 @@ int sogl_ctx_sync(sogl_ctx* ctx);
 #define SOGL_EXEC_PERSISTENT 0 /* one persistent kernel pulling ready tasks from a device queue */
 #define SOGL_EXEC_LEVELS 1     /* one fused launch per dependency level */
 #define SOGL_EXEC_PER_OP 2     /* one launch per unit op (the unfused comparison point) */
+#define SOGL_EXEC_GRAPH 3      /* the level schedule captured once into a hipGraph, then replayed */
+#define SOGL_EXEC_AUTO 4       /* the configuration sogl_factor_autotune picked (tunes on first use) */
 
 typedef struct sogl_factor_desc {
     int32_t n_ids;          /* storage_count */
@@ -168,6 +170,18 @@ int sogl_factor_load_plan(sogl_ctx* ctx, const sogl_plan* plan, int32_t precisio
  * called from decompose_solveLU (solver.cpp:106).  Re-running refactors from the
  * resident input tiles. */
 int sogl_factor_run(sogl_ctx* ctx, int32_t factor_id, int32_t exec_mode);
+/* Same with the persistent executor's queue settings: chunk = tasks taken per fetch
+ * (1..64), wg_per_cu = resident workgroups per CU (0: as many as fit). */
+int sogl_factor_run_config(sogl_ctx* ctx, int32_t factor_id, int32_t exec_mode, int32_t chunk,
+                           int32_t wg_per_cu);
+/* Autotuner: times each executor configuration `trials` times (best of) on this
+ * factor, keeps the fastest for SOGL_EXEC_AUTO and returns its index.  Candidates, in
+ * order: persistent chunk 1 / 2 / 4 at full occupancy, persistent chunk 1 at one
+ * workgroup per CU, levels, captured level graph.  times_ms (may be NULL) receives
+ * SOGL_TUNE_CANDIDATES executor times.  The factor is left computed. */
+#define SOGL_TUNE_CANDIDATES 6
+int sogl_factor_autotune(sogl_ctx* ctx, int32_t factor_id, int32_t trials, double* times_ms,
+                         int32_t* best);
 /* Triangular solve: replaces BlockPlanner::solve() (BlockPlanner.cpp:834-862), called
  * from decompose_solveLU (solver.cpp:115).  bp, xp: host arrays in the permuted padded
  * order, 64*block_rows x nrhs column-major.  y (may be NULL) receives the forward

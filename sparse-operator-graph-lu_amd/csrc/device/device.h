@@ -75,6 +75,10 @@ struct Factor {
     double growth_limit = 1e4;
     bool trace_on = false;
     DevBuf trace_d;
+    // captured level schedule (SOGL_EXEC_GRAPH) and the autotuner's choice
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int32_t tuned = -1;  // index into the tuning candidates, -1: not tuned
 };
 
 struct Context {

@@ -108,6 +108,7 @@ struct Args {
     int32_t ntasks;
     int32_t epoch;
     int32_t task_offset;  // levels executor: first task of the launch
+    int32_t chunk;        // persistent executor: tasks taken per fetch from the queue
     double growth_limit;
     long long* trace;     // optional: per task {taken, sources ready, done, last wait ended}
 };
@@ -199,12 +200,16 @@ __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
     for (;;) {
         if (tid == 0) {
             s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_task = s_abort ? a.ntasks : atomicAdd(a.ctrl, 1);
+            s_task = s_abort ? a.ntasks : atomicAdd(a.ctrl, a.chunk);
         }
         __syncthreads();
         // wave-uniform loop control: the exit branch must not depend on the lane
-        const int ti = __builtin_amdgcn_readfirstlane(s_task);
-        if (ti >= a.ntasks) break;
+        const int base = __builtin_amdgcn_readfirstlane(s_task);
+        if (base >= a.ntasks) break;
+        // a chunk is run in index order; every dependency points to a smaller index,
+        // so with all workgroups resident no two chunks can wait on each other
+        const int end = min(base + a.chunk, a.ntasks);
+      for (int ti = base; ti < end; ti++) {
         const DevTask t = a.tasks[ti];
         long long t_taken = 0, t_ready = 0, t_wait = 0;
         if (a.trace && tid == 0) t_taken = wall_clock64();
@@ -243,6 +248,7 @@ __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
         // keeps the publishing branch and the next fetch branch apart, so the
         // back-edge cannot join them into a lane-dependent inner loop
         __syncthreads();
+      }
     }
 }
 
@@ -848,6 +854,8 @@ int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
 }
 
 void free_factor(Factor* f) {
+    if (f->graph_exec) (void)hipGraphExecDestroy(f->graph_exec);
+    if (f->graph) (void)hipGraphDestroy(f->graph);
     for (DevBuf* b : {&f->pool, &f->tasks_d, &f->pairs_d, &f->deps_d, &f->ready_d, &f->ctrl_d,
                       &f->op_tasks_d, &f->op_pairs_d, &f->inputs_d, &f->l_rowptr, &f->l_col,
                       &f->l_slot, &f->u_rowptr, &f->u_col, &f->u_slot, &f->l_diag_d, &f->u_diag_d,
@@ -858,9 +866,10 @@ void free_factor(Factor* f) {
 }
 
 template <typename T, bool BF16>
-int launch_factor(Context* c, Factor* f, int mode) {
+int launch_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
     hipStream_t st = c->stream;
     Args a{};
+    a.chunk = std::max(1, chunk);
     a.pool = f->pool.p;
     a.ready = (int*)f->ready_d.p;
     a.ctrl = (int*)f->ctrl_d.p;
@@ -868,18 +877,7 @@ int launch_factor(Context* c, Factor* f, int mode) {
     a.growth_limit = f->growth_limit;
     a.deps = (const int32_t*)f->deps_d.p;
     a.trace = (mode == SOGL_EXEC_PERSISTENT && f->trace_on) ? (long long*)f->trace_d.p : nullptr;
-    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, st, a.ctrl);
-    HIP_TRY(c, hipEventRecord(c->ex0, st));  // executor alone, for kernel-level rates
-    if (mode == SOGL_EXEC_PERSISTENT) {
-        a.tasks = (const DevTask*)f->tasks_d.p;
-        a.pairs = (const Pair*)f->pairs_d.p;
-        a.ntasks = (int32_t)f->tasks.size();
-        int per_cu = 0;
-        HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_persistent<T, BF16>, NT, 0));
-        if (per_cu < 1) per_cu = 1;
-        int grid = std::max(1, std::min<int>(a.ntasks, c->num_cu * per_cu));
-        if (a.ntasks > 0) hipLaunchKernelGGL((k_factor_persistent<T, BF16>), dim3(grid), dim3(NT), 0, st, a);
-    } else if (mode == SOGL_EXEC_LEVELS) {
+    auto launch_levels = [&]() {
         a.tasks = (const DevTask*)f->tasks_d.p;
         a.pairs = (const Pair*)f->pairs_d.p;
         a.ntasks = (int32_t)f->tasks.size();
@@ -889,6 +887,35 @@ int launch_factor(Context* c, Factor* f, int mode) {
             a.task_offset = f->level_begin[v];
             hipLaunchKernelGGL((k_factor_batch<T, BF16, false>), dim3(cnt), dim3(NT), 0, st, a);
         }
+    };
+    if (mode == SOGL_EXEC_GRAPH && !f->graph_exec) {
+        // the level schedule, captured once per factor: the control reset is a kernel
+        // (no memset nodes) and every launch's arguments are fixed for the factor
+        HIP_TRY(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, st, a.ctrl);
+        launch_levels();
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &g);
+        if (e != hipSuccess) return check_hip(c, e, "level schedule capture");
+        f->graph = g;
+        HIP_TRY(c, hipGraphInstantiate(&f->graph_exec, g, nullptr, nullptr, 0));
+    }
+    if (mode != SOGL_EXEC_GRAPH) hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, st, a.ctrl);
+    HIP_TRY(c, hipEventRecord(c->ex0, st));  // executor alone, for kernel-level rates
+    if (mode == SOGL_EXEC_PERSISTENT) {
+        a.tasks = (const DevTask*)f->tasks_d.p;
+        a.pairs = (const Pair*)f->pairs_d.p;
+        a.ntasks = (int32_t)f->tasks.size();
+        int per_cu = 0;
+        HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_persistent<T, BF16>, NT, 0));
+        if (per_cu < 1) per_cu = 1;
+        if (wg_per_cu > 0) per_cu = std::min(per_cu, wg_per_cu);  // never more than fit: all must be resident
+        int grid = std::max(1, std::min<int>(a.ntasks, c->num_cu * per_cu));
+        if (a.ntasks > 0) hipLaunchKernelGGL((k_factor_persistent<T, BF16>), dim3(grid), dim3(NT), 0, st, a);
+    } else if (mode == SOGL_EXEC_LEVELS) {
+        launch_levels();
+    } else if (mode == SOGL_EXEC_GRAPH) {
+        HIP_TRY(c, hipGraphLaunch(f->graph_exec, st));
     } else if (mode == SOGL_EXEC_PER_OP) {
         if (BF16) return set_error(c, SOGL_ERR_UNSUPPORTED, "per-op executor has no bf16 variant");
         // results start at zero; every op then runs alone, C += A*B reading C
@@ -1041,19 +1068,18 @@ extern "C" int sogl_factor_load_plan(sogl_ctx* x, const sogl_plan* plan, int32_t
     return sogl_factor_load(x, &d, factor_id);
 }
 
-extern "C" int sogl_factor_run(sogl_ctx* x, int32_t factor_id, int32_t exec_mode) {
-    if (!x) return set_error(nullptr, SOGL_ERR_ARG, "null context");
-    Context* c = &x->c;
-    std::lock_guard<std::mutex> g(c->mu);
-    Factor* f = find_factor(c, factor_id);
-    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+namespace sogl_dev {
+namespace {
+// One factor run on the context stream with an explicit executor configuration
+// (caller holds the context lock).
+int run_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
     HIP_TRY(c, hipSetDevice(c->device));
     f->epoch++;
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     int rc;
-    if (f->precision == SOGL_FP64) rc = launch_factor<double, false>(c, f, exec_mode);
-    else if (f->precision == SOGL_FP32) rc = launch_factor<float, false>(c, f, exec_mode);
-    else rc = launch_factor<float, true>(c, f, exec_mode);
+    if (f->precision == SOGL_FP64) rc = launch_factor<double, false>(c, f, mode, chunk, wg_per_cu);
+    else if (f->precision == SOGL_FP32) rc = launch_factor<float, false>(c, f, mode, chunk, wg_per_cu);
+    else rc = launch_factor<float, true>(c, f, mode, chunk, wg_per_cu);
     if (rc != SOGL_OK) return rc;
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     HIP_TRY(c, hipEventSynchronize(c->ev1));
@@ -1069,6 +1095,81 @@ extern "C" int sogl_factor_run(sogl_ctx* x, int32_t factor_id, int32_t exec_mode
     std::memcpy(&gm, &ctrl[3], 4);
     f->max_growth = gm;
     if (ctrl[1]) return set_error(c, SOGL_ERR_TIMEOUT, "factor: a dependency wait exceeded its bound");
+    return SOGL_OK;
+}
+
+// Executor configurations the autotuner times, in this order.
+struct TuneCand {
+    int mode, chunk, wg_per_cu;
+};
+constexpr TuneCand kTune[SOGL_TUNE_CANDIDATES] = {
+    {SOGL_EXEC_PERSISTENT, 1, 0}, {SOGL_EXEC_PERSISTENT, 2, 0}, {SOGL_EXEC_PERSISTENT, 4, 0},
+    {SOGL_EXEC_PERSISTENT, 1, 1}, {SOGL_EXEC_LEVELS, 1, 0},     {SOGL_EXEC_GRAPH, 1, 0},
+};
+
+int autotune(Context* c, Factor* f, int trials, double* times_ms) {
+    trials = std::max(1, trials);
+    int best = -1;
+    double best_ms = 0;
+    for (int k = 0; k < SOGL_TUNE_CANDIDATES; k++) {
+        const TuneCand& q = kTune[k];
+        if (q.mode == SOGL_EXEC_GRAPH && !f->graph_exec) {  // capture outside the timed trials
+            int rc = run_factor(c, f, q.mode, q.chunk, q.wg_per_cu);
+            if (rc != SOGL_OK) return rc;
+        }
+        double t = 1e30;
+        for (int r = 0; r < trials; r++) {
+            int rc = run_factor(c, f, q.mode, q.chunk, q.wg_per_cu);
+            if (rc != SOGL_OK) return rc;
+            t = std::min(t, f->exec_ms);
+        }
+        if (times_ms) times_ms[k] = t;
+        if (best < 0 || t < best_ms) {
+            best = k;
+            best_ms = t;
+        }
+    }
+    f->tuned = best;
+    return SOGL_OK;
+}
+}  // namespace
+}  // namespace sogl_dev
+
+extern "C" int sogl_factor_run_config(sogl_ctx* x, int32_t factor_id, int32_t exec_mode, int32_t chunk,
+                                      int32_t wg_per_cu) {
+    if (!x) return set_error(nullptr, SOGL_ERR_ARG, "null context");
+    Context* c = &x->c;
+    std::lock_guard<std::mutex> g(c->mu);
+    Factor* f = find_factor(c, factor_id);
+    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+    if (chunk < 1 || chunk > 64 || wg_per_cu < 0) return set_error(c, SOGL_ERR_ARG, "bad executor configuration");
+    if (exec_mode == SOGL_EXEC_AUTO) {
+        if (f->tuned < 0) {
+            int rc = autotune(c, f, 1, nullptr);
+            if (rc != SOGL_OK) return rc;
+        }
+        const TuneCand& q = kTune[f->tuned];
+        return run_factor(c, f, q.mode, q.chunk, q.wg_per_cu);
+    }
+    if (exec_mode < SOGL_EXEC_PERSISTENT || exec_mode > SOGL_EXEC_GRAPH)
+        return set_error(c, SOGL_ERR_ARG, "unknown executor mode");
+    return run_factor(c, f, exec_mode, chunk, wg_per_cu);
+}
+
+extern "C" int sogl_factor_run(sogl_ctx* x, int32_t factor_id, int32_t exec_mode) {
+    return sogl_factor_run_config(x, factor_id, exec_mode, 1, 0);
+}
+
+extern "C" int sogl_factor_autotune(sogl_ctx* x, int32_t factor_id, int32_t trials, double* times_ms,
+                                    int32_t* best) {
+    if (!x) return set_error(nullptr, SOGL_ERR_ARG, "null context");
+    Context* c = &x->c;
+    std::lock_guard<std::mutex> g(c->mu);
+    Factor* f = find_factor(c, factor_id);
+    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+    int rc = autotune(c, f, trials, times_ms);
+    if (rc != SOGL_OK) return rc;
+    if (best) *best = f->tuned;
     return SOGL_OK;
 }
 

@@ -23,6 +23,8 @@
 namespace sogl_dev {
 
 constexpr long long SOLVE_SPIN_LIMIT = 20LL * 100 * 1000 * 1000;
+constexpr int TRSM_MIN_RHS = 16;   // from this many right-hand sides the block-row sums run on MFMA
+constexpr int TRSM_MAX_RHS = 256;  // columns per MFMA solve launch (4 panels of 64)
 
 struct SolveArgs {
     const void* pool;
@@ -157,6 +159,139 @@ __global__ __launch_bounds__(NT) void k_trsv_persistent(SolveArgs a) {
     }
 }
 
+// Many right-hand sides: the same block-row schedule with the sums on MFMA.  Row I's
+// right-hand sides go in panels of 64 columns; for each panel the workgroup keeps
+// R = B_I - sum_J L_IJ X_J as a 64x64 accumulator (wave w: quadrant (w/2, w%2) as 2x2
+// v_mfma_f64_16x16x4 blocks), each product staged through LDS like the factor's
+// GEMM (tiles of any storage precision widened to FP64), then X_I = Linv_II R.
+// Columns past nrhs read as zero and are not stored.
+template <typename T>
+__device__ inline void tile_to_lds_f64(const T* __restrict__ g, double* __restrict__ s, int ld, int tid) {
+    const int r = tid >> 2, c0 = (tid & 3) * 16;
+#pragma unroll
+    for (int q = 0; q < 16; q++) s[r * ld + c0 + q] = (double)g[r * TS + c0 + q];
+}
+
+// rows row0 .. row0+63 of the column-major vector block v (column stride ld),
+// columns col0 .. col0+63 (zero past ncol), into LDS row-major
+__device__ inline void panel_to_lds(const double* __restrict__ v, int64_t ld, int64_t row0, int col0, int ncol,
+                                    double* __restrict__ s, int lds_ld, int tid) {
+    const int r = tid & 63, cq = tid >> 6;  // lanes walk rows: coalesced within a column
+#pragma unroll 4
+    for (int k = 0; k < 16; k++) {
+        const int cc = cq * 16 + k;
+        const int col = col0 + cc;
+        s[r * lds_ld + cc] = col < ncol ? v[row0 + r + (int64_t)col * ld] : 0.0;
+    }
+}
+
+template <typename T, int NP>
+__global__ __launch_bounds__(NT, 2) void k_trsm_persistent(SolveArgs a) {
+    __shared__ __attribute__((aligned(16))) double la[TS * LDA];
+    __shared__ __attribute__((aligned(16))) double lb[TS * LDB];
+    __shared__ int s_row, s_abort;
+    typedef MfmaOp<double> M;
+    typedef M::acc_t acc_t;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32, li = lane & 15, lk = lane >> 4;
+    const T* pool = reinterpret_cast<const T*>(a.pool);
+    auto mma_lds = [&](acc_t (&acc)[2][2], double sg) {
+#pragma unroll 4
+        for (int kk = 0; kk < TS / 4; kk++) {
+            const int k = 4 * kk + lk;
+            double af[2], bf[2];
+#pragma unroll
+            for (int m = 0; m < 2; m++) af[m] = sg * la[(wr + m * 16 + li) * LDA + k];
+#pragma unroll
+            for (int n = 0; n < 2; n++) bf[n] = lb[k * LDB + wc + n * 16 + li];
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+                for (int n = 0; n < 2; n++) acc[m][n] = M::mma(af[m], bf[n], acc[m][n]);
+        }
+    };
+    for (;;) {
+        if (tid == 0) {
+            s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_row = s_abort ? a.nbr : atomicAdd(a.ctrl, 1);
+        }
+        __syncthreads();
+        const int q = __builtin_amdgcn_readfirstlane(s_row);  // wave-uniform loop control
+        if (q >= a.nbr) break;
+        const int I = a.forward ? q : a.nbr - 1 - q;
+        const int e0 = a.rowptr[I], e1 = a.rowptr[I + 1], cnt = e1 - e0;
+        const int64_t rowI = (int64_t)I * TS;
+        // R = B_I for every panel of 64 columns (NP panels cover nrhs)
+        acc_t acc[NP][2][2];
+#pragma unroll
+        for (int p = 0; p < NP; p++)
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+                for (int n = 0; n < 2; n++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int row = wr + m * 16 + M::row(lk, r), col = p * TS + wc + n * 16 + li;
+                        acc[p][m][n][r] = col < a.nrhs ? a.in[rowI + row + (int64_t)col * a.ld] : 0.0;
+                    }
+        // dependencies oldest first; each L tile is staged once for all panels
+        for (int i = 0; i < cnt; i++) {
+            const int e = a.forward ? e0 + i : e1 - 1 - i;
+            const int J = a.col[e];
+            if (tid == 0) {
+                spin_row(a, J);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
+            __syncthreads();  // also: the previous products are done with la / lb
+            tile_to_lds_f64(pool + (size_t)a.slot[e] * TE, la, LDA, tid);
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                if (p) __syncthreads();  // the previous panel's product is done with lb
+                panel_to_lds(a.out, a.ld, (int64_t)J * TS, p * TS, a.nrhs, lb, LDB, tid);
+                __syncthreads();
+                mma_lds(acc[p], -1.0);
+            }
+        }
+        // X_I = Dinv_II R, panel by panel: R goes through LDS as the right operand
+        __syncthreads();
+        tile_to_lds_f64(pool + (size_t)a.dinv[I] * TE, la, LDA, tid);
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            if (p) __syncthreads();
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+                for (int n = 0; n < 2; n++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        lb[(wr + m * 16 + M::row(lk, r)) * LDB + wc + n * 16 + li] = acc[p][m][n][r];
+            __syncthreads();
+            acc_t o[2][2];
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+                for (int n = 0; n < 2; n++) o[m][n] = acc_t{0, 0, 0, 0};
+            mma_lds(o, 1.0);
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+                for (int n = 0; n < 2; n++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int row = wr + m * 16 + M::row(lk, r), col = p * TS + wc + n * 16 + li;
+                        if (col < a.nrhs) a.out[rowI + row + (int64_t)col * a.ld] = o[m][n][r];
+                    }
+        }
+        __syncthreads();  // every wave's stores are complete (workgroup release)
+        if (tid == 0) {   // one device-scope release for the workgroup, then the flag
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_store(a.ready + I, a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();  // separates the publish branch from the next fetch branch
+    }
+}
+
 __global__ void k_solve_ctrl_reset(int* ctrl) {
     if (threadIdx.x < 4) ctrl[threadIdx.x] = 0;
 }
@@ -192,12 +327,27 @@ int launch_pass(Context* c, Factor* f, bool forward, const double* in, double* o
     a.ctrl = (int*)f->solve_ctrl.p;
     hipLaunchKernelGGL(k_solve_ctrl_reset, dim3(1), dim3(64), 0, c->stream, a.ctrl);
     int per_cu = 0;
-    const void* kern = nrhs == 1 ? (const void*)k_trsv_persistent<T, 1> : (const void*)k_trsv_persistent<T, 8>;
+    const bool mfma = nrhs >= TRSM_MIN_RHS;
+    const int np = (nrhs + TS - 1) / TS;  // panels of 64 columns (at most TRSM_MAX_RHS / 64)
+    const void* kern = !mfma ? (nrhs == 1 ? (const void*)k_trsv_persistent<T, 1> : (const void*)k_trsv_persistent<T, 8>)
+                       : np == 1 ? (const void*)k_trsm_persistent<T, 1>
+                       : np == 2 ? (const void*)k_trsm_persistent<T, 2>
+                       : np == 3 ? (const void*)k_trsm_persistent<T, 3> : (const void*)k_trsm_persistent<T, 4>;
     HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, 0));
     if (per_cu < 1) per_cu = 1;
     const int grid = std::max(1, std::min(a.nbr, c->num_cu * per_cu));
-    if (nrhs == 1) hipLaunchKernelGGL((k_trsv_persistent<T, 1>), dim3(grid), dim3(NT), 0, c->stream, a);
-    else hipLaunchKernelGGL((k_trsv_persistent<T, 8>), dim3(grid), dim3(NT), 0, c->stream, a);
+    if (!mfma) {
+        if (nrhs == 1) hipLaunchKernelGGL((k_trsv_persistent<T, 1>), dim3(grid), dim3(NT), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_trsv_persistent<T, 8>), dim3(grid), dim3(NT), 0, c->stream, a);
+    } else if (np == 1) {
+        hipLaunchKernelGGL((k_trsm_persistent<T, 1>), dim3(grid), dim3(NT), 0, c->stream, a);
+    } else if (np == 2) {
+        hipLaunchKernelGGL((k_trsm_persistent<T, 2>), dim3(grid), dim3(NT), 0, c->stream, a);
+    } else if (np == 3) {
+        hipLaunchKernelGGL((k_trsm_persistent<T, 3>), dim3(grid), dim3(NT), 0, c->stream, a);
+    } else {
+        hipLaunchKernelGGL((k_trsm_persistent<T, 4>), dim3(grid), dim3(NT), 0, c->stream, a);
+    }
     HIP_TRY(c, hipGetLastError());
     return SOGL_OK;
 }
@@ -214,12 +364,17 @@ int solve_device(Context* c, Factor* f, const double* d_b, double* d_x, double* 
         HIP_TRY(c, hipGetLastError());
     }
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-    if (f->elem == 8) {
-        if ((rc = launch_pass<double>(c, f, true, d_b, d_y, nrhs))) return rc;
-        if ((rc = launch_pass<double>(c, f, false, d_y, d_x, nrhs))) return rc;
-    } else {
-        if ((rc = launch_pass<float>(c, f, true, d_b, d_y, nrhs))) return rc;
-        if ((rc = launch_pass<float>(c, f, false, d_y, d_x, nrhs))) return rc;
+    const int64_t ld = (int64_t)R * TS;
+    for (int c0 = 0; c0 < nrhs; c0 += TRSM_MAX_RHS) {  // column groups of one MFMA solve launch
+        const int m = std::min(TRSM_MAX_RHS, nrhs - c0);
+        const int64_t off = (int64_t)c0 * ld;
+        if (f->elem == 8) {
+            if ((rc = launch_pass<double>(c, f, true, d_b + off, d_y + off, m))) return rc;
+            if ((rc = launch_pass<double>(c, f, false, d_y + off, d_x + off, m))) return rc;
+        } else {
+            if ((rc = launch_pass<float>(c, f, true, d_b + off, d_y + off, m))) return rc;
+            if ((rc = launch_pass<float>(c, f, false, d_y + off, d_x + off, m))) return rc;
+        }
     }
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     HIP_TRY(c, hipEventSynchronize(c->ev1));

@@ -17,7 +17,9 @@ BUILD = os.path.join(HERE, "build")
 
 OK = 0
 FP64, FP32, BF16 = 0, 1, 2
-EXEC_PERSISTENT, EXEC_LEVELS, EXEC_PER_OP = 0, 1, 2
+EXEC_PERSISTENT, EXEC_LEVELS, EXEC_PER_OP, EXEC_GRAPH, EXEC_AUTO = 0, 1, 2, 3, 4
+TUNE_CANDIDATES = ("persistent chunk 1", "persistent chunk 2", "persistent chunk 4",
+                   "persistent chunk 1, 1 workgroup/CU", "levels", "level graph")
 OP_NAMES = {0: "inv", 1: "lu", 2: "lowerInv", 3: "upperInv", 4: "sub", 5: "add", 6: "neg",
             7: "copy", 8: "mul", 9: "mulneg", 10: "llt", 11: "mult", 12: "noop"}
 
@@ -108,6 +110,8 @@ def load(name: str = "libsogl.so") -> C.CDLL:
             "sogl_factor_load_plan": (C.c_int, [vp, vp, i32, P(i32)]),
             "sogl_factor_load": (C.c_int, [vp, P(FactorDesc), P(i32)]),
             "sogl_factor_run": (C.c_int, [vp, i32, i32]),
+            "sogl_factor_run_config": (C.c_int, [vp, i32, i32, i32, i32]),
+            "sogl_factor_autotune": (C.c_int, [vp, i32, i32, P(dbl), P(i32)]),
             "sogl_solve": (C.c_int, [vp, i32, P(dbl), P(dbl), P(dbl), i32]),
             "sogl_solve_device": (C.c_int, [vp, i32, vp, vp, i32]),
             "sogl_factor_release": (C.c_int, [vp, i32]),
@@ -338,6 +342,16 @@ class Device:
         self._check(self.lib.sogl_factor_load(self.h, C.byref(d), C.byref(fid)))
         return fid.value
 
+    def factor_config(self, fid: int, mode: int = EXEC_PERSISTENT, chunk: int = 1, wg_per_cu: int = 0):
+        self._check(self.lib.sogl_factor_run_config(self.h, fid, mode, chunk, wg_per_cu))
+
+    def autotune(self, fid: int, trials: int = 1):
+        """(index of the fastest configuration in TUNE_CANDIDATES, executor ms per candidate)"""
+        t = np.zeros(len(TUNE_CANDIDATES))
+        best = C.c_int32()
+        self._check(self.lib.sogl_factor_autotune(self.h, fid, trials, _ptr(t, C.c_double), C.byref(best)))
+        return best.value, t
+
     def factor(self, fid: int, mode: int = EXEC_PERSISTENT):
         self._check(self.lib.sogl_factor_run(self.h, fid, mode))
 
@@ -438,6 +452,9 @@ class Device:
 
 
 def oracle_residual(n, row, col, val, x, b) -> float:
-    """||b - A x||_2 / (||A||_F ||x||_2 + ||b||_2), FP64."""
+    """||b - A x||_2 / (||A||_F ||x||_2 + ||b||_2), FP64; the largest over columns
+    when x and b are (n, k)."""
+    if np.ndim(x) == 2:
+        return max(oracle_residual(n, row, col, val, x[:, j], b[:, j]) for j in range(x.shape[1]))
     r = b - np.bincount(row, weights=val * x[col], minlength=n)
     return float(np.linalg.norm(r) / (np.linalg.norm(val) * np.linalg.norm(x) + np.linalg.norm(b)))

@@ -144,6 +144,59 @@ def test_executor_modes(dev, name):
     assert np.linalg.norm(xs[0] - xs[2]) / np.linalg.norm(xs[2]) < 1e-13
 
 
+def _factor_tiles(dev, pl, run):
+    fid = dev.load(pl, sogl.FP64)
+    run(fid)
+    lt = dev.factor_tiles(fid, 0, pl.info.n_l_tiles)
+    ut = dev.factor_tiles(fid, 1, pl.info.n_u_tiles)
+    return fid, lt, ut
+
+
+@pytest.mark.parametrize("name", ["c1", "diffusion3d_20"])
+def test_level_graph_replays(dev, name):
+    """The captured level schedule gives the levels executor's bits on every replay,
+    with eager work (a solve on the same stream) between the replays."""
+    p = CASES[name]()
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
+    fid0, l0, u0 = _factor_tiles(dev, pl, lambda f: dev.factor(f, sogl.EXEC_LEVELS))
+    dev.release(fid0)
+    fid = dev.load(pl, sogl.FP64)
+    bp = pl.permute_rhs(p.b)
+    for _ in range(3):
+        dev.factor(fid, sogl.EXEC_GRAPH)
+        assert np.array_equal(dev.factor_tiles(fid, 0, pl.info.n_l_tiles), l0)
+        assert np.array_equal(dev.factor_tiles(fid, 1, pl.info.n_u_tiles), u0)
+        dev.solve(fid, bp)
+    dev.release(fid)
+
+
+@pytest.mark.parametrize("chunk,wg", [(2, 0), (4, 0), (1, 1)])
+def test_persistent_queue_settings(dev, chunk, wg):
+    """Chunked fetches and one workgroup per CU change only the schedule: same bits."""
+    p = W.diffusion_varcoef(20, 3, 5)
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
+    fid0, l0, u0 = _factor_tiles(dev, pl, lambda f: dev.factor(f, sogl.EXEC_PERSISTENT))
+    dev.release(fid0)
+    fid, l1, u1 = _factor_tiles(dev, pl, lambda f: dev.factor_config(f, sogl.EXEC_PERSISTENT, chunk, wg))
+    dev.release(fid)
+    assert np.array_equal(l0, l1) and np.array_equal(u0, u1)
+
+
+def test_autotune_and_auto_mode(dev):
+    p = W.convdiff2d(128)
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
+    fid0, l0, u0 = _factor_tiles(dev, pl, lambda f: dev.factor(f, sogl.EXEC_LEVELS))
+    dev.release(fid0)
+    fid = dev.load(pl, sogl.FP64)
+    best, times = dev.autotune(fid, 1)
+    assert 0 <= best < len(sogl.TUNE_CANDIDATES) and np.all(times > 0)
+    assert times[best] == times.min()
+    dev.factor(fid, sogl.EXEC_AUTO)
+    assert np.array_equal(dev.factor_tiles(fid, 0, pl.info.n_l_tiles), l0)
+    assert np.array_equal(dev.factor_tiles(fid, 1, pl.info.n_u_tiles), u0)
+    dev.release(fid)
+
+
 def test_resident_factor_repeat_solves(dev):
     p = W.convdiff2d(128)
     pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
@@ -159,8 +212,11 @@ def test_resident_factor_repeat_solves(dev):
     assert sogl.oracle_residual(p.n, p.row, p.col, p.val, pl.unpermute(x2), b2) < 1e-15
 
 
-@pytest.mark.parametrize("nrhs", [3, 20])
+@pytest.mark.parametrize("nrhs", [3, 20, 70, 300])
 def test_multi_rhs(dev, nrhs):
+    """Few right-hand sides: the per-row dot-product kernel; from 16 on: block-row
+    sums on MFMA in panels of 64 columns (70 = one full and one partial panel; 300 =
+    a launch of 256 columns and one of 44)."""
     p = W.diffusion_varcoef(20, 3, 5)
     B = W.uniform_pm1(5, p.n * nrhs).reshape(nrhs, p.n).T.copy()
     pl = sogl.Plan(p.n, p.row, p.col, p.val, B[:, 0])
@@ -168,11 +224,32 @@ def test_multi_rhs(dev, nrhs):
     dev.factor(fid)
     Xp = dev.solve(fid, pl.permute_rhs(B))
     x0 = dev.solve(fid, pl.permute_rhs(B[:, 0]))
+    if nrhs < 16:
+        assert np.array_equal(Xp[:, 0], x0)  # columns are independent
+    else:
+        X17 = dev.solve(fid, pl.permute_rhs(B[:, :17]))
+        assert np.array_equal(Xp[:, :17], X17)  # columns are independent on MFMA too
+        assert np.linalg.norm(Xp[:, 0] - x0) / np.linalg.norm(x0) < 1e-14
     dev.release(fid)
-    assert np.array_equal(Xp[:, 0], x0)  # columns are independent
     X = pl.unpermute(Xp)
     for j in range(nrhs):
         assert sogl.oracle_residual(p.n, p.row, p.col, p.val, X[:, j], B[:, j]) < 1e-15
+
+
+def test_fp32_refined_many_rhs(dev):
+    """FP32 factor, 40 right-hand sides refined in FP64 to 1e-12 each (MFMA block solves)."""
+    p = W.convdiff2d(128)
+    nrhs = 40
+    B = W.uniform_pm1(21, p.n * nrhs).reshape(nrhs, p.n).T.copy()
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, B[:, 0])
+    fid = dev.load(pl, sogl.FP32)
+    dev.factor(fid)
+    dev.attach_refine(fid, pl)
+    X, it, res, rc = dev.solve_refined(fid, B, tol=1e-12, max_iter=20)
+    dev.release(fid)
+    assert rc == 0 and res <= 1e-12
+    for j in range(nrhs):
+        assert sogl.oracle_residual(p.n, p.row, p.col, p.val, X[:, j], B[:, j]) <= 1e-12
 
 
 @pytest.mark.parametrize("precision", [sogl.FP32, sogl.BF16])
@@ -247,3 +324,18 @@ def test_refined_device_resident_matches_host(dev):
     dev.release(fid)
     assert rc == 0 and rc_d == 0 and it_h == it_d
     assert np.array_equal(xh, xd)
+
+
+def test_serving_mix_matches_alone():
+    """Requests served concurrently (3 slots, resident factors, batched RHS) give the
+    same x as each request served alone; every x meets the FP64 residual bar."""
+    import serve
+    reqs = serve.make_stream(18, 2000.0, seed=5, max_side_2d=72, max_side_3d=20)
+    m, mixed = serve.serve(reqs, slots=3)
+    assert m["requests"] == 18 and m["resident_hits"] >= 1
+    for r in mixed:
+        alone = serve.make_stream(18, 2000.0, seed=5, max_side_2d=72, max_side_3d=20)[r.rid]
+        _, (a,) = serve.serve([alone], slots=1)
+        assert np.linalg.norm(a.x - r.x) <= 1e-13 * np.linalg.norm(a.x)
+        prob = serve.matrix_problem(r.matrix)
+        assert sogl.oracle_residual(prob.n, prob.row, prob.col, prob.val, r.x, r.b) < 1e-14
