@@ -210,9 +210,9 @@ int sogl_factor_get_tiles(sogl_ctx* ctx, int32_t factor_id, int32_t which, doubl
 /* Diagnostics: per-task device timestamps of the persistent executor.
  * tasks4[4t..]: kind (0 gemm, 1 sub, 2 lu, 3 lower inverse, 4 upper inverse), pair
  * count, dependency level, dependency count; times4[4t..]: wall-clock ticks (100 MHz)
- * when the task was taken, when its sources were ready (GEMM tasks: when taken; they
- * wait pair by pair), when it finished, and when its last dependency wait ended.
- * Arrays hold n_tasks * 4 entries. */
+ * when the task was taken, when its work was done (before publishing), when its
+ * outputs were published, and when its last dependency wait ended (GEMM tasks wait
+ * pair by pair).  Arrays hold n_tasks * 4 entries. */
 int sogl_factor_set_trace(sogl_ctx* ctx, int32_t factor_id, int32_t enable);
 int sogl_factor_get_trace(sogl_ctx* ctx, int32_t factor_id, int64_t* tasks4, int64_t* times4);
 /* Diagnostics: the executor's task graph.  n_deps / n_slots receive the sizes; with

@@ -110,7 +110,7 @@ struct Args {
     int32_t task_offset;  // levels executor: first task of the launch
     int32_t chunk;        // persistent executor: tasks taken per fetch from the queue
     double growth_limit;
-    long long* trace;     // optional: per task {taken, sources ready, done, last wait ended}
+    long long* trace;     // optional: per task {taken, work done, published, last wait ended}
 };
 
 // Waits (thread 0 polls, relaxed) until ready[s] >= epoch; on abort or timeout it
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
             }
             __syncthreads();
         }
-        if (a.trace && tid == 0) t_ready = t_wait = wall_clock64();
+        if (a.trace && tid == 0) t_wait = wall_clock64();
         if (tid == 0) s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         if (__builtin_amdgcn_readfirstlane(s_abort)) break;
@@ -228,6 +228,7 @@ __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
         // publish: every wave's stores complete (the barrier's workgroup release), then
         // thread 0 makes them visible at device scope once and raises the flags
         __syncthreads();
+        if (a.trace && tid == 0) t_ready = wall_clock64();  // work done, publish starts
         if (tid == 0) {
             release_for_workgroup();
             __hip_atomic_store(a.ready + t.out0, a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -29,6 +29,7 @@ from __future__ import annotations
 import collections
 import concurrent.futures as cf
 import dataclasses
+import sys
 import threading
 import time
 
@@ -337,7 +338,7 @@ class Server:
             self.completed += len(batch)
             if self.progress and self.completed // self.progress != (self.completed - len(batch)) // self.progress:
                 print(f"[serve] {self.completed} requests done at {t:.1f} s, {len(self.resident)} factors "
-                      f"resident, {self.evictions} evicted", flush=True)
+                      f"resident, {self.evictions} evicted", file=sys.stderr, flush=True)
 
 
 def serve(reqs: list[Request], slots: int = 4, precision: int = sogl.FP64, time_scale: float = 1.0,

@@ -399,7 +399,7 @@ class Device:
         self._check(self.lib.sogl_factor_set_trace(self.h, fid, 1 if enable else 0))
 
     def get_trace(self, fid: int, n_tasks: int, times: bool = True):
-        """(tasks (n,4): kind, pairs, level, deps; times (n,4): taken, ready, done, last wait end)"""
+        """(tasks (n,4): kind, pairs, level, deps; times (n,4): taken, work done, published, last wait end)"""
         tk = np.empty((n_tasks, 4), np.int64)
         tm = np.empty((n_tasks, 4), np.int64) if times else None
         self._check(self.lib.sogl_factor_get_trace(self.h, fid, _ptr(tk, C.c_int64),
