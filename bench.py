@@ -39,6 +39,10 @@ import workloads as W  # noqa: E402
 METRIC = "factor+solve s & block-GEMM TFLOP/s at rel. residual ≤1e-10, FP32+IR vs FP64"
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense MFMA TFLOP/s
 HBM_PEAK = 8000.0  # GB/s
+# HBM bytes per launch of the FP32 factor kernel, from two rocprofv3 --pmc runs
+# (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the scales a plain 1 GiB copy reads):
+# profiles/k_factor_persistent_c2_pmc_traffic.md
+TRAFFIC_FP32_FACTOR = {"c2": 23.256e9}
 
 WORKLOADS = {
     "c1": ("2D 5-point Poisson 64x64 (n=4,096), 1 RHS U[-1,1] seed 0", 1e-12),
@@ -306,7 +310,7 @@ def main():
                    "gemm_gflop": round(gemm_flops / 1e9, 3), "host_plan_s": round(t_plan, 3)},
         "roofline": {"bound": "mfma", "achieved": round(head["gemm_tflops"], 4), "peak": PEAK["fp32"],
                      "unit": "TFLOP/s", "frac": round(head["gemm_tflops"] / PEAK["fp32"], 6),
-                     "traffic": None,
+                     "traffic": TRAFFIC_FP32_FACTOR.get(args.config),
                      "kernel": "k_factor_persistent<float> (FP32 factor executor); achieved = "
                                "(mul+mulneg ops) x 2*64^3 flop / executor time from HIP events"},
         "cpu_baseline": cpu,
