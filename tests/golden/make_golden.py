@@ -40,7 +40,7 @@ CASES = {
     "convdiff2d_128": (lambda: W.convdiff2d(128), True, False),
     "diffusion3d_20": (lambda: W.diffusion_varcoef(20, 3, 5), True, False),
     "c2": (lambda: W.config("c2"), False, False),
-    "c5": (lambda: W.config("c5"), False, True),
+    "c5": (lambda: W.config("c5"), False, False),
 }
 
 

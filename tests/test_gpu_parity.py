@@ -294,6 +294,32 @@ def test_c2_full_size(dev, golden):
     assert np.linalg.norm(x32[idx] - ref) / np.linalg.norm(ref) < 1e-9
 
 
+def test_c5_full_size(dev, golden):
+    """2D convection-diffusion 1000^2 (n=1,000,000, 3.44M unit ops) at its real size:
+    FP64 vs the reference x (sample), FP32 factor + FP64 refinement to residual 1e-10.
+    The FP32 path's x error is the condition number times its residual (about 1e-6);
+    the bar for it is the residual."""
+    g = golden("c5")
+    p = W.config("c5")
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
+    idx, ref = g["xorig_idx"], g["xorig_sample"]
+    fid = dev.load(pl, sogl.FP64)
+    dev.factor(fid)
+    x = pl.unpermute(dev.solve(fid, pl.permute_rhs(p.b)))
+    dev.release(fid)
+    assert np.linalg.norm(x[idx] - ref) / np.linalg.norm(ref) < 1e-12
+    assert abs(np.linalg.norm(x) - float(g["xorig_norm"])) / float(g["xorig_norm"]) < 1e-12
+    assert sogl.oracle_residual(p.n, p.row, p.col, p.val, x, p.b) < 1e-16
+    fid = dev.load(pl, sogl.FP32)
+    dev.factor(fid)
+    dev.attach_refine(fid, pl)
+    x32, it, res, rc = dev.solve_refined(fid, p.b, tol=1e-10, max_iter=30)
+    dev.release(fid)
+    assert rc == 0 and res <= 1e-10
+    assert sogl.oracle_residual(p.n, p.row, p.col, p.val, x32, p.b) <= 1e-10
+    assert np.linalg.norm(x32[idx] - ref) / np.linalg.norm(ref) < 1e-4
+
+
 def test_drop_in_solveLU(dev):
     import ctypes as C
     p = W.config("c1")
