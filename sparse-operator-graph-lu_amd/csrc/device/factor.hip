@@ -31,6 +31,7 @@
 namespace sogl {
 void set_last_error(const std::string& s);
 bool plan_factor_desc(const sogl_plan* p, int32_t precision, sogl_factor_desc* d);
+bool plan_input_cells(const sogl_plan* p, const int64_t** begin, const uint16_t** pos, const double** val);
 }
 
 namespace sogl_dev {
@@ -267,6 +268,19 @@ __global__ void k_f64_to_f32(const double* in, float* out, size_t n) {
     for (; i < n; i += stride) out[i] = (float)in[i];
 }
 
+// Input tiles given as cells: tile t holds cells [begin[t], begin[t+1]), each an
+// in-tile position (row*64 + col, < 4096) and a value; positions are distinct within
+// a tile, so every store has one writer. The tiles are zeroed beforehand.
+template <typename T>
+__global__ void k_scatter_cells(T* pool, const int64_t* begin, const uint16_t* pos, const double* val,
+                                int64_t n_tiles) {
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        T* tile = pool + (size_t)t * TE;
+        const int64_t q1 = begin[t + 1];
+        for (int64_t q = begin[t] + threadIdx.x; q < q1; q += blockDim.x) tile[pos[q]] = (T)val[q];
+    }
+}
+
 __global__ void k_f32_to_f64(const float* in, double* out, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -284,6 +298,13 @@ __global__ void k_gather_tiles(const char* pool, const int32_t* slots, char* out
 
 // ------------------------------------------------------------------ task graph
 namespace {
+
+// Input tiles as cells (the plan's storage): see k_scatter_cells.
+struct InputCells {
+    const int64_t* begin = nullptr;  // n_inputs + 1
+    const uint16_t* pos = nullptr;
+    const double* val = nullptr;
+};
 
 // Host arrays of a task graph that the factor keeps only on the device.
 struct HostGraph {
@@ -770,7 +791,8 @@ int validate_graph(Context* c, const sogl_factor_desc* d, const Factor* f, const
 
 // Device arrays of a checked graph: pool, tasks, pairs, dependencies, flags, solve
 // structure, and the input tiles copied in.
-int upload_graph(Context* c, const sogl_factor_desc* d, Factor* f, const HostGraph& hg) {
+int upload_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells, Factor* f,
+                 const HostGraph& hg) {
     const std::vector<Pair>& pairs = hg.pairs;
     const std::vector<Pair>& op_pairs = hg.op_pairs;
     const std::vector<int32_t>& deps = hg.deps;
@@ -822,7 +844,39 @@ int upload_graph(Context* c, const sogl_factor_desc* d, Factor* f, const HostGra
     HIP_TRY(c, hipGetLastError());
     // input tiles: contiguous slots 0..n_inputs-1
     const size_t nin = (size_t)d->n_inputs * TE;
-    if (nin) {
+    if (nin && cells) {
+        // zero the input slots, then scatter the cells (FP64 values rounded on the device)
+        const int64_t ntl = d->n_inputs;
+        const int64_t ncell = cells->begin[ntl];
+        for (int64_t t = 0; t < ntl; t++)
+            if (cells->begin[t + 1] < cells->begin[t]) return set_error(c, SOGL_ERR_ARG, "bad input cells");
+        if (f->elem == 8)
+            hipLaunchKernelGGL(k_fill<double>, dim3(4096), dim3(256), 0, st, (double*)f->pool.p, nin, 0.0);
+        else
+            hipLaunchKernelGGL(k_fill<float>, dim3(4096), dim3(256), 0, st, (float*)f->pool.p, nin, 0.0f);
+        HIP_TRY(c, hipGetLastError());
+        DevBuf cb, cp, cv;
+        auto done = [&](int code) { dev_free(cb); dev_free(cp); dev_free(cv); return code; };
+        if ((rc = dev_alloc(c, cb, (size_t)(ntl + 1) * 8, "cell index")) ||
+            (rc = dev_alloc(c, cp, std::max<int64_t>(ncell, 1) * 2, "cell positions")) ||
+            (rc = dev_alloc(c, cv, std::max<int64_t>(ncell, 1) * 8, "cell values")))
+            return done(rc);
+        hipError_t e = hipMemcpy(cb.p, cells->begin, (size_t)(ntl + 1) * 8, hipMemcpyHostToDevice);
+        if (e == hipSuccess && ncell) e = hipMemcpy(cp.p, cells->pos, (size_t)ncell * 2, hipMemcpyHostToDevice);
+        if (e == hipSuccess && ncell) e = hipMemcpy(cv.p, cells->val, (size_t)ncell * 8, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return done(check_hip(c, e, "input cell upload"));
+        const unsigned grid = (unsigned)std::min<int64_t>(ntl, 65536);
+        if (f->elem == 8)
+            hipLaunchKernelGGL(k_scatter_cells<double>, dim3(grid), dim3(256), 0, st, (double*)f->pool.p,
+                               (const int64_t*)cb.p, (const uint16_t*)cp.p, (const double*)cv.p, ntl);
+        else
+            hipLaunchKernelGGL(k_scatter_cells<float>, dim3(grid), dim3(256), 0, st, (float*)f->pool.p,
+                               (const int64_t*)cb.p, (const uint16_t*)cp.p, (const double*)cv.p, ntl);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return done(check_hip(c, e, "input cell scatter"));
+        done(SOGL_OK);
+    } else if (nin) {
         if (f->elem == 8) {
             HIP_TRY(c, hipMemcpy(f->pool.p, d->input_tiles, nin * 8, hipMemcpyHostToDevice));
         } else {
@@ -846,11 +900,11 @@ int upload_graph(Context* c, const sogl_factor_desc* d, Factor* f, const HostGra
     return SOGL_OK;
 }
 
-int build_graph(Context* c, const sogl_factor_desc* d, Factor* f) {
+int build_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells, Factor* f) {
     HostGraph hg;
     int rc = build_host_graph(c, d, f, hg);
     if (rc == SOGL_OK) rc = validate_graph(c, d, f, hg);
-    if (rc == SOGL_OK) rc = upload_graph(c, d, f, hg);
+    if (rc == SOGL_OK) rc = upload_graph(c, d, cells, f, hg);
     return rc;
 }
 
@@ -1009,12 +1063,14 @@ extern "C" int sogl_ctx_sync(sogl_ctx* x) {
     return SOGL_OK;
 }
 
-extern "C" int sogl_factor_load(sogl_ctx* x, const sogl_factor_desc* d, int32_t* factor_id) {
+namespace {
+// Loads a factor whose input tiles are dense (desc) or cells (plan storage).
+int load_factor(sogl_ctx* x, const sogl_factor_desc* d, const InputCells* cells, int32_t* factor_id) {
     if (!x || !d || !factor_id) return set_error(x ? &x->c : nullptr, SOGL_ERR_ARG, "null argument");
     Context* c = &x->c;
     std::lock_guard<std::mutex> g(c->mu);
     if (d->n_ids <= 0 || d->n_ops < 0 || d->n_inputs < 0 || d->block_rows <= 0 ||
-        (d->n_ops && !d->ops) || (d->n_inputs && (!d->input_ids || !d->input_tiles)))
+        (d->n_ops && !d->ops) || (d->n_inputs && (!d->input_ids || (!d->input_tiles && !cells))))
         return set_error(c, SOGL_ERR_ARG, "bad factor description");
     if (d->precision != SOGL_FP64 && d->precision != SOGL_FP32 && d->precision != SOGL_BF16)
         return set_error(c, SOGL_ERR_ARG, "unknown precision");
@@ -1022,7 +1078,7 @@ extern "C" int sogl_factor_load(sogl_ctx* x, const sogl_factor_desc* d, int32_t*
     Factor* f = new Factor();
     f->precision = d->precision;
     f->elem = d->precision == SOGL_FP64 ? 8 : 4;
-    int rc = build_graph(c, d, f);
+    int rc = build_graph(c, d, cells, f);
     if (rc != SOGL_OK) {
         free_factor(f);
         return rc;
@@ -1031,6 +1087,11 @@ extern "C" int sogl_factor_load(sogl_ctx* x, const sogl_factor_desc* d, int32_t*
     c->factors[f->id] = f;
     *factor_id = f->id;
     return SOGL_OK;
+}
+}  // namespace
+
+extern "C" int sogl_factor_load(sogl_ctx* x, const sogl_factor_desc* d, int32_t* factor_id) {
+    return load_factor(x, d, nullptr, factor_id);
 }
 
 extern "C" int sogl_graph_check(const sogl_factor_desc* d, int64_t* info) {
@@ -1064,9 +1125,11 @@ extern "C" int sogl_graph_check_plan(const sogl_plan* plan, int32_t precision, i
 extern "C" int sogl_factor_load_plan(sogl_ctx* x, const sogl_plan* plan, int32_t precision,
                                      int32_t* factor_id) {
     sogl_factor_desc d;
-    if (!sogl::plan_factor_desc(plan, precision, &d))
+    InputCells cells;
+    if (!sogl::plan_factor_desc(plan, precision, &d) ||
+        !sogl::plan_input_cells(plan, &cells.begin, &cells.pos, &cells.val))
         return set_error(x ? &x->c : nullptr, SOGL_ERR_ARG, "null plan");
-    return sogl_factor_load(x, &d, factor_id);
+    return load_factor(x, &d, &cells, factor_id);
 }
 
 namespace sogl_dev {

@@ -108,6 +108,12 @@ struct PlanView {
     const int32_t* col;
     const double* val;
     int64_t nnz_orig;  // entries of the original matrix (the rest is padding)
+    int64_t n_inputs;  // input tiles: block coordinates and cells (position row*64+col, value)
+    const int32_t* in_bi;
+    const int32_t* in_bj;
+    const int64_t* cell_begin;
+    const uint16_t* cell_pos;
+    const double* cell_val;
 };
 bool plan_view(const sogl_plan* p, PlanView* v);
 void set_last_error(const std::string& s);
@@ -123,19 +129,38 @@ extern "C" int sogl_refine_attach(sogl_ctx* x, int32_t factor_id, const sogl_pla
     if (!sogl::plan_view(plan, &v)) return set_error(c, SOGL_ERR_ARG, "bad plan");
     if (v.N != (int64_t)f->block_rows * TS) return set_error(c, SOGL_ERR_ARG, "plan does not match factor");
     HIP_TRY(c, hipSetDevice(c->device));
-    // CSR of the permuted padded matrix
+    // CSR of the permuted padded matrix, taken from the input tiles' cells so that the
+    // residual uses the matrix the factor holds (where the COO repeats a position, the
+    // tiles keep one value; the rows >= n hold the padding diagonal only)
+    const int64_t nc = v.n_inputs ? v.cell_begin[v.n_inputs] : 0;
     std::vector<int64_t> ptr((size_t)v.N + 1, 0);
-    for (int64_t k = 0; k < v.nnz; k++) ptr[(size_t)v.row[k] + 1]++;
+    for (int64_t t = 0; t < v.n_inputs; t++)
+        for (int64_t q = v.cell_begin[t]; q < v.cell_begin[t + 1]; q++)
+            ptr[(size_t)((int64_t)v.in_bi[t] * TS + (v.cell_pos[q] >> 6)) + 1]++;
     for (int64_t i = 0; i < v.N; i++) ptr[(size_t)i + 1] += ptr[(size_t)i];
-    std::vector<int32_t> col((size_t)v.nnz);
-    std::vector<double> val((size_t)v.nnz);
+    std::vector<int32_t> col((size_t)nc);
+    std::vector<double> val((size_t)nc);
     std::vector<int64_t> pos(ptr.begin(), ptr.end() - 1);
+    for (int64_t t = 0; t < v.n_inputs; t++)
+        for (int64_t q = v.cell_begin[t]; q < v.cell_begin[t + 1]; q++) {
+            const int64_t r = (int64_t)v.in_bi[t] * TS + (v.cell_pos[q] >> 6);
+            const int64_t p = pos[(size_t)r]++;
+            col[(size_t)p] = v.in_bj[t] * TS + (v.cell_pos[q] & 63);
+            val[(size_t)p] = v.cell_val[q];
+        }
+    // each row by column, and the Frobenius norm over the rows of the original matrix
     double fro = 0;
-    for (int64_t k = 0; k < v.nnz; k++) {
-        const int64_t p = pos[(size_t)v.row[k]]++;
-        col[(size_t)p] = v.col[k];
-        val[(size_t)p] = v.val[k];
-        if (k < v.nnz_orig) fro += v.val[k] * v.val[k];
+    std::vector<std::pair<int32_t, double>> row;
+    for (int64_t i = 0; i < v.N; i++) {
+        const int64_t b = ptr[(size_t)i], e = ptr[(size_t)i + 1];
+        row.clear();
+        for (int64_t p = b; p < e; p++) row.emplace_back(col[(size_t)p], val[(size_t)p]);
+        std::sort(row.begin(), row.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+        for (int64_t p = b; p < e; p++) {
+            col[(size_t)p] = row[(size_t)(p - b)].first;
+            val[(size_t)p] = row[(size_t)(p - b)].second;
+            if (i < v.n) fro += val[(size_t)p] * val[(size_t)p];
+        }
     }
     int rc;
     if ((rc = dev_alloc(c, f->csr_ptr, ptr.size() * 8, "csr ptr")) ||
@@ -148,7 +173,7 @@ extern "C" int sogl_refine_attach(sogl_ctx* x, int32_t factor_id, const sogl_pla
         HIP_TRY(c, hipMemcpy(f->csr_val.p, val.data(), val.size() * 8, hipMemcpyHostToDevice));
     }
     f->n = v.n;
-    f->nnz = v.nnz;
+    f->nnz = nc;
     f->new_order.assign(v.new_order, v.new_order + v.n);
     if ((rc = dev_alloc(c, f->perm_d, (size_t)v.n * 4, "permutation"))) return rc;
     HIP_TRY(c, hipMemcpy(f->perm_d.p, v.new_order, (size_t)v.n * 4, hipMemcpyHostToDevice));

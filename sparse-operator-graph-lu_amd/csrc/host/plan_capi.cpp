@@ -38,6 +38,14 @@ struct PlanView {
     const int32_t* col;
     const double* val;
     int64_t nnz_orig;
+    // the matrix as the factor holds it: input tiles as cells (a COO position that
+    // repeats keeps one value, as in the tiles)
+    int64_t n_inputs;
+    const int32_t* in_bi;
+    const int32_t* in_bj;
+    const int64_t* cell_begin;
+    const uint16_t* cell_pos;
+    const double* cell_val;
 };
 bool plan_view(const sogl_plan* p, PlanView* v) {
     if (!p || !v) return false;
@@ -50,6 +58,19 @@ bool plan_view(const sogl_plan* p, PlanView* v) {
     v->col = P.pcol.data();
     v->val = P.pval.data();
     v->nnz_orig = v->nnz - (v->N - v->n);
+    v->n_inputs = (int64_t)P.inputIds.size();
+    v->in_bi = P.inputBi.data();
+    v->in_bj = P.inputBj.data();
+    v->cell_begin = P.inputCellBegin.data();
+    v->cell_pos = P.cellPos.data();
+    v->cell_val = P.cellVal.data();
+    return true;
+}
+bool plan_input_cells(const sogl_plan* p, const int64_t** begin, const uint16_t** pos, const double** val) {
+    if (!p) return false;
+    *begin = p->plan.inputCellBegin.data();
+    *pos = p->plan.cellPos.data();
+    *val = p->plan.cellVal.data();
     return true;
 }
 bool plan_factor_desc(const sogl_plan* p, int32_t precision, sogl_factor_desc* d) {
@@ -60,7 +81,7 @@ bool plan_factor_desc(const sogl_plan* p, int32_t precision, sogl_factor_desc* d
     d->ops = P.ops.data();
     d->n_ops = (int64_t)P.ops.size();
     d->input_ids = P.inputIds.data();
-    d->input_tiles = P.inputTiles.data();
+    d->input_tiles = nullptr;  // the plan keeps cells: see plan_input_cells
     d->n_inputs = (int64_t)P.inputIds.size();
     d->l_bi = P.lBi.data();
     d->l_bj = P.lBj.data();
@@ -224,7 +245,7 @@ extern "C" int sogl_plan_get_inputs(const sogl_plan* p, int32_t* ids, int32_t* b
     if (ids) std::memcpy(ids, P.inputIds.data(), n * sizeof(int32_t));
     if (bi) std::memcpy(bi, P.inputBi.data(), n * sizeof(int32_t));
     if (bj) std::memcpy(bj, P.inputBj.data(), n * sizeof(int32_t));
-    if (tiles) std::memcpy(tiles, P.inputTiles.data(), P.inputTiles.size() * sizeof(double));
+    if (tiles) P.input_tiles_dense(tiles);
     return SOGL_OK;
 }
 

@@ -407,7 +407,7 @@ struct Planner {
         // (vi) order by (stage, group, result, src, seq).  The reference buckets by
         // stage%700, sorts each bucket and merges by stage (:278-333); seq is unique,
         // so this is a total order and one global sort gives the same sequence.
-        std::sort(graph.begin(), graph.end(), [](const sogl_op& x, const sogl_op& y) {
+        std::stable_sort(graph.begin(), graph.end(), [](const sogl_op& x, const sogl_op& y) {
             if (x.stage != y.stage) return x.stage < y.stage;
             if (x.group != y.group) return x.group < y.group;
             if (x.result != y.result) return x.result < y.result;
@@ -578,20 +578,49 @@ void build_plan(const Coo& a, const double* b, const PlanOptions& opt, Plan& pla
         // ids follow first touch in that order, values scatter into dense tiles
         struct Cell { int row; int col; double val; };
         std::vector<Cell> cells(nnz);
-        for (size_t k = 0; k < nnz; k++) cells[k] = Cell{plan.prow[k], plan.pcol[k], plan.pval[k]};
-        std::sort(cells.begin(), cells.end(), [](const Cell& x, const Cell& y) {
-            if ((x.row >> 6) == (y.row >> 6)) return (x.col >> 6) < (y.col >> 6);
-            return (x.row >> 6) < (y.row >> 6);
-        });
-        size_t distinct = 0;
-        for (size_t k = 0; k < nnz; k++)
-            if (k == 0 || (cells[k].row >> 6) != (cells[k - 1].row >> 6) ||
-                (cells[k].col >> 6) != (cells[k - 1].col >> 6))
-                distinct++;
-        distinct += (size_t)cfg.blockRows;  // upper bound on padding diagonal tiles
-        plan.inputTiles.reserve(distinct * kTile * kTile);
-        std::vector<int64_t> tileOf;  // id -> index into inputTiles
-        auto tile_for = [&](int64_t bi, int64_t bj) -> double* {
+        // The reference orders the cells by tile only (BlockPlanner.cpp:1498,
+        // std::sort), so the order inside a tile is the sort's own and matters only where
+        // the COO repeats a position: the value stored last wins. Without repeats any
+        // order by tile stores the same tiles, and two stable counting passes (column
+        // tile, then row tile) do it in linear time; with repeats the same std::sort runs.
+        {
+            const size_t nb = (size_t)blockRows + 1;
+            std::vector<uint32_t> a(nnz), b(nnz);
+            std::vector<int64_t> cnt(nb + 1);
+            auto pass = [&](const int32_t* key, const std::vector<uint32_t>* in, std::vector<uint32_t>& out) {
+                std::fill(cnt.begin(), cnt.end(), 0);
+                for (size_t k = 0; k < nnz; k++) cnt[(size_t)(key[in ? (*in)[k] : k] >> 6) + 1]++;
+                for (size_t t = 0; t < nb; t++) cnt[t + 1] += cnt[t];
+                for (size_t k = 0; k < nnz; k++) {
+                    const uint32_t e = in ? (*in)[k] : (uint32_t)k;
+                    out[(size_t)cnt[(size_t)(key[e] >> 6)]++] = e;
+                }
+            };
+            pass(plan.pcol.data(), nullptr, a);
+            pass(plan.prow.data(), &a, b);
+            bool repeats = false;
+            std::vector<int64_t> seen(kTile * kTile, -1);
+            int64_t group = -1, prevKey = -1;
+            for (size_t k = 0; k < nnz && !repeats; k++) {
+                const int32_t r = plan.prow[b[k]], c = plan.pcol[b[k]];
+                const int64_t key = (int64_t)(r >> 6) * blockRows + (c >> 6);
+                if (key != prevKey) { prevKey = key; group++; }
+                int64_t& m = seen[(size_t)((r & 63) * kTile + (c & 63))];
+                if (m == group) repeats = true;
+                m = group;
+            }
+            if (!repeats) {
+                for (size_t k = 0; k < nnz; k++) cells[k] = Cell{plan.prow[b[k]], plan.pcol[b[k]], plan.pval[b[k]]};
+            } else {
+                for (size_t k = 0; k < nnz; k++) cells[k] = Cell{plan.prow[k], plan.pcol[k], plan.pval[k]};
+                std::sort(cells.begin(), cells.end(), [](const Cell& x, const Cell& y) {
+                    if ((x.row >> 6) == (y.row >> 6)) return (x.col >> 6) < (y.col >> 6);
+                    return (x.row >> 6) < (y.row >> 6);
+                });
+            }
+        }
+        std::vector<int64_t> tileOf;  // id -> input index
+        auto tile_for = [&](int64_t bi, int64_t bj) -> int32_t {
             int64_t key = bi * blockRows + bj;
             int64_t id = Planner::tree_get(fblocks, key);
             if (id == 0) {
@@ -600,22 +629,54 @@ void build_plan(const Coo& a, const double* b, const PlanOptions& opt, Plan& pla
                 plan.inputIds.push_back((int32_t)id);
                 plan.inputBi.push_back((int32_t)bi);
                 plan.inputBj.push_back((int32_t)bj);
-                plan.inputTiles.resize(plan.inputTiles.size() + (size_t)kTile * kTile, 0.0);
                 if ((int64_t)tileOf.size() <= id) tileOf.resize((size_t)id + 1, -1);
                 tileOf[(size_t)id] = (int64_t)plan.inputIds.size() - 1;
             }
-            return plan.inputTiles.data() + (size_t)tileOf[(size_t)id] * kTile * kTile;
+            return (int32_t)tileOf[(size_t)id];
         };
+        // cells in scatter order: the sorted COO, then the padding diagonal
+        const size_t npad = (size_t)(N - n);
+        std::vector<int32_t> ck(nnz + npad);
+        std::vector<uint16_t> cp(nnz + npad);
+        std::vector<double> cv(nnz + npad);
         for (size_t k = 0; k < nnz; k++) {
             const Cell& c = cells[k];
-            double* t = tile_for(c.row >> 6, c.col >> 6);
-            t[(c.row & 63) * kTile + (c.col & 63)] = c.val;
+            ck[k] = tile_for(c.row >> 6, c.col >> 6);
+            cp[k] = (uint16_t)((c.row & 63) * kTile + (c.col & 63));
+            cv[k] = c.val;
         }
         for (int64_t i = n; i < N; i++) {
             int64_t bi = i / blockSize;
             int64_t ri = i - bi * blockSize;
-            double* t = tile_for(bi, bi);
-            t[ri * kTile + ri] = 1.0;
+            const size_t k = nnz + (size_t)(i - n);
+            ck[k] = tile_for(bi, bi);
+            cp[k] = (uint16_t)(ri * kTile + ri);
+            cv[k] = 1.0;
+        }
+        // group by input tile keeping scatter order (counting sort), then keep only the
+        // last cell of each position, which is the value a dense scatter leaves
+        const size_t ntiles = plan.inputIds.size();
+        std::vector<int64_t> start(ntiles + 1, 0);
+        for (int32_t k : ck) start[(size_t)k + 1]++;
+        for (size_t t = 0; t < ntiles; t++) start[t + 1] += start[t];
+        std::vector<int64_t> fill(start.begin(), start.end() - 1);
+        std::vector<int64_t> order(ck.size());
+        for (size_t k = 0; k < ck.size(); k++) order[(size_t)fill[(size_t)ck[k]]++] = (int64_t)k;
+        plan.inputCellBegin.assign(ntiles + 1, 0);
+        plan.cellPos.reserve(ck.size());
+        plan.cellVal.reserve(ck.size());
+        std::vector<int32_t> lastAt(kTile * kTile, -1);
+        for (size_t t = 0; t < ntiles; t++) {
+            const int64_t b0 = start[t], b1 = start[t + 1];
+            for (int64_t q = b0; q < b1; q++) lastAt[cp[(size_t)order[(size_t)q]]] = (int32_t)q;
+            for (int64_t q = b0; q < b1; q++) {
+                const size_t k = (size_t)order[(size_t)q];
+                if (lastAt[cp[k]] != (int32_t)q) continue;  // overwritten later
+                plan.cellPos.push_back(cp[k]);
+                plan.cellVal.push_back(cv[k]);
+            }
+            for (int64_t q = b0; q < b1; q++) lastAt[cp[(size_t)order[(size_t)q]]] = -1;
+            plan.inputCellBegin[t + 1] = (int64_t)plan.cellPos.size();
         }
     }
     zoom_set(cblocks, fblocks, l2);
@@ -671,6 +732,15 @@ void build_plan(const Coo& a, const double* b, const PlanOptions& opt, Plan& pla
     plan.tPlan = std::chrono::duration<double>(clk::now() - tp).count();
     log += fmt_seconds("plan time: ", plan.tPlan);
     flush(mark);
+}
+
+void Plan::input_tiles_dense(double* out) const {
+    const size_t n = inputIds.size();
+    std::memset(out, 0, n * kTile * kTile * sizeof(double));
+    for (size_t t = 0; t < n; t++) {
+        double* tile = out + t * kTile * kTile;
+        for (int64_t q = inputCellBegin[t]; q < inputCellBegin[t + 1]; q++) tile[cellPos[(size_t)q]] = cellVal[(size_t)q];
+    }
 }
 
 void unpermute(const Plan& plan, const double* xperm, double* xorig) {

@@ -79,9 +79,16 @@ struct Plan {
     std::vector<int32_t> tileStage;  // per id: stage of its last write (data::stage)
     int32_t maxStage = 0;
 
-    // input tiles (fine ids in allocation order), dense 64x64 row-major
+    // input tiles (fine ids in allocation order) as their nonzero cells: the cells of
+    // input k are [inputCellBegin[k], inputCellBegin[k+1]), each a position
+    // row*64 + col inside the tile and a value, one cell per position (where the COO
+    // repeats an entry, the value scattered last in the reference's order).
     std::vector<int32_t> inputIds, inputBi, inputBj;
-    std::vector<double> inputTiles;
+    std::vector<int64_t> inputCellBegin;
+    std::vector<uint16_t> cellPos;
+    std::vector<double> cellVal;
+    // dense 64x64 row-major tiles of all inputs (n_inputs * 4096 doubles)
+    void input_tiles_dense(double* out) const;
 
     // factor tiles reachable from the L and U quadtrees
     std::vector<int32_t> lBi, lBj, lId, uBi, uBj, uId;

@@ -318,3 +318,27 @@ def edge_unsym_pattern(n: int = 3000, seed: int = 11) -> Problem:
     vals[diag] = rowsum[rows[diag]] + 1.0
     r, c, v = _sorted(rows, cols, vals)
     return Problem(f"unsym_{n}", n, r, c, v, uniform_pm1(seed + 2, n))
+
+
+def edge_repeated_entries(n: int = 1200, seed: int = 13) -> Problem:
+    """Band matrix whose COO repeats about one position in eight with another value,
+    in a scattered (unsorted) order: the reader keeps repeats, and the tile builder's
+    sort decides which value a tile keeps. Every value keeps the diagonal dominant."""
+    k = 5
+    ri = np.repeat(np.arange(n, dtype=np.int64), k)
+    off = (draws(seed, n * k) % np.uint64(81)).astype(np.int64) - 40
+    rj = np.clip(ri + off, 0, n - 1)
+    rows = np.concatenate([np.arange(n), ri])
+    cols = np.concatenate([np.arange(n), rj])
+    key = rows * n + cols
+    _, first = np.unique(key, return_index=True)
+    rows, cols = rows[first], cols[first]
+    rep = np.nonzero(draws(seed + 1, rows.size) % np.uint64(8) == 0)[0]
+    rows = np.concatenate([rows, rows[rep], rows[rep[::3]]])
+    cols = np.concatenate([cols, cols[rep], cols[rep[::3]]])
+    vals = 0.5 * uniform_pm1(seed + 2, rows.size)
+    diag = rows == cols
+    vals[diag] = 4.0 * k + 2.0 + uniform01(seed + 3, int(diag.sum()))
+    order = np.argsort(draws(seed + 4, rows.size), kind="stable")
+    return Problem(f"repeated_{n}", n, rows[order].astype(np.int32), cols[order].astype(np.int32),
+                   vals[order], uniform_pm1(seed + 5, n))

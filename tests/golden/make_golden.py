@@ -4,7 +4,7 @@ Runs only where the reference sources are present: `make -C oracle ref`
 compiles them (in place, read-only) into oracle/_ref, and its harness
 oracle/_ref/ref_dump replays SOGLU::solveLU step by step on each workload,
 dumping the permutation, the coarse and fine op lists, tile stages, input
-tile map, factor tile map and the FP64 solution.  This script packs those
+tile map and values, factor tile map and the FP64 solution.  This script packs those
 dumps into tests/golden/*.npz (allow_pickle=False) -- data only:
 
   small cases   full arrays
@@ -39,6 +39,7 @@ CASES = {
     "unsym_3000": (lambda: W.edge_unsym_pattern(3000), True, False),
     "convdiff2d_128": (lambda: W.convdiff2d(128), True, False),
     "diffusion3d_20": (lambda: W.diffusion_varcoef(20, 3, 5), True, False),
+    "repeated_1200": (lambda: W.edge_repeated_entries(1200), True, False),
     "c2": (lambda: W.config("c2"), False, False),
     "c5": (lambda: W.config("c5"), False, False),
 }
@@ -46,6 +47,10 @@ CASES = {
 
 def sha(a: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).astype("<i4").tobytes()).hexdigest()
+
+
+def sha_f64(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).astype("<f8").tobytes()).hexdigest()
 
 
 def run_case(name: str) -> None:
@@ -76,7 +81,7 @@ def run_case(name: str) -> None:
                    sha_perm=np.str_(sha(perm)), sha_ops=np.str_(sha(ops)),
                    sha_coarse_ops=np.str_(sha(cops)), sha_tiles=np.str_(sha(tiles)),
                    sha_inputs=np.str_(sha(inputs)), sha_ltiles=np.str_(sha(lt)),
-                   sha_utiles=np.str_(sha(ut)),
+                   sha_utiles=np.str_(sha(ut)), sha_input_vals=np.str_(sha_f64(rd("inputs.f64", np.float64))),
                    ref_t_factor=np.float64(float(meta["t_factor"])),
                    ref_t_solve=np.float64(float(meta["t_solve"])),
                    ref_stdout=np.str_(r.stdout))

@@ -152,6 +152,25 @@ def _factor_tiles(dev, pl, run):
     return fid, lt, ut
 
 
+@pytest.mark.parametrize("name", ["c1", "repeated_1200"])
+@pytest.mark.parametrize("precision", [sogl.FP64, sogl.FP32])
+def test_plan_cells_match_dense_inputs(dev, name, precision):
+    """A plan loads its input tiles as cells scattered on the device; the same tiles
+    given densely through the factor description must give the same factor bits."""
+    p = W.edge_repeated_entries(1200) if name == "repeated_1200" else CASES[name]()
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
+    ids, _, _, tiles = pl.inputs()
+    f1 = dev.load(pl, precision)
+    f2 = dev.load_ops(pl.info.storage_count, pl.ops(0), ids, tiles, np.stack(pl.factor_tiles(0), 1),
+                      np.stack(pl.factor_tiles(1), 1), pl.info.block_rows, precision)
+    out = []
+    for fid in (f1, f2):
+        dev.factor(fid)
+        out.append((dev.factor_tiles(fid, 0, pl.info.n_l_tiles), dev.factor_tiles(fid, 1, pl.info.n_u_tiles)))
+        dev.release(fid)
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 @pytest.mark.parametrize("name", ["c1", "diffusion3d_20"])
 def test_level_graph_replays(dev, name):
     """The captured level schedule gives the levels executor's bits on every replay,
@@ -267,6 +286,27 @@ def test_low_precision_refined(dev, precision, name, golden):
     # forward error is bounded by cond(A) x residual: cond(A) ~ 2e3 for these grids
     ref = golden(name)["xorig"]
     assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-7
+
+
+def test_repeated_entries_refined(dev, golden):
+    """Where the COO repeats a position, the tiles keep one value (the one the
+    reference's tile builder stores last); the FP64 refinement residual uses the same
+    matrix, so FP32 + refinement converges to the reference's FP64 solution."""
+    p = W.edge_repeated_entries(1200)
+    ref = golden("repeated_1200")["xorig"]
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
+    fid = dev.load(pl, sogl.FP64)
+    dev.factor(fid)
+    x64 = pl.unpermute(dev.solve(fid, pl.permute_rhs(p.b)))
+    dev.release(fid)
+    assert np.linalg.norm(x64 - ref) / np.linalg.norm(ref) < 1e-13
+    fid = dev.load(pl, sogl.FP32)
+    dev.factor(fid)
+    dev.attach_refine(fid, pl)
+    x, it, res, rc = dev.solve_refined(fid, p.b, tol=1e-13, max_iter=40)
+    dev.release(fid)
+    assert rc == 0, (it, res)
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-12
 
 
 @pytest.mark.slow

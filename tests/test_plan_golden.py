@@ -17,11 +17,16 @@ SMALL = {
     "unsym_3000": lambda: W.edge_unsym_pattern(3000),
     "convdiff2d_128": lambda: W.convdiff2d(128),
     "diffusion3d_20": lambda: W.diffusion_varcoef(20, 3, 5),
+    "repeated_1200": lambda: W.edge_repeated_entries(1200),
 }
 
 
 def sha(a):
     return hashlib.sha256(np.ascontiguousarray(a).astype("<i4").tobytes()).hexdigest()
+
+
+def sha_f64(a):
+    return hashlib.sha256(np.ascontiguousarray(a).astype("<f8").tobytes()).hexdigest()
 
 
 def plan_arrays(p):
@@ -34,7 +39,7 @@ def plan_arrays(p):
                coarse_ops=sogl.canonical_ops(pl.ops(1)),
                tiles=np.concatenate([[pl.info.storage_count], ls, st]).astype(np.int32),
                inputs=np.stack([ids, bi, bj], 1), ltiles=np.stack(pl.factor_tiles(0), 1),
-               utiles=np.stack(pl.factor_tiles(1), 1), bperm=pl.permute_rhs(b))
+               utiles=np.stack(pl.factor_tiles(1), 1), bperm=pl.permute_rhs(b), input_vals=tiles)
     return pl, out
 
 
@@ -45,6 +50,8 @@ def test_small_bit_exact(name, golden):
     for k in ("perm", "ops", "coarse_ops", "tiles", "inputs", "ltiles", "utiles"):
         assert np.array_equal(a[k], g[k]), f"{name}: {k} differs from the reference"
     assert np.array_equal(a["bperm"], g["bperm"])
+    # input tile values, bit for bit (with repeated COO positions: the same value kept)
+    assert sha_f64(a["input_vals"]) == str(g["sha_input_vals"]), f"{name}: input tile values differ"
 
 
 @pytest.mark.parametrize("name", ["c2", "c5"])
@@ -55,6 +62,7 @@ def test_large_hashes(name, golden):
     assert pl.info.storage_count == int(g["storage"])
     for k in ("perm", "ops", "coarse_ops", "tiles", "inputs", "ltiles", "utiles"):
         assert sha(a[k]) == str(g["sha_" + k]), f"{name}: {k} hash differs from the reference"
+    assert sha_f64(a["input_vals"]) == str(g["sha_input_vals"]), f"{name}: input tile values differ"
 
 
 def test_log_lines_match_reference(golden):
