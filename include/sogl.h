@@ -95,6 +95,20 @@ int sogl_plan_create(int32_t dim, int64_t valcount, int32_t symmetric,
                      const int32_t* index_i, const int32_t* index_j,
                      const double* vals, const double* b, int32_t verbose,
                      sogl_plan** out);
+/* Plan cache support (no reference counterpart: the reference plans every solve from
+ * scratch, GPSOrder.cpp:494-635 and BlockPlanner.cpp:103-374, 1332-1460).
+ * sogl_plan_create_like gives the same plan as sogl_plan_create (general matrix,
+ * quiet) for a matrix with the same sparsity pattern as `like` -- the same dim and
+ * the same (index_i, index_j) sequence -- by reusing `like`'s ordering, op list and
+ * tile maps; only values, b and input tiles are new.  Where the pattern differs, or
+ * the value-dependent sortInBlock step (GPSOrder.cpp:55-150) reorders for either
+ * matrix, it plans from scratch.  *reused (may be NULL) is 1 when the plan was reused.
+ * `like` may be NULL.  sogl_pattern_hash keys a cache of plans by pattern. */
+int sogl_plan_create_like(const sogl_plan* like, int32_t dim, int64_t valcount,
+                          const int32_t* index_i, const int32_t* index_j, const double* vals,
+                          const double* b, sogl_plan** out, int32_t* reused);
+int sogl_pattern_hash(int32_t dim, int64_t valcount, const int32_t* index_i,
+                      const int32_t* index_j, uint64_t* hash);
 void sogl_plan_destroy(sogl_plan* p);
 const char* sogl_plan_log(const sogl_plan* p);  /* reference stdout lines, or last error */
 int sogl_plan_get_info(const sogl_plan* p, sogl_plan_info* info);

@@ -331,10 +331,19 @@ void gps_reorder(int n, std::vector<int32_t>& prow, std::vector<int32_t>& pcol,
 // GOrder::sortInBlock (GPSOrder.cpp:55-150).  For every full 64-block whose first 32
 // diagonal entries include one with |d| <= 1e-8, a compare-swap network moves larger
 // signed diagonals forward; the block-local permutation composes into the ordering.
-void sort_in_block(int n, int blockSize, std::vector<int32_t>& prow, std::vector<int32_t>& pcol,
+// The block test of GOrder::sortInBlock (GPSOrder.cpp:79-90): a block strictly inside
+// [0, n) is reordered when one of its first blockSize/2 diagonal entries has |d| <= 1e-8.
+bool block_needs_sort(const double* diag, int i, int n, int blockSize) {
+    const double TINY = 1e-8;
+    if (i + blockSize >= n) return false;
+    for (int j = 0; j < blockSize / 2; j++)
+        if (!(diag[i + j] > TINY || diag[i + j] < -TINY)) return true;
+    return false;
+}
+
+bool sort_in_block(int n, int blockSize, std::vector<int32_t>& prow, std::vector<int32_t>& pcol,
                    const std::vector<double>& pval, std::vector<double>* bperm,
                    std::vector<int32_t>& newOrder, std::vector<int32_t>& reverseOrder) {
-    const double TINY = 1e-8;
     std::vector<double> tmpv((size_t)n, 0.0);
     std::vector<int> tmpi((size_t)n, -1);
     for (size_t k = 0; k < prow.size(); k++)
@@ -343,14 +352,8 @@ void sort_in_block(int n, int blockSize, std::vector<int32_t>& prow, std::vector
         if (tmpi[(size_t)i] == -1) tmpi[(size_t)i] = i;
     bool any = false;
     for (int i = 0; i < n; i += blockSize) {
-        if (i + blockSize >= n) continue;  // GPSOrder.cpp:79: only blocks strictly inside
+        if (!block_needs_sort(tmpv.data(), i, n, blockSize)) continue;
         int step = blockSize / 2;
-        int zcount = 0;
-        for (int j = 0; j < step; j++) {
-            if (tmpv[(size_t)(i + j)] > TINY || tmpv[(size_t)(i + j)] < -TINY) continue;
-            zcount++;
-        }
-        if (zcount == 0) continue;
         any = true;
         while (step > 0) {
             for (int j = 0; j < step; j++) {
@@ -363,7 +366,7 @@ void sort_in_block(int n, int blockSize, std::vector<int32_t>& prow, std::vector
             step /= 2;
         }
     }
-    if (!any) return;  // identity: nothing below changes
+    if (!any) return false;  // identity: nothing below changes
     std::vector<int> tmpn((size_t)n, -1);
     for (int i = 0; i < n; i++) tmpn[(size_t)tmpi[(size_t)i]] = i;
     for (size_t k = 0; k < prow.size(); k++) {
@@ -381,6 +384,7 @@ void sort_in_block(int n, int blockSize, std::vector<int32_t>& prow, std::vector
         newOrder[(size_t)i] = norder[(size_t)i];
         reverseOrder[(size_t)norder[(size_t)i]] = i;
     }
+    return true;
 }
 
 }  // namespace sogl

@@ -175,6 +175,44 @@ extern "C" int sogl_plan_create(int32_t dim, int64_t valcount, int32_t symmetric
     return SOGL_OK;
 }
 
+extern "C" int sogl_plan_create_like(const sogl_plan* like, int32_t dim, int64_t valcount,
+                                     const int32_t* index_i, const int32_t* index_j, const double* vals,
+                                     const double* b, sogl_plan** out, int32_t* reused) {
+    if (reused) *reused = 0;
+    if (!out) return fail(SOGL_ERR_ARG, "null output");
+    *out = nullptr;
+    if (dim <= 0 || valcount < 0 || (valcount > 0 && (!index_i || !index_j || !vals)))
+        return fail(SOGL_ERR_ARG, "bad matrix arguments");
+    if (like) {
+        sogl_plan* p = new (std::nothrow) sogl_plan();
+        if (!p) return fail(SOGL_ERR_OOM, "out of host memory");
+        try {
+            sogl::Coo a;
+            a.n = dim;
+            a.row.assign(index_i, index_i + valcount);
+            a.col.assign(index_j, index_j + valcount);
+            a.val.assign(vals, vals + valcount);
+            if (sogl::rebind_plan(like->plan, a, b, p->plan)) {
+                if (reused) *reused = 1;
+                *out = p;
+                return SOGL_OK;
+            }
+        } catch (const std::bad_alloc&) {
+            delete p;
+            return fail(SOGL_ERR_OOM, "out of host memory while planning");
+        }
+        delete p;
+    }
+    return sogl_plan_create(dim, valcount, 0, index_i, index_j, vals, b, 0, out);
+}
+
+extern "C" int sogl_pattern_hash(int32_t dim, int64_t valcount, const int32_t* index_i,
+                                 const int32_t* index_j, uint64_t* hash) {
+    if (!hash || valcount < 0 || (valcount > 0 && (!index_i || !index_j))) return fail(SOGL_ERR_ARG, "bad arguments");
+    *hash = sogl::pattern_hash(dim, valcount, index_i, index_j);
+    return SOGL_OK;
+}
+
 extern "C" void sogl_plan_destroy(sogl_plan* p) { delete p; }
 
 extern "C" const char* sogl_plan_log(const sogl_plan* p) {

@@ -24,9 +24,10 @@ void gps_reorder(int n, std::vector<int32_t>& prow, std::vector<int32_t>& pcol,
                  std::vector<int32_t>& newOrder, std::vector<int32_t>& reverseOrder,
                  int& outLevels, int& outWidth, int& outStart, bool& symmetricPattern,
                  int threads, std::string& log);
-void sort_in_block(int n, int blockSize, std::vector<int32_t>& prow, std::vector<int32_t>& pcol,
+bool sort_in_block(int n, int blockSize, std::vector<int32_t>& prow, std::vector<int32_t>& pcol,
                    const std::vector<double>& pval, std::vector<double>* bperm,
                    std::vector<int32_t>& newOrder, std::vector<int32_t>& reverseOrder);
+bool block_needs_sort(const double* diag, int i, int n, int blockSize);
 
 // config::set + roundup (config.cpp:30-49)
 SizeConfig SizeConfig::set(int32_t dim) {
@@ -525,8 +526,9 @@ void build_plan(const Coo& a, const double* b, const PlanOptions& opt, Plan& pla
         std::vector<double> t(plan.bPerm);
         for (int32_t i = 0; i < n; i++) plan.bPerm[(size_t)i] = t[(size_t)plan.newOrder[(size_t)i]];
     }
-    sort_in_block(n, cfg.blockSize, plan.prow, plan.pcol, plan.pval, b ? &plan.bPerm : nullptr,
-                  plan.newOrder, plan.reverseOrder);
+    plan.nnzOrig = (int64_t)nnz;
+    plan.blockSorted = sort_in_block(n, cfg.blockSize, plan.prow, plan.pcol, plan.pval,
+                                     b ? &plan.bPerm : nullptr, plan.newOrder, plan.reverseOrder);
     plan.tReorder = std::chrono::duration<double>(clk::now() - t0).count();
     log += fmt_seconds("re Order time: ", plan.tReorder);
     flush(mark);
@@ -576,7 +578,8 @@ void build_plan(const Coo& a, const double* b, const PlanOptions& opt, Plan& pla
     {
         // fine tiles: cells sorted block-major with std::sort (cellCompare, operation.cpp:86-92);
         // ids follow first touch in that order, values scatter into dense tiles
-        struct Cell { int row; int col; double val; };
+        struct Cell { int row; int col; double val; uint32_t src; };
+        if (nnz + (size_t)(N - n) >= ((size_t)1 << 32)) throw std::runtime_error("too many matrix entries");
         std::vector<Cell> cells(nnz);
         // The reference orders the cells by tile only (BlockPlanner.cpp:1498,
         // std::sort), so the order inside a tile is the sort's own and matters only where
@@ -610,9 +613,9 @@ void build_plan(const Coo& a, const double* b, const PlanOptions& opt, Plan& pla
                 m = group;
             }
             if (!repeats) {
-                for (size_t k = 0; k < nnz; k++) cells[k] = Cell{plan.prow[b[k]], plan.pcol[b[k]], plan.pval[b[k]]};
+                for (size_t k = 0; k < nnz; k++) cells[k] = Cell{plan.prow[b[k]], plan.pcol[b[k]], plan.pval[b[k]], b[k]};
             } else {
-                for (size_t k = 0; k < nnz; k++) cells[k] = Cell{plan.prow[k], plan.pcol[k], plan.pval[k]};
+                for (size_t k = 0; k < nnz; k++) cells[k] = Cell{plan.prow[k], plan.pcol[k], plan.pval[k], (uint32_t)k};
                 std::sort(cells.begin(), cells.end(), [](const Cell& x, const Cell& y) {
                     if ((x.row >> 6) == (y.row >> 6)) return (x.col >> 6) < (y.col >> 6);
                     return (x.row >> 6) < (y.row >> 6);
@@ -639,11 +642,13 @@ void build_plan(const Coo& a, const double* b, const PlanOptions& opt, Plan& pla
         std::vector<int32_t> ck(nnz + npad);
         std::vector<uint16_t> cp(nnz + npad);
         std::vector<double> cv(nnz + npad);
+        std::vector<uint32_t> cs(nnz + npad);
         for (size_t k = 0; k < nnz; k++) {
             const Cell& c = cells[k];
             ck[k] = tile_for(c.row >> 6, c.col >> 6);
             cp[k] = (uint16_t)((c.row & 63) * kTile + (c.col & 63));
             cv[k] = c.val;
+            cs[k] = c.src;
         }
         for (int64_t i = n; i < N; i++) {
             int64_t bi = i / blockSize;
@@ -652,6 +657,7 @@ void build_plan(const Coo& a, const double* b, const PlanOptions& opt, Plan& pla
             ck[k] = tile_for(bi, bi);
             cp[k] = (uint16_t)(ri * kTile + ri);
             cv[k] = 1.0;
+            cs[k] = (uint32_t)k;  // the padding entries follow the COO in prow/pcol/pval
         }
         // group by input tile keeping scatter order (counting sort), then keep only the
         // last cell of each position, which is the value a dense scatter leaves
@@ -665,6 +671,7 @@ void build_plan(const Coo& a, const double* b, const PlanOptions& opt, Plan& pla
         plan.inputCellBegin.assign(ntiles + 1, 0);
         plan.cellPos.reserve(ck.size());
         plan.cellVal.reserve(ck.size());
+        plan.cellSrc.reserve(ck.size());
         std::vector<int32_t> lastAt(kTile * kTile, -1);
         for (size_t t = 0; t < ntiles; t++) {
             const int64_t b0 = start[t], b1 = start[t + 1];
@@ -674,6 +681,7 @@ void build_plan(const Coo& a, const double* b, const PlanOptions& opt, Plan& pla
                 if (lastAt[cp[k]] != (int32_t)q) continue;  // overwritten later
                 plan.cellPos.push_back(cp[k]);
                 plan.cellVal.push_back(cv[k]);
+                plan.cellSrc.push_back(cs[k]);
             }
             for (int64_t q = b0; q < b1; q++) lastAt[cp[(size_t)order[(size_t)q]]] = -1;
             plan.inputCellBegin[t + 1] = (int64_t)plan.cellPos.size();
@@ -732,6 +740,56 @@ void build_plan(const Coo& a, const double* b, const PlanOptions& opt, Plan& pla
     plan.tPlan = std::chrono::duration<double>(clk::now() - tp).count();
     log += fmt_seconds("plan time: ", plan.tPlan);
     flush(mark);
+}
+
+bool rebind_plan(const Plan& like, const Coo& a, const double* b, Plan& out) {
+    using clk = std::chrono::steady_clock;
+    auto t0 = clk::now();
+    const int32_t n = a.n;
+    const size_t nnz = a.val.size();
+    if (a.symmetric || n != like.cfg.mSize || like.blockSorted || (int64_t)nnz != like.nnzOrig ||
+        a.row.size() != nnz || a.col.size() != nnz)
+        return false;
+    // the same pattern: every entry lands where the earlier plan put it
+    const int32_t* rev = like.reverseOrder.data();
+    for (size_t k = 0; k < nnz; k++) {
+        const int32_t r = a.row[k], c = a.col[k];
+        if (r < 0 || r >= n || c < 0 || c >= n) return false;
+        if (rev[r] != like.prow[k] || rev[c] != like.pcol[k]) return false;
+    }
+    // sortInBlock stays the identity for the new diagonal (GPSOrder.cpp:55-150)
+    std::vector<double> diag((size_t)n, 0.0);
+    for (size_t k = 0; k < nnz; k++)
+        if (like.prow[k] == like.pcol[k]) diag[(size_t)like.prow[k]] = a.val[k];
+    for (int32_t i = 0; i < n; i += like.cfg.blockSize)
+        if (block_needs_sort(diag.data(), i, n, like.cfg.blockSize)) return false;
+    out = like;
+    for (size_t k = 0; k < nnz; k++) out.pval[k] = a.val[k];
+    const int64_t N = (int64_t)like.cfg.blockRows * like.cfg.blockSize;
+    out.bPerm.clear();
+    if (b) {
+        out.bPerm.assign((size_t)N, 1.0);
+        for (int32_t i = 0; i < n; i++) out.bPerm[(size_t)i] = b[like.newOrder[(size_t)i]];
+    }
+    for (size_t q = 0; q < out.cellSrc.size(); q++) out.cellVal[q] = out.pval[out.cellSrc[q]];
+    out.tReorder = 0;
+    out.tPlan = std::chrono::duration<double>(clk::now() - t0).count();
+    out.log = "plan reused: same sparsity pattern as an earlier plan\n" + fmt_seconds("plan time: ", out.tPlan);
+    return true;
+}
+
+uint64_t pattern_hash(int32_t n, int64_t nnz, const int32_t* row, const int32_t* col) {
+    // splitmix64 finaliser over (n, nnz, then each (row, col) pair), chained
+    auto mix = [](uint64_t z) {
+        z += 0x9e3779b97f4a7c15ull;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    };
+    uint64_t h = mix((uint64_t)(uint32_t)n ^ ((uint64_t)nnz << 32));
+    for (int64_t k = 0; k < nnz; k++)
+        h = mix(h ^ (((uint64_t)(uint32_t)row[k] << 32) | (uint32_t)col[k]));
+    return h;
 }
 
 void Plan::input_tiles_dense(double* out) const {

@@ -87,6 +87,7 @@ struct Plan {
     std::vector<int64_t> inputCellBegin;
     std::vector<uint16_t> cellPos;
     std::vector<double> cellVal;
+    std::vector<uint32_t> cellSrc;  // per cell: the entry of the padded permuted COO it holds
     // dense 64x64 row-major tiles of all inputs (n_inputs * 4096 doubles)
     void input_tiles_dense(double* out) const;
 
@@ -96,6 +97,9 @@ struct Plan {
     // permuted COO of the padded problem (indices after GPS + sortInBlock)
     std::vector<int32_t> prow, pcol;
     std::vector<double> pval;
+
+    int64_t nnzOrig = 0;        // COO entries given (the rest of prow/pcol/pval is padding)
+    bool blockSorted = false;   // sortInBlock changed the GPS order (it depends on values)
 
     std::vector<double> bPerm;  // permuted, padded right-hand side (may be empty)
     double tReorder = 0, tPlan = 0;
@@ -112,6 +116,17 @@ struct PlanOptions {
 // decompose_solveLU's planning (solver.cpp:50-100).  b may be null.
 // Throws std::runtime_error on inputs the reference cannot handle.
 void build_plan(const Coo& a, const double* b, const PlanOptions& opt, Plan& plan);
+
+// The plan of a matrix with the same sparsity pattern as `like` (the same n and the
+// same row/column sequence) from `like`'s ordering and op list: only the values, the
+// right-hand side and the input cells are new.  The result equals build_plan's.  The
+// ordering depends on values only through sortInBlock, so this returns false (and
+// leaves `out` untouched) where the pattern differs or sortInBlock would reorder for
+// either matrix; build_plan is the way then.  Has no reference counterpart.
+bool rebind_plan(const Plan& like, const Coo& a, const double* b, Plan& out);
+
+// 64-bit key of a sparsity pattern (n and the row/column sequence) for plan caches.
+uint64_t pattern_hash(int32_t n, int64_t nnz, const int32_t* row, const int32_t* col);
 
 // GOrder::reOrderResult (GPSOrder.cpp:41-53): x_orig[i] = x_perm[reverseOrder[i]].
 void unpermute(const Plan& plan, const double* xperm, double* xorig);

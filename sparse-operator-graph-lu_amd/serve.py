@@ -14,6 +14,9 @@ the C library releases the GIL while it works.
   memory, less a reserve, covers its pool, task graph and vectors. Otherwise
   least-recently-used idle factors are evicted first, and the request waits if
   that is not enough.
+* Plans by sparsity pattern: matrices of one grid shape share a pattern, and the
+  planner pool reuses the ordering and op list of an earlier plan of the same
+  pattern (the same plan as one made from scratch), so only values are new.
 * Every result is the same alone and in the mix: the factor and the solve of a
   request depend only on its own matrix and right-hand sides. The solve path is
   the same for the same number of columns; across batch widths it agrees to
@@ -98,9 +101,36 @@ class Resident:
     busy: int = 0
 
 
-def make_plan(key: tuple):
-    prob = matrix_problem(key)
-    return prob, sogl.Plan(prob.n, prob.row, prob.col, prob.val)
+class PatternPlans:
+    """Plans by sparsity pattern: a new matrix whose pattern was planned before reuses
+    that plan's ordering and op list (sogl_plan_create_like: the same plan, in a
+    fraction of the time). Least recently used patterns are dropped first once the
+    kept plans hold more than max_ops unit ops."""
+
+    def __init__(self, max_ops: int = 60_000_000):
+        self.max_ops = max_ops
+        self.lock = threading.Lock()
+        self.plans: collections.OrderedDict[int, sogl.Plan] = collections.OrderedDict()
+        self.reused = 0
+        self.planned = 0
+
+    def plan(self, prob: W.Problem) -> sogl.Plan:
+        h = sogl.pattern_hash(prob.n, prob.row, prob.col)
+        with self.lock:
+            like = self.plans.get(h)
+            if like is not None:
+                self.plans.move_to_end(h)
+        pl = sogl.Plan(prob.n, prob.row, prob.col, prob.val, like=like)
+        with self.lock:
+            self.reused += pl.reused
+            self.planned += 1
+            if h not in self.plans:
+                self.plans[h] = pl
+                total = sum(p.info.n_ops for p in self.plans.values())
+                while total > self.max_ops and len(self.plans) > 1:
+                    _, old = self.plans.popitem(last=False)
+                    total -= old.info.n_ops
+        return pl
 
 
 class Server:
@@ -112,6 +142,7 @@ class Server:
         self.planner = cf.ThreadPoolExecutor(max_workers=planners)
         self.plan_futs: collections.OrderedDict[tuple, cf.Future] = collections.OrderedDict()
         self.plan_cache_ops = plan_cache_ops
+        self.patterns = PatternPlans()
         self.precision = precision
         self.exec_mode = exec_mode
         self.reserve = int(reserve_gb * 1e9)
@@ -131,7 +162,7 @@ class Server:
         """The plan of a matrix, made by the planner pool (caller holds the lock)."""
         f = self.plan_futs.get(key)
         if f is None:
-            f = self.plan_futs[key] = self.planner.submit(make_plan, key)
+            f = self.plan_futs[key] = self.planner.submit(self._make_plan, key)
             # drop the oldest plans of matrices that are neither resident nor wanted
             total = sum(v.result()[1].info.n_ops for v in self.plan_futs.values() if v.done())
             for k in list(self.plan_futs):
@@ -146,6 +177,10 @@ class Server:
         else:
             self.plan_futs.move_to_end(key)
         return f
+
+    def _make_plan(self, key: tuple):
+        prob = matrix_problem(key)
+        return prob, self.patterns.plan(prob)
 
     def close(self):
         self.planner.shutdown(wait=True)
@@ -219,7 +254,8 @@ class Server:
                     p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
                     mean_ms=float(lat.mean() * 1e3), solves_per_s=sum(r.nrhs for r in reqs) / wall,
                     requests_per_s=len(reqs) / wall, resident_hits=sum(r.resident_hit for r in reqs),
-                    evictions=self.evictions, max_batch=max(r.batched_with for r in reqs))
+                    evictions=self.evictions, max_batch=max(r.batched_with for r in reqs),
+                    plans=self.patterns.planned, plans_reused=self.patterns.reused)
 
     def _take(self, s: int):
         """Next work for slot s (caller holds the lock): a batch of solves on one of its

@@ -81,6 +81,8 @@ def load(name: str = "libsogl.so") -> C.CDLL:
     P = C.POINTER
     sig = {
         "sogl_plan_create": (C.c_int, [i32, i64, i32, P(i32), P(i32), P(dbl), P(dbl), i32, P(vp)]),
+        "sogl_plan_create_like": (C.c_int, [vp, i32, i64, P(i32), P(i32), P(dbl), P(dbl), P(vp), P(i32)]),
+        "sogl_pattern_hash": (C.c_int, [i32, i64, P(i32), P(i32), P(C.c_uint64)]),
         "sogl_plan_destroy": (None, [vp]),
         "sogl_plan_log": (C.c_char_p, [vp]),
         "sogl_plan_get_info": (C.c_int, [vp, P(PlanInfo)]),
@@ -153,22 +155,42 @@ def ops_array(ops) -> np.ndarray:
         np.zeros((0, 8), np.int32)
 
 
+def pattern_hash(n: int, row, col, lib: str = "libsogl.so") -> int:
+    """64-bit key of a sparsity pattern (n and the row/column sequence)."""
+    row = np.ascontiguousarray(row, dtype=np.int32)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    h = C.c_uint64()
+    rc = load(lib).sogl_pattern_hash(n, row.size, _ptr(row, C.c_int32), _ptr(col, C.c_int32), C.byref(h))
+    if rc != OK:
+        raise SoglError(rc, load(lib).sogl_last_error().decode())
+    return h.value
+
+
 class Plan:
     """Host plan of one matrix: GPS ordering + staged 64x64 unit-op list."""
 
-    def __init__(self, n: int, row, col, val, b=None, verbose=False, lib: str = "libsogl.so"):
+    def __init__(self, n: int, row, col, val, b=None, verbose=False, lib: str = "libsogl.so",
+                 like: "Plan | None" = None):
+        """like: an earlier plan; where the matrix has its sparsity pattern, its ordering
+        and op list are reused (same result, see sogl_plan_create_like); self.reused
+        says whether they were."""
         self.lib = load(lib)
         row = np.ascontiguousarray(row, dtype=np.int32)
         col = np.ascontiguousarray(col, dtype=np.int32)
         val = np.ascontiguousarray(val, dtype=np.float64)
         bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
         h = C.c_void_p()
-        rc = self.lib.sogl_plan_create(n, val.size, 0, _ptr(row, C.c_int32), _ptr(col, C.c_int32),
-                                       _ptr(val, C.c_double),
-                                       None if bb is None else _ptr(bb, C.c_double),
-                                       1 if verbose else 0, C.byref(h))
+        bptr = None if bb is None else _ptr(bb, C.c_double)
+        reused = C.c_int32(0)
+        if like is not None:
+            rc = self.lib.sogl_plan_create_like(like.h, n, val.size, _ptr(row, C.c_int32), _ptr(col, C.c_int32),
+                                                _ptr(val, C.c_double), bptr, C.byref(h), C.byref(reused))
+        else:
+            rc = self.lib.sogl_plan_create(n, val.size, 0, _ptr(row, C.c_int32), _ptr(col, C.c_int32),
+                                           _ptr(val, C.c_double), bptr, 1 if verbose else 0, C.byref(h))
         if rc != OK:
             raise SoglError(rc, self.lib.sogl_last_error().decode())
+        self.reused = bool(reused.value)
         self.h = h
         self.n = n
         self.info = PlanInfo()
