@@ -98,3 +98,79 @@ def test_roundtrip_generated(tmp_path):
     assert np.array_equal(np.ctypeslib.as_array(v, (p.nnz,)), p.val)
     assert np.array_equal(np.ctypeslib.as_array(b, (p.n,)), p.b)  # %.17g round-trips exactly
     lib.sogl_mtx_free(h)
+
+
+def _restated(text):
+    """The per-line rules of mtx.cpp:44-117 in Python, for files whose numbers are in
+    plain forms (where float() and strtod agree)."""
+    import re
+    lines = text.split("\n")
+    if text.endswith("\n"):
+        lines = lines[:-1]
+    sym = "symmetric" in lines[0]
+    num_i = re.compile(r"\s*([+-]?\d+)")
+
+    def strtol(s):
+        m = num_i.match(s)
+        return (int(m.group(1)), s[m.end():]) if m else (0, s)
+    it = iter(lines)
+    n = None
+    for ln in it:
+        if len(ln) <= 3 or "%" in ln:
+            continue
+        r, rest = strtol(ln)
+        c, rest = strtol(rest)
+        n = min(r, c)
+        break
+    rows, cols, vals = [], [], []
+    num_f = re.compile(r"\s*([+-]?(\d+\.?\d*|\.\d+)([eE][+-]?\d+)?)")
+    for ln in it:
+        if len(ln) <= 3 or "%" in ln or len(ln) >= 1000:
+            continue
+        r, rest = strtol(ln)
+        c, rest = strtol(rest)
+        m = num_f.match(rest)
+        v = float(m.group(1)) if m else 0.0
+        if v == 0 or r > n or c > n:
+            continue
+        rows.append(r - 1)
+        cols.append(c - 1)
+        vals.append(v)
+    return n, sym, rows, cols, vals
+
+
+def test_chunked_parse_matches_line_rules(tmp_path):
+    """A file of several parallel chunks with every skip rule spread through it."""
+    rng = np.random.default_rng(5)
+    n = 3000
+    out = ["%%MatrixMarket matrix coordinate real general", "% c", f"{n} {n} 60000"]
+    for k in range(60000):
+        r, c = int(rng.integers(1, n + 1)), int(rng.integers(1, n + 1))
+        v = float(rng.normal())
+        kind = k % 97
+        if kind == 0:
+            out.append(f"{r} {c} 0")                     # explicit zero
+        elif kind == 1:
+            out.append(f"{r} {c} {v!r} % note")          # comment: skipped
+        elif kind == 2:
+            out.append("")                               # short
+        elif kind == 3:
+            out.append(f"{r} {c}")                       # no value: 0, dropped
+        elif kind == 4:
+            out.append(f"{r} {c} {v!r}\r")               # CRLF
+        elif kind == 5:
+            out.append(f"{r} {c} {v!r}" + " " * 1000)    # long line: skipped
+        elif kind == 6:
+            out.append(f"{n + 3} {c} {v!r}")             # beyond the size
+        elif kind == 7:
+            out.append(f"  {r}\t{c}   {v:.6e}")          # other spacing and form
+        else:
+            out.append(f"{r} {c} {v!r}")
+    text = "\n".join(out)  # no final line end
+    assert len(text) > 4 * 65536
+    rc, m = _read(tmp_path, text)
+    assert rc == 0
+    n2, sym, rows, cols, vals = _restated(text)
+    assert m["n"] == n2 and m["sym"] == int(sym)
+    assert m["row"].tolist() == rows and m["col"].tolist() == cols
+    assert np.array_equal(m["val"], np.array(vals))
