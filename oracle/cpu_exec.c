@@ -14,6 +14,10 @@
  *   oracle_inv_upper   column elimination from the bottom + long double row
  *                      scaling (the textbook inv_upper, 2804-2827)
  *   oracle_sub         r = a - b (2938-2947), copy / negate for missing operands
+ *   oracle_llt         Cholesky of a tile, l_jj = sqrt(max(a_jj - sum, 1e-20)) (the
+ *                      textbook lltdcmpSimple, 2599-2622)
+ *   oracle_gemm_t      C += A*B^T, also C += A*A^T (the textbook mat_mult 704-716,
+ *                      771-783)
  *   oracle_factor      BlockPlanner::calculate (BlockPlanner.cpp:376-651): stages in
  *                      order; inside a stage, ops grouped by result tile run in
  *                      list order (the reference's result%47 streams keep that
@@ -21,7 +25,9 @@
  *                      freed after their last reading stage
  *   oracle_solve       BlockPlanner::solve/lowerSolver/upperSolver (669-862):
  *                      block forward then backward substitution, dividing by
- *                      the diagonal entries like the leaf loops do
+ *                      the diagonal entries like the leaf loops do; with no U
+ *                      tiles (an L L^T factor) the backward pass runs on L^T
+ *                      (upperSolverT / updateVectorT, 741-797)
  *
  * Dense 64x64 row-major tiles replace the reference's 64x72 tiles with 8x8
  * masks; skipping a masked (zero) block is exact, so the only difference is
@@ -35,7 +41,8 @@
 #define TS 64
 #define TE (TS * TS)
 
-enum { OP_LU = 1, OP_LOWER_INV = 2, OP_UPPER_INV = 3, OP_SUB = 4, OP_MUL = 8, OP_MULNEG = 9 };
+enum { OP_LU = 1, OP_LOWER_INV = 2, OP_UPPER_INV = 3, OP_SUB = 4, OP_MUL = 8, OP_MULNEG = 9, OP_LLT = 10,
+       OP_MULT = 11 };
 
 typedef struct {
     int32_t src, src2, op, result, result2, stage, seq, group;
@@ -47,6 +54,34 @@ void oracle_gemm(const double* a, const double* b, double* c, int sign) {
             const double aik = sign * a[i * TS + k];
             for (int j = 0; j < TS; j++) c[i * TS + j] += aik * b[k * TS + j];
         }
+}
+
+void oracle_gemm_t(const double* a, const double* b, double* c) {
+    for (int i = 0; i < TS; i++)
+        for (int j = 0; j < TS; j++) {
+            double sum = 0;
+            for (int k = 0; k < TS; k++) sum += a[i * TS + k] * b[j * TS + k];
+            c[i * TS + j] += sum;
+        }
+}
+
+void oracle_llt(const double* a, double* l) {
+    const double TINY = 1e-20;
+    memset(l, 0, TE * sizeof(double));
+    for (int j = 0; j < TS; j++) {
+        double sum = 0;
+        for (int k = 0; k < j; k++) sum += l[j * TS + k] * l[j * TS + k];
+        double p = a[j * TS + j] - sum;
+        if (p < TINY) p = TINY;
+        p = sqrt(p);
+        l[j * TS + j] = p;
+        p = 1 / p;
+        for (int i = j + 1; i < TS; i++) {
+            double sun = 0;
+            for (int k = 0; k < j; k++) sun += l[i * TS + k] * l[j * TS + k];
+            l[i * TS + j] = p * (a[i * TS + j] - sun);
+        }
+    }
 }
 
 void oracle_sub(const double* a, const double* b, double* r) {
@@ -163,6 +198,8 @@ int oracle_factor(int32_t n_ids, const op_t* ops, int64_t n_ops, const int32_t* 
                     case OP_LU: if (a) oracle_lu(a, r, tiles[o->result2]); break;
                     case OP_LOWER_INV: if (a) oracle_inv_lower(a, r); break;
                     case OP_UPPER_INV: if (a) oracle_inv_upper(a, r); break;
+                    case OP_LLT: if (a) oracle_llt(a, r); break;
+                    case OP_MULT: if (a && b) oracle_gemm_t(a, b, r); break;
                     default: break;
                 }
             }
@@ -202,7 +239,8 @@ void oracle_free_tiles(double** tiles, int32_t n_ids) {
 /*
  * Block triangular solve of the permuted padded system, nrhs columns (column-major,
  * ld = 64*block_rows).  L / U tiles given as (bi, bj, id) triples; b is overwritten
- * with y (like the reference leaves y in data::b), x receives the solution.
+ * as the reference overwrites data::b (with y for a unit-diagonal L), x receives the
+ * solution.
  */
 int oracle_solve(int32_t block_rows, double** tiles, const int32_t* lbi, const int32_t* lbj,
                  const int32_t* lid, int64_t nl, const int32_t* ubi, const int32_t* ubj,
@@ -220,11 +258,20 @@ int oracle_solve(int32_t block_rows, double** tiles, const int32_t* lbi, const i
     for (int64_t k = 0; k < nl; k++) lk[lp[lbi[k]] + fill[lbi[k]]++] = k;
     memset(fill, 0, (size_t)block_rows * sizeof(int64_t));
     for (int64_t k = 0; k < nu; k++) uk[up[ubi[k]] + fill[ubi[k]]++] = k;
+    /* per block column lists of L (the L^T pass of an L L^T factor) */
+    int64_t* cp = (int64_t*)calloc((size_t)block_rows + 1, sizeof(int64_t));
+    int64_t* ck = (int64_t*)malloc((size_t)(nl > 0 ? nl : 1) * sizeof(int64_t));
+    for (int64_t k = 0; k < nl; k++) cp[lbj[k] + 1]++;
+    for (int32_t i = 0; i < block_rows; i++) cp[i + 1] += cp[i];
+    memset(fill, 0, (size_t)block_rows * sizeof(int64_t));
+    for (int64_t k = 0; k < nl; k++) ck[cp[lbj[k]] + fill[lbj[k]]++] = k;
     int rc = 0;
     for (int c = 0; c < nrhs; c++) {
         double* bc = b + c * N;
         double* xc = x + c * N;
-        /* forward: y_I = L_II^-1 (b_I - sum_J L_IJ y_J), y stored in b */
+        /* forward: y_I = L_II^-1 (b_I - sum_J L_IJ y_J), y in x; b keeps each row's
+           value before its division by l_ii, as lowerSolver leaves data::b (that is y
+           itself where L has a unit diagonal) */
         for (int32_t I = 0; I < block_rows && !rc; I++) {
             const double* diag = NULL;
             for (int64_t q = lp[I]; q < lp[I + 1]; q++) {
@@ -232,7 +279,7 @@ int oracle_solve(int32_t block_rows, double** tiles, const int32_t* lbi, const i
                 const double* t = tiles[lid[k]];
                 if (!t) { rc = -1; break; }
                 if (lbj[k] == I) { diag = t; continue; }
-                const double* yj = bc + (int64_t)lbj[k] * TS;
+                const double* yj = xc + (int64_t)lbj[k] * TS;
                 for (int i = 0; i < TS; i++) {
                     double s = 0;
                     for (int j = 0; j < TS; j++) s += t[i * TS + j] * yj[j];
@@ -240,15 +287,42 @@ int oracle_solve(int32_t block_rows, double** tiles, const int32_t* lbi, const i
                 }
             }
             if (!diag) { rc = -1; break; }
-            double* yi = bc + (int64_t)I * TS;
+            double* bi = bc + (int64_t)I * TS;
+            double* yi = xc + (int64_t)I * TS;
             for (int i = 0; i < TS; i++) {
                 double s = 0;
                 for (int j = 0; j < i; j++) s += diag[i * TS + j] * yi[j];
-                yi[i] = (yi[i] - s) / diag[i * TS + i];
+                bi[i] -= s;
+                yi[i] = bi[i] / diag[i * TS + i];
             }
         }
-        /* backward: x_I = U_II^-1 (y_I - sum_J U_IJ x_J) */
-        for (int64_t e = 0; e < N; e++) xc[e] = bc[e];
+        /* backward: x_I = U_II^-1 (y_I - sum_J U_IJ x_J), in place on y */
+        if (nu == 0) {
+            /* L L^T: U_IJ = L_JI^T; block column I of L holds block row I of U */
+            for (int32_t I = block_rows - 1; I >= 0 && !rc; I--) {
+                const double* diag = NULL;
+                for (int64_t q = cp[I]; q < cp[I + 1]; q++) {
+                    const int64_t k = ck[q];
+                    const double* t = tiles[lid[k]];
+                    if (!t) { rc = -1; break; }
+                    if (lbi[k] == I) { diag = t; continue; }
+                    const double* xj = xc + (int64_t)lbi[k] * TS;
+                    for (int i = 0; i < TS; i++) {
+                        double s = 0;
+                        for (int j = 0; j < TS; j++) s += t[j * TS + i] * xj[j];
+                        xc[(int64_t)I * TS + i] -= s;
+                    }
+                }
+                if (!diag) { rc = -1; break; }
+                double* xi = xc + (int64_t)I * TS;
+                for (int i = TS - 1; i >= 0; i--) {
+                    double s = 0;
+                    for (int j = i + 1; j < TS; j++) s += diag[j * TS + i] * xi[j];
+                    xi[i] = (xi[i] - s) / diag[i * TS + i];
+                }
+            }
+            continue;
+        }
         for (int32_t I = block_rows - 1; I >= 0 && !rc; I--) {
             const double* diag = NULL;
             for (int64_t q = up[I]; q < up[I + 1]; q++) {
@@ -272,7 +346,7 @@ int oracle_solve(int32_t block_rows, double** tiles, const int32_t* lbi, const i
             }
         }
     }
-    free(lp); free(up); free(lk); free(uk); free(fill);
+    free(lp); free(up); free(lk); free(uk); free(fill); free(cp); free(ck);
     return rc;
 }
 

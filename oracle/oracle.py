@@ -28,6 +28,8 @@ def lib() -> C.CDLL:
         L.oracle_lu.argtypes = [dp, dp, dp]
         L.oracle_inv_lower.argtypes = [dp, dp]
         L.oracle_inv_upper.argtypes = [dp, dp]
+        L.oracle_llt.argtypes = [dp, dp]
+        L.oracle_gemm_t.argtypes = [dp, dp, dp]
         L.oracle_factor_solve.restype = C.c_int
         L.oracle_factor_solve.argtypes = [C.c_int32, ip, C.c_int64, ip, dp, C.c_int64, ip, C.c_int32,
                                           ip, ip, ip, C.c_int64, ip, ip, ip, C.c_int64, dp, dp, C.c_int,
@@ -50,6 +52,22 @@ def gemm(a, b, c=None, sign=1):
     c = np.zeros((64, 64)) if c is None else np.array(c, np.float64, copy=True)
     lib().oracle_gemm(_d(a), _d(b), _d(c), sign)
     return c
+
+
+def gemm_t(a, b, c=None):
+    """c + a b^T"""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    c = np.zeros((64, 64)) if c is None else np.array(c, np.float64, copy=True)
+    lib().oracle_gemm_t(_d(a), _d(b), _d(c))
+    return c
+
+
+def llt(a):
+    a = np.ascontiguousarray(a, np.float64)
+    l = np.empty((64, 64))
+    lib().oracle_llt(_d(a), _d(l))
+    return l
 
 
 def sub(a, b):

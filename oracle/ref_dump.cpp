@@ -111,15 +111,25 @@ int main(int argc, char** argv) {
     data::mSize_out = dim;
     data::valcount = valcount;
     data::symmetric = mtx::symmetric;
-    if (data::symmetric) { std::fprintf(stderr, "symmetric input not dumped\n"); return 1; }
-    data::indexi = (int*)memutil::getSmallMem(1, sizeof(int) * valcount);
-    data::indexj = (int*)memutil::getSmallMem(1, sizeof(int) * valcount);
-    data::vals = (double*)memutil::getSmallMem(1, sizeof(double) * valcount);
+    // symmetric input: each off-diagonal entry followed by its mirror (solver.cpp:130-149)
+    const int cap = data::symmetric ? valcount * 2 : valcount;
+    data::indexi = (int*)memutil::getSmallMem(1, sizeof(int) * cap);
+    data::indexj = (int*)memutil::getSmallMem(1, sizeof(int) * cap);
+    data::vals = (double*)memutil::getSmallMem(1, sizeof(double) * cap);
+    int ii = 0;
     for (int i = 0; i < valcount; i++) {
-        data::indexi[i] = mtx::indexi[i];
-        data::indexj[i] = mtx::indexj[i];
-        data::vals[i] = mtx::vals[i];
+        data::indexi[ii] = mtx::indexi[i];
+        data::indexj[ii] = mtx::indexj[i];
+        data::vals[ii] = mtx::vals[i];
+        ii++;
+        if (data::symmetric && mtx::indexi[i] != mtx::indexj[i]) {
+            data::indexi[ii] = mtx::indexj[i];
+            data::indexj[ii] = mtx::indexi[i];
+            data::vals[ii] = mtx::vals[i];
+            ii++;
+        }
     }
+    data::valcount = ii;
     config::set(dim);
     data::blockRows = config::blockRows;
     data::blockSize = config::blockSize;
@@ -153,15 +163,20 @@ int main(int argc, char** argv) {
     int levels = __builtin_popcount(data::blockRows - 1);
     matrix* bl = memutil::newmatrix(0, n, levels);
     matrix* bu = memutil::newmatrix(0, n, levels);
-    BlockPlanner::blockMatrixLU(data::blocks, bl, bu, n, NULL, NULL);
-    BlockPlanner::blockPlan(bl, bu);
+    if (data::symmetric) {
+        BlockPlanner::blockMatrixLLT(data::blocks, bl, n, NULL);
+        BlockPlanner::blockPlan(bl, NULL);
+    } else {
+        BlockPlanner::blockMatrixLU(data::blocks, bl, bu, n, NULL, NULL);
+        BlockPlanner::blockPlan(bl, bu);
+    }
     dump_ops("coarse_ops.i32");
     data::PlanL2 = false;
     data::blockSize = BLOCK64;
     int levelall = __builtin_popcount(data::blockRows - 1);
     matrix* bl2 = memutil::newmatrix(0, data::blockRows, levelall);
     matrix* bu2 = memutil::newmatrix(0, data::blockRows, levelall);
-    BlockPlanner::copyOperatorL2(data::blocks, bl, bu, bl2, bu2, n);
+    BlockPlanner::copyOperatorL2(data::blocks, bl, data::symmetric ? NULL : bu, bl2, bu2, n);
     memutil::freeAllSmallMem(0);
     BlockPlanner::blockPlan(bl2, bu2);
     double t_plan = std::chrono::duration<double>(std::chrono::steady_clock::now() - tp).count();

@@ -306,6 +306,66 @@ struct Planner {
         u->sub[3] = u1;
     }
 
+    // blockMatrixMulT (BlockPlanner.cpp:1203-1229): c += a * b^T over present children
+    void mul_t(Node* a, Node* b, Node* c, int32_t group) {
+        if (a == nullptr || b == nullptr) return;
+        if (a->level == 0) {
+            if (a->id > 0 && b->id > 0) {
+                if (c->id <= 0) c->id = append_storage();
+                emit((int32_t)a->id, (int32_t)b->id, SOGL_OP_MULT, (int32_t)c->id, 0, group);
+            }
+            return;
+        }
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 2; j++)
+                for (int k = 0; k < 2; k++) {
+                    if (a->sub[i * 2 + k] != nullptr && b->sub[j * 2 + k] != nullptr) {
+                        if (c->sub[i * 2 + j] == nullptr)
+                            c->sub[i * 2 + j] = node(c->blockrows2 / 2, c->level - 1);
+                        mul_t(a->sub[i * 2 + k], b->sub[j * 2 + k], c->sub[i * 2 + j], group);
+                    }
+                }
+    }
+    // blockMatrixLLT (BlockPlanner.cpp:941-989): a = l l^T; l3, when given, receives
+    // the inverse of the top-left quadrant's factor for the caller to reuse
+    void llt(Node* a, Node* l, Node* l3, int32_t group) {
+        if (a->level == 0) {
+            if (a->id > 0) {
+                if (l->id <= 0) l->id = append_storage();
+                emit((int32_t)a->id, 0, SOGL_OP_LLT, (int32_t)l->id, 0, group);
+            }
+            return;
+        }
+        const int32_t br = a->blockrows2, lv = a->level;
+        Node* ha = a->sub[0];
+        need(ha, "A11");
+        Node* l1 = node(br / 2, lv - 1);
+        Node* l2 = l3 != nullptr ? l3 : node(br / 2, lv - 1);
+        Node* hl1 = lv > 1 ? node(br / 4, lv - 2) : nullptr;
+        llt(ha, l1, hl1, group);
+        inv_lower(l1, l2, hl1, group);  // L11^-1
+        l->sub[0] = l1;
+        Node* d = a->sub[3];
+        Node* c = a->sub[2];
+        need(d, "A22");
+        Node* dsub;
+        if (c != nullptr) {
+            Node* ca1 = node(br / 2, lv - 1);
+            Node* ca2 = node(br / 2, lv - 1);
+            mul_t(c, l2, ca1, group);  // L21 = A21 L11^-T
+            l->sub[2] = ca1;
+            mul_t(ca1, ca1, ca2, group);  // L21 L21^T
+            dsub = node(br / 2, lv - 1);
+            sub(d, ca2, dsub, group);  // Schur complement A22 - L21 L21^T
+        } else {
+            dsub = d;
+        }
+        l1 = node(br / 2, lv - 1);
+        Node* dl1 = lv > 1 ? node(br / 4, lv - 2) : nullptr;
+        llt(dsub, l1, dl1, group);
+        l->sub[3] = l1;
+    }
+
     // ---------------------------------------------------------- blockPlan
     void add_input(Node* a, std::vector<int32_t>& lastuse) {  // BlockPlanner.cpp:73-87
         if (a->level == 0 && a->id > 0) {
@@ -495,14 +555,36 @@ void build_plan(const Coo& a, const double* b, const PlanOptions& opt, Plan& pla
     using clk = std::chrono::steady_clock;
     auto t0 = clk::now();
     plan = Plan();
-    if (a.symmetric)
-        throw std::runtime_error("symmetric input selects the LL^T path, which this build does not provide");
     const int32_t n = a.n;
     if (n < kTile) throw std::runtime_error("matrix dimension below 64 is outside the planner's sizing");
-    const size_t nnz = a.val.size();
-    for (size_t k = 0; k < nnz; k++)
+    for (size_t k = 0; k < a.val.size(); k++)
         if (a.row[k] < 0 || a.row[k] >= n || a.col[k] < 0 || a.col[k] >= n)
             throw std::runtime_error("COO index out of range");
+    // symmetric input: every off-diagonal entry is followed by its mirror
+    // (solver.cpp:136-149), and the factorisation is L L^T
+    const bool sym = a.symmetric;
+    Coo mirrored;
+    if (sym) {
+        mirrored.n = n;
+        mirrored.symmetric = true;
+        const size_t m = a.val.size();
+        mirrored.row.reserve(2 * m);
+        mirrored.col.reserve(2 * m);
+        mirrored.val.reserve(2 * m);
+        for (size_t k = 0; k < m; k++) {
+            mirrored.row.push_back(a.row[k]);
+            mirrored.col.push_back(a.col[k]);
+            mirrored.val.push_back(a.val[k]);
+            if (a.row[k] != a.col[k]) {
+                mirrored.row.push_back(a.col[k]);
+                mirrored.col.push_back(a.row[k]);
+                mirrored.val.push_back(a.val[k]);
+            }
+        }
+    }
+    const Coo& in = sym ? mirrored : a;
+    const size_t nnz = in.val.size();
+    plan.symmetric = sym;
     std::string& log = plan.log;
     auto flush = [&](size_t from) {
         if (opt.verbose) { std::fputs(log.c_str() + from, stdout); std::fflush(stdout); }
@@ -512,9 +594,9 @@ void build_plan(const Coo& a, const double* b, const PlanOptions& opt, Plan& pla
     plan.cfg = SizeConfig::set(n);
     const SizeConfig& cfg = plan.cfg;
     const int64_t N = (int64_t)cfg.blockRows * cfg.blockSize;
-    plan.prow = a.row;
-    plan.pcol = a.col;
-    plan.pval = a.val;
+    plan.prow = in.row;
+    plan.pcol = in.col;
+    plan.pval = in.val;
     if (b) {
         plan.bPerm.assign((size_t)N, 1.0);
         for (int32_t i = 0; i < n; i++) plan.bPerm[(size_t)i] = b[i];
@@ -558,9 +640,14 @@ void build_plan(const Coo& a, const double* b, const PlanOptions& opt, Plan& pla
     const int levels = ilog2(blockRows);
     Node* bl = P.node(blockRows, levels);
     Node* bu = P.node(blockRows, levels);
-    P.lu(cblocks, bl, bu, nullptr, nullptr, 0);
+    if (sym) {
+        log += "symmetric\n";  // solver.cpp:52
+        P.llt(cblocks, bl, nullptr, 0);
+    } else {
+        P.lu(cblocks, bl, bu, nullptr, nullptr, 0);
+    }
     log += blocks_line(blockRows, blockSize, n, P.graph.size(), P.storage);
-    P.block_plan(cblocks, bl, bu);
+    P.block_plan(cblocks, bl, sym ? nullptr : bu);
     if (opt.keepCoarse) plan.coarseOps = P.graph;
 
     // L2 expansion (BlockPlanner::copyOperatorL2, BlockPlanner.cpp:1332-1460)
@@ -706,6 +793,8 @@ void build_plan(const Coo& a, const double* b, const PlanOptions& opt, Plan& pla
             case SOGL_OP_LOWER_INV: P.inv_lower(s1, r1, nullptr, g); break;
             case SOGL_OP_UPPER_INV: P.inv_upper(s1, r1, nullptr, g); break;
             case SOGL_OP_LU: P.lu(s1, r1, r2, nullptr, nullptr, g); break;
+            case SOGL_OP_LLT: P.llt(s1, r1, nullptr, g); break;
+            case SOGL_OP_MULT: P.mul_t(s1, s2, r1, g); break;
             case SOGL_OP_MUL: P.mul(s1, s2, r1, g, SOGL_OP_MUL); break;
             case SOGL_OP_MULNEG: P.mul(s1, s2, r1, g, SOGL_OP_MULNEG); break;
             case SOGL_OP_SUB:
@@ -720,7 +809,7 @@ void build_plan(const Coo& a, const double* b, const PlanOptions& opt, Plan& pla
                 throw std::runtime_error("unexpected coarse op " + std::to_string(o.op));
         }
     }
-    zoom_update(P, bu, UL2, l2);
+    if (!sym) zoom_update(P, bu, UL2, l2);
     zoom_update(P, bl, LL2, l2);
     log += blocks_line(blockRows, blockSize, n, P.graph.size(), P.storage);
     P.block_plan(fblocks, LL2, UL2);

@@ -71,6 +71,7 @@ struct Plan {
     std::vector<int32_t> newOrder, reverseOrder;
     int32_t gpsLevels = 0, gpsBandwidth = 0, gpsStart = 0;
     bool structureSymmetric = false;
+    bool symmetric = false;  // L L^T plan: factor tiles are L only (U = L^T)
 
     std::vector<sogl_op> coarseOps;  // coarse list after its blockPlan
     std::vector<sogl_op> ops;        // fine 64x64 unit operations, staged and sorted

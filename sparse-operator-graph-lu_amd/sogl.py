@@ -170,10 +170,11 @@ class Plan:
     """Host plan of one matrix: GPS ordering + staged 64x64 unit-op list."""
 
     def __init__(self, n: int, row, col, val, b=None, verbose=False, lib: str = "libsogl.so",
-                 like: "Plan | None" = None):
+                 like: "Plan | None" = None, symmetric: bool = False):
         """like: an earlier plan; where the matrix has its sparsity pattern, its ordering
         and op list are reused (same result, see sogl_plan_create_like); self.reused
-        says whether they were."""
+        says whether they were.  symmetric: the COO holds one triangle of a symmetric
+        matrix (a Matrix Market `symmetric` file); it is mirrored and factored as L L^T."""
         self.lib = load(lib)
         row = np.ascontiguousarray(row, dtype=np.int32)
         col = np.ascontiguousarray(col, dtype=np.int32)
@@ -182,11 +183,12 @@ class Plan:
         h = C.c_void_p()
         bptr = None if bb is None else _ptr(bb, C.c_double)
         reused = C.c_int32(0)
-        if like is not None:
+        if like is not None and not symmetric:
             rc = self.lib.sogl_plan_create_like(like.h, n, val.size, _ptr(row, C.c_int32), _ptr(col, C.c_int32),
                                                 _ptr(val, C.c_double), bptr, C.byref(h), C.byref(reused))
         else:
-            rc = self.lib.sogl_plan_create(n, val.size, 0, _ptr(row, C.c_int32), _ptr(col, C.c_int32),
+            rc = self.lib.sogl_plan_create(n, val.size, 1 if symmetric else 0, _ptr(row, C.c_int32),
+                                           _ptr(col, C.c_int32),
                                            _ptr(val, C.c_double), bptr, 1 if verbose else 0, C.byref(h))
         if rc != OK:
             raise SoglError(rc, self.lib.sogl_last_error().decode())

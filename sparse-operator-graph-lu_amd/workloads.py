@@ -69,6 +69,7 @@ class Problem:
     col: np.ndarray
     val: np.ndarray  # float64
     b: np.ndarray    # float64, shape (n,) or (n, nrhs) column-major when written
+    symmetric: bool = False  # the COO is the lower triangle of a symmetric matrix
 
     @property
     def nnz(self) -> int:
@@ -243,7 +244,7 @@ def write_mtx(prob: Problem, path: str, rhs: bool = True) -> None:
     """Matrix Market coordinate file (1-based, %.17g) and, with rhs, the
     `<base>_b.mtx` array file the reference CLI reads next to it."""
     with open(path, "w") as f:
-        f.write("%%MatrixMarket matrix coordinate real general\n")
+        f.write("%%MatrixMarket matrix coordinate real " + ("symmetric" if prob.symmetric else "general") + "\n")
         f.write(f"{prob.n} {prob.n} {prob.nnz}\n")
         body = np.char.add(
             np.char.add(np.char.add((prob.row + 1).astype(str), " "),
@@ -342,3 +343,20 @@ def edge_repeated_entries(n: int = 1200, seed: int = 13) -> Problem:
     order = np.argsort(draws(seed + 4, rows.size), kind="stable")
     return Problem(f"repeated_{n}", n, rows[order].astype(np.int32), cols[order].astype(np.int32),
                    vals[order], uniform_pm1(seed + 5, n))
+
+
+def symmetric_lower(p: Problem, name: str | None = None) -> Problem:
+    """The lower triangle (row >= col) of a symmetric matrix, in the COO's order, as a
+    Matrix Market `symmetric` file stores it."""
+    keep = p.row >= p.col
+    return Problem(name or f"{p.name}_sym", p.n, p.row[keep].copy(), p.col[keep].copy(), p.val[keep].copy(),
+                   p.b, symmetric=True)
+
+
+def full_matrix(p: Problem) -> tuple:
+    """(row, col, val) of the whole matrix a symmetric Problem stands for (mirrored)."""
+    if not p.symmetric:
+        return p.row, p.col, p.val
+    off = p.row != p.col
+    return (np.concatenate([p.row, p.col[off]]), np.concatenate([p.col, p.row[off]]),
+            np.concatenate([p.val, p.val[off]]))

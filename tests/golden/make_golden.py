@@ -40,6 +40,10 @@ CASES = {
     "convdiff2d_128": (lambda: W.convdiff2d(128), True, False),
     "diffusion3d_20": (lambda: W.diffusion_varcoef(20, 3, 5), True, False),
     "repeated_1200": (lambda: W.edge_repeated_entries(1200), True, False),
+    "sym_poisson2d_64": (lambda: W.symmetric_lower(W.poisson2d(64)), True, False),
+    "sym_diffusion3d_16": (lambda: W.symmetric_lower(W.diffusion_varcoef(16, 3, 21)), True, False),
+    "sym_diffusion2d_100": (lambda: W.symmetric_lower(W.diffusion_varcoef(100, 2, 22)), True, False),
+    "sym_diffusion3d_48": (lambda: W.symmetric_lower(W.diffusion_varcoef(48, 3, 23)), False, False),
     "c2": (lambda: W.config("c2"), False, False),
     "c5": (lambda: W.config("c5"), False, False),
 }

@@ -66,6 +66,57 @@ def test_triangular_inverses():
     assert np.allclose(g @ oracle.inv_lower(g), np.eye(64), atol=1e-13)
 
 
+def spd_tile():
+    m = rng.standard_normal((64, 64))
+    return m @ m.T + 64 * np.eye(64)
+
+
+def test_gemm_t_matches_numpy():
+    a, b, c = rng.standard_normal((3, 64, 64))
+    assert np.allclose(oracle.gemm_t(a, b, c), c + a @ b.T, rtol=0, atol=1e-12)
+    assert np.allclose(oracle.gemm_t(a, a), a @ a.T, rtol=0, atol=1e-12)
+    assert not np.allclose(oracle.gemm_t(a, b), a @ b)  # the transpose is there
+
+
+def test_llt_matches_cholesky_and_clamp():
+    a = spd_tile()
+    l = oracle.llt(a)
+    assert np.all(np.triu(l, 1) == 0)
+    assert np.allclose(l, np.linalg.cholesky(a), rtol=0, atol=1e-12 * np.abs(l).max())
+    assert np.allclose(l @ l.T, a, rtol=0, atol=1e-12 * np.abs(a).max())
+    # the upper triangle is never read
+    b = a.copy()
+    b[np.triu_indices(64, 1)] = 7.0
+    assert np.array_equal(oracle.llt(b), l)
+    # a non-positive pivot is clamped to 1e-20 before the square root
+    z = np.eye(64)
+    z[3, 3] = -2.0
+    assert oracle.llt(z)[3, 3] == np.sqrt(1e-20)
+
+
+SYM = {
+    "sym_poisson2d_64": lambda: W.symmetric_lower(W.poisson2d(64)),
+    "sym_diffusion3d_16": lambda: W.symmetric_lower(W.diffusion_varcoef(16, 3, 21)),
+    "sym_diffusion2d_100": lambda: W.symmetric_lower(W.diffusion_varcoef(100, 2, 22)),
+}
+
+
+@pytest.mark.parametrize("name", list(SYM))
+def test_llt_factor_solve_vs_reference(name, golden):
+    """Symmetric input: L L^T factor and the L, L^T substitutions against the
+    reference's own symmetric solve."""
+    g = golden(name)
+    p = SYM[name]()
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b, lib="libsogl_host.so", symmetric=True)
+    xp, yp = oracle.factor_solve(pl, pl.permute_rhs(p.b), threads=8)
+    x = pl.unpermute(xp)
+    ref = g["xorig"]
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-12
+    assert np.linalg.norm(yp - g["y"]) / np.linalg.norm(g["y"]) < 1e-12
+    r, c, v = W.full_matrix(p)
+    assert sogl.oracle_residual(p.n, r, c, v, x, p.b) < 1e-15
+
+
 CASES = {
     "c1": lambda: W.config("c1"),
     "disconnected_40": lambda: W.edge_disconnected(40),

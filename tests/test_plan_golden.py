@@ -18,6 +18,9 @@ SMALL = {
     "convdiff2d_128": lambda: W.convdiff2d(128),
     "diffusion3d_20": lambda: W.diffusion_varcoef(20, 3, 5),
     "repeated_1200": lambda: W.edge_repeated_entries(1200),
+    "sym_poisson2d_64": lambda: W.symmetric_lower(W.poisson2d(64)),
+    "sym_diffusion3d_16": lambda: W.symmetric_lower(W.diffusion_varcoef(16, 3, 21)),
+    "sym_diffusion2d_100": lambda: W.symmetric_lower(W.diffusion_varcoef(100, 2, 22)),
 }
 
 
@@ -31,7 +34,7 @@ def sha_f64(a):
 
 def plan_arrays(p):
     b = p.b if p.b.ndim == 1 else p.b[:, 0]
-    pl = sogl.Plan(p.n, p.row, p.col, p.val, b, lib="libsogl_host.so")
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, b, lib="libsogl_host.so", symmetric=p.symmetric)
     no, ro = pl.perm()
     ls, st = pl.tile_stages()
     ids, bi, bj, tiles = pl.inputs()
@@ -54,10 +57,14 @@ def test_small_bit_exact(name, golden):
     assert sha_f64(a["input_vals"]) == str(g["sha_input_vals"]), f"{name}: input tile values differ"
 
 
-@pytest.mark.parametrize("name", ["c2", "c5"])
+LARGE = {"c2": lambda: W.config("c2"), "c5": lambda: W.config("c5"),
+         "sym_diffusion3d_48": lambda: W.symmetric_lower(W.diffusion_varcoef(48, 3, 23))}
+
+
+@pytest.mark.parametrize("name", list(LARGE))
 def test_large_hashes(name, golden):
     g = golden(name)
-    pl, a = plan_arrays(W.config(name))
+    pl, a = plan_arrays(LARGE[name]())
     assert pl.info.n_ops == int(g["n_ops"])
     assert pl.info.storage_count == int(g["storage"])
     for k in ("perm", "ops", "coarse_ops", "tiles", "inputs", "ltiles", "utiles"):
@@ -105,13 +112,17 @@ def test_plan_invariants():
     assert np.all(np.diff(stage) >= 0)
 
 
-def test_symmetric_rejected():
-    p = W.poisson2d(16)
-    lib = sogl.load("libsogl_host.so")
-    import ctypes as C
-    h = C.c_void_p()
-    row = p.row.astype(np.int32); col = p.col.astype(np.int32)
-    rc = lib.sogl_plan_create(p.n, p.nnz, 1, row.ctypes.data_as(C.POINTER(C.c_int32)),
-                              col.ctypes.data_as(C.POINTER(C.c_int32)),
-                              p.val.ctypes.data_as(C.POINTER(C.c_double)), None, 0, C.byref(h))
-    assert rc == 6  # SOGL_ERR_UNSUPPORTED
+def test_symmetric_input_plans_llt():
+    """A symmetric matrix (one triangle in the COO) is mirrored and planned as L L^T:
+    llt / mult ops instead of lu / mul, no U factor tiles (U = L^T)."""
+    p = W.symmetric_lower(W.poisson2d(16 * 4))
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b, lib="libsogl_host.so", symmetric=True)
+    ops = pl.ops(0)
+    kinds = set(np.unique(ops[:, 2]).tolist())
+    assert 10 in kinds and 11 in kinds and 1 not in kinds  # llt, mult; no lu
+    assert pl.info.n_u_tiles == 0 and pl.info.n_l_tiles > 0
+    assert "symmetric" in pl.log.splitlines()
+    # the mirrored matrix enters the plan: its permuted COO has every entry twice
+    r, c, v = pl.perm_coo()
+    n_off = int((p.row != p.col).sum())
+    assert r.size == p.nnz + n_off + (pl.N - p.n)
