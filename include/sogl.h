@@ -33,7 +33,7 @@ extern "C" {
 #define SOGL_ERR_OOM 3           /* host/device allocation failed or admission refused */
 #define SOGL_ERR_DEVICE 4        /* HIP runtime error */
 #define SOGL_ERR_NAN 5           /* NaN found in the solution */
-#define SOGL_ERR_UNSUPPORTED 6   /* feature outside this path (e.g. symmetric LL^T) */
+#define SOGL_ERR_UNSUPPORTED 6   /* feature outside this path (e.g. a unit op kind it lacks) */
 #define SOGL_ERR_NOT_CONVERGED 7 /* iterative refinement missed its tolerance */
 #define SOGL_ERR_TIMEOUT 8       /* a device dependency wait hit its bound */
 #define SOGL_ERR_GROWTH 9        /* pivot-free growth above threshold in a low-precision factor */
@@ -86,11 +86,17 @@ typedef struct sogl_plan_info {
     int64_t n_ops, n_coarse_ops, n_inputs, n_l_tiles, n_u_tiles;
     int64_t n_mul, n_mulneg, n_sub, n_lu, n_lower_inv, n_upper_inv;
     double t_reorder, t_plan; /* seconds, the reference's timer boundaries */
+    int64_t n_llt, n_mult;    /* L L^T plans (symmetric input): llt and mult ops */
+    int32_t symmetric;        /* 1: L L^T plan */
+    int32_t reserved;
 } sogl_plan_info;
 
 /* solveLU's host part (solver.cpp:121-161, GPSOrder.cpp, solver.cpp:50-100):
  * copy COO, size config, GPS + sortInBlock, coarse plan, L2 expansion, fine
- * plan.  b may be NULL.  verbose != 0 echoes the reference's log lines. */
+ * plan.  b may be NULL.  verbose != 0 echoes the reference's log lines.
+ * symmetric != 0: the COO holds one triangle of a symmetric matrix; each off-diagonal
+ * entry is mirrored (solver.cpp:136-149) and the plan is the L L^T one
+ * (blockMatrixLLT, BlockPlanner.cpp:941-989): llt / mult ops and no U tiles. */
 int sogl_plan_create(int32_t dim, int64_t valcount, int32_t symmetric,
                      const int32_t* index_i, const int32_t* index_j,
                      const double* vals, const double* b, int32_t verbose,
@@ -198,8 +204,11 @@ int sogl_factor_autotune(sogl_ctx* ctx, int32_t factor_id, int32_t trials, doubl
                          int32_t* best);
 /* Triangular solve: replaces BlockPlanner::solve() (BlockPlanner.cpp:834-862), called
  * from decompose_solveLU (solver.cpp:115).  bp, xp: host arrays in the permuted padded
- * order, 64*block_rows x nrhs column-major.  y (may be NULL) receives the forward
- * substitution result, which the reference leaves in data::b. */
+ * order, 64*block_rows x nrhs column-major.  y (may be NULL) receives what the
+ * reference leaves in data::b: the forward substitution result y of an LU factor, and
+ * diag(L) y for an L L^T factor (lowerSolver stores b_i - sum before dividing by l_ii,
+ * BlockPlanner.cpp:741-767).  An L L^T factor (op lists with llt ops, symmetric input)
+ * solves L y = b, then L^T x = y (upperSolverT, BlockPlanner.cpp:769-797). */
 int sogl_solve(sogl_ctx* ctx, int32_t factor_id, const double* bp, double* xp, double* y,
                int32_t nrhs);
 /* Same, on device-resident arrays (hipMalloc'd by the caller) on the ctx stream. */
@@ -262,7 +271,8 @@ int sogl_device_copy(sogl_ctx* ctx, void* dst, const void* src, size_t bytes, in
 
 /* ------------------------------------------------ drop-in solveLU */
 /* Mirrors SOGLU::solveLU (solver.h:24, solver.cpp:121-184): plan on the host,
- * factor and solve on device 0 in FP64, x (length dim) in the original order. */
+ * factor and solve on device 0 in FP64, x (length dim) in the original order;
+ * symmetric != 0 takes the L L^T path. */
 int sogl_solveLU(int32_t dim, int64_t valcount, int32_t symmetric, const int32_t* index_i,
                  const int32_t* index_j, const double* vals, const double* b, double* x);
 const char* sogl_last_error(void);

@@ -13,7 +13,7 @@
 
 namespace sogl_dev {
 
-enum TaskKind : int32_t { TK_GEMM = 0, TK_SUB = 1, TK_LU = 2, TK_LINV = 3, TK_UINV = 4 };
+enum TaskKind : int32_t { TK_GEMM = 0, TK_SUB = 1, TK_LU = 2, TK_LINV = 3, TK_UINV = 4, TK_LLT = 5, TK_TRANS = 6 };
 
 // One device task = every unit op that writes one result tile (an lu writes two).
 // Writers of one tile always share a stage in the reference's plan (the "fix"
@@ -22,10 +22,11 @@ struct DevTask {
     int32_t kind;
     int32_t out0, out1;   // result slots (out1: U of an lu, else -1)
     int32_t a, b;         // GEMM: pair begin / pair count; SUB: minuend / subtrahend slot
-                          // (-1 = zero); LU, LINV, UINV: source slot
+                          // (-1 = zero); LU, LINV, UINV, LLT, TRANS: source slot
     int32_t dep_begin, dep_count;  // slots this task waits for (persistent executor)
     int32_t level;        // dependency level (0-based)
-    int32_t out2, out3;   // lu: slots of L^-1 and U^-1 made by the same task (-1: none)
+    int32_t out2, out3;   // lu: slots of L^-1 and U^-1 made by the same task, llt: of L^-1
+                          // (-1: none)
     int32_t add = -1;     // GEMM: slot of a tile added to the sum (-1: none)
     int32_t add_neg = 0;  // GEMM: the added tile is subtracted instead
 };
@@ -79,6 +80,7 @@ struct Factor {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int32_t tuned = -1;  // index into the tuning candidates, -1: not tuned
+    bool llt = false;    // L L^T factor: the U tiles of the solve are transposes of L tiles
 };
 
 struct Context {

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -155,34 +156,43 @@ struct PairWait {
     }
 };
 
-template <typename T, bool BF16, bool ACC, typename Wait = NoWait>
+// LLT: the kernel of an L L^T factor (llt, transpose and A B^T tasks, no lu); the LU
+// kernels are built without those, which keeps their register use as it was.
+template <typename T, bool BF16, bool ACC, bool LLT, typename Wait = NoWait>
 __device__ __attribute__((always_inline)) void run_task(const DevTask& t, const Args& a, char* lds, const Wait& wait = Wait()) {
     T* pool = reinterpret_cast<T*>(a.pool);
     switch (t.kind) {
         case TK_GEMM:
             if constexpr (BF16 && !ACC) {
-                gemm_task_bf16(reinterpret_cast<float*>(a.pool), a.pairs + t.a, t.b, t.out0, lds, wait, t.add,
-                               t.add_neg != 0);
+                gemm_task_bf16<LLT>(reinterpret_cast<float*>(a.pool), a.pairs + t.a, t.b, t.out0, lds, wait, t.add,
+                                    t.add_neg != 0);
             } else {
-                gemm_task<T, ACC>(pool, a.pairs + t.a, t.b, t.out0, lds, wait, t.add, t.add_neg != 0);
+                gemm_task<T, ACC, LLT>(pool, a.pairs + t.a, t.b, t.out0, lds, wait, t.add, t.add_neg != 0);
             }
             break;
         case TK_SUB:
             sub_task<T>(pool, t.b, t.a, t.out0);
             break;
-        case TK_LU: {
-            double g = lu_task<T>(pool, t.a, t.out0, t.out1, t.out2, t.out3, lds, a.growth_limit, a.ctrl + 2);
-            if (threadIdx.x == 0 && sizeof(T) == 4) {
-                float gf = (float)g;
-                atomicMax(a.ctrl + 3, __float_as_int(gf));
+        case TK_LU:
+            if constexpr (!LLT) {
+                double g = lu_task<T>(pool, t.a, t.out0, t.out1, t.out2, t.out3, lds, a.growth_limit, a.ctrl + 2);
+                if (threadIdx.x == 0 && sizeof(T) == 4) {
+                    float gf = (float)g;
+                    atomicMax(a.ctrl + 3, __float_as_int(gf));
+                }
             }
             break;
-        }
         case TK_LINV:
             tri_inv_task<T, true>(pool, t.a, t.out0, lds);
             break;
         case TK_UINV:
             tri_inv_task<T, false>(pool, t.a, t.out0, lds);
+            break;
+        case TK_LLT:
+            if constexpr (LLT) llt_task<T>(pool, t.a, t.out0, t.out2, lds);
+            break;
+        case TK_TRANS:
+            if constexpr (LLT) trans_task<T>(pool, t.a, t.out0, lds);
             break;
         default:
             break;
@@ -192,7 +202,7 @@ __device__ __attribute__((always_inline)) void run_task(const DevTask& t, const 
 // Persistent executor: every workgroup loops taking the next task index.  GEMM
 // tasks wait pair by pair (pairs are ordered by when their operands become
 // ready); other tasks wait for all their sources first.
-template <typename T, bool BF16>
+template <typename T, bool BF16, bool LLT>
 __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
     __shared__ __attribute__((aligned(16))) char lds[lds_bytes<T>()];
     __shared__ int s_task;
@@ -225,7 +235,7 @@ __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
         if (tid == 0) s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         if (__builtin_amdgcn_readfirstlane(s_abort)) break;
-        run_task<T, BF16, false>(t, a, lds, PairWait{&a, &t_wait});
+        run_task<T, BF16, false, LLT>(t, a, lds, PairWait{&a, &t_wait});
         // publish: every wave's stores complete (the barrier's workgroup release), then
         // thread 0 makes them visible at device scope once and raises the flags
         __syncthreads();
@@ -255,11 +265,11 @@ __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
 }
 
 // Levels / per-op executors: one workgroup per task of the launch.
-template <typename T, bool BF16, bool ACC>
+template <typename T, bool BF16, bool ACC, bool LLT>
 __global__ __launch_bounds__(NT) void k_factor_batch(Args a) {
     __shared__ __attribute__((aligned(16))) char lds[lds_bytes<T>()];
     const DevTask t = a.tasks[a.task_offset + blockIdx.x];
-    run_task<T, BF16, ACC>(t, a, lds);
+    run_task<T, BF16, ACC, LLT>(t, a, lds);
 }
 
 __global__ void k_f64_to_f32(const double* in, float* out, size_t n) {
@@ -348,6 +358,18 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
             readers[(size_t)id[k]]++;
         }
     }
+    // an op list with llt ops is an L L^T factorisation: its U is L^T, made on the
+    // device by transposing the L tiles after they are final
+    bool llt = false, has_lu = false, has_mult = false;
+    for (int64_t k = 0; k < d->n_ops; k++) {
+        llt |= d->ops[k].op == SOGL_OP_LLT;
+        has_lu |= d->ops[k].op == SOGL_OP_LU;
+        has_mult |= d->ops[k].op == SOGL_OP_MULT;
+    }
+    if (llt && has_lu) return set_error(c, SOGL_ERR_ARG, "an op list has both lu and llt ops");
+    if (has_mult && !llt) return set_error(c, SOGL_ERR_ARG, "mult ops belong to an L L^T op list (llt ops)");
+    if (llt && d->n_u != 0) return set_error(c, SOGL_ERR_ARG, "an L L^T factor takes no U tiles");
+    f->llt = llt;
     f->zero_slot = next++;
 
     // producer task of each slot (-1: input or never written); sealed: the slot has
@@ -379,14 +401,15 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
         int32_t rs = slot[(size_t)o.result];
         int kind;
         switch (o.op) {
-            case SOGL_OP_MUL: case SOGL_OP_MULNEG: kind = TK_GEMM; break;
+            case SOGL_OP_MUL: case SOGL_OP_MULNEG: case SOGL_OP_MULT: kind = TK_GEMM; break;
             case SOGL_OP_SUB: kind = TK_SUB; break;
             case SOGL_OP_LU: kind = TK_LU; break;
+            case SOGL_OP_LLT: kind = TK_LLT; break;
             case SOGL_OP_LOWER_INV: kind = TK_LINV; break;
             case SOGL_OP_UPPER_INV: kind = TK_UINV; break;
             default:
                 return set_error(c, SOGL_ERR_UNSUPPORTED,
-                                 "unit op " + std::to_string(o.op) + " is outside the LU path");
+                                 "unit op " + std::to_string(o.op) + " is outside the factor path");
         }
         DevTask ot{};
         ot.kind = kind;
@@ -401,7 +424,8 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
         if (kind == TK_GEMM) {
             if (s1 < 0) s1 = f->zero_slot;
             if (s2 < 0) s2 = f->zero_slot;
-            pr = Pair{s1, o.op == SOGL_OP_MULNEG ? -s2 - 1 : s2};
+            // mult: C += A B^T, the transposed second operand flagged on a
+            pr = Pair{o.op == SOGL_OP_MULT ? -s1 - 1 : s1, o.op == SOGL_OP_MULNEG ? -s2 - 1 : s2};
             ot.a = (int32_t)op_pairs.size();
             ot.b = 1;
             op_pairs.push_back(pr);
@@ -475,7 +499,8 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
         if (kind == TK_LINV || kind == TK_UINV) {
             const int32_t pt = s1 >= 0 ? prod[(size_t)s1] : -1;
             const bool lower = kind == TK_LINV;
-            if (pt >= 0 && tasks[(size_t)pt].kind == TK_LU && rs < 0 &&
+            const int32_t pk = pt >= 0 ? tasks[(size_t)pt].kind : -1;
+            if ((pk == TK_LU || (pk == TK_LLT && lower)) && rs < 0 &&
                 (lower ? tasks[(size_t)pt].out0 : tasks[(size_t)pt].out1) == s1) {
                 int32_t inv = lower ? tasks[(size_t)pt].out2 : tasks[(size_t)pt].out3;
                 if (inv < 0) {
@@ -525,17 +550,18 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
                 (which == 0 ? f->l_diag : f->u_diag)[(size_t)bi[k]] = slot[(size_t)id[k]];
     }
     for (int32_t i = 0; i < R; i++)
-        if (f->l_diag[(size_t)i] < 0 || f->u_diag[(size_t)i] < 0)
+        if (f->l_diag[(size_t)i] < 0 || (!llt && f->u_diag[(size_t)i] < 0))
             return set_error(c, SOGL_ERR_ARG, "factor lacks a diagonal tile");
     // inverses of the diagonal factor tiles for the triangular solve: made by the lu
     // task of each diagonal tile, into the plan's inverse tile when the plan has one
     f->linv_slot.assign((size_t)R, -1);
     f->uinv_slot.assign((size_t)R, -1);
     for (int32_t i = 0; i < R; i++) {
-        for (int which = 0; which < 2; which++) {
+        for (int which = 0; which < (llt ? 1 : 2); which++) {
             const int32_t ds = (which == 0 ? f->l_diag : f->u_diag)[(size_t)i];
             const int32_t pt = ds >= 0 && ds < (int32_t)prod.size() ? prod[(size_t)ds] : -1;
-            const bool from_lu = pt >= 0 && tasks[(size_t)pt].kind == TK_LU &&
+            const int32_t pk = pt >= 0 ? tasks[(size_t)pt].kind : -1;
+            const bool from_lu = (pk == TK_LU || (pk == TK_LLT && which == 0)) &&
                                  (which == 0 ? tasks[(size_t)pt].out0 : tasks[(size_t)pt].out1) == ds;
             int32_t inv;
             if (from_lu) {
@@ -566,6 +592,39 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
             (which == 0 ? f->linv_slot : f->uinv_slot)[(size_t)i] = inv;
         }
     }
+    // L L^T: every L tile and every diagonal inverse transposed into a slot of its own
+    // (U_JI = L_IJ^T, U_II^-1 = (L_II^-1)^T), by transpose tasks after their sources
+    std::map<int32_t, int32_t> trans_of;
+    auto transposed = [&](int32_t src) -> int32_t {
+        auto it = trans_of.find(src);
+        if (it != trans_of.end()) return it->second;
+        const int32_t ts = new_slot();
+        DevTask nt{};
+        nt.kind = TK_TRANS;
+        nt.out0 = ts;
+        nt.out1 = nt.out2 = nt.out3 = -1;
+        nt.a = src;
+        tasks.push_back(nt);
+        tpairs.emplace_back();
+        tstage.push_back(0);
+        lu_op_task.push_back(-1);
+        prod[(size_t)ts] = (int32_t)tasks.size() - 1;
+        f->op_tasks.push_back(nt);
+        trans_of[src] = ts;
+        return ts;
+    };
+    std::vector<int32_t> l_trans;
+    if (llt) {
+        for (int64_t k = 0; k < d->n_l; k++) {
+            const int32_t s0 = slot[(size_t)d->l_ids[k]];
+            if (s0 < 0) return set_error(c, SOGL_ERR_ARG, "factor tile never produced");
+            l_trans.push_back(transposed(s0));
+        }
+        for (int32_t i = 0; i < R; i++) {
+            f->u_diag[(size_t)i] = transposed(f->l_diag[(size_t)i]);
+            f->uinv_slot[(size_t)i] = transposed(f->linv_slot[(size_t)i]);
+        }
+    }
     f->n_slots = next;
 
     // dependencies and levels
@@ -582,8 +641,8 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
         const DevTask& k = tasks[t];
         if (k.kind == TK_GEMM) {
             for (const Pair& p : tpairs[t]) {
-                add_dep(t, p.a);
-                add_dep(t, p.b < 0 ? -p.b - 1 : p.b);
+                add_dep(t, pair_a(p));
+                add_dep(t, pair_b(p));
             }
             add_dep(t, k.add);
         } else if (k.kind == TK_SUB) {
@@ -610,7 +669,7 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
     // waits only for its last pair once the earlier ones are done
     auto ready_level = [&](int32_t s) -> int32_t { return prod[(size_t)s] < 0 ? -1 : level[(size_t)prod[(size_t)s]]; };
     auto pair_level = [&](const Pair& x) -> int32_t {
-        return std::max(ready_level(x.a), ready_level(x.b < 0 ? -x.b - 1 : x.b));
+        return std::max(ready_level(pair_a(x)), ready_level(pair_b(x)));
     };
     for (size_t t = 0; t < T; t++) {
         if (tasks[t].kind != TK_GEMM) continue;
@@ -651,6 +710,12 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
     std::vector<std::vector<std::pair<int32_t, int32_t>>> lrow((size_t)R), urow((size_t)R);
     f->l_slots.clear();
     f->u_slots.clear();
+    if (llt)  // U = L^T: block (j, i) of U is the transpose of L's block (i, j)
+        for (int64_t k = 0; k < d->n_l; k++)
+            if (d->l_bi[k] != d->l_bj[k]) {
+                if (d->l_bj[k] > d->l_bi[k]) return set_error(c, SOGL_ERR_ARG, "factor tile on the wrong side of the diagonal");
+                urow[(size_t)d->l_bj[k]].push_back({d->l_bi[k], l_trans[(size_t)k]});
+            }
     for (int which = 0; which < 2; which++) {
         const int64_t cnt = which == 0 ? d->n_l : d->n_u;
         const int32_t* bi = which == 0 ? d->l_bi : d->u_bi;
@@ -726,8 +791,8 @@ int validate_graph(Context* c, const sogl_factor_desc* d, const Factor* f, const
                     return bad("pair range of task " + std::to_string(t));
                 for (int32_t q = k.a; q < k.a + k.b; q++) {
                     const Pair& p = hg.pairs[(size_t)q];
-                    reads.push_back(p.a);
-                    reads.push_back(p.b < 0 ? -p.b - 1 : p.b);
+                    reads.push_back(pair_a(p));
+                    reads.push_back(pair_b(p));
                 }
                 if (!ok_opt(k.add)) return bad("added tile of task " + std::to_string(t));
                 if (k.add >= 0) reads.push_back(k.add);
@@ -737,7 +802,7 @@ int validate_graph(Context* c, const sogl_factor_desc* d, const Factor* f, const
                 if (k.a >= 0) reads.push_back(k.a);
                 if (k.b >= 0) reads.push_back(k.b);
                 break;
-            case TK_LU: case TK_LINV: case TK_UINV:
+            case TK_LU: case TK_LINV: case TK_UINV: case TK_LLT: case TK_TRANS:
                 reads.push_back(k.a);
                 break;
             default:
@@ -767,7 +832,7 @@ int validate_graph(Context* c, const sogl_factor_desc* d, const Factor* f, const
             if (k.a < 0 || k.b < 1 || (size_t)k.a + (size_t)k.b > hg.op_pairs.size()) return bad("per-op pairs");
             for (int32_t q = k.a; q < k.a + k.b; q++) {
                 const Pair& p = hg.op_pairs[(size_t)q];
-                if (!ok_slot(p.a) || !ok_slot(p.b < 0 ? -p.b - 1 : p.b)) return bad("per-op pair slot");
+                if (!ok_slot(pair_a(p)) || !ok_slot(pair_b(p))) return bad("per-op pair slot");
             }
         } else if (!ok_opt(k.a) || !ok_opt(k.b)) {
             return bad("per-op operand");
@@ -920,7 +985,7 @@ void free_factor(Factor* f) {
     delete f;
 }
 
-template <typename T, bool BF16>
+template <typename T, bool BF16, bool LLT>
 int launch_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
     hipStream_t st = c->stream;
     Args a{};
@@ -940,7 +1005,7 @@ int launch_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
             int cnt = f->level_begin[v + 1] - f->level_begin[v];
             if (cnt <= 0) continue;
             a.task_offset = f->level_begin[v];
-            hipLaunchKernelGGL((k_factor_batch<T, BF16, false>), dim3(cnt), dim3(NT), 0, st, a);
+            hipLaunchKernelGGL((k_factor_batch<T, BF16, false, LLT>), dim3(cnt), dim3(NT), 0, st, a);
         }
     };
     if (mode == SOGL_EXEC_GRAPH && !f->graph_exec) {
@@ -962,11 +1027,11 @@ int launch_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
         a.pairs = (const Pair*)f->pairs_d.p;
         a.ntasks = (int32_t)f->tasks.size();
         int per_cu = 0;
-        HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_persistent<T, BF16>, NT, 0));
+        HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_persistent<T, BF16, LLT>, NT, 0));
         if (per_cu < 1) per_cu = 1;
         if (wg_per_cu > 0) per_cu = std::min(per_cu, wg_per_cu);  // never more than fit: all must be resident
         int grid = std::max(1, std::min<int>(a.ntasks, c->num_cu * per_cu));
-        if (a.ntasks > 0) hipLaunchKernelGGL((k_factor_persistent<T, BF16>), dim3(grid), dim3(NT), 0, st, a);
+        if (a.ntasks > 0) hipLaunchKernelGGL((k_factor_persistent<T, BF16, LLT>), dim3(grid), dim3(NT), 0, st, a);
     } else if (mode == SOGL_EXEC_LEVELS) {
         launch_levels();
     } else if (mode == SOGL_EXEC_GRAPH) {
@@ -983,7 +1048,7 @@ int launch_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
         a.ntasks = (int32_t)f->op_tasks.size();
         for (int32_t k = 0; k < a.ntasks; k++) {
             a.task_offset = k;
-            hipLaunchKernelGGL((k_factor_batch<T, false, true>), dim3(1), dim3(NT), 0, st, a);
+            hipLaunchKernelGGL((k_factor_batch<T, false, true, LLT>), dim3(1), dim3(NT), 0, st, a);
         }
     } else {
         return set_error(c, SOGL_ERR_ARG, "unknown executor mode");
@@ -1141,9 +1206,15 @@ int run_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
     f->epoch++;
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     int rc;
-    if (f->precision == SOGL_FP64) rc = launch_factor<double, false>(c, f, mode, chunk, wg_per_cu);
-    else if (f->precision == SOGL_FP32) rc = launch_factor<float, false>(c, f, mode, chunk, wg_per_cu);
-    else rc = launch_factor<float, true>(c, f, mode, chunk, wg_per_cu);
+    if (f->llt) {
+        if (f->precision == SOGL_FP64) rc = launch_factor<double, false, true>(c, f, mode, chunk, wg_per_cu);
+        else if (f->precision == SOGL_FP32) rc = launch_factor<float, false, true>(c, f, mode, chunk, wg_per_cu);
+        else rc = launch_factor<float, true, true>(c, f, mode, chunk, wg_per_cu);
+    } else {
+        if (f->precision == SOGL_FP64) rc = launch_factor<double, false, false>(c, f, mode, chunk, wg_per_cu);
+        else if (f->precision == SOGL_FP32) rc = launch_factor<float, false, false>(c, f, mode, chunk, wg_per_cu);
+        else rc = launch_factor<float, true, false>(c, f, mode, chunk, wg_per_cu);
+    }
     if (rc != SOGL_OK) return rc;
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     HIP_TRY(c, hipEventSynchronize(c->ev1));

@@ -300,6 +300,19 @@ __global__ void k_zero_i32(int* p, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
+// out = diag(L) y over the padded rows, every column: the forward result as the
+// reference's lowerSolver leaves it in its right-hand side, b_i = l_ii y_i (for a
+// unit-diagonal L that is y itself).
+template <typename T>
+__global__ void k_scale_by_ldiag(const T* __restrict__ pool, const int32_t* __restrict__ l_diag,
+                                 const double* __restrict__ y, double* __restrict__ out, int64_t ld, int64_t n) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e % ld;
+        const int64_t i = r / TS, q = r % TS;
+        out[e] = (double)pool[(size_t)l_diag[i] * TE + q * TS + q] * y[e];
+    }
+}
+
 namespace {
 
 #define HIP_TRY(ctx, expr)                                       \
@@ -417,7 +430,22 @@ extern "C" int sogl_solve(sogl_ctx* x, int32_t factor_id, const double* bp, doub
     if ((rc = solve_device(c, f, (const double*)f->vec_b.p, (double*)f->vec_x.p, (double*)f->vec_y.p, nrhs)))
         return rc;
     HIP_TRY(c, hipMemcpyAsync(xp, f->vec_x.p, bytes, hipMemcpyDeviceToHost, c->stream));
-    if (y) HIP_TRY(c, hipMemcpyAsync(y, f->vec_y.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    if (y && f->llt) {
+        // L L^T: the reference's right-hand side ends as diag(L) y (the input b in
+        // vec_b is no longer needed)
+        const int64_t ld = (int64_t)f->block_rows * TS, n = ld * nrhs;
+        const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+        if (f->elem == 8)
+            hipLaunchKernelGGL(k_scale_by_ldiag<double>, dim3(grid), dim3(256), 0, c->stream, (const double*)f->pool.p,
+                               (const int32_t*)f->l_diag_d.p, (const double*)f->vec_y.p, (double*)f->vec_b.p, ld, n);
+        else
+            hipLaunchKernelGGL(k_scale_by_ldiag<float>, dim3(grid), dim3(256), 0, c->stream, (const float*)f->pool.p,
+                               (const int32_t*)f->l_diag_d.p, (const double*)f->vec_y.p, (double*)f->vec_b.p, ld, n);
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipMemcpyAsync(y, f->vec_b.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    } else if (y) {
+        HIP_TRY(c, hipMemcpyAsync(y, f->vec_y.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return SOGL_OK;
 }

@@ -7,6 +7,11 @@
 //   lu_task   : A = L U, pivot-free, |u_kk| < 1e-9 clamped to +-1e-9
 //   linv_task : Y = L^-1 (general diagonal)
 //   uinv_task : Y = U^-1
+//   llt_task  : A = L L^T (lower triangle read), l_jj = sqrt(max(p, 1e-20)), and L^-1
+//   trans_task: R = A^T (the U = L^T tiles of an L L^T factor, for the solve)
+//
+// A GEMM pair may take its second operand transposed (C += A B^T, the mult op of the
+// L L^T path, MatrixStdDouble.cpp mat_mult 704-716, 771-783).
 //
 // The arithmetic follows the reference's tile kernels (MatrixStdDouble.cpp:
 // C+=AB 2043-2507, C-=AB 237-701, ludcmpSimple 2711-2784, inv_lower
@@ -99,10 +104,33 @@ struct MfmaOp<float> {
 };
 
 // One (A,B) pair of a GEMM task.  b < 0 encodes a subtracted product (mulneg):
-// slot = -b - 1.
+// slot = -b - 1; a < 0 encodes a transposed second operand (A B^T): slot = -a - 1.
 struct Pair {
     int32_t a, b;
 };
+__host__ __device__ inline int pair_a(const Pair& p) { return p.a < 0 ? -p.a - 1 : p.a; }
+__host__ __device__ inline int pair_b(const Pair& p) { return p.b < 0 ? -p.b - 1 : p.b; }
+
+// acc += sg * A B (or A B^T) from LDS: A rows with stride LDA; B row-major with stride
+// LDB, or, transposed, stored row-major with stride LDA and read by columns.
+template <typename T, bool TB, typename acc_t>
+__device__ __forceinline__ void gemm_lds(acc_t (&acc)[2][2], const T* la, const T* lb, T sg, int wr, int wc,
+                                         int li, int lk) {
+#pragma unroll 4
+    for (int kk = 0; kk < TS / 4; kk++) {
+        const int k = 4 * kk + lk;
+        T af[2], bf[2];
+#pragma unroll
+        for (int m = 0; m < 2; m++) af[m] = sg * la[(wr + m * 16 + li) * LDA + k];
+#pragma unroll
+        for (int n = 0; n < 2; n++)
+            bf[n] = TB ? lb[(wc + n * 16 + li) * LDA + k] : lb[k * LDB + wc + n * 16 + li];
+#pragma unroll
+        for (int m = 0; m < 2; m++)
+#pragma unroll
+            for (int n = 0; n < 2; n++) acc[m][n] = MfmaOp<T>::mma(af[m], bf[n], acc[m][n]);
+    }
+}
 
 // C = sum_s sign_s A_s B_s over npairs pairs, full 64x64 output per workgroup.
 // Wave w computes the 32x32 quadrant (w/2, w%2) as 2x2 blocks of 16x16 MFMA
@@ -132,7 +160,9 @@ __device__ __forceinline__ void add_tile(acc_t (&acc)[2][2], const T* __restrict
             }
 }
 
-template <typename T, bool ACC = false, typename Wait = NoWait>
+// TBOK: pairs may carry a transposed second operand (only L L^T factors have them; the
+// LU kernels are built without that branch).
+template <typename T, bool ACC = false, bool TBOK = false, typename Wait = NoWait>
 __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, const Pair* __restrict__ pairs, int npairs,
                           int out, char* lds_raw, const Wait& wait = Wait(), int add = -1,
                           bool add_neg = false) {
@@ -157,26 +187,15 @@ __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, c
         }
     for (int s = 0; s < npairs; s++) {
         const Pair p = pairs[s];
-        const bool neg = p.b < 0;
-        const int sb = neg ? -p.b - 1 : p.b;
-        wait(p.a, sb);
-        tile_to_lds(pool + (size_t)p.a * TE, la, LDA, tid);
-        tile_to_lds(pool + (size_t)sb * TE, lb, LDB, tid);
+        const bool neg = p.b < 0, tb = p.a < 0;
+        const int sa = pair_a(p), sb = pair_b(p);
+        wait(sa, sb);
+        tile_to_lds(pool + (size_t)sa * TE, la, LDA, tid);
+        tile_to_lds(pool + (size_t)sb * TE, lb, TBOK && tb ? LDA : LDB, tid);
         __syncthreads();
         const T sg = neg ? T(-1) : T(1);
-#pragma unroll 4
-        for (int kk = 0; kk < TS / 4; kk++) {
-            const int k = 4 * kk + lk;
-            T af[2], bf[2];
-#pragma unroll
-            for (int m = 0; m < 2; m++) af[m] = sg * la[(wr + m * 16 + li) * LDA + k];
-#pragma unroll
-            for (int n = 0; n < 2; n++) bf[n] = lb[k * LDB + wc + n * 16 + li];
-#pragma unroll
-            for (int m = 0; m < 2; m++)
-#pragma unroll
-                for (int n = 0; n < 2; n++) acc[m][n] = MfmaOp<T>::mma(af[m], bf[n], acc[m][n]);
-        }
+        if (TBOK && tb) gemm_lds<T, TBOK>(acc, la, lb, sg, wr, wc, li, lk);
+        else gemm_lds<T, false>(acc, la, lb, sg, wr, wc, li, lk);
     }
     if (add >= 0) {
         wait(add, add);
@@ -194,7 +213,7 @@ __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, c
 
 // bf16 experiment: FP32 tiles, operands rounded to bf16, FP32 accumulation
 // (v_mfma_f32_16x16x32_bf16: A[i][8q..8q+7] in lane i + 16q, same D layout).
-template <typename Wait = NoWait>
+template <bool TBOK = false, typename Wait = NoWait>
 __device__ __attribute__((always_inline)) void gemm_task_bf16(float* __restrict__ pool, const Pair* __restrict__ pairs,
                                       int npairs, int out, char* lds_raw, const Wait& wait = Wait(),
                                       int add = -1, bool add_neg = false) {
@@ -210,13 +229,15 @@ __device__ __attribute__((always_inline)) void gemm_task_bf16(float* __restrict_
         for (int n = 0; n < 2; n++) acc[m][n] = v4f{0, 0, 0, 0};
     for (int s = 0; s < npairs; s++) {
         const Pair p = pairs[s];
-        const bool neg = p.b < 0;
-        const int sb = neg ? -p.b - 1 : p.b;
-        wait(p.a, sb);
-        tile_to_lds(pool + (size_t)p.a * TE, la, LDA, tid);
-        tile_to_lds(pool + (size_t)sb * TE, lb, LDB, tid);
+        const bool neg = p.b < 0, tb = p.a < 0;
+        const int sa = pair_a(p), sb = pair_b(p);
+        wait(sa, sb);
+        tile_to_lds(pool + (size_t)sa * TE, la, LDA, tid);
+        tile_to_lds(pool + (size_t)sb * TE, lb, TBOK && tb ? LDA : LDB, tid);
         __syncthreads();
         const float sg = neg ? -1.f : 1.f;
+        // B[k][j] at k * LDB + j, or (transposed operand) at j * LDA + k
+        const int bsk = TBOK && tb ? 1 : LDB, bsj = TBOK && tb ? LDA : 1;
 #pragma unroll
         for (int kk = 0; kk < TS / 32; kk++) {
             v8bf af[2], bf[2];
@@ -229,7 +250,7 @@ __device__ __attribute__((always_inline)) void gemm_task_bf16(float* __restrict_
             for (int n = 0; n < 2; n++)
 #pragma unroll
                 for (int e = 0; e < 8; e++)
-                    bf[n][e] = (__bf16)lb[(kk * 32 + lk * 8 + e) * LDB + wc + n * 16 + li];
+                    bf[n][e] = (__bf16)lb[(kk * 32 + lk * 8 + e) * bsk + (wc + n * 16 + li) * bsj];
 #pragma unroll
             for (int m = 0; m < 2; m++)
 #pragma unroll
@@ -842,6 +863,76 @@ __device__ __attribute__((always_inline)) void tri_inv_task(T* __restrict__ pool
     if (LOWER) inv_lower_regs<C, false>(ts, v, bro, lane, w);
     else inv_upper_regs<C>(ts, v, bro, lane, w);
     store_rowseg(pool + (size_t)out * TE, v, lane, w);
+}
+
+// The llt task: Cholesky of one tile (its lower triangle), right-looking in LDS, two
+// barriers per column: every thread takes l_kk = sqrt(max(p, 1e-20)) from the current
+// pivot p (lltdcmpSimple, MatrixStdDouble.cpp:2629-2668, clamp 2612), the column below
+// it is scaled, then the trailing lower triangle takes the rank-1 update.  FP32 tiles
+// factor in FP32.  When outinv >= 0 the task also writes L^-1 (general diagonal).
+template <typename T>
+__device__ __attribute__((always_inline)) void llt_task(T* __restrict__ pool, int src, int out, int outinv,
+                                                        char* lds_raw) {
+    typedef typename std::conditional<sizeof(T) == 8, double, float>::type C;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
+    C* a = reinterpret_cast<C*>(lds_raw);  // the tile, row stride LDA
+    C* lc = a + TS * LDA;                  // the scaled column of the step
+    C* bro = lc + TS + 16 * w;             // row broadcast of the inverse sweep
+    const int i = tid >> 2, j0 = (tid & 3) * 16;
+    const T* A = pool + (size_t)src * TE;
+#pragma unroll
+    for (int q = 0; q < 16; q++) a[i * LDA + j0 + q] = C(A[i * TS + j0 + q]);
+    __syncthreads();
+    const C TINY = C(1e-20);
+#pragma unroll 1
+    for (int k = 0; k < TS; k++) {
+        C p = a[k * LDA + k];
+        if (p < TINY) p = TINY;
+        const C s = sqrt(p);
+        const C inv = C(1) / s;
+        if (tid < TS && tid > k) {
+            const C l = a[tid * LDA + k] * inv;
+            lc[tid] = l;
+            a[tid * LDA + k] = l;
+        }
+        __syncthreads();
+        if (tid == 0) a[k * LDA + k] = s;
+        if (i > k) {
+            const C li = lc[i];
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int j = j0 + q;
+                if (j > k && j <= i) a[i * LDA + j] -= li * lc[j];
+            }
+        }
+        __syncthreads();
+    }
+    T* L = pool + (size_t)out * TE;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int j = j0 + q;
+        L[i * TS + j] = j <= i ? T(a[i * LDA + j]) : T(0);
+    }
+    if (outinv < 0) return;
+    C y[16];
+    inv_lower_regs<C, false>(a, y, bro, lane, w);
+    store_rowseg(pool + (size_t)outinv * TE, y, lane, w);
+}
+
+// R = A^T through LDS (odd row stride: the column reads hit distinct banks).
+template <typename T>
+__device__ __attribute__((always_inline)) void trans_task(T* __restrict__ pool, int src, int out, char* lds_raw) {
+    T* s = reinterpret_cast<T*>(lds_raw);
+    const int tid = threadIdx.x;
+    const int r = tid >> 2, c0 = (tid & 3) * 16;
+    const T* A = pool + (size_t)src * TE;
+#pragma unroll
+    for (int q = 0; q < 16; q++) s[r * LDA + c0 + q] = A[r * TS + c0 + q];
+    __syncthreads();
+    T* R = pool + (size_t)out * TE;
+#pragma unroll
+    for (int q = 0; q < 16; q++) R[r * TS + c0 + q] = s[(c0 + q) * LDA + r];
 }
 
 }  // namespace sogl_dev

@@ -169,7 +169,7 @@ extern "C" int sogl_plan_create(int32_t dim, int64_t valcount, int32_t symmetric
     } catch (const std::exception& e) {
         delete p;
         std::string w = e.what();
-        return fail(w.find("LL^T") != std::string::npos ? SOGL_ERR_UNSUPPORTED : SOGL_ERR_ARG, w);
+        return fail(SOGL_ERR_ARG, w);
     }
     *out = p;
     return SOGL_OK;
@@ -244,9 +244,12 @@ extern "C" int sogl_plan_get_info(const sogl_plan* p, sogl_plan_info* info) {
             case SOGL_OP_LU: info->n_lu++; break;
             case SOGL_OP_LOWER_INV: info->n_lower_inv++; break;
             case SOGL_OP_UPPER_INV: info->n_upper_inv++; break;
+            case SOGL_OP_LLT: info->n_llt++; break;
+            case SOGL_OP_MULT: info->n_mult++; break;
             default: break;
         }
     }
+    info->symmetric = P.symmetric ? 1 : 0;
     info->t_reorder = P.tReorder;
     info->t_plan = P.tPlan;
     return SOGL_OK;

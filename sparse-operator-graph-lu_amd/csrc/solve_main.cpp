@@ -65,16 +65,10 @@ int main(int argc, char** argv) {
     std::vector<double> b;
     sogl::read_array(base + "_b.mtx", a.n, b, &log);
     std::cout << log;
-    if (a.symmetric) {
-        std::cout << "symmetric" << std::endl;
-        std::fprintf(stderr, "symmetric input selects the LL^T path, which this build does not provide\n");
-        return 1;
-    }
-
     auto t0 = std::chrono::steady_clock::now();
     sogl_plan* plan = nullptr;
-    int rc = sogl_plan_create(a.n, (int64_t)a.val.size(), 0, a.row.data(), a.col.data(), a.val.data(),
-                              b.data(), 1, &plan);
+    int rc = sogl_plan_create(a.n, (int64_t)a.val.size(), a.symmetric ? 1 : 0, a.row.data(), a.col.data(),
+                              a.val.data(), b.data(), 1, &plan);
     if (rc) return die("plan", rc, sogl_last_error());
     sogl_plan_info info;
     sogl_plan_get_info(plan, &info);

@@ -43,7 +43,8 @@ class PlanInfo(C.Structure):
                 ("n_l_tiles", C.c_int64), ("n_u_tiles", C.c_int64), ("n_mul", C.c_int64),
                 ("n_mulneg", C.c_int64), ("n_sub", C.c_int64), ("n_lu", C.c_int64),
                 ("n_lower_inv", C.c_int64), ("n_upper_inv", C.c_int64),
-                ("t_reorder", C.c_double), ("t_plan", C.c_double)]
+                ("t_reorder", C.c_double), ("t_plan", C.c_double),
+                ("n_llt", C.c_int64), ("n_mult", C.c_int64), ("symmetric", C.c_int32), ("reserved", C.c_int32)]
 
 
 class FactorDesc(C.Structure):

@@ -96,6 +96,153 @@ def test_sub_copy_neg_tiles_exact(dev):
     assert np.array_equal(r, -b)
 
 
+# ---- L L^T (symmetric input)
+LLT, MULT, LINV, SUBOP = 10, 11, 2, 4
+
+
+def spd_tile(scale=1.0):
+    m = rng.standard_normal((64, 64))
+    return (m @ m.T + 64 * np.eye(64)) * scale
+
+
+def run_llt(dev, n_ids, ops, inputs, ltiles, block_rows, precision=sogl.FP64, b=None):
+    """Load an L L^T op list, factor, return the L tiles and (with b) x and y."""
+    ids = np.arange(1, len(inputs) + 1, dtype=np.int32)
+    fid = dev.load_ops(n_ids, np.array(ops, np.int32), ids, np.stack(inputs).astype(np.float64),
+                       np.array(ltiles, np.int32), np.zeros((0, 3), np.int32), block_rows, precision)
+    dev.factor(fid, sogl.EXEC_PERSISTENT)
+    lt = dev.factor_tiles(fid, 0, len(ltiles))
+    out = (lt,)
+    if b is not None:
+        x, y = dev.solve(fid, b, want_y=True)
+        out = (lt, x, y)
+    dev.release(fid)
+    return out
+
+
+@pytest.mark.parametrize("precision,tol", [(sogl.FP64, 1e-13), (sogl.FP32, 2e-5)])
+def test_llt_tile(dev, precision, tol):
+    a = spd_tile()
+    b = rng.standard_normal(64)
+    (l,), x, y = run_llt(dev, 3, [op(1, 0, LLT, 2)], [a], [[0, 0, 2]], 1, precision, b)
+    want = oracle.llt(a)
+    assert np.all(np.triu(l, 1) == 0)
+    assert np.abs(l - want).max() / np.abs(want).max() < tol
+    # L y = b, L^T x = y through the inverse and transposed tiles; y as the reference
+    # leaves it: diag(L) y
+    xs = np.linalg.solve(a, b)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 50 * tol
+    yl = np.linalg.solve(want, b)
+    assert np.linalg.norm(y - np.diag(want) * yl) / np.linalg.norm(yl) < 50 * tol
+
+
+def test_llt_tile_clamp(dev):
+    z = np.eye(64)
+    z[3, 3] = -2.0
+    l = run_llt(dev, 3, [op(1, 0, LLT, 2)], [z], [[0, 0, 2]], 1)[0][0]
+    assert l[3, 3] == np.sqrt(1e-20) and l[4, 4] == 1.0
+
+
+@pytest.mark.parametrize("precision,tol", [(sogl.FP64, 1e-12), (sogl.FP32, 2e-5), (sogl.BF16, 2e-2)])
+def test_llt_two_by_two_blocks(dev, precision, tol):
+    """The L L^T recursion one level up, on 2 x 2 tiles: L11 = llt(A11), X = L11^-1,
+    L21 = A21 X^T (mult), S = L21 L21^T (mult, self product), D = A22 - S (folded into
+    the GEMM), L22 = llt(D); against the Cholesky factor of the 128 x 128 matrix."""
+    m = rng.standard_normal((128, 128))
+    a = m @ m.T + 128 * np.eye(128)
+    a11, a21, a22 = a[:64, :64], a[64:, :64], a[64:, 64:]
+    ops = [op(1, 0, LLT, 4, stage=2, seq=0), op(4, 0, LINV, 5, stage=3, seq=1),
+           op(2, 5, MULT, 6, stage=4, seq=2), op(6, 6, MULT, 7, stage=5, seq=3),
+           op(7, 3, SUBOP, 8, stage=6, seq=4), op(8, 0, LLT, 9, stage=7, seq=5)]
+    b = rng.standard_normal(128)
+    lt, x, y = run_llt(dev, 10, ops, [a11, a21, a22], [[0, 0, 4], [1, 0, 6], [1, 1, 9]], 2, precision, b)
+    L = np.zeros((128, 128))
+    L[:64, :64], L[64:, :64], L[64:, 64:] = lt[0], lt[1], lt[2]
+    want = np.linalg.cholesky(a)
+    assert np.abs(L - want).max() / np.abs(want).max() < tol
+    xs = np.linalg.solve(a, b)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 100 * tol
+
+
+SYM = {
+    "sym_poisson2d_64": lambda: W.symmetric_lower(W.poisson2d(64)),
+    "sym_diffusion3d_16": lambda: W.symmetric_lower(W.diffusion_varcoef(16, 3, 21)),
+    "sym_diffusion2d_100": lambda: W.symmetric_lower(W.diffusion_varcoef(100, 2, 22)),
+}
+
+
+@pytest.mark.parametrize("name", list(SYM))
+def test_llt_solve_vs_reference(dev, name, golden):
+    """Symmetric input end to end in FP64 against the reference's own symmetric solve:
+    x, the right-hand side as the reference leaves it, the residual, the oracle."""
+    g = golden(name)
+    p = SYM[name]()
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b, symmetric=True)
+    fid = dev.load(pl, sogl.FP64)
+    dev.factor(fid)
+    xp, yp = dev.solve(fid, pl.permute_rhs(p.b), want_y=True)
+    dev.release(fid)
+    x = pl.unpermute(xp)
+    ref = g["xorig"]
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-12
+    assert np.linalg.norm(yp - g["y"]) / np.linalg.norm(g["y"]) < 1e-12
+    r, c, v = W.full_matrix(p)
+    assert sogl.oracle_residual(p.n, r, c, v, x, p.b) < 1e-15
+    xo, _ = oracle.factor_solve(pl, pl.permute_rhs(p.b), threads=8)
+    assert np.linalg.norm(xp - xo) / np.linalg.norm(xo) < 1e-12
+
+
+def test_llt_executor_modes_and_refinement(dev, golden):
+    p = SYM["sym_diffusion2d_100"]()
+    ref = golden("sym_diffusion2d_100")["xorig"]
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b, symmetric=True)
+    xs = []
+    for mode in (sogl.EXEC_PERSISTENT, sogl.EXEC_LEVELS, sogl.EXEC_GRAPH, sogl.EXEC_PER_OP):
+        fid = dev.load(pl, sogl.FP64)
+        dev.factor(fid, mode)
+        xs.append(dev.solve(fid, pl.permute_rhs(p.b)))
+        dev.release(fid)
+    assert np.array_equal(xs[0], xs[1]) and np.array_equal(xs[0], xs[2])
+    assert np.linalg.norm(xs[0] - xs[3]) / np.linalg.norm(xs[3]) < 1e-13
+    r, c, v = W.full_matrix(p)
+    for prec in (sogl.FP32, sogl.BF16):
+        fid = dev.load(pl, prec)
+        dev.factor(fid)
+        dev.attach_refine(fid, pl)
+        x, it, res, rc = dev.solve_refined(fid, p.b, tol=1e-12, max_iter=60)
+        dev.release(fid)
+        assert rc == 0, (prec, it, res)
+        assert sogl.oracle_residual(p.n, r, c, v, x, p.b) <= 1e-12
+        # forward error <= cond(A) x residual; log-normal coefficients give cond ~ 1e5
+        assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-6
+
+
+@pytest.mark.slow
+def test_llt_full_size(dev, golden):
+    """3D 48^3 variable-coefficient diffusion, symmetric input (n=110,592): FP64 vs the
+    reference's symmetric solve (sample), FP32 factor + FP64 refinement to 1e-12."""
+    g = golden("sym_diffusion3d_48")
+    p = W.symmetric_lower(W.diffusion_varcoef(48, 3, 23))
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b, symmetric=True)
+    fid = dev.load(pl, sogl.FP64)
+    dev.factor(fid)
+    x = pl.unpermute(dev.solve(fid, pl.permute_rhs(p.b)))
+    dev.release(fid)
+    idx, ref = g["xorig_idx"], g["xorig_sample"]
+    assert np.linalg.norm(x[idx] - ref) / np.linalg.norm(ref) < 1e-12
+    assert abs(np.linalg.norm(x) - float(g["xorig_norm"])) / float(g["xorig_norm"]) < 1e-12
+    r, c, v = W.full_matrix(p)
+    assert sogl.oracle_residual(p.n, r, c, v, x, p.b) < 1e-15
+    fid = dev.load(pl, sogl.FP32)
+    dev.factor(fid)
+    dev.attach_refine(fid, pl)
+    x32, it, res, rc = dev.solve_refined(fid, p.b, tol=1e-12, max_iter=30)
+    dev.release(fid)
+    assert rc == 0 and res <= 1e-12
+    # forward error <= cond(A) x residual (log-normal coefficients: cond ~ 1e5)
+    assert np.linalg.norm(x32[idx] - ref) / np.linalg.norm(ref) < 1e-6
+
+
 CASES = {
     "c1": lambda: W.config("c1"),
     "disconnected_40": lambda: W.edge_disconnected(40),
@@ -360,18 +507,22 @@ def test_c5_full_size(dev, golden):
     assert np.linalg.norm(x32[idx] - ref) / np.linalg.norm(ref) < 1e-4
 
 
-def test_drop_in_solveLU(dev):
+@pytest.mark.parametrize("name", ["c1", "sym_poisson2d_64"])
+def test_drop_in_solveLU(dev, name, golden):
     import ctypes as C
-    p = W.config("c1")
+    p = W.config("c1") if name == "c1" else SYM[name]()
     lib = sogl.load()
     x = np.empty(p.n)
     row, col = p.row.astype(np.int32), p.col.astype(np.int32)
-    rc = lib.sogl_solveLU(p.n, p.nnz, 0, row.ctypes.data_as(C.POINTER(C.c_int32)),
+    rc = lib.sogl_solveLU(p.n, p.nnz, 1 if p.symmetric else 0, row.ctypes.data_as(C.POINTER(C.c_int32)),
                           col.ctypes.data_as(C.POINTER(C.c_int32)),
                           p.val.ctypes.data_as(C.POINTER(C.c_double)),
                           p.b.ctypes.data_as(C.POINTER(C.c_double)), x.ctypes.data_as(C.POINTER(C.c_double)))
     assert rc == 0
-    assert sogl.oracle_residual(p.n, p.row, p.col, p.val, x, p.b) < 1e-15
+    r, c, v = W.full_matrix(p)
+    assert sogl.oracle_residual(p.n, r, c, v, x, p.b) < 1e-15
+    ref = golden(name)["xorig"]
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-12
 
 
 def test_refined_device_resident_matches_host(dev):

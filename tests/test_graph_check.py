@@ -20,23 +20,27 @@ CASES = {
     "convdiff2d_128": lambda: W.convdiff2d(128),
     "diffusion3d_20": lambda: W.diffusion_varcoef(20, 3, 5),
     "random_band_2048": lambda: W.random_band(2048),
+    "sym_poisson2d_64": lambda: W.symmetric_lower(W.poisson2d(64)),
+    "sym_diffusion3d_16": lambda: W.symmetric_lower(W.diffusion_varcoef(16, 3, 21)),
 }
 
 
 @pytest.mark.parametrize("name", list(CASES))
 def test_plan_graphs_pass(name):
     p = CASES[name]()
-    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b, symmetric=p.symmetric)
     info = pl.info
     for prec in (sogl.FP64, sogl.FP32):
         g = sogl.graph_check(pl, prec)
-        assert g["pairs"] == info.n_mul + info.n_mulneg
+        assert g["pairs"] == info.n_mul + info.n_mulneg + info.n_mult
         # folded ops are sub ops (copy / neg / sub), never more than the plan has
         assert 0 <= g["folded"] <= info.n_sub
         assert g["inv_aliased"] <= info.n_lower_inv + info.n_upper_inv
+        # L L^T: one transpose task per L tile and per diagonal inverse (U = L^T)
+        extra = info.n_l_tiles + info.block_rows if info.symmetric else 0
         # every unit op is a task, one more pair of a GEMM task, folded or aliased
-        assert g["tasks"] <= info.n_ops - g["folded"] - g["inv_aliased"]
-        assert g["levels"] > 0 and g["pool_tiles"] <= info.storage_count + 2 * info.block_rows + 1
+        assert g["tasks"] <= info.n_ops - g["folded"] - g["inv_aliased"] + extra
+        assert g["levels"] > 0 and g["pool_tiles"] <= info.storage_count + 2 * info.block_rows + 1 + extra
 
 
 def test_c2_graph_passes():
@@ -80,3 +84,21 @@ def test_rejects_write_after_read():
 def test_rejects_two_writers():
     with pytest.raises(sogl.SoglError, match="more than one non-accumulating"):
         _check([[1, 0, LU, 3, 4, 2, 0, 0], [2, 0, LU, 3, 7, 3, 1, 0]])
+
+
+def test_rejects_mixed_factor_kinds():
+    LLT, MULT = 10, 11
+    with pytest.raises(sogl.SoglError, match="both lu and llt"):
+        _check([[1, 0, LU, 3, 4, 2, 0, 0], [2, 0, LLT, 5, 0, 2, 1, 0]])
+    with pytest.raises(sogl.SoglError, match="mult ops belong"):
+        _check([[1, 2, MULT, 5, 0, 2, 0, 0], [1, 0, LU, 3, 4, 2, 1, 0]])
+    with pytest.raises(sogl.SoglError, match="no U tiles"):
+        _check([[1, 0, LLT, 3, 0, 2, 0, 0]])  # _check passes a U tile list
+
+
+def test_llt_hand_built_graph():
+    LLT = 10
+    g = sogl.graph_check_ops(8, np.array([[1, 0, LLT, 3, 0, 2, 0, 0]], np.int32), np.array([1, 2], np.int32),
+                             np.zeros((2, 64, 64)), np.array([[0, 0, 3]], np.int32), np.zeros((0, 3), np.int32), 1)
+    # the llt task (with its inverse) and transposes of L and of L^-1 for the solve
+    assert g["tasks"] == 3
