@@ -53,7 +53,12 @@ WORKLOADS = {
     "c3": ("random nonsymmetric n=16,384 (the largest size whose fill fits the planner; n=262,144 "
            "fills to ~550 GB), band half-width 32 + 4 random entries per row seed 2, N(0,1) values, "
            "diagonal sum|a_ij|+1, 128 RHS N(0,1) seed 1002", 1e-12),
+    "sym": ("symmetric input (L L^T path): 3D 7-point variable-coefficient diffusion 48^3 (n=110,592), "
+            "log-normal (sigma 1) node coefficients seed 23, lower triangle in the file, 1 RHS U[-1,1] "
+            "seed 1023", 1e-12),
 }
+# workloads beyond the named configurations, and the fixture holding their reference x
+EXTRA = {"sym": (lambda: W.symmetric_lower(W.diffusion_varcoef(48, 3, 23)), "sym_diffusion3d_48")}
 
 
 def compare_executors(dev, plan, per_op: bool):
@@ -203,7 +208,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS) + ["c4"])
+    ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS) + ["c4"],
+                    help="c1..c5: the named workloads; sym: a symmetric-input (L L^T) workload")
     ap.add_argument("--requests", type=int, default=200, help="c4: requests in the serving stream")
     ap.add_argument("--rate", type=float, default=20.0, help="c4: mean arrivals per second")
     ap.add_argument("--slots", type=int, default=4, help="c4: concurrent requests on the GPU")
@@ -228,12 +234,13 @@ def main():
         return
     mode = sogl.EXEC_PERSISTENT if args.exec == "persistent" else sogl.EXEC_LEVELS
     desc, tol = WORKLOADS[args.config]
-    prob = W.config(args.config)
+    prob = EXTRA[args.config][0]() if args.config in EXTRA else W.config(args.config)
     t0 = time.perf_counter()
-    plan = sogl.Plan(prob.n, prob.row, prob.col, prob.val, prob.b if prob.b.ndim == 1 else prob.b[:, 0])
+    plan = sogl.Plan(prob.n, prob.row, prob.col, prob.val, prob.b if prob.b.ndim == 1 else prob.b[:, 0],
+                     symmetric=prob.symmetric)
     t_plan = time.perf_counter() - t0
     info = plan.info
-    gemm_flops = (info.n_mul + info.n_mulneg) * 2.0 * 64 ** 3
+    gemm_flops = (info.n_mul + info.n_mulneg + info.n_mult) * 2.0 * 64 ** 3
     nrhs = 1 if prob.b.ndim == 1 else prob.b.shape[1]
     solve_bytes = {k: (info.n_l_tiles + info.n_u_tiles) * 64 * 64 * s + 2 * plan.N * 8 * nrhs
                    for k, s in (("fp64", 8), ("fp32", 4))}
@@ -246,6 +253,16 @@ def main():
     executors = None
     if args.compare_executors or args.config == "c3":
         executors = compare_executors(dev, plan, per_op=True)
+    lu_same = None
+    if prob.symmetric:  # the same matrix given in full (both triangles) takes the LU path
+        r, c, v = W.full_matrix(prob)
+        full = W.Problem(prob.name + "_full", prob.n, r, c, v, prob.b)
+        lu_plan = sogl.Plan(full.n, full.row, full.col, full.val, full.b)
+        lu_same = run_path(dev, lu_plan, full, sogl.FP64, tol, 0, args.steps, args.warmup, mode)
+        lu_same.pop("x")
+        lu_same = {k: (float(f"{v:.6g}") if isinstance(v, float) else v) for k, v in lu_same.items()}
+        lu_same["unit_ops"] = int(lu_plan.info.n_ops)
+        del lu_plan
     dev.close()
 
     ref = None
@@ -255,12 +272,13 @@ def main():
 
     # parity: vs the reference's FP64 x (live reference run, else the golden fixture)
     # and the FP64 oracle residual
-    gold = os.path.join(ROOT, "tests", "golden", f"{args.config}.npz")
+    gold = os.path.join(ROOT, "tests", "golden", f"{EXTRA[args.config][1] if args.config in EXTRA else args.config}.npz")
+    mat = W.full_matrix(prob)  # the whole matrix (a symmetric input stores one triangle)
     g = np.load(gold, allow_pickle=False) if os.path.exists(gold) else None
     out_paths = {}
     for k, p in paths.items():
         x = p.pop("x")
-        resid = sogl.oracle_residual(prob.n, prob.row, prob.col, prob.val, x, prob.b)
+        resid = sogl.oracle_residual(prob.n, mat[0], mat[1], mat[2], x, prob.b)
         err = None
         x0 = x if x.ndim == 1 else x[:, 0]
         if ref is not None and ref["xref"] is not None:
@@ -306,13 +324,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.config}: {desc}", "tol": tol, "executor": args.exec,
                    "n": prob.n, "nnz": prob.nnz, "unit_ops": int(info.n_ops),
-                   "gemm_ops": int(info.n_mul + info.n_mulneg), "stages": int(info.n_stages),
+                   "gemm_ops": int(info.n_mul + info.n_mulneg + info.n_mult), "stages": int(info.n_stages),
                    "gemm_gflop": round(gemm_flops / 1e9, 3), "host_plan_s": round(t_plan, 3)},
         "roofline": {"bound": "mfma", "achieved": round(head["gemm_tflops"], 4), "peak": PEAK["fp32"],
                      "unit": "TFLOP/s", "frac": round(head["gemm_tflops"] / PEAK["fp32"], 6),
                      "traffic": TRAFFIC_FP32_FACTOR.get(args.config),
                      "kernel": "k_factor_persistent<float> (FP32 factor executor); achieved = "
-                               "(mul+mulneg ops) x 2*64^3 flop / executor time from HIP events"},
+                               "(mul+mulneg+mult ops) x 2*64^3 flop / executor time from HIP events"},
         "cpu_baseline": cpu,
         "fp32_ir": out_paths["fp32_ir"],
         "fp64": out_paths["fp64"],
@@ -321,6 +339,8 @@ def main():
         line["executors_fp64"] = executors
     if "bf16_ir" in out_paths:
         line["bf16_ir_experiment"] = out_paths["bf16_ir"]
+    if lu_same is not None:
+        line["fp64_lu_same_matrix"] = lu_same
     print(json.dumps(line))
 
 

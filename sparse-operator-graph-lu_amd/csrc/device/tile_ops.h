@@ -416,8 +416,9 @@ constexpr int BS = 17;        // LDS row stride of a 16x16 block
 constexpr int BE = 16 * BS;   // LDS elements of a 16x16 block
 
 // LDS elements (in the compute type) of lu_inv_blocked: the tile, 8 diagonal-block
-// inverses, 4 first-level inverse blocks, 2 temporaries per wave, 8 reduction slots
-constexpr int lu_lds_elems() { return TS * LDA + 8 * BE + 4 * BE + 8 * BE + 8; }
+// inverses, 4 first-level inverse blocks, 2 temporaries per wave, 8 reduction slots,
+// and the 2 x 64 column scales of the Cholesky form
+constexpr int lu_lds_elems() { return TS * LDA + 8 * BE + 4 * BE + 8 * BE + 8 + 2 * TS; }
 
 // Lane k of each 16-lane row to every lane of that row (DPP row_newbcast).  A DPP
 // read of a lane that EXEC disables returns 0, so callers evaluate it where every
@@ -481,6 +482,25 @@ __device__ __forceinline__ void blk_to_global(const C* pb, int ld, T* tile, int 
     }
 }
 
+// the same two stores with row i of the tile scaled by rs[i] (rs == nullptr: none)
+template <typename T, typename C>
+__device__ __forceinline__ void st16_global_rs(const typename MfmaOp<C>::acc_t& acc, T* tile, int I, int J,
+                                               int li, int lk, C sg, const C* rs) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int row = 16 * I + MfmaOp<C>::row(lk, r);
+        tile[row * TS + 16 * J + li] = T(sg * acc[r] * rs[row]);
+    }
+}
+template <typename T, typename C>
+__device__ __forceinline__ void blk_to_global_rs(const C* pb, int ld, T* tile, int I, int J, int lane, const C* rs) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int r = 4 * q + (lane >> 4), c = lane & 15;
+        tile[(16 * I + r) * TS + 16 * J + c] = T(pb[r * ld + c] * rs[16 * I + r]);
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ void zero_blk_global(T* tile, int I, int J, int lane) {
 #pragma unroll
@@ -499,7 +519,19 @@ __device__ __forceinline__ void zero_blk_global(T* tile, int I, int J, int lane)
 // of the forward substitution for L^-1 (rows, lanes 0-15: l_ik) and for (U^T)^-1
 // (rows, lanes 16-31: u_ki / u_kk, the 1/u_ii scaling applied at the end), so both
 // inverses advance in the same 16 steps as the factorisation.
-template <typename C>
+// The pivot rule: |p| < 1e-9 -> +-1e-9 (ludcmpSimple); CHOL: p < 1e-20 -> 1e-20, the
+// rule lltdcmpSimple applies before its square root (MatrixStdDouble.cpp:2612).
+template <bool CHOL, typename C>
+__device__ __forceinline__ C clamp_pivot(C p) {
+    if constexpr (CHOL) {
+        return p < C(1e-20) ? C(1e-20) : p;
+    } else {
+        const C TINY2 = C(1e-9);
+        return (p < TINY2 && p > -TINY2) ? ((p < 0) ? -TINY2 : TINY2) : p;
+    }
+}
+
+template <typename C, bool CHOL = false>
 __device__ inline void diag16(C* a, int r0, C* xd, C* zd, int lane) {
     const int i = lane & 15, g = lane >> 4;
     const bool colw = g == 1;
@@ -511,7 +543,6 @@ __device__ inline void diag16(C* a, int r0, C* xd, C* zd, int lane) {
         y[c] = c == i ? C(1) : C(0);
     }
     C dq = C(1);  // 1 / u_ii, taken at step i
-    const C TINY2 = C(1e-9);
     static_for<0, 16>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
         C bc[16], yb[16];
@@ -519,8 +550,7 @@ __device__ inline void diag16(C* a, int r0, C* xd, C* zd, int lane) {
         for (int c = k; c < 16; c++) bc[c] = rbcast<k>(x[c]);
 #pragma unroll
         for (int c = 0; c <= k; c++) yb[c] = rbcast<k>(y[c]);
-        C p = bc[k];
-        if (p < TINY2 && p > -TINY2) p = (p < 0) ? -TINY2 : TINY2;
+        const C p = clamp_pivot<CHOL>(bc[k]);
         const C rp = recip(p);
         const C own = x[k];
         const C f = i > k ? own * rp : C(0);
@@ -548,17 +578,15 @@ __device__ inline void diag16(C* a, int r0, C* xd, C* zd, int lane) {
 // executor has at two workgroups per CU.  Lanes i, i+16, i+32, i+48 hold row i;
 // after the LU, lanes 0-15 invert L by rows and lanes 16-31 invert U^T by rows with
 // the same instructions (forward substitution, 1/u_ii scaling deferred to the end).
-template <typename C>
+template <typename C, bool CHOL = false>
 __device__ inline void diag16_split(C* a, int r0, C* xd, C* zd, int lane) {
     const int i = lane & 15, g = lane >> 4;
     C v[16];
 #pragma unroll
     for (int c = 0; c < 16; c++) v[c] = a[(r0 + i) * LDA + r0 + c];
-    const C TINY2 = C(1e-9);
     static_for<0, 16>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        C p = rbcast<k>(v[k]);
-        if (p < TINY2 && p > -TINY2) p = (p < 0) ? -TINY2 : TINY2;
+        const C p = clamp_pivot<CHOL>(rbcast<k>(v[k]));
         const C l = i > k ? v[k] * recip(p) : C(0);
         v[k] = i > k ? l : (i == k ? p : v[k]);
 #pragma unroll
@@ -598,7 +626,11 @@ __device__ inline void diag16_split(C* a, int r0, C* xd, C* zd, int lane) {
 // LU of one tile in compute type C plus, when their slots are given (>= 0), the
 // inverses of its unit-lower L and upper U: the reference's lowerInv / upperInv
 // of the same diagonal tile, fused into the producing task.
-template <typename T, typename C>
+// CHOL: the Cholesky factor of a symmetric tile from the same elimination.  Only the
+// lower triangle is read (mirrored on load), pivots follow the Cholesky clamp, and
+// with d_j = sqrt(u_jj) the outputs are L = L_u diag(d) and L^-1 = diag(1/d) L_u^-1
+// (A = L_u U with U = diag(u) L_u^T, so L L^T = A); outu and outui are unused.
+template <typename T, typename C, bool CHOL = false>
 __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ pool, int src, int outl, int outu,
                                                               int outli, int outui, char* lds_raw,
                                                               float* amax_out, float* umax_out) {
@@ -612,15 +644,25 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
     C* x1 = zd + 4 * BE;                   // X10, X32, Z01, Z23 (first-level inverse blocks)
     C* tw = x1 + 4 * BE + 2 * BE * w;      // 2 temporaries of this wave
     C* red = x1 + 4 * BE + 8 * BE;         // 8 reduction slots
+    C* dsc = red + 8;                      // CHOL: d_j = sqrt(u_jj), then 1 / d_j
     {
         const T* A = pool + (size_t)src * TE;
         const int r = tid >> 2, c0 = (tid & 3) * 16;
         C m = 0;
 #pragma unroll
         for (int q = 0; q < 16; q++) {
-            const C x = C(A[r * TS + c0 + q]);
-            a[r * LDA + c0 + q] = x;
+            const int c = c0 + q;
+            const C x = C(A[r * TS + c]);
+            a[r * LDA + c] = x;
             m = fabs(x) > m ? fabs(x) : m;
+        }
+        if constexpr (CHOL) {  // only the lower triangle is read: mirror it
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int c = c0 + q;
+                if (c > r) a[r * LDA + c] = a[c * LDA + r];
+            }
         }
         if (amax_out) *amax_out = (float)block_absmax(m, red, tid);
         else __syncthreads();
@@ -629,8 +671,8 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
     for (int K = 0; K < 4; K++) {
         const int r0 = 16 * K;
         if (w == 0) {
-            if constexpr (sizeof(C) == 8) diag16_split(a, r0, xd + K * BE, zd + K * BE, lane);
-            else diag16(a, r0, xd + K * BE, zd + K * BE, lane);
+            if constexpr (sizeof(C) == 8) diag16_split<C, CHOL>(a, r0, xd + K * BE, zd + K * BE, lane);
+            else diag16<C, CHOL>(a, r0, xd + K * BE, zd + K * BE, lane);
         }
         __syncthreads();
         if (K == 3) break;
@@ -659,8 +701,25 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
         }
         __syncthreads();
     }
-    // L (unit lower) and U to HBM
-    {
+    if constexpr (CHOL) {
+        // L = L_u diag(d) to HBM; the row scales of L^-1 stay in LDS
+        if (tid < TS) {
+            const C d = sqrt(a[tid * LDA + tid]);
+            dsc[tid] = d;
+            dsc[TS + tid] = C(1) / d;
+        }
+        __syncthreads();
+        const int r = tid >> 2, c0 = (tid & 3) * 16;
+        T* Lg = pool + (size_t)outl * TE + r * TS + c0;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int j = c0 + q;
+            Lg[q] = T(r > j ? a[r * LDA + j] * dsc[j] : (r == j ? dsc[j] : C(0)));
+        }
+        if (outli < 0) return;
+        // L_u^-1 is assembled below as for the LU; its rows are stored scaled by 1 / d
+    } else {
+        // L (unit lower) and U to HBM
         const int r = tid >> 2, c0 = (tid & 3) * 16;
         T* Lg = pool + (size_t)outl * TE + r * TS + c0;
         T* Ug = pool + (size_t)outu * TE + r * TS + c0;
@@ -675,8 +734,8 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
             um = fabs(u) > um ? fabs(u) : um;
         }
         if (umax_out) *umax_out = (float)block_absmax(um, red + 4, tid);
+        if (outli < 0 && outui < 0) return;
     }
-    if (outli < 0 && outui < 0) return;
     // X = L^-1 (lower) and Z = U^-1 (upper) from the diagonal-block inverses.
     // First level, one chain per wave: X10 = -X11 L10 X00, X32 = -X33 L32 X22,
     // Z01 = -Z00 U01 Z11, Z23 = -Z22 U23 Z33.
@@ -694,11 +753,16 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
         if (lo) mm16<C, 4>(acc, xd + (b + 1) * BE, BS, tw, BS, li, lk);
         else mm16<C, 4>(acc, zd + b * BE, BS, tw, BS, li, lk);
         st16(acc, x1 + w * BE, BS, li, lk, C(-1));
-        if (lo && Xg) st16_global(acc, Xg, b + 1, b, li, lk, C(-1));
+        if constexpr (CHOL) {
+            if (lo && Xg) st16_global_rs(acc, Xg, b + 1, b, li, lk, C(-1), (const C*)(dsc + TS));
+        } else {
+            if (lo && Xg) st16_global(acc, Xg, b + 1, b, li, lk, C(-1));
+        }
         if (!lo && Zg) st16_global(acc, Zg, b, b + 1, li, lk, C(-1));
         // diagonal blocks and the zero half of each inverse
         if (Xg) {
-            blk_to_global(xd + w * BE, BS, Xg, w, w, lane);
+            if constexpr (CHOL) blk_to_global_rs(xd + w * BE, BS, Xg, w, w, lane, (const C*)(dsc + TS));
+            else blk_to_global(xd + w * BE, BS, Xg, w, w, lane);
             for (int J = w + 1; J < 4; J++) zero_blk_global(Xg, w, J, lane);
         }
         if (Zg) {
@@ -751,8 +815,13 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
             mm16<C, 4>(c1, X32, BS, t0, BS, li, lk);
             mm16<C, 4>(c1, xd + 3 * BE, BS, t1, BS, li, lk);
             if (Xg) {
-                st16_global(c0, Xg, 2, col, li, lk, C(-1));
-                st16_global(c1, Xg, 3, col, li, lk, C(-1));
+                if constexpr (CHOL) {
+                    st16_global_rs(c0, Xg, 2, col, li, lk, C(-1), (const C*)(dsc + TS));
+                    st16_global_rs(c1, Xg, 3, col, li, lk, C(-1), (const C*)(dsc + TS));
+                } else {
+                    st16_global(c0, Xg, 2, col, li, lk, C(-1));
+                    st16_global(c1, Xg, 3, col, li, lk, C(-1));
+                }
             }
         } else {  // Z0c' = -(Z00 T0c + Z01 T1c), Z1c' = -Z11 T1c  (c' = 2 + col)
             mm16<C, 4>(c0, zd, BS, t0, BS, li, lk);
@@ -865,59 +934,15 @@ __device__ __attribute__((always_inline)) void tri_inv_task(T* __restrict__ pool
     store_rowseg(pool + (size_t)out * TE, v, lane, w);
 }
 
-// The llt task: Cholesky of one tile (its lower triangle), right-looking in LDS, two
-// barriers per column: every thread takes l_kk = sqrt(max(p, 1e-20)) from the current
-// pivot p (lltdcmpSimple, MatrixStdDouble.cpp:2629-2668, clamp 2612), the column below
-// it is scaled, then the trailing lower triangle takes the rank-1 update.  FP32 tiles
-// factor in FP32.  When outinv >= 0 the task also writes L^-1 (general diagonal).
+// The llt task: Cholesky of one tile (its lower triangle) and, when outinv >= 0, L^-1,
+// from the blocked elimination of the lu task in its Cholesky form (lu_inv_blocked,
+// CHOL): the pivots are lltdcmpSimple's p_j (MatrixStdDouble.cpp:2629-2668), clamped
+// to 1e-20 before the square root as it does (2612).  FP32 tiles factor in FP32.
 template <typename T>
 __device__ __attribute__((always_inline)) void llt_task(T* __restrict__ pool, int src, int out, int outinv,
                                                         char* lds_raw) {
     typedef typename std::conditional<sizeof(T) == 8, double, float>::type C;
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
-    C* a = reinterpret_cast<C*>(lds_raw);  // the tile, row stride LDA
-    C* lc = a + TS * LDA;                  // the scaled column of the step
-    C* bro = lc + TS + 16 * w;             // row broadcast of the inverse sweep
-    const int i = tid >> 2, j0 = (tid & 3) * 16;
-    const T* A = pool + (size_t)src * TE;
-#pragma unroll
-    for (int q = 0; q < 16; q++) a[i * LDA + j0 + q] = C(A[i * TS + j0 + q]);
-    __syncthreads();
-    const C TINY = C(1e-20);
-#pragma unroll 1
-    for (int k = 0; k < TS; k++) {
-        C p = a[k * LDA + k];
-        if (p < TINY) p = TINY;
-        const C s = sqrt(p);
-        const C inv = C(1) / s;
-        if (tid < TS && tid > k) {
-            const C l = a[tid * LDA + k] * inv;
-            lc[tid] = l;
-            a[tid * LDA + k] = l;
-        }
-        __syncthreads();
-        if (tid == 0) a[k * LDA + k] = s;
-        if (i > k) {
-            const C li = lc[i];
-#pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const int j = j0 + q;
-                if (j > k && j <= i) a[i * LDA + j] -= li * lc[j];
-            }
-        }
-        __syncthreads();
-    }
-    T* L = pool + (size_t)out * TE;
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const int j = j0 + q;
-        L[i * TS + j] = j <= i ? T(a[i * LDA + j]) : T(0);
-    }
-    if (outinv < 0) return;
-    C y[16];
-    inv_lower_regs<C, false>(a, y, bro, lane, w);
-    store_rowseg(pool + (size_t)outinv * TE, y, lane, w);
+    lu_inv_blocked<T, C, true>(pool, src, out, -1, outinv, -1, lds_raw, nullptr, nullptr);
 }
 
 // R = A^T through LDS (odd row stride: the column reads hit distinct banks).
