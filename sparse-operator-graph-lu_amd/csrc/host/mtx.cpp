@@ -110,11 +110,7 @@ int64_t read_mtx(const std::string& path, Coo& out, std::string* log) {
             if (rows > cols) rows = cols;
         }
         out.n = rows;
-        if (counts > 0) {
-            out.row.reserve((size_t)counts);
-            out.col.reserve((size_t)counts);
-            out.val.reserve((size_t)counts);
-        }
+        (void)counts;  // the entry count comes from the lines themselves
         have_size = true;
         break;
     }
@@ -139,6 +135,10 @@ int64_t read_mtx(const std::string& path, Coo& out, std::string* log) {
         Entries& e = part[(size_t)t];
         size_t q = cut[(size_t)t];
         const size_t stop = cut[(size_t)t + 1];
+        const size_t guess = (stop - q) / 16 + 16;  // entry lines are rarely shorter
+        e.row.reserve(guess);
+        e.col.reserve(guess);
+        e.val.reserve(guess);
         while (q < stop) {
             size_t x = line_end(q);
             const char* b = data + q;
@@ -149,10 +149,17 @@ int64_t read_mtx(const std::string& path, Coo& out, std::string* log) {
             parse_entry(b, len, rows, e);
         }
     }
-    for (const Entries& e : part) {
-        out.row.insert(out.row.end(), e.row.begin(), e.row.end());
-        out.col.insert(out.col.end(), e.col.begin(), e.col.end());
-        out.val.insert(out.val.end(), e.val.begin(), e.val.end());
+    std::vector<size_t> at((size_t)nchunk + 1, 0);
+    for (int t = 0; t < nchunk; t++) at[(size_t)t + 1] = at[(size_t)t] + part[(size_t)t].val.size();
+    out.row.resize(at[(size_t)nchunk]);
+    out.col.resize(at[(size_t)nchunk]);
+    out.val.resize(at[(size_t)nchunk]);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int t = 0; t < nchunk; t++) {
+        const Entries& e = part[(size_t)t];
+        std::copy(e.row.begin(), e.row.end(), out.row.begin() + (ptrdiff_t)at[(size_t)t]);
+        std::copy(e.col.begin(), e.col.end(), out.col.begin() + (ptrdiff_t)at[(size_t)t]);
+        std::copy(e.val.begin(), e.val.end(), out.val.begin() + (ptrdiff_t)at[(size_t)t]);
     }
     return (int64_t)out.val.size();
 }
