@@ -159,6 +159,15 @@ int sogl_ctx_sync(sogl_ctx* ctx);
 #define SOGL_EXEC_GRAPH 3      /* the level schedule captured once into a hipGraph, then replayed */
 #define SOGL_EXEC_AUTO 4       /* the configuration sogl_factor_autotune picked (tunes on first use) */
 
+/* 16x16 block masks (the reference's 8x8 sub-block masks, MatrixStdDouble.cpp:120-154,
+ * at MFMA block size): a block with no |v| > 1e-18 counts as zero, and products and
+ * substitutions skip it.  ON always, OFF never (dense tiles), AUTO (default): the first
+ * factor run is masked, later runs keep the masks when at least a quarter of the factor
+ * tiles' blocks came out zero.  sogl_factor_stats reports the share and the choice. */
+#define SOGL_MASKS_OFF 0
+#define SOGL_MASKS_ON 1
+#define SOGL_MASKS_AUTO 2
+
 typedef struct sogl_factor_desc {
     int32_t n_ids;          /* storage_count */
     int32_t block_rows;     /* fine block rows */
@@ -202,6 +211,9 @@ int sogl_factor_run_config(sogl_ctx* ctx, int32_t factor_id, int32_t exec_mode, 
 #define SOGL_TUNE_CANDIDATES 6
 int sogl_factor_autotune(sogl_ctx* ctx, int32_t factor_id, int32_t trials, double* times_ms,
                          int32_t* best);
+/* Block-mask policy of a factor (SOGL_MASKS_*); takes effect at the next factor run.
+ * No reference counterpart beyond the masks themselves. */
+int sogl_factor_set_masks(sogl_ctx* ctx, int32_t factor_id, int32_t mode);
 /* Triangular solve: replaces BlockPlanner::solve() (BlockPlanner.cpp:834-862), called
  * from decompose_solveLU (solver.cpp:115).  bp, xp: host arrays in the permuted padded
  * order, 64*block_rows x nrhs column-major.  y (may be NULL) receives what the
@@ -225,6 +237,9 @@ typedef struct sogl_factor_stats {
     int32_t n_levels;       /* dependency levels of the task graph */
     int32_t growth_tiles;   /* low precision only: diagonal tiles factored in FP64 */
     double max_growth;      /* largest pivot-free growth factor seen */
+    double zero_blocks;     /* share of the factor tiles' 16x16 blocks that are zero (last masked run) */
+    int32_t block_masks;    /* 1 when the next run uses block masks */
+    int32_t reserved;
 } sogl_factor_stats;
 int sogl_factor_get_stats(sogl_ctx* ctx, int32_t factor_id, sogl_factor_stats* st);
 /* Copy factor tiles back (FP64 view).  which 0 = L, 1 = U, in the order given at load. */

@@ -53,6 +53,7 @@ struct Factor {
     int64_t n_folded = 0;       // copies, negations and subtractions folded into other tasks
     // device arrays
     DevBuf pool, tasks_d, pairs_d, deps_d, ready_d, ctrl_d, op_tasks_d, op_pairs_d, inputs_d;
+    DevBuf mask_d;                         // per slot: 16x16 block mask (uint32)
     int32_t epoch = 0;
     // triangular solve structure (block CSR by block row, off-diagonal only)
     std::vector<int32_t> l_diag, u_diag, linv_slot, uinv_slot;  // per block row
@@ -81,6 +82,11 @@ struct Factor {
     hipGraphExec_t graph_exec = nullptr;
     int32_t tuned = -1;  // index into the tuning candidates, -1: not tuned
     bool llt = false;    // L L^T factor: the U tiles of the solve are transposes of L tiles
+    // 16x16 block masks: SOGL_MASKS_OFF / ON / AUTO (masked first run, then kept when a
+    // quarter or more of the factor's blocks are zero)
+    int32_t mask_mode = SOGL_MASKS_AUTO;
+    bool masks_on = true, masks_decided = false, graph_masks = false;
+    double zero_blocks = 0;  // share of zero blocks in the last masked run's factor tiles
 };
 
 struct Context {

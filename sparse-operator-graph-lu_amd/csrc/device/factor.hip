@@ -105,7 +105,7 @@ struct Args {
     const DevTask* tasks;
     const Pair* pairs;
     const int32_t* deps;
-    int* ready;
+    unsigned long long* ready;  // per slot: epoch << 32 | block mask of the tile made in that run
     int* ctrl;
     int32_t ntasks;
     int32_t epoch;
@@ -113,22 +113,27 @@ struct Args {
     int32_t chunk;        // persistent executor: tasks taken per fetch from the queue
     double growth_limit;
     long long* trace;     // optional: per task {taken, work done, published, last wait ended}
+    uint32_t* mask;       // per slot: 16x16 block mask of the tile (see tile_ops.h)
 };
 
-// Waits (thread 0 polls, relaxed) until ready[s] >= epoch; on abort or timeout it
-// raises ctrl[1] and returns.  Relaxed polling keeps the spin loop free of cache
+// Waits (thread 0 polls, relaxed) until the epoch in ready[s] reaches this run's and
+// returns the flag (its low word is the tile's block mask); on abort or timeout it
+// raises ctrl[1] and returns 0.  Relaxed polling keeps the spin loop free of cache
 // invalidations; the caller issues one acquire fence after the barrier.
-__device__ inline void spin_ready(const Args& a, int s) {
-    int* flag = a.ready + s;
-    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= a.epoch) return;
+__device__ inline unsigned long long spin_ready(const Args& a, int s) {
+    unsigned long long* flag = a.ready + s;
+    const unsigned long long want = (unsigned long long)(unsigned)a.epoch << 32;
+    unsigned long long v = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (v >= want) return v;
     const long long t0 = wall_clock64();
     for (;;) {
         __builtin_amdgcn_s_sleep(1);
-        if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= a.epoch) return;
-        if (__hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+        v = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (v >= want) return v;
+        if (__hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return 0;
         if (wall_clock64() - t0 > SPIN_LIMIT_TICKS) {
             __hip_atomic_store(a.ctrl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return;
+            return 0;
         }
     }
 }
@@ -142,36 +147,45 @@ __device__ inline void spin_ready(const Args& a, int s) {
 __device__ __forceinline__ void acquire_for_workgroup() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 __device__ __forceinline__ void release_for_workgroup() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
 
+// The masks come with the flags: thread 0 leaves them in LDS, in one of two slots by
+// the pair's parity (a wave may still be reading the previous pair's pair of masks when
+// thread 0 writes the next one; every other pair has a barrier in between).
 struct PairWait {
     const Args* a;
     long long* t_wait;  // tracing: wall clock when the latest wait ended (thread 0)
-    __device__ inline void operator()(int sa, int sb) const {
+    uint32_t* s_pm;     // LDS, 4 words
+    __device__ inline void operator()(int s, int sa, int sb, uint32_t& ma, uint32_t& mb) const {
+        uint32_t* pm = s_pm + 2 * (s & 1);
         if (threadIdx.x == 0) {
-            spin_ready(*a, sa);
-            spin_ready(*a, sb);
+            pm[0] = (uint32_t)spin_ready(*a, sa);
+            pm[1] = (uint32_t)spin_ready(*a, sb);
             acquire_for_workgroup();
             if (a->trace) *t_wait = wall_clock64();
         }
         __syncthreads();
+        ma = pm[0];
+        mb = pm[1];
     }
 };
 
 // LLT: the kernel of an L L^T factor (llt, transpose and A B^T tasks, no lu); the LU
 // kernels are built without those, which keeps their register use as it was.
-template <typename T, bool BF16, bool ACC, bool LLT, typename Wait = NoWait>
-__device__ __attribute__((always_inline)) void run_task(const DevTask& t, const Args& a, char* lds, const Wait& wait = Wait()) {
+template <typename T, bool BF16, bool ACC, bool LLT, bool MASKS, typename Wait>
+__device__ __attribute__((always_inline)) void run_task(const DevTask& t, const Args& a, char* lds, uint32_t* s_mask,
+                                                        const Wait& wait) {
     T* pool = reinterpret_cast<T*>(a.pool);
     switch (t.kind) {
         case TK_GEMM:
             if constexpr (BF16 && !ACC) {
-                gemm_task_bf16<LLT>(reinterpret_cast<float*>(a.pool), a.pairs + t.a, t.b, t.out0, lds, wait, t.add,
-                                    t.add_neg != 0);
+                gemm_task_bf16<LLT, MASKS>(reinterpret_cast<float*>(a.pool), a.pairs + t.a, t.b, t.out0, lds, s_mask,
+                                    wait, t.add, t.add_neg != 0);
             } else {
-                gemm_task<T, ACC, LLT>(pool, a.pairs + t.a, t.b, t.out0, lds, wait, t.add, t.add_neg != 0);
+                gemm_task<T, ACC, LLT, MASKS>(pool, a.pairs + t.a, t.b, t.out0, lds, s_mask, wait, t.add,
+                                       t.add_neg != 0);
             }
             break;
         case TK_SUB:
-            sub_task<T>(pool, t.b, t.a, t.out0);
+            sub_task<T, MASKS>(pool, t.b, t.a, t.out0, s_mask);
             break;
         case TK_LU:
             if constexpr (!LLT) {
@@ -192,21 +206,37 @@ __device__ __attribute__((always_inline)) void run_task(const DevTask& t, const 
             if constexpr (LLT) llt_task<T>(pool, t.a, t.out0, t.out2, lds);
             break;
         case TK_TRANS:
-            if constexpr (LLT) trans_task<T>(pool, t.a, t.out0, lds);
+            if constexpr (LLT) trans_task<T, MASKS>(pool, t.a, t.out0, lds, s_mask);
             break;
         default:
             break;
     }
 }
 
+// Block mask of a finished task's outputs (thread 0, after the task's last barrier):
+// GEMM, sub and transpose tasks leave their waves' bits in s_mask; the tiles of lu /
+// llt and the inverses, and everything when masks are off, count as full.
+template <bool MASKS>
+__device__ inline uint32_t task_mask(const DevTask& t, const uint32_t* s_mask) {
+    if (MASKS && (t.kind == TK_GEMM || t.kind == TK_SUB || t.kind == TK_TRANS))
+        return s_mask[0] | s_mask[1] | s_mask[2] | s_mask[3];
+    return MASK_ALL;
+}
+__device__ inline void write_masks(const DevTask& t, const Args& a, uint32_t m) {
+    for (int32_t o : {t.out0, t.out1, t.out2, t.out3})
+        if (o >= 0) a.mask[o] = m;
+}
+
 // Persistent executor: every workgroup loops taking the next task index.  GEMM
 // tasks wait pair by pair (pairs are ordered by when their operands become
 // ready); other tasks wait for all their sources first.
-template <typename T, bool BF16, bool LLT>
+template <typename T, bool BF16, bool LLT, bool MASKS>
 __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
     __shared__ __attribute__((aligned(16))) char lds[lds_bytes<T>()];
     __shared__ int s_task;
     __shared__ int s_abort;
+    __shared__ uint32_t s_mask[NT / 64];
+    __shared__ uint32_t s_pm[4];
     const int tid = threadIdx.x;
     for (;;) {
         if (tid == 0) {
@@ -235,20 +265,27 @@ __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
         if (tid == 0) s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         if (__builtin_amdgcn_readfirstlane(s_abort)) break;
-        run_task<T, BF16, false, LLT>(t, a, lds, PairWait{&a, &t_wait});
+        run_task<T, BF16, false, LLT, MASKS>(t, a, lds, s_mask, PairWait{&a, &t_wait, s_pm});
         // publish: every wave's stores complete (the barrier's workgroup release), then
-        // thread 0 makes them visible at device scope once and raises the flags
+        // thread 0 writes the output masks, makes everything visible at device scope
+        // once and raises the flags, which carry the masks to the consumers
         __syncthreads();
         if (a.trace && tid == 0) t_ready = wall_clock64();  // work done, publish starts
         if (tid == 0) {
+            const uint32_t m = task_mask<MASKS>(t, s_mask);
             release_for_workgroup();
-            __hip_atomic_store(a.ready + t.out0, a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long fv = ((unsigned long long)(unsigned)a.epoch << 32) | m;
+            __hip_atomic_store(a.ready + t.out0, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (t.out1 >= 0)
-                __hip_atomic_store(a.ready + t.out1, a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(a.ready + t.out1, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (t.out2 >= 0)
-                __hip_atomic_store(a.ready + t.out2, a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(a.ready + t.out2, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (t.out3 >= 0)
-                __hip_atomic_store(a.ready + t.out3, a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(a.ready + t.out3, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // consumers in this run read the masks from the flags; the mask array serves
+            // the solve and the other executors, after the kernel, so its stores stay off
+            // the release above
+            write_masks(t, a, m);
             if (a.trace) {
                 long long* tr = a.trace + (size_t)ti * 4;
                 tr[0] = t_taken;
@@ -265,11 +302,14 @@ __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
 }
 
 // Levels / per-op executors: one workgroup per task of the launch.
-template <typename T, bool BF16, bool ACC, bool LLT>
+template <typename T, bool BF16, bool ACC, bool LLT, bool MASKS>
 __global__ __launch_bounds__(NT) void k_factor_batch(Args a) {
     __shared__ __attribute__((aligned(16))) char lds[lds_bytes<T>()];
+    __shared__ uint32_t s_mask[NT / 64];
     const DevTask t = a.tasks[a.task_offset + blockIdx.x];
-    run_task<T, BF16, ACC, LLT>(t, a, lds);
+    run_task<T, BF16, ACC, LLT, MASKS>(t, a, lds, s_mask, NoWait{a.mask});
+    __syncthreads();
+    if (threadIdx.x == 0) write_masks(t, a, task_mask<MASKS>(t, s_mask));
 }
 
 __global__ void k_f64_to_f32(const double* in, float* out, size_t n) {
@@ -288,6 +328,25 @@ __global__ void k_scatter_cells(T* pool, const int64_t* begin, const uint16_t* p
         T* tile = pool + (size_t)t * TE;
         const int64_t q1 = begin[t + 1];
         for (int64_t q = begin[t] + threadIdx.x; q < q1; q += blockDim.x) tile[pos[q]] = (T)val[q];
+    }
+}
+
+// Block masks of tiles already in the pool (the inputs), one workgroup per tile; their
+// ready flags say "always ready" (epoch field 0x7fffffff) and carry the mask.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_tile_masks(const T* __restrict__ pool, uint32_t* mask,
+                                                   unsigned long long* ready, int64_t n_tiles) {
+    __shared__ uint32_t s[NT / 64];
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const uint32_t b = tile_mask(pool + (size_t)t * TE, threadIdx.x);
+        if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = b;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t m = s[0] | s[1] | s[2] | s[3];
+            mask[t] = m;
+            ready[t] = (0x7fffffffull << 32) | m;
+        }
+        __syncthreads();
     }
 }
 
@@ -869,7 +928,8 @@ int upload_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells,
     if ((rc = dev_alloc(c, f->tasks_d, std::max<size_t>(T, 1) * sizeof(DevTask), "tasks")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->pairs_d, std::max<size_t>(pairs.size(), 1) * sizeof(Pair), "pairs")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->deps_d, std::max<size_t>(deps.size(), 1) * 4, "deps")) != SOGL_OK) return rc;
-    if ((rc = dev_alloc(c, f->ready_d, (size_t)f->n_slots * 4, "ready flags")) != SOGL_OK) return rc;
+    if ((rc = dev_alloc(c, f->ready_d, (size_t)f->n_slots * 8, "ready flags")) != SOGL_OK) return rc;
+    if ((rc = dev_alloc(c, f->mask_d, (size_t)f->n_slots * 4, "block masks")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->ctrl_d, CTRL_WORDS * 4, "control")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->op_tasks_d, std::max<size_t>(f->op_tasks.size(), 1) * sizeof(DevTask), "op tasks")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->op_pairs_d, std::max<size_t>(op_pairs.size(), 1) * sizeof(Pair), "op pairs")) != SOGL_OK) return rc;
@@ -892,13 +952,12 @@ int upload_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells,
         (rc = up_i32(f->linv_d, f->linv_slot, "L inv")) || (rc = up_i32(f->uinv_d, f->uinv_slot, "U inv")) ||
         (rc = up_i32(f->inputs_d, f->input_slots, "input slots")))
         return rc;
-    hipLaunchKernelGGL(k_fill<int>, dim3(256), dim3(256), 0, st, (int*)f->ready_d.p, (size_t)f->n_slots, 0);
+    hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(256), dim3(256), 0, st, (unsigned long long*)f->ready_d.p,
+                       (size_t)f->n_slots, 0ull);
     // input tiles (slots 0..n_inputs-1) and the zero tile never change: always ready
-    if (d->n_inputs > 0)
-        hipLaunchKernelGGL(k_fill<int>, dim3(256), dim3(256), 0, st, (int*)f->ready_d.p, (size_t)d->n_inputs,
-                           0x7fffffff);
-    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, st, (int*)f->ready_d.p + f->zero_slot, (size_t)1,
-                       0x7fffffff);
+    // (the inputs' flags are set with their masks after the upload, below)
+    hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(64), 0, st,
+                       (unsigned long long*)f->ready_d.p + f->zero_slot, (size_t)1, 0x7fffffffull << 32);
     // the zero tile
     if (f->elem == 8)
         hipLaunchKernelGGL(k_fill<double>, dim3(16), dim3(256), 0, st,
@@ -961,6 +1020,19 @@ int upload_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells,
             dev_free(tmp);
         }
     }
+    // block masks: the inputs' from their values, every other slot (the zero tile
+    // included) empty until its task writes it
+    hipLaunchKernelGGL(k_fill<int>, dim3(256), dim3(256), 0, st, (int*)f->mask_d.p, (size_t)f->n_slots, 0);
+    if (d->n_inputs > 0) {
+        const unsigned grid = (unsigned)std::min<int64_t>(d->n_inputs, 65536);
+        if (f->elem == 8)
+            hipLaunchKernelGGL(k_tile_masks<double>, dim3(grid), dim3(NT), 0, st, (const double*)f->pool.p,
+                               (uint32_t*)f->mask_d.p, (unsigned long long*)f->ready_d.p, (int64_t)d->n_inputs);
+        else
+            hipLaunchKernelGGL(k_tile_masks<float>, dim3(grid), dim3(NT), 0, st, (const float*)f->pool.p,
+                               (uint32_t*)f->mask_d.p, (unsigned long long*)f->ready_d.p, (int64_t)d->n_inputs);
+    }
+    HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(st));
     return SOGL_OK;
 }
@@ -976,7 +1048,7 @@ int build_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells, 
 void free_factor(Factor* f) {
     if (f->graph_exec) (void)hipGraphExecDestroy(f->graph_exec);
     if (f->graph) (void)hipGraphDestroy(f->graph);
-    for (DevBuf* b : {&f->pool, &f->tasks_d, &f->pairs_d, &f->deps_d, &f->ready_d, &f->ctrl_d,
+    for (DevBuf* b : {&f->pool, &f->mask_d, &f->tasks_d, &f->pairs_d, &f->deps_d, &f->ready_d, &f->ctrl_d,
                       &f->op_tasks_d, &f->op_pairs_d, &f->inputs_d, &f->l_rowptr, &f->l_col,
                       &f->l_slot, &f->u_rowptr, &f->u_col, &f->u_slot, &f->l_diag_d, &f->u_diag_d,
                       &f->linv_d, &f->uinv_d, &f->solve_ready, &f->solve_ctrl, &f->vec_b, &f->vec_x,
@@ -985,14 +1057,15 @@ void free_factor(Factor* f) {
     delete f;
 }
 
-template <typename T, bool BF16, bool LLT>
+template <typename T, bool BF16, bool LLT, bool MASKS>
 int launch_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
     hipStream_t st = c->stream;
     Args a{};
     a.chunk = std::max(1, chunk);
     a.pool = f->pool.p;
-    a.ready = (int*)f->ready_d.p;
+    a.ready = (unsigned long long*)f->ready_d.p;
     a.ctrl = (int*)f->ctrl_d.p;
+    a.mask = (uint32_t*)f->mask_d.p;
     a.epoch = f->epoch;
     a.growth_limit = f->growth_limit;
     a.deps = (const int32_t*)f->deps_d.p;
@@ -1005,7 +1078,7 @@ int launch_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
             int cnt = f->level_begin[v + 1] - f->level_begin[v];
             if (cnt <= 0) continue;
             a.task_offset = f->level_begin[v];
-            hipLaunchKernelGGL((k_factor_batch<T, BF16, false, LLT>), dim3(cnt), dim3(NT), 0, st, a);
+            hipLaunchKernelGGL((k_factor_batch<T, BF16, false, LLT, MASKS>), dim3(cnt), dim3(NT), 0, st, a);
         }
     };
     if (mode == SOGL_EXEC_GRAPH && !f->graph_exec) {
@@ -1027,11 +1100,11 @@ int launch_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
         a.pairs = (const Pair*)f->pairs_d.p;
         a.ntasks = (int32_t)f->tasks.size();
         int per_cu = 0;
-        HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_persistent<T, BF16, LLT>, NT, 0));
+        HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_persistent<T, BF16, LLT, MASKS>, NT, 0));
         if (per_cu < 1) per_cu = 1;
         if (wg_per_cu > 0) per_cu = std::min(per_cu, wg_per_cu);  // never more than fit: all must be resident
         int grid = std::max(1, std::min<int>(a.ntasks, c->num_cu * per_cu));
-        if (a.ntasks > 0) hipLaunchKernelGGL((k_factor_persistent<T, BF16, LLT>), dim3(grid), dim3(NT), 0, st, a);
+        if (a.ntasks > 0) hipLaunchKernelGGL((k_factor_persistent<T, BF16, LLT, MASKS>), dim3(grid), dim3(NT), 0, st, a);
     } else if (mode == SOGL_EXEC_LEVELS) {
         launch_levels();
     } else if (mode == SOGL_EXEC_GRAPH) {
@@ -1043,12 +1116,13 @@ int launch_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
         const size_t nres = (size_t)f->n_slots - first_result;
         hipLaunchKernelGGL(k_fill<T>, dim3(4096), dim3(256), 0, st,
                            (T*)f->pool.p + first_result * TE, nres * TE, T(0));
+        hipLaunchKernelGGL(k_fill<int>, dim3(256), dim3(256), 0, st, (int*)f->mask_d.p + first_result, nres, 0);
         a.tasks = (const DevTask*)f->op_tasks_d.p;
         a.pairs = (const Pair*)f->op_pairs_d.p;
         a.ntasks = (int32_t)f->op_tasks.size();
         for (int32_t k = 0; k < a.ntasks; k++) {
             a.task_offset = k;
-            hipLaunchKernelGGL((k_factor_batch<T, false, true, LLT>), dim3(1), dim3(NT), 0, st, a);
+            hipLaunchKernelGGL((k_factor_batch<T, false, true, LLT, MASKS>), dim3(1), dim3(NT), 0, st, a);
         }
     } else {
         return set_error(c, SOGL_ERR_ARG, "unknown executor mode");
@@ -1206,15 +1280,24 @@ int run_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
     f->epoch++;
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
     int rc;
-    if (f->llt) {
-        if (f->precision == SOGL_FP64) rc = launch_factor<double, false, true>(c, f, mode, chunk, wg_per_cu);
-        else if (f->precision == SOGL_FP32) rc = launch_factor<float, false, true>(c, f, mode, chunk, wg_per_cu);
-        else rc = launch_factor<float, true, true>(c, f, mode, chunk, wg_per_cu);
-    } else {
-        if (f->precision == SOGL_FP64) rc = launch_factor<double, false, false>(c, f, mode, chunk, wg_per_cu);
-        else if (f->precision == SOGL_FP32) rc = launch_factor<float, false, false>(c, f, mode, chunk, wg_per_cu);
-        else rc = launch_factor<float, true, false>(c, f, mode, chunk, wg_per_cu);
+    const bool masks = f->mask_mode == SOGL_MASKS_ON || (f->mask_mode == SOGL_MASKS_AUTO && f->masks_on);
+    if (f->graph_exec && f->graph_masks != masks) {  // the captured schedule is of the other variant
+        (void)hipGraphExecDestroy(f->graph_exec);
+        (void)hipGraphDestroy(f->graph);
+        f->graph_exec = nullptr;
+        f->graph = nullptr;
     }
+    f->graph_masks = masks;
+    auto go = [&](auto llt_c, auto masks_c) {
+        constexpr bool L = decltype(llt_c)::value, M = decltype(masks_c)::value;
+        if (f->precision == SOGL_FP64) return launch_factor<double, false, L, M>(c, f, mode, chunk, wg_per_cu);
+        if (f->precision == SOGL_FP32) return launch_factor<float, false, L, M>(c, f, mode, chunk, wg_per_cu);
+        return launch_factor<float, true, L, M>(c, f, mode, chunk, wg_per_cu);
+    };
+    using TT = std::true_type;
+    using FF = std::false_type;
+    if (f->llt) rc = masks ? go(TT{}, TT{}) : go(TT{}, FF{});
+    else rc = masks ? go(FF{}, TT{}) : go(FF{}, FF{});
     if (rc != SOGL_OK) return rc;
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     HIP_TRY(c, hipEventSynchronize(c->ev1));
@@ -1230,6 +1313,24 @@ int run_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
     std::memcpy(&gm, &ctrl[3], 4);
     f->max_growth = gm;
     if (ctrl[1]) return set_error(c, SOGL_ERR_TIMEOUT, "factor: a dependency wait exceeded its bound");
+    if (masks) {
+        // the share of the factor's 16x16 blocks that count as zero
+        std::vector<uint32_t> m((size_t)f->n_slots);
+        HIP_TRY(c, hipMemcpy(m.data(), f->mask_d.p, m.size() * 4, hipMemcpyDeviceToHost));
+        int64_t zero = 0, total = 0;
+        for (const auto* v : {&f->l_slots, &f->u_slots})
+            for (int32_t sl : *v) {
+                zero += 16 - __builtin_popcount(m[(size_t)sl] & MASK_ALL);
+                total += 16;
+            }
+        f->zero_blocks = total ? (double)zero / (double)total : 0.0;
+        // automatic: after the first masked run keep the masks only where they pay, a
+        // factor with a quarter or more of its blocks zero
+        if (f->mask_mode == SOGL_MASKS_AUTO && !f->masks_decided) {
+            f->masks_on = f->zero_blocks >= 0.25;
+            f->masks_decided = true;
+        }
+    }
     return SOGL_OK;
 }
 
@@ -1334,6 +1435,25 @@ extern "C" int sogl_factor_get_stats(sogl_ctx* x, int32_t factor_id, sogl_factor
     st->n_levels = (int32_t)f->level_begin.size() - 1;
     st->growth_tiles = f->growth_tiles;
     st->max_growth = f->max_growth;
+    st->zero_blocks = f->zero_blocks;
+    st->block_masks = (f->mask_mode == SOGL_MASKS_ON || (f->mask_mode == SOGL_MASKS_AUTO && f->masks_on)) ? 1 : 0;
+    st->reserved = 0;
+    return SOGL_OK;
+}
+
+extern "C" int sogl_factor_set_masks(sogl_ctx* x, int32_t factor_id, int32_t mode) {
+    if (!x) return set_error(nullptr, SOGL_ERR_ARG, "null argument");
+    Context* c = &x->c;
+    std::lock_guard<std::mutex> g(c->mu);
+    Factor* f = find_factor(c, factor_id);
+    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+    if (mode != SOGL_MASKS_OFF && mode != SOGL_MASKS_ON && mode != SOGL_MASKS_AUTO)
+        return set_error(c, SOGL_ERR_ARG, "unknown block-mask mode");
+    f->mask_mode = mode;
+    if (mode == SOGL_MASKS_AUTO) {  // decide again from the next (masked) run
+        f->masks_on = true;
+        f->masks_decided = false;
+    }
     return SOGL_OK;
 }
 

@@ -41,6 +41,7 @@ struct SolveArgs {
     int32_t epoch;
     int* ready;
     int* ctrl;            // [0] row counter, [1] abort
+    const uint32_t* mask; // per slot: the factor's 16x16 block masks (a zero tile is skipped)
 };
 
 // thread (r, kp) of a row task owns tile row r, columns 16kp .. 16kp+15
@@ -99,8 +100,16 @@ __global__ __launch_bounds__(NT) void k_trsv_persistent(SolveArgs a) {
             for (int i = 0; i < cnt; i++) {
                 const int e = a.forward ? e0 + i : e1 - 1 - i;
                 const int J = a.col[e];
+                // a tile whose blocks all count as zero adds nothing: no wait for row J
+                const uint32_t tm = a.mask[a.slot[e]];
+                if (tm == 0) continue;
                 double seg[16];
-                load_seg(pool + (size_t)a.slot[e] * TE, r, kp, seg);
+                if ((tm >> (4 * (r >> 4) + kp)) & 1u) {
+                    load_seg(pool + (size_t)a.slot[e] * TE, r, kp, seg);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 16; k++) seg[k] = 0.0;
+                }
                 if (c0 == 0) {
                     // one thread waits and acquires for the workgroup; the barrier
                     // orders the other waves after it
@@ -166,10 +175,12 @@ __global__ __launch_bounds__(NT) void k_trsv_persistent(SolveArgs a) {
 // GEMM (tiles of any storage precision widened to FP64), then X_I = Linv_II R.
 // Columns past nrhs read as zero and are not stored.
 template <typename T>
-__device__ inline void tile_to_lds_f64(const T* __restrict__ g, double* __restrict__ s, int ld, int tid) {
+__device__ inline void tile_to_lds_f64(const T* __restrict__ g, double* __restrict__ s, int ld, int tid,
+                                       uint32_t m = MASK_ALL) {
     const int r = tid >> 2, c0 = (tid & 3) * 16;
+    const bool on = (m >> (4 * (r >> 4) + (tid & 3))) & 1u;  // zero blocks are written, not read
 #pragma unroll
-    for (int q = 0; q < 16; q++) s[r * ld + c0 + q] = (double)g[r * TS + c0 + q];
+    for (int q = 0; q < 16; q++) s[r * ld + c0 + q] = on ? (double)g[r * TS + c0 + q] : 0.0;
 }
 
 // rows row0 .. row0+63 of the column-major vector block v (column stride ld),
@@ -239,12 +250,14 @@ __global__ __launch_bounds__(NT, 2) void k_trsm_persistent(SolveArgs a) {
         for (int i = 0; i < cnt; i++) {
             const int e = a.forward ? e0 + i : e1 - 1 - i;
             const int J = a.col[e];
+            const uint32_t tm = a.mask[a.slot[e]];
+            if (tm == 0) continue;  // a zero tile: no product, no wait for row J
             if (tid == 0) {
                 spin_row(a, J);
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
             __syncthreads();  // also: the previous products are done with la / lb
-            tile_to_lds_f64(pool + (size_t)a.slot[e] * TE, la, LDA, tid);
+            tile_to_lds_f64(pool + (size_t)a.slot[e] * TE, la, LDA, tid, tm);
 #pragma unroll
             for (int p = 0; p < NP; p++) {
                 if (p) __syncthreads();  // the previous panel's product is done with lb
@@ -338,6 +351,7 @@ int launch_pass(Context* c, Factor* f, bool forward, const double* in, double* o
     a.epoch = ++f->solve_epoch;
     a.ready = (int*)f->solve_ready.p;
     a.ctrl = (int*)f->solve_ctrl.p;
+    a.mask = (const uint32_t*)f->mask_d.p;
     hipLaunchKernelGGL(k_solve_ctrl_reset, dim3(1), dim3(64), 0, c->stream, a.ctrl);
     int per_cu = 0;
     const bool mfma = nrhs >= TRSM_MIN_RHS;

@@ -54,6 +54,7 @@ constexpr int lds_bytes() {
 
 // ------------------------------------------------------------------ loads
 // thread t moves row t/4, columns (t%4)*16 .. +15 of a tile between global and LDS
+// (that segment lies in 16x16 block (t/64, t%4))
 template <typename T>
 __device__ inline void tile_to_lds(const T* __restrict__ g, T* __restrict__ s, int ld, int tid) {
     const int r = tid >> 2, c0 = (tid & 3) * 16;
@@ -79,6 +80,40 @@ __device__ inline void tile_to_lds(const T* __restrict__ g, T* __restrict__ s, i
         }
     }
 }
+
+// The same with the blocks whose mask bit is clear written as zeros, not loaded.
+template <typename T>
+__device__ inline void tile_to_lds_masked(const T* __restrict__ g, T* __restrict__ s, int ld, int tid, uint32_t m) {
+    const int r = tid >> 2, c0 = (tid & 3) * 16;
+    if ((m >> (4 * (r >> 4) + (tid & 3))) & 1u) {
+        tile_to_lds(g, s, ld, tid);
+    } else {
+        T* dst = s + r * ld + c0;
+#pragma unroll
+        for (int q = 0; q < 16; q++) dst[q] = T(0);
+    }
+}
+
+// ------------------------------------------------------------------ 16x16 block masks
+// Bit 4*I + J of a tile's mask is set when its 16x16 block (I, J) holds an entry with
+// |v| > 1e-18.  A block whose bit is clear counts as zero in every product that reads
+// the tile: it is neither loaded nor multiplied, and a tile with no bit set takes no
+// part at all.  This is the reference's 8x8 sub-block masking (MatrixStdDouble.cpp:
+// 120-154, 268-289; GEMM stores set a block's bit where some |v| > 1e-18, 622-623),
+// at the 16x16 granularity of the MFMA blocks.
+constexpr double MASK_TINY = 1e-18;
+constexpr uint32_t MASK_ALL = 0xFFFFu;
+
+__host__ __device__ inline uint32_t mask_transpose(uint32_t m) {
+    uint32_t t = 0;
+    for (int I = 0; I < 4; I++)
+        for (int J = 0; J < 4; J++)
+            if ((m >> (4 * I + J)) & 1u) t |= 1u << (4 * J + I);
+    return t;
+}
+
+// the four blocks of block row I: bits 4I .. 4I+3
+__host__ __device__ inline uint32_t mask_row(uint32_t m, int I) { return (m >> (4 * I)) & 0xFu; }
 
 // ------------------------------------------------------------------ GEMM
 template <typename T>
@@ -113,34 +148,68 @@ __host__ __device__ inline int pair_b(const Pair& p) { return p.b < 0 ? -p.b - 1
 
 // acc += sg * A B (or A B^T) from LDS: A rows with stride LDA; B row-major with stride
 // LDB, or, transposed, stored row-major with stride LDA and read by columns.
+// kuse: bit K set when the wave's products need the 16-wide k block K (k ascending,
+// as in the dense loop, so nothing changes where every block is used).
 template <typename T, bool TB, typename acc_t>
 __device__ __forceinline__ void gemm_lds(acc_t (&acc)[2][2], const T* la, const T* lb, T sg, int wr, int wc,
-                                         int li, int lk) {
-#pragma unroll 4
-    for (int kk = 0; kk < TS / 4; kk++) {
-        const int k = 4 * kk + lk;
-        T af[2], bf[2];
+                                         int li, int lk, uint32_t kuse) {
+    for (int K = 0; K < 4; K++) {
+        if (!((kuse >> K) & 1u)) continue;  // wave-uniform
 #pragma unroll
-        for (int m = 0; m < 2; m++) af[m] = sg * la[(wr + m * 16 + li) * LDA + k];
+        for (int k4 = 0; k4 < 4; k4++) {
+            const int k = 16 * K + 4 * k4 + lk;
+            T af[2], bf[2];
+#pragma unroll
+            for (int m = 0; m < 2; m++) af[m] = sg * la[(wr + m * 16 + li) * LDA + k];
+#pragma unroll
+            for (int n = 0; n < 2; n++)
+                bf[n] = TB ? lb[(wc + n * 16 + li) * LDA + k] : lb[k * LDB + wc + n * 16 + li];
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+                for (int n = 0; n < 2; n++) acc[m][n] = MfmaOp<T>::mma(af[m], bf[n], acc[m][n]);
+        }
+    }
+}
+
+// k blocks a wave with row blocks I0, I0+1 and column blocks J0, J0+1 needs, from the
+// masks of A and of B in (k, j) orientation
+__device__ __forceinline__ uint32_t k_blocks_used(uint32_t ma, uint32_t mb, int I0, int J0) {
+    const uint32_t acols = mask_row(ma, I0) | mask_row(ma, I0 + 1);             // bit K: A(I, K) nonzero
+    const uint32_t bt = mask_transpose(mb);                                      // bit 4J + K: B(K, J)
+    const uint32_t brows = mask_row(bt, J0) | mask_row(bt, J0 + 1);             // bit K: B(K, J) nonzero
+    return acols & brows;
+}
+
+// Mask of a wave's 2 x 2 output blocks (block (I0+m, J0+n) counts when some lane saw
+// |v| > 1e-18 in it: nz[m][n]), in the tile's bit positions; every lane returns it.
+__device__ __forceinline__ uint32_t quad_mask(const bool (&nz)[2][2], int I0, int J0) {
+    uint32_t bits = 0;
+#pragma unroll
+    for (int m = 0; m < 2; m++)
 #pragma unroll
         for (int n = 0; n < 2; n++)
-            bf[n] = TB ? lb[(wc + n * 16 + li) * LDA + k] : lb[k * LDB + wc + n * 16 + li];
-#pragma unroll
-        for (int m = 0; m < 2; m++)
-#pragma unroll
-            for (int n = 0; n < 2; n++) acc[m][n] = MfmaOp<T>::mma(af[m], bf[n], acc[m][n]);
-    }
+            if (__ballot(nz[m][n]) != 0) bits |= 1u << (4 * (I0 + m) + J0 + n);
+    return bits;
 }
 
 // C = sum_s sign_s A_s B_s over npairs pairs, full 64x64 output per workgroup.
 // Wave w computes the 32x32 quadrant (w/2, w%2) as 2x2 blocks of 16x16 MFMA
 // outputs; v_mfma_*_16x16x4 lane layout: A[i][k] in lane i + 16k, B[k][j] in
 // lane j + 16k, D[i][j] in lane j + 16g with (g, r) given by MfmaOp<T>::row.
-// A GEMM task may start before all of its pairs' operands exist: `wait(a, b)` is
+// A GEMM task may start before all of its pairs' operands exist: `wait(s, a, b, ...)` is
 // called by every thread before pair (a, b) is read and returns once both tiles
 // are ready (it includes the barrier that precedes the operand loads).
+// The wait also hands every thread the block masks of both tiles (`ma`, `mb`); `s` is
+// the pair's index in the task (the add tile comes last).  NoWait: all operands exist
+// already (level / per-op launches), the masks come from the mask array.
 struct NoWait {
-    __device__ inline void operator()(int, int) const { __syncthreads(); }
+    const uint32_t* mask;
+    __device__ inline void operator()(int, int sa, int sb, uint32_t& ma, uint32_t& mb) const {
+        __syncthreads();
+        ma = mask[sa];
+        mb = mask[sb];
+    }
 };
 
 // A task made by folding a later sub/neg into the GEMM adds (or subtracts) one more
@@ -162,9 +231,12 @@ __device__ __forceinline__ void add_tile(acc_t (&acc)[2][2], const T* __restrict
 
 // TBOK: pairs may carry a transposed second operand (only L L^T factors have them; the
 // LU kernels are built without that branch).
-template <typename T, bool ACC = false, bool TBOK = false, typename Wait = NoWait>
+// MASKS: skip zero products and zero k blocks by the operands' masks, and leave the
+// mask bits of wave w's quadrant of the result in out_mask[w]; without, every pair is
+// multiplied in full and nothing is left (the result counts as full).
+template <typename T, bool ACC = false, bool TBOK = false, bool MASKS = true, typename Wait = NoWait>
 __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, const Pair* __restrict__ pairs, int npairs,
-                          int out, char* lds_raw, const Wait& wait = Wait(), int add = -1,
+                          int out, char* lds_raw, uint32_t* out_mask, const Wait& wait, int add = -1,
                           bool add_neg = false) {
     typedef typename MfmaOp<T>::acc_t acc_t;
     T* la = reinterpret_cast<T*>(lds_raw);
@@ -187,35 +259,54 @@ __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, c
         }
     for (int s = 0; s < npairs; s++) {
         const Pair p = pairs[s];
-        const bool neg = p.b < 0, tb = p.a < 0;
+        const bool neg = p.b < 0, tb = TBOK && p.a < 0;
         const int sa = pair_a(p), sb = pair_b(p);
-        wait(sa, sb);
-        tile_to_lds(pool + (size_t)sa * TE, la, LDA, tid);
-        tile_to_lds(pool + (size_t)sb * TE, lb, TBOK && tb ? LDA : LDB, tid);
+        uint32_t ma, mbs;                                       // the tiles' masks as stored
+        wait(s, sa, sb, ma, mbs);
+        uint32_t kuse = 0xFu;
+        if constexpr (MASKS) {
+            const uint32_t mb = tb ? mask_transpose(mbs) : mbs;  // B (or B^T) in (k, j) orientation
+            if (ma == 0 || mb == 0) continue;                    // a zero product: every thread skips
+            tile_to_lds_masked(pool + (size_t)sa * TE, la, LDA, tid, ma);
+            tile_to_lds_masked(pool + (size_t)sb * TE, lb, tb ? LDA : LDB, tid, mbs);
+            kuse = k_blocks_used(ma, mb, wr >> 4, wc >> 4);
+        } else {
+            tile_to_lds(pool + (size_t)sa * TE, la, LDA, tid);
+            tile_to_lds(pool + (size_t)sb * TE, lb, tb ? LDA : LDB, tid);
+        }
         __syncthreads();
         const T sg = neg ? T(-1) : T(1);
-        if (TBOK && tb) gemm_lds<T, TBOK>(acc, la, lb, sg, wr, wc, li, lk);
-        else gemm_lds<T, false>(acc, la, lb, sg, wr, wc, li, lk);
+        if (tb) gemm_lds<T, TBOK>(acc, la, lb, sg, wr, wc, li, lk, kuse);
+        else gemm_lds<T, false>(acc, la, lb, sg, wr, wc, li, lk, kuse);
     }
     if (add >= 0) {
-        wait(add, add);
+        uint32_t m0, m1;
+        wait(npairs, add, add, m0, m1);
         add_tile<T>(acc, pool + (size_t)add * TE, add_neg, wr, wc, li, lk);
     }
     T* C = pool + (size_t)out * TE;
+    bool nz[2][2] = {{false, false}, {false, false}};
 #pragma unroll
     for (int m = 0; m < 2; m++)
 #pragma unroll
         for (int n = 0; n < 2; n++)
 #pragma unroll
-            for (int r = 0; r < 4; r++)
-                C[(wr + m * 16 + MfmaOp<T>::row(lk, r)) * TS + wc + n * 16 + li] = acc[m][n][r];
+            for (int r = 0; r < 4; r++) {
+                const T v = acc[m][n][r];
+                C[(wr + m * 16 + MfmaOp<T>::row(lk, r)) * TS + wc + n * 16 + li] = v;
+                if constexpr (MASKS) nz[m][n] |= fabs(v) > T(MASK_TINY);
+            }
+    if constexpr (MASKS) {
+        const uint32_t qm = quad_mask(nz, wr >> 4, wc >> 4);
+        if (lane == 0) out_mask[w] = qm;
+    }
 }
 
 // bf16 experiment: FP32 tiles, operands rounded to bf16, FP32 accumulation
 // (v_mfma_f32_16x16x32_bf16: A[i][8q..8q+7] in lane i + 16q, same D layout).
-template <bool TBOK = false, typename Wait = NoWait>
+template <bool TBOK = false, bool MASKS = true, typename Wait = NoWait>
 __device__ __attribute__((always_inline)) void gemm_task_bf16(float* __restrict__ pool, const Pair* __restrict__ pairs,
-                                      int npairs, int out, char* lds_raw, const Wait& wait = Wait(),
+                                      int npairs, int out, char* lds_raw, uint32_t* out_mask, const Wait& wait,
                                       int add = -1, bool add_neg = false) {
     float* la = reinterpret_cast<float*>(lds_raw);
     float* lb = la + TS * LDA;
@@ -231,9 +322,16 @@ __device__ __attribute__((always_inline)) void gemm_task_bf16(float* __restrict_
         const Pair p = pairs[s];
         const bool neg = p.b < 0, tb = p.a < 0;
         const int sa = pair_a(p), sb = pair_b(p);
-        wait(sa, sb);
-        tile_to_lds(pool + (size_t)sa * TE, la, LDA, tid);
-        tile_to_lds(pool + (size_t)sb * TE, lb, TBOK && tb ? LDA : LDB, tid);
+        uint32_t ma, mbs;
+        wait(s, sa, sb, ma, mbs);
+        if constexpr (MASKS) {
+            if (ma == 0 || mbs == 0) continue;  // a zero product: every thread skips
+            tile_to_lds_masked(pool + (size_t)sa * TE, la, LDA, tid, ma);
+            tile_to_lds_masked(pool + (size_t)sb * TE, lb, TBOK && tb ? LDA : LDB, tid, mbs);
+        } else {
+            tile_to_lds(pool + (size_t)sa * TE, la, LDA, tid);
+            tile_to_lds(pool + (size_t)sb * TE, lb, TBOK && tb ? LDA : LDB, tid);
+        }
         __syncthreads();
         const float sg = neg ? -1.f : 1.f;
         // B[k][j] at k * LDB + j, or (transposed operand) at j * LDA + k
@@ -259,17 +357,44 @@ __device__ __attribute__((always_inline)) void gemm_task_bf16(float* __restrict_
         }
     }
     if (add >= 0) {  // the bf16 kernel's accumulator rows follow the f32 layout
-        wait(add, add);
+        uint32_t m0, m1;
+        wait(npairs, add, add, m0, m1);
         add_tile<float>(acc, pool + (size_t)add * TE, add_neg, wr, wc, li, lk);
     }
     float* C = pool + (size_t)out * TE;
+    bool nz[2][2] = {{false, false}, {false, false}};
 #pragma unroll
     for (int m = 0; m < 2; m++)
 #pragma unroll
         for (int n = 0; n < 2; n++)
 #pragma unroll
-            for (int r = 0; r < 4; r++)
-                C[(wr + m * 16 + 4 * lk + r) * TS + wc + n * 16 + li] = acc[m][n][r];
+            for (int r = 0; r < 4; r++) {
+                const float v = acc[m][n][r];
+                C[(wr + m * 16 + 4 * lk + r) * TS + wc + n * 16 + li] = v;
+                if constexpr (MASKS) nz[m][n] |= fabsf(v) > float(MASK_TINY);
+            }
+    if constexpr (MASKS) {
+        const uint32_t qm = quad_mask(nz, wr >> 4, wc >> 4);
+        if (lane == 0) out_mask[w] = qm;
+    }
+}
+
+// Mask of a stored tile, by one workgroup: wave w covers block row w; lane l reads
+// row 16w + l/4 (and + 16 more rows each round), columns 16(l%4) .. +15.
+template <typename T>
+__device__ inline uint32_t tile_mask(const T* __restrict__ g, int tid) {
+    const int lane = tid & 63, w = tid >> 6;
+    bool any = false;
+    // the thread's segments: rows 16w + lane/4 (lanes cover rows 16w .. 16w+15), cols 16 * (lane % 4)
+    const T* p = g + (16 * w + (lane >> 2)) * TS + 16 * (lane & 3);
+#pragma unroll
+    for (int q = 0; q < 16; q++) any |= fabs((double)p[q]) > MASK_TINY;
+    const unsigned long long b = __ballot(any);
+    uint32_t bits = 0;
+#pragma unroll
+    for (int J = 0; J < 4; J++)
+        if (b & (0x1111111111111111ull << J)) bits |= 1u << (4 * w + J);
+    return bits;
 }
 
 // ------------------------------------------------------------------ sub
@@ -277,9 +402,12 @@ __device__ __attribute__((always_inline)) void gemm_task_bf16(float* __restrict_
 // reads as zero, which gives copy and negate (BlockPlanner.cpp:553-572).  All
 // loads are issued before the stores (the operands may not alias the result,
 // but the compiler cannot know that).
-template <typename T>
-__device__ __attribute__((always_inline)) void sub_task(T* __restrict__ pool, int src, int src2, int out) {
-    const int tid = threadIdx.x;
+// Wave w leaves the mask bits of the rows it wrote in out_mask[w] (element q*256 + tid
+// is row 4q + w, column tid % 64).
+template <typename T, bool MASKS = true>
+__device__ __attribute__((always_inline)) void sub_task(T* __restrict__ pool, int src, int src2, int out,
+                                                        uint32_t* out_mask) {
+    const int tid = threadIdx.x, lane = tid & 63;
     constexpr int Q = TE / NT;  // 16 elements per thread
     T a[Q], b[Q];
     const T* A = src2 >= 0 ? pool + (size_t)src2 * TE : nullptr;
@@ -290,8 +418,19 @@ __device__ __attribute__((always_inline)) void sub_task(T* __restrict__ pool, in
         b[q] = B ? B[q * NT + tid] : T(0);
     }
     T* R = pool + (size_t)out * TE;
+    uint32_t bits = 0;
 #pragma unroll
-    for (int q = 0; q < Q; q++) R[q * NT + tid] = a[q] - b[q];
+    for (int q = 0; q < Q; q++) {
+        const T v = a[q] - b[q];
+        R[q * NT + tid] = v;
+        if constexpr (MASKS) {
+            const unsigned long long bb = __ballot(fabs(v) > T(MASK_TINY));
+#pragma unroll
+            for (int J = 0; J < 4; J++)
+                if (bb & (0xFFFFull << (16 * J))) bits |= 1u << (4 * (q >> 2) + J);
+        }
+    }
+    if (MASKS && lane == 0) out_mask[tid >> 6] = bits;
 }
 
 // ------------------------------------------------------------------ register-resident tile algebra
@@ -945,19 +1084,33 @@ __device__ __attribute__((always_inline)) void llt_task(T* __restrict__ pool, in
     lu_inv_blocked<T, C, true>(pool, src, out, -1, outinv, -1, lds_raw, nullptr, nullptr);
 }
 
-// R = A^T through LDS (odd row stride: the column reads hit distinct banks).
-template <typename T>
-__device__ __attribute__((always_inline)) void trans_task(T* __restrict__ pool, int src, int out, char* lds_raw) {
+// R = A^T through LDS (odd row stride: the column reads hit distinct banks); wave w
+// leaves the mask bits of block row w of R in out_mask[w].
+template <typename T, bool MASKS = true>
+__device__ __attribute__((always_inline)) void trans_task(T* __restrict__ pool, int src, int out, char* lds_raw,
+                                                          uint32_t* out_mask) {
     T* s = reinterpret_cast<T*>(lds_raw);
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63;
     const int r = tid >> 2, c0 = (tid & 3) * 16;
     const T* A = pool + (size_t)src * TE;
 #pragma unroll
     for (int q = 0; q < 16; q++) s[r * LDA + c0 + q] = A[r * TS + c0 + q];
     __syncthreads();
     T* R = pool + (size_t)out * TE;
+    bool any = false;
 #pragma unroll
-    for (int q = 0; q < 16; q++) R[r * TS + c0 + q] = s[(c0 + q) * LDA + r];
+    for (int q = 0; q < 16; q++) {
+        const T v = s[(c0 + q) * LDA + r];
+        R[r * TS + c0 + q] = v;
+        any |= fabs(v) > T(MASK_TINY);
+    }
+    if constexpr (!MASKS) return;
+    const unsigned long long b = __ballot(any);  // lane l: block (w, l % 4)
+    uint32_t bits = 0;
+#pragma unroll
+    for (int J = 0; J < 4; J++)
+        if (b & (0x1111111111111111ull << J)) bits |= 1u << (4 * (tid >> 6) + J);
+    if (lane == 0) out_mask[tid >> 6] = bits;
 }
 
 }  // namespace sogl_dev

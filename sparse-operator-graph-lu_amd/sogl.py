@@ -18,6 +18,7 @@ BUILD = os.path.join(HERE, "build")
 OK = 0
 FP64, FP32, BF16 = 0, 1, 2
 EXEC_PERSISTENT, EXEC_LEVELS, EXEC_PER_OP, EXEC_GRAPH, EXEC_AUTO = 0, 1, 2, 3, 4
+MASKS_OFF, MASKS_ON, MASKS_AUTO = 0, 1, 2
 TUNE_CANDIDATES = ("persistent chunk 1", "persistent chunk 2", "persistent chunk 4",
                    "persistent chunk 1, 1 workgroup/CU", "levels", "level graph")
 OP_NAMES = {0: "inv", 1: "lu", 2: "lowerInv", 3: "upperInv", 4: "sub", 5: "add", 6: "neg",
@@ -60,7 +61,8 @@ class FactorStats(C.Structure):
     _fields_ = [("factor_ms", C.c_double), ("exec_ms", C.c_double), ("solve_ms", C.c_double),
                 ("pool_bytes", C.c_int64),
                 ("n_tasks", C.c_int64), ("n_levels", C.c_int32), ("growth_tiles", C.c_int32),
-                ("max_growth", C.c_double)]
+                ("max_growth", C.c_double), ("zero_blocks", C.c_double), ("block_masks", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 def _ptr(a: np.ndarray, ctype):
@@ -120,6 +122,7 @@ def load(name: str = "libsogl.so") -> C.CDLL:
             "sogl_factor_release": (C.c_int, [vp, i32]),
             "sogl_factor_get_stats": (C.c_int, [vp, i32, P(FactorStats)]),
             "sogl_factor_get_tiles": (C.c_int, [vp, i32, i32, P(dbl)]),
+        "sogl_factor_set_masks": (C.c_int, [vp, i32, i32]),
             "sogl_refine_attach": (C.c_int, [vp, i32, vp]),
             "sogl_solve_refined": (C.c_int, [vp, i32, P(dbl), P(dbl), i32, dbl, i32, P(i32),
                                              P(dbl)]),
@@ -379,6 +382,10 @@ class Device:
 
     def factor(self, fid: int, mode: int = EXEC_PERSISTENT):
         self._check(self.lib.sogl_factor_run(self.h, fid, mode))
+
+    def set_masks(self, fid: int, mode: int):
+        """16x16 block-mask policy: MASKS_OFF, MASKS_ON or MASKS_AUTO (the default)."""
+        self._check(self.lib.sogl_factor_set_masks(self.h, fid, mode))
 
     def solve(self, fid: int, bp: np.ndarray, want_y=False):
         b2 = np.asfortranarray(bp.reshape(bp.shape[0], -1), dtype=np.float64)

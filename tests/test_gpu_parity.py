@@ -291,6 +291,41 @@ def test_executor_modes(dev, name):
     assert np.linalg.norm(xs[0] - xs[2]) / np.linalg.norm(xs[2]) < 1e-13
 
 
+@pytest.mark.parametrize("name", ["c1", "convdiff2d_128", "random_band_2048"])
+def test_block_masks(dev, name, golden):
+    """16x16 block masks: a masked factor (zero blocks skipped in products and
+    substitutions) against a dense one and the reference x; the automatic policy keeps
+    the masks after the first run only when a quarter of the factor's blocks are zero."""
+    p = CASES[name]()
+    b = p.b if p.b.ndim == 1 else p.b[:, 0]
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, b)
+    ref = golden(name)["xorig"]
+    xs, zs = {}, None
+    for mode in (sogl.MASKS_ON, sogl.MASKS_OFF):
+        fid = dev.load(pl, sogl.FP64)
+        dev.set_masks(fid, mode)
+        dev.factor(fid)
+        st = dev.stats(fid)
+        if mode == sogl.MASKS_ON:
+            zs = st.zero_blocks
+            assert 0.0 <= zs <= 1.0 and st.block_masks == 1
+        else:
+            assert st.block_masks == 0
+        xs[mode] = pl.unpermute(dev.solve(fid, pl.permute_rhs(b)))
+        dev.release(fid)
+    for x in xs.values():
+        assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-12
+    assert np.linalg.norm(xs[sogl.MASKS_ON] - xs[sogl.MASKS_OFF]) / np.linalg.norm(ref) < 1e-13
+    fid = dev.load(pl, sogl.FP64)  # automatic (the default)
+    dev.factor(fid)
+    st = dev.stats(fid)
+    assert abs(st.zero_blocks - zs) < 1e-12 and st.block_masks == (1 if zs >= 0.25 else 0)
+    dev.factor(fid)
+    x2 = pl.unpermute(dev.solve(fid, pl.permute_rhs(b)))
+    dev.release(fid)
+    assert np.linalg.norm(x2 - ref) / np.linalg.norm(ref) < 1e-12
+
+
 def _factor_tiles(dev, pl, run):
     fid = dev.load(pl, sogl.FP64)
     run(fid)
