@@ -39,10 +39,11 @@ import workloads as W  # noqa: E402
 METRIC = "factor+solve s & block-GEMM TFLOP/s at rel. residual ≤1e-10, FP32+IR vs FP64"
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense MFMA TFLOP/s
 HBM_PEAK = 8000.0  # GB/s
-# HBM bytes per launch of the FP32 factor kernel, from two rocprofv3 --pmc runs
-# (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the scales a plain 1 GiB copy reads):
-# profiles/k_factor_persistent_c2_pmc_traffic.md
-TRAFFIC_FP32_FACTOR = {"c2": 23.256e9}
+# HBM bytes per launch of the FP32 factor kernel in the timed runs, from two rocprofv3
+# --pmc runs (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the scales a plain 1 GiB copy
+# reads): profiles/k_factor_persistent_c2_pmc_traffic.md (dense tiles) and
+# profiles/k_factor_persistent_c5_pmc_traffic.md (16x16 block masks on)
+TRAFFIC_FP32_FACTOR = {"c2": 23.275e9, "c5": 24.17e9}
 
 WORKLOADS = {
     "c1": ("2D 5-point Poisson 64x64 (n=4,096), 1 RHS U[-1,1] seed 0", 1e-12),
