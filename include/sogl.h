@@ -188,8 +188,9 @@ int sogl_factor_load(sogl_ctx* ctx, const sogl_factor_desc* desc, int32_t* facto
  * touched) and checks it: every slot index inside the pool, one writer per tile,
  * every operand produced at an earlier dependency level and waited for.
  * sogl_factor_load runs the same check before any upload.  info (may be NULL)
- * receives 7 counts: tasks, pool tiles, dependency levels, GEMM pairs,
- * dependencies, folded copy/negate/subtract ops, aliased repeated inverses. */
+ * receives 8 counts: tasks, pool tiles, dependency levels, GEMM pairs,
+ * dependencies, folded copy/negate/subtract ops, aliased repeated inverses, and lu /
+ * llt tasks chained to the GEMM that makes their source (same workgroup, no hand-off). */
 int sogl_graph_check(const sogl_factor_desc* desc, int64_t* info);
 int sogl_graph_check_plan(const sogl_plan* plan, int32_t precision, int64_t* info);
 /* Convenience: sogl_factor_load from a host plan. */

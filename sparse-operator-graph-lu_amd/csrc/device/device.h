@@ -29,6 +29,9 @@ struct DevTask {
                           // (-1: none)
     int32_t add = -1;     // GEMM: slot of a tile added to the sum (-1: none)
     int32_t add_neg = 0;  // GEMM: the added tile is subtracted instead
+    int32_t next = -1;    // persistent executor: the task the same workgroup runs right after
+                          // this one (the lu / llt that alone reads this GEMM's sum), or -1
+    int32_t pad = 0;
 };
 
 struct DevBuf {
@@ -51,8 +54,10 @@ struct Factor {
     int64_t n_pairs = 0, n_deps = 0;
     int64_t n_inv_aliased = 0;  // repeated inverses of one factor tile that share its slot
     int64_t n_folded = 0;       // copies, negations and subtractions folded into other tasks
+    int64_t n_chained = 0;      // lu / llt tasks run by the workgroup that made their source
     // device arrays
-    DevBuf pool, tasks_d, pairs_d, deps_d, ready_d, ctrl_d, op_tasks_d, op_pairs_d, inputs_d;
+    DevBuf pool, tasks_d, pairs_d, deps_d, ready_d, ctrl_d, op_tasks_d, op_pairs_d, inputs_d, queue_d;
+    std::vector<int32_t> queue;            // persistent executor: task indices handed out, in order
     DevBuf mask_d;                         // per slot: 16x16 block mask (uint32)
     int32_t epoch = 0;
     // triangular solve structure (block CSR by block row, off-diagonal only)

@@ -111,6 +111,8 @@ struct Args {
     int32_t epoch;
     int32_t task_offset;  // levels executor: first task of the launch
     int32_t chunk;        // persistent executor: tasks taken per fetch from the queue
+    const int32_t* queue; // persistent executor: the task indices handed out, in order
+    int32_t nqueue;
     double growth_limit;
     long long* trace;     // optional: per task {taken, work done, published, last wait ended}
     uint32_t* mask;       // per slot: 16x16 block mask of the tile (see tile_ops.h)
@@ -238,23 +240,32 @@ __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
     __shared__ uint32_t s_mask[NT / 64];
     __shared__ uint32_t s_pm[4];
     const int tid = threadIdx.x;
+    bool aborted = false;
     for (;;) {
         if (tid == 0) {
             s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_task = s_abort ? a.ntasks : atomicAdd(a.ctrl, a.chunk);
+            s_task = s_abort ? a.nqueue : atomicAdd(a.ctrl, a.chunk);
         }
         __syncthreads();
         // wave-uniform loop control: the exit branch must not depend on the lane
         const int base = __builtin_amdgcn_readfirstlane(s_task);
-        if (base >= a.ntasks) break;
-        // a chunk is run in index order; every dependency points to a smaller index,
-        // so with all workgroups resident no two chunks can wait on each other
-        const int end = min(base + a.chunk, a.ntasks);
-      for (int ti = base; ti < end; ti++) {
+        if (base >= a.nqueue || aborted) break;
+        // a chunk is run in queue order; every dependency points to a task handed out
+        // earlier (or to one chained to it), so with all workgroups resident no two
+        // chunks can wait on each other
+        const int end = min(base + a.chunk, a.nqueue);
+      for (int qi = base; qi < end && !aborted; qi++) {
+        // the task and its chain: a task with `next` hands its sum to the lu / llt that
+        // alone reads it, which this workgroup runs at once, without a wait (the sum is
+        // its only source, and the barrier below makes it visible to the workgroup)
+        int ti = a.queue[qi];
+        bool chained_in = false;
+        for (;;) {
         const DevTask t = a.tasks[ti];
+        const int nxt = __builtin_amdgcn_readfirstlane(t.next);
         long long t_taken = 0, t_ready = 0, t_wait = 0;
         if (a.trace && tid == 0) t_taken = wall_clock64();
-        if (t.kind != TK_GEMM) {
+        if (t.kind != TK_GEMM && !chained_in) {
             if (tid == 0) {
                 for (int d = 0; d < t.dep_count; d++) spin_ready(a, a.deps[t.dep_begin + d]);
                 acquire_for_workgroup();
@@ -264,24 +275,30 @@ __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
         if (a.trace && tid == 0) t_wait = wall_clock64();
         if (tid == 0) s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
-        if (__builtin_amdgcn_readfirstlane(s_abort)) break;
+        if (__builtin_amdgcn_readfirstlane(s_abort)) {
+            aborted = true;
+            break;
+        }
         run_task<T, BF16, false, LLT, MASKS>(t, a, lds, s_mask, PairWait{&a, &t_wait, s_pm});
         // publish: every wave's stores complete (the barrier's workgroup release), then
         // thread 0 writes the output masks, makes everything visible at device scope
-        // once and raises the flags, which carry the masks to the consumers
+        // once and raises the flags, which carry the masks to the consumers.  A sum that
+        // goes to the chained task is read by nobody else: no release, no flag.
         __syncthreads();
         if (a.trace && tid == 0) t_ready = wall_clock64();  // work done, publish starts
         if (tid == 0) {
             const uint32_t m = task_mask<MASKS>(t, s_mask);
-            release_for_workgroup();
-            const unsigned long long fv = ((unsigned long long)(unsigned)a.epoch << 32) | m;
-            __hip_atomic_store(a.ready + t.out0, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (t.out1 >= 0)
-                __hip_atomic_store(a.ready + t.out1, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (t.out2 >= 0)
-                __hip_atomic_store(a.ready + t.out2, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (t.out3 >= 0)
-                __hip_atomic_store(a.ready + t.out3, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (nxt < 0) {
+                release_for_workgroup();
+                const unsigned long long fv = ((unsigned long long)(unsigned)a.epoch << 32) | m;
+                __hip_atomic_store(a.ready + t.out0, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (t.out1 >= 0)
+                    __hip_atomic_store(a.ready + t.out1, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (t.out2 >= 0)
+                    __hip_atomic_store(a.ready + t.out2, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (t.out3 >= 0)
+                    __hip_atomic_store(a.ready + t.out3, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
             // consumers in this run read the masks from the flags; the mask array serves
             // the solve and the other executors, after the kernel, so its stores stay off
             // the release above
@@ -297,6 +314,10 @@ __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
         // keeps the publishing branch and the next fetch branch apart, so the
         // back-edge cannot join them into a lane-dependent inner loop
         __syncthreads();
+        if (nxt < 0) break;
+        ti = nxt;
+        chained_in = true;
+        }
       }
     }
 }
@@ -765,6 +786,49 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
     f->n_pairs = (int64_t)pairs.size();
     f->n_deps = (int64_t)deps.size();
 
+    // chains for the persistent executor: an lu / llt whose source is a GEMM sum that
+    // nothing else reads runs in the workgroup that made the sum, right after it (no
+    // publish, hand-off or wait in between); it is then not handed out by the queue
+    {
+        std::vector<int32_t> nread((size_t)next, 0);
+        auto rd = [&](int32_t sl) { if (sl >= 0 && sl < next) nread[(size_t)sl]++; };
+        for (size_t t = 0; t < tasks.size(); t++) {
+            const DevTask& k = tasks[t];
+            if (k.kind == TK_GEMM) {
+                for (const Pair& p : tpairs[t]) { rd(pair_a(p)); rd(pair_b(p)); }
+                rd(k.add);
+            } else if (k.kind == TK_SUB) {
+                rd(k.a);
+                rd(k.b);
+            } else {
+                rd(k.a);
+            }
+        }
+        for (int64_t k = 0; k < d->n_l; k++) rd(slot[(size_t)d->l_ids[k]]);
+        for (int64_t k = 0; k < d->n_u; k++) rd(slot[(size_t)d->u_ids[k]]);
+        for (const auto* v : {&f->l_diag, &f->u_diag, &f->linv_slot, &f->uinv_slot})
+            for (int32_t sl : *v) rd(sl);
+        std::vector<int32_t> final_of(tasks.size(), -1);
+        for (size_t i = 0; i < order.size(); i++) final_of[(size_t)order[i]] = (int32_t)i;
+        std::vector<char> chained(f->tasks.size(), 0);
+        f->n_chained = 0;
+        for (size_t u = 0; u < f->tasks.size(); u++) {
+            const DevTask& k = f->tasks[u];
+            if (k.kind != TK_LU && k.kind != TK_LLT) continue;
+            const int32_t sl = k.a;
+            const int32_t po = sl >= 0 && sl < (int32_t)prod.size() ? prod[(size_t)sl] : -1;
+            if (po < 0 || tasks[(size_t)po].kind != TK_GEMM || nread[(size_t)sl] != 1) continue;
+            DevTask& g = f->tasks[(size_t)final_of[(size_t)po]];
+            if (g.next >= 0 || g.out0 != sl) continue;
+            g.next = (int32_t)u;
+            chained[u] = 1;
+            f->n_chained++;
+        }
+        f->queue.clear();
+        for (size_t u = 0; u < f->tasks.size(); u++)
+            if (!chained[u]) f->queue.push_back((int32_t)u);
+    }
+
     // triangular-solve structure
     std::vector<std::vector<std::pair<int32_t, int32_t>>> lrow((size_t)R), urow((size_t)R);
     f->l_slots.clear();
@@ -885,6 +949,26 @@ int validate_graph(Context* c, const sogl_factor_desc* d, const Factor* f, const
         for (int32_t q = k.dep_begin; q < k.dep_begin + k.dep_count; q++)
             if (!ok_slot(hg.deps[(size_t)q])) return bad("dependency slot of task " + std::to_string(t));
     }
+    // chains and the queue: each link goes from a GEMM to the lu / llt of its sum, at a
+    // later level; every task is handed out by the queue or reached by one link, once
+    {
+        std::vector<char> seen(T, 0);
+        for (int32_t q : f->queue) {
+            if (q < 0 || (size_t)q >= T || seen[(size_t)q]) return bad("queue entry " + std::to_string(q));
+            seen[(size_t)q] = 1;
+        }
+        for (size_t t = 0; t < T; t++) {
+            const DevTask& k = f->tasks[t];
+            if (k.next < 0) continue;
+            if ((size_t)k.next >= T || k.kind != TK_GEMM) return bad("chain of task " + std::to_string(t));
+            const DevTask& u = f->tasks[(size_t)k.next];
+            if ((u.kind != TK_LU && u.kind != TK_LLT) || u.a != k.out0 || u.level <= k.level || seen[(size_t)k.next])
+                return bad("chain of task " + std::to_string(t));
+            seen[(size_t)k.next] = 1;
+        }
+        for (size_t t = 0; t < T; t++)
+            if (!seen[t]) return bad("task " + std::to_string(t) + " is neither queued nor chained");
+    }
     for (const DevTask& k : f->op_tasks) {
         if (!ok_slot(k.out0) || !ok_opt(k.out1) || !ok_opt(k.out2) || !ok_opt(k.out3)) return bad("per-op output");
         if (k.kind == TK_GEMM) {
@@ -933,6 +1017,7 @@ int upload_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells,
     if ((rc = dev_alloc(c, f->ctrl_d, CTRL_WORDS * 4, "control")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->op_tasks_d, std::max<size_t>(f->op_tasks.size(), 1) * sizeof(DevTask), "op tasks")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->op_pairs_d, std::max<size_t>(op_pairs.size(), 1) * sizeof(Pair), "op pairs")) != SOGL_OK) return rc;
+    if ((rc = dev_alloc(c, f->queue_d, std::max<size_t>(f->queue.size(), 1) * 4, "task queue")) != SOGL_OK) return rc;
     auto up_i32 = [&](DevBuf& b, const std::vector<int32_t>& v, const char* what) -> int {
         int r2 = dev_alloc(c, b, std::max<size_t>(v.size(), 1) * 4, what);
         if (r2 != SOGL_OK) return r2;
@@ -945,6 +1030,7 @@ int upload_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells,
     if (!deps.empty()) HIP_TRY(c, hipMemcpy(f->deps_d.p, deps.data(), deps.size() * 4, hipMemcpyHostToDevice));
     if (!f->op_tasks.empty()) HIP_TRY(c, hipMemcpy(f->op_tasks_d.p, f->op_tasks.data(), f->op_tasks.size() * sizeof(DevTask), hipMemcpyHostToDevice));
     if (!op_pairs.empty()) HIP_TRY(c, hipMemcpy(f->op_pairs_d.p, op_pairs.data(), op_pairs.size() * sizeof(Pair), hipMemcpyHostToDevice));
+    if (!f->queue.empty()) HIP_TRY(c, hipMemcpy(f->queue_d.p, f->queue.data(), f->queue.size() * 4, hipMemcpyHostToDevice));
     if ((rc = up_i32(f->l_rowptr, hg.lp, "L rows")) || (rc = up_i32(f->l_col, hg.lc, "L cols")) ||
         (rc = up_i32(f->l_slot, hg.ls, "L slots")) || (rc = up_i32(f->u_rowptr, hg.up, "U rows")) ||
         (rc = up_i32(f->u_col, hg.uc, "U cols")) || (rc = up_i32(f->u_slot, hg.us, "U slots")) ||
@@ -1048,7 +1134,7 @@ int build_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells, 
 void free_factor(Factor* f) {
     if (f->graph_exec) (void)hipGraphExecDestroy(f->graph_exec);
     if (f->graph) (void)hipGraphDestroy(f->graph);
-    for (DevBuf* b : {&f->pool, &f->mask_d, &f->tasks_d, &f->pairs_d, &f->deps_d, &f->ready_d, &f->ctrl_d,
+    for (DevBuf* b : {&f->pool, &f->mask_d, &f->queue_d, &f->tasks_d, &f->pairs_d, &f->deps_d, &f->ready_d, &f->ctrl_d,
                       &f->op_tasks_d, &f->op_pairs_d, &f->inputs_d, &f->l_rowptr, &f->l_col,
                       &f->l_slot, &f->u_rowptr, &f->u_col, &f->u_slot, &f->l_diag_d, &f->u_diag_d,
                       &f->linv_d, &f->uinv_d, &f->solve_ready, &f->solve_ctrl, &f->vec_b, &f->vec_x,
@@ -1099,12 +1185,14 @@ int launch_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
         a.tasks = (const DevTask*)f->tasks_d.p;
         a.pairs = (const Pair*)f->pairs_d.p;
         a.ntasks = (int32_t)f->tasks.size();
+        a.queue = (const int32_t*)f->queue_d.p;
+        a.nqueue = (int32_t)f->queue.size();
         int per_cu = 0;
         HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_persistent<T, BF16, LLT, MASKS>, NT, 0));
         if (per_cu < 1) per_cu = 1;
         if (wg_per_cu > 0) per_cu = std::min(per_cu, wg_per_cu);  // never more than fit: all must be resident
-        int grid = std::max(1, std::min<int>(a.ntasks, c->num_cu * per_cu));
-        if (a.ntasks > 0) hipLaunchKernelGGL((k_factor_persistent<T, BF16, LLT, MASKS>), dim3(grid), dim3(NT), 0, st, a);
+        int grid = std::max(1, std::min<int>(a.nqueue, c->num_cu * per_cu));
+        if (a.nqueue > 0) hipLaunchKernelGGL((k_factor_persistent<T, BF16, LLT, MASKS>), dim3(grid), dim3(NT), 0, st, a);
     } else if (mode == SOGL_EXEC_LEVELS) {
         launch_levels();
     } else if (mode == SOGL_EXEC_GRAPH) {
@@ -1251,6 +1339,7 @@ extern "C" int sogl_graph_check(const sogl_factor_desc* d, int64_t* info) {
         info[4] = (int64_t)hg.deps.size();
         info[5] = f.n_folded;
         info[6] = f.n_inv_aliased;
+        info[7] = f.n_chained;
     }
     return rc;
 }

@@ -296,7 +296,7 @@ def make_desc(n_ids: int, ops, input_ids, input_tiles, ltiles, utiles, block_row
     return d, keep
 
 
-GRAPH_INFO = ("tasks", "pool_tiles", "levels", "pairs", "deps", "folded", "inv_aliased")
+GRAPH_INFO = ("tasks", "pool_tiles", "levels", "pairs", "deps", "folded", "inv_aliased", "chained")
 
 
 def graph_check_ops(n_ids: int, ops, input_ids, input_tiles, ltiles, utiles, block_rows: int,

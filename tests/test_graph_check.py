@@ -49,6 +49,8 @@ def test_c2_graph_passes():
     g = sogl.graph_check(pl, sogl.FP32)
     assert g["pairs"] == 1_000_386
     assert g["levels"] < pl.info.n_stages  # fused tasks: fewer dependency levels than stages
+    # most lu tasks follow the Schur-complement GEMM of their tile in the same workgroup
+    assert 0 < g["chained"] <= pl.info.n_lu
 
 
 # ---- planted mistakes in hand-built op lists (ids 1, 2 are inputs)
