@@ -567,6 +567,46 @@ __device__ __forceinline__ C rbcast(C x) {
     return __builtin_amdgcn_update_dpp(C(0), x, 0x150 + K, 0xF, 0xF, true);
 }
 
+// x += (x of lane K of this 16-lane row) * s, one v_fmac with the row broadcast as
+// its DPP source operand.  The compiler folds a row broadcast into v_add / v_mul but
+// not into the tied-destination v_fmac, which would leave a separate v_mov_dpp per
+// update.  A DPP source register needs two wait states after a VALU instruction
+// writes it, which the compiler does not track through asm statements, so the
+// callers keep the rule themselves: dpp_fence before the first update (the values'
+// VALU definitions come before it, then s_nop 1), fmac_rbcast_nop for the column the
+// next step's pivot broadcast reads, and dpp_step_end after each step (the next
+// step's updates read what this step's wrote).  The asm statements are volatile, so
+// they keep their order; the emitted assembly is checked for the rule by a test.
+// Every lane of the wave must be active, as for rbcast.
+template <int K>
+__device__ __forceinline__ void fmac_rbcast(float& x, float s) {
+    asm volatile("v_fmac_f32_dpp %0, %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf"
+                 : "+v"(x) : "v"(s), "n"(K));
+}
+template <int K>
+__device__ __forceinline__ void fmac_rbcast(double& x, double s) {
+    asm volatile("v_fmac_f64_dpp %0, %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf"
+                 : "+v"(x) : "v"(s), "n"(K));
+}
+template <int K>
+__device__ __forceinline__ void fmac_rbcast_nop(float& x, float s) {
+    asm volatile("v_fmac_f32_dpp %0, %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf\n\ts_nop 1"
+                 : "+v"(x) : "v"(s), "n"(K));
+}
+template <int K>
+__device__ __forceinline__ void fmac_rbcast_nop(double& x, double s) {
+    asm volatile("v_fmac_f64_dpp %0, %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf\n\ts_nop 1"
+                 : "+v"(x) : "v"(s), "n"(K));
+}
+__device__ __forceinline__ void dpp_step_end() { asm volatile("s_nop 1"); }
+template <typename C>
+__device__ __forceinline__ void dpp_fence(C (&v)[16]) {
+    asm volatile("s_nop 1"
+                 : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]), "+v"(v[4]), "+v"(v[5]), "+v"(v[6]),
+                   "+v"(v[7]), "+v"(v[8]), "+v"(v[9]), "+v"(v[10]), "+v"(v[11]), "+v"(v[12]),
+                   "+v"(v[13]), "+v"(v[14]), "+v"(v[15]));
+}
+
 // 1/p for a clamped pivot (1e-9 <= |p|, finite): v_rcp plus two Newton steps for
 // FP64 (each step squares the relative error of the hardware estimate), v_rcp_f32
 // (1 ulp) for FP32.  Shorter than the IEEE division, which the pivot chain waits on.
@@ -682,23 +722,21 @@ __device__ inline void diag16(C* a, int r0, C* xd, C* zd, int lane) {
         y[c] = c == i ? C(1) : C(0);
     }
     C dq = C(1);  // 1 / u_ii, taken at step i
+    dpp_fence(x);
+    dpp_fence(y);
     static_for<0, 16>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        C bc[16], yb[16];
-#pragma unroll
-        for (int c = k; c < 16; c++) bc[c] = rbcast<k>(x[c]);
-#pragma unroll
-        for (int c = 0; c <= k; c++) yb[c] = rbcast<k>(y[c]);
-        const C p = clamp_pivot<CHOL>(bc[k]);
+        const C p = clamp_pivot<CHOL>(rbcast<k>(x[k]));
         const C rp = recip(p);
         const C own = x[k];
-        const C f = i > k ? own * rp : C(0);
-        x[k] = i > k ? (colw ? own : f) : (i == k ? p : own);
+        // nf = -f; -0 where f = 0 keeps lanes <= k bit for bit (x + bc * -0 == x)
+        const C nf = i > k ? own * -rp : C(-0.0);
+        x[k] = i > k ? (colw ? own : -nf) : (i == k ? p : own);
         dq = i == k ? rp : dq;
-#pragma unroll
-        for (int c = k + 1; c < 16; c++) x[c] -= f * bc[c];  // f = 0 leaves lanes <= k as they are
-#pragma unroll
-        for (int c = 0; c <= k; c++) y[c] -= f * yb[c];
+        if constexpr (k < 15) fmac_rbcast_nop<k>(x[k + 1], nf);  // the next pivot
+        static_for<k + 2, 16>([&](auto cc) { fmac_rbcast<k>(x[decltype(cc)::value], nf); });
+        static_for<0, k + 1>([&](auto cc) { fmac_rbcast<k>(y[decltype(cc)::value], nf); });
+        dpp_step_end();
     });
     if (g == 0) {
 #pragma unroll
@@ -723,13 +761,15 @@ __device__ inline void diag16_split(C* a, int r0, C* xd, C* zd, int lane) {
     C v[16];
 #pragma unroll
     for (int c = 0; c < 16; c++) v[c] = a[(r0 + i) * LDA + r0 + c];
+    dpp_fence(v);
     static_for<0, 16>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
         const C p = clamp_pivot<CHOL>(rbcast<k>(v[k]));
-        const C l = i > k ? v[k] * recip(p) : C(0);
-        v[k] = i > k ? l : (i == k ? p : v[k]);
-#pragma unroll
-        for (int c = k + 1; c < 16; c++) v[c] -= l * rbcast<k>(v[c]);  // lane k's row is unchanged here
+        const C nl = i > k ? v[k] * -recip(p) : C(-0.0);  // -l_ik; lanes <= k unchanged
+        v[k] = i > k ? -nl : (i == k ? p : v[k]);
+        if constexpr (k < 15) fmac_rbcast_nop<k>(v[k + 1], nl);  // the next pivot
+        static_for<k + 2, 16>([&](auto cc) { fmac_rbcast<k>(v[decltype(cc)::value], nl); });
+        dpp_step_end();
     });
     if (g == 0) {
 #pragma unroll
@@ -746,12 +786,13 @@ __device__ inline void diag16_split(C* a, int r0, C* xd, C* zd, int lane) {
     C y[16];
 #pragma unroll
     for (int c = 0; c < 16; c++) y[c] = c == i ? C(1) : C(0);
+    dpp_fence(y);
     static_for<0, 16>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
         const C qk = rbcast<k>(di);  // outside the conditional: every lane must run it
-        const C f = i > k ? m[k] * qk : C(0);
-#pragma unroll
-        for (int c = 0; c <= k; c++) y[c] -= f * rbcast<k>(y[c]);
+        const C nf = i > k ? m[k] * -qk : C(-0.0);
+        static_for<0, k + 1>([&](auto cc) { fmac_rbcast<k>(y[decltype(cc)::value], nf); });
+        dpp_step_end();
     });
     if (g == 0) {
 #pragma unroll

@@ -1,0 +1,111 @@
+"""Wait states around the hand-issued DPP updates of the tile LU.
+
+The diagonal-block elimination issues its row-broadcast FMAs as asm statements
+(v_fmac_*_dpp row_newbcast), which the compiler's hazard recognizer does not see
+through.  A DPP source VGPR must not be written by a VALU instruction in the two
+wait states before the DPP instruction, and EXEC must not be written by a VALU
+instruction in the five before.  This test compiles the factor kernels to gfx950
+assembly and checks every DPP instruction in it against those two rules, counting
+one wait state per instruction and N + 1 per s_nop N, and treating a label inside
+the window as a violation (the branch into it could come from anywhere).
+"""
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "sparse-operator-graph-lu_amd")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+REG = re.compile(r"\bv(\d+)\b|\bv\[(\d+):(\d+)\]")
+
+
+def vregs(text):
+    out = set()
+    for m in REG.finditer(text):
+        if m.group(1) is not None:
+            out.add(int(m.group(1)))
+        else:
+            out.update(range(int(m.group(2)), int(m.group(3)) + 1))
+    return out
+
+
+def instructions(asm):
+    out = []
+    for line in asm.splitlines():
+        s = line.split(";")[0].strip()
+        if not s or s.startswith("."):
+            continue
+        out.append(("label", s) if s.endswith(":") else ("inst", s))
+    return out
+
+
+def dpp_violations(asm, vgpr_states=2, exec_states=5):
+    insts = instructions(asm)
+    bad, ndpp = [], 0
+    for idx, (kind, s) in enumerate(insts):
+        if kind != "inst":
+            continue
+        op = s.split()[0]
+        if "_dpp" not in op and "row_newbcast" not in s:
+            continue
+        ndpp += 1
+        src0 = vregs(s.split(None, 1)[1].split(",")[1])
+        waits, j = 0, idx - 1
+        while j >= 0 and waits < exec_states:
+            k2, t = insts[j]
+            if k2 == "label":
+                if waits < vgpr_states:
+                    bad.append((s, "label %d wait states before" % waits))
+                break
+            top = t.split()[0]
+            if top == "s_nop":
+                waits += int(t.split()[1], 0) + 1
+                j -= 1
+                continue
+            if top.startswith("v_"):
+                dst = t.split(None, 1)[1].split(",")[0] if " " in t else ""
+                if top.startswith("v_cmpx") or dst.strip() == "exec":
+                    bad.append((s, "EXEC written %d wait states before by %s" % (waits, t)))
+                elif (waits < vgpr_states and not top.startswith(("v_cmp", "v_readlane", "v_readfirstlane"))
+                      and vregs(dst) & src0):
+                    bad.append((s, "source written %d wait states before by %s" % (waits, t)))
+            waits += 1
+            j -= 1
+    return ndpp, bad
+
+
+def test_checker_finds_planted_hazards():
+    asm = "\n".join([
+        "f:", "\ts_nop 4",
+        "\tv_fmac_f32_dpp v27, v27, v53 row_newbcast:4 row_mask:0xf bank_mask:0xf",
+        "\tv_mov_b32_dpp v51, v27 row_newbcast:5 row_mask:0xf bank_mask:0xf",   # 0 states after v27's write
+        "\tv_add_f64 v[4:5], v[0:1], v[2:3]",
+        "\tv_add_f32_e32 v9, v1, v2",
+        "\tv_fmac_f64_dpp v[4:5], v[4:5], v[0:1] row_newbcast:4 row_mask:0xf bank_mask:0xf",  # 1 state
+        "\tv_mov_b32_e32 v8, v1", "\ts_nop 1",
+        "\tv_fmac_f32_dpp v8, v8, v53 row_newbcast:4 row_mask:0xf bank_mask:0xf",  # 2 states: fine
+        "\tv_cmpx_lt_f32_e32 vcc, v1, v2", "\ts_nop 1",
+        "\tv_mov_b32_dpp v50, v7 row_newbcast:1 row_mask:0xf bank_mask:0xf",  # EXEC 2 states before
+    ])
+    n, bad = dpp_violations(asm)
+    assert n == 5
+    assert len(bad) == 3
+    assert "v27" in bad[0][1] and "v[4:5]" in bad[1][1] and "EXEC" in bad[2][1]
+
+
+@pytest.mark.skipif(not (os.path.exists(HIPCC) or shutil.which("hipcc")), reason="hipcc not installed")
+def test_factor_kernels_keep_dpp_wait_states(tmp_path):
+    out = tmp_path / "factor.s"
+    cmd = [HIPCC, "-std=c++17", "-O3", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", "-Wno-unused-result", "-I" + os.path.join(ROOT, "include"),
+           "--cuda-device-only", "-S", "-o", str(out), os.path.join(PKG, "csrc", "device", "factor.hip")]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    asm = out.read_text()
+    n, bad = dpp_violations(asm)
+    assert asm.count("v_fmac_f32_dpp") > 0 and asm.count("v_fmac_f64_dpp") > 0
+    assert n > 1000
+    assert not bad, bad[:5]
