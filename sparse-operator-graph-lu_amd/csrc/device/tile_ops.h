@@ -710,41 +710,36 @@ __device__ __forceinline__ C clamp_pivot(C p) {
     }
 }
 
+// The 16 elimination steps on the registers of diag16 (FP32), one scheduled asm
+// statement written by gen_diag16.py: the pivot clamp, the multipliers, both
+// inverses' updates, x[k] = l_ik for li > k, and pd = the clamped pivot of step li.
+template <bool CHOL>
+__device__ __forceinline__ void diag16_elim_f32(float (&x)[16], float (&y)[16], float& pd, int li);
+#include "diag16_asm.h"
+
 template <typename C, bool CHOL = false>
 __device__ inline void diag16(C* a, int r0, C* xd, C* zd, int lane) {
+    static_assert(sizeof(C) == 4, "FP32 only; FP64 tiles use diag16_split");
     const int i = lane & 15, g = lane >> 4;
     const bool colw = g == 1;
     const int rs = colw ? 1 : LDA, cs = colw ? LDA : 1;
-    C x[16], y[16];
+    float x[16], y[16];
 #pragma unroll
     for (int c = 0; c < 16; c++) {
         x[c] = a[r0 * LDA + r0 + i * rs + c * cs];
-        y[c] = c == i ? C(1) : C(0);
+        y[c] = c == i ? 1.f : 0.f;
     }
-    C dq = C(1);  // 1 / u_ii, taken at step i
-    dpp_fence(x);
-    dpp_fence(y);
-    static_for<0, 16>([&](auto kc) {
-        constexpr int k = decltype(kc)::value;
-        const C p = clamp_pivot<CHOL>(rbcast<k>(x[k]));
-        const C rp = recip(p);
-        const C own = x[k];
-        // nf = -f; -0 where f = 0 keeps lanes <= k bit for bit (x + bc * -0 == x)
-        const C nf = i > k ? own * -rp : C(-0.0);
-        x[k] = i > k ? (colw ? own : -nf) : (i == k ? p : own);
-        dq = i == k ? rp : dq;
-        if constexpr (k < 15) fmac_rbcast_nop<k>(x[k + 1], nf);  // the next pivot
-        static_for<k + 2, 16>([&](auto cc) { fmac_rbcast<k>(x[decltype(cc)::value], nf); });
-        static_for<0, k + 1>([&](auto cc) { fmac_rbcast<k>(y[decltype(cc)::value], nf); });
-        dpp_step_end();
-    });
+    float pd = 1.f;
+    diag16_elim_f32<CHOL>(x, y, pd, i);
     if (g == 0) {
 #pragma unroll
         for (int c = 0; c < 16; c++) {
             a[(r0 + i) * LDA + r0 + c] = x[c];
             xd[i * BS + c] = y[c];
         }
+        a[(r0 + i) * LDA + r0 + i] = pd;  // the clamped pivot over u_ii (same lane, after)
     } else if (colw) {
+        const float dq = recip(pd);  // 1 / u_ii as taken at step i
 #pragma unroll
         for (int c = 0; c < 16; c++) zd[c * BS + i] = y[c] * dq;
     }
