@@ -555,9 +555,9 @@ constexpr int BS = 17;        // LDS row stride of a 16x16 block
 constexpr int BE = 16 * BS;   // LDS elements of a 16x16 block
 
 // LDS elements (in the compute type) of lu_inv_blocked: the tile, 8 diagonal-block
-// inverses, 4 first-level inverse blocks, 2 temporaries per wave, 8 reduction slots,
-// and the 2 x 64 column scales of the Cholesky form
-constexpr int lu_lds_elems() { return TS * LDA + 8 * BE + 4 * BE + 8 * BE + 8 + 2 * TS; }
+// inverses, 12 blocks of partial inverse products, 8 reduction slots, and the 2 x 64
+// column scales of the Cholesky form
+constexpr int lu_lds_elems() { return TS * LDA + 8 * BE + 12 * BE + 8 + 2 * TS; }
 
 // Lane k of each 16-lane row to every lane of that row (DPP row_newbcast).  A DPP
 // read of a lane that EXEC disables returns 0, so callers evaluate it where every
@@ -798,6 +798,36 @@ __device__ inline void diag16_split(C* a, int r0, C* xd, C* zd, int lane) {
     }
 }
 
+// Factor blocks of the LDS tile a (row stride LDA) to global block (I, J) of L or U,
+// 4 elements per lane (rows 4q + lane/16, column lane%16).  L (I >= J): the strict
+// lower part of a, 1 on the diagonal, 0 above; CHOL: L_u diag(d) with d = dsc.
+// U (I <= J): a on and above the diagonal (the clamped pivots on it), 0 below; um
+// collects max |u| for the growth check.
+template <typename T, typename C, bool CHOL>
+__device__ __forceinline__ void l_blk_to_global(const C* a, T* Lg, int I, int J, int lane, const C* dsc) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int r = 4 * q + (lane >> 4), c = lane & 15;
+        const int gr = 16 * I + r, gc = 16 * J + c;
+        const C x = a[gr * LDA + gc];
+        C v = (I > J || r > c) ? x : (r == c ? C(1) : C(0));
+        if constexpr (CHOL) v = (I == J && r == c) ? dsc[gc] : v * dsc[gc];
+        Lg[gr * TS + gc] = T(v);
+    }
+}
+template <typename T, typename C>
+__device__ __forceinline__ void u_blk_to_global(const C* a, T* Ug, int I, int J, int lane, C& um) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int r = 4 * q + (lane >> 4), c = lane & 15;
+        const int gr = 16 * I + r, gc = 16 * J + c;
+        const C x = a[gr * LDA + gc];
+        const C v = (I < J || c >= r) ? x : C(0);
+        Ug[gr * TS + gc] = T(v);
+        um = fabs(v) > um ? fabs(v) : um;
+    }
+}
+
 // LU of one tile in compute type C plus, when their slots are given (>= 0), the
 // inverses of its unit-lower L and upper U: the reference's lowerInv / upperInv
 // of the same diagonal tile, fused into the producing task.
@@ -805,6 +835,16 @@ __device__ inline void diag16_split(C* a, int r0, C* xd, C* zd, int lane) {
 // lower triangle is read (mirrored on load), pivots follow the Cholesky clamp, and
 // with d_j = sqrt(u_jj) the outputs are L = L_u diag(d) and L^-1 = diag(1/d) L_u^-1
 // (A = L_u U with U = diag(u) L_u^T, so L L^T = A); outu and outui are unused.
+//
+// Schedule: step K factors diagonal block K on wave 0 (diag16), then all waves form
+// block row / column K (panel) and update the trailing blocks.  While wave 0 factors
+// block K, waves 1-3 finish what block row / column K-1 allows: the stores of its L,
+// U and inverse blocks, and the block products of the inverses
+//   X_IJ = -X_II N_IJ,  N_IJ = sum_{J <= K < I} L_IK X_KJ      (X = L^-1)
+//   Z_IJ = -M_IJ Z_JJ,  M_IJ = sum_{I <= K < J} Z_IK U_KJ      (Z = U^-1)
+// accumulated term by term as the factors of column / row K appear (N and M live in
+// LDS; X_IJ and Z_IJ replace them once complete).  After block 3, X_3J and Z_I3 take
+// one product each, spread over the four waves.
 template <typename T, typename C, bool CHOL = false>
 __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ pool, int src, int outl, int outu,
                                                               int outli, int outui, char* lds_raw,
@@ -813,13 +853,77 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     const int li = lane & 15, lk = lane >> 4;
+    // The stores and products of waves 1-3 use bl, bli, blk, re-defined opaquely where
+    // that work starts in each step: their block addresses then stay inside the step
+    // instead of being hoisted out of the step loop and held in registers throughout
+    // (wave 0's own loop-invariant values stay hoisted).
+    int bl = lane, bli = li, blk = lk;
     C* a = reinterpret_cast<C*>(lds_raw);  // the tile, row stride LDA
-    C* xd = a + TS * LDA;                  // L_II^-1, I = 0..3
-    C* zd = xd + 4 * BE;                   // U_II^-1
-    C* x1 = zd + 4 * BE;                   // X10, X32, Z01, Z23 (first-level inverse blocks)
-    C* tw = x1 + 4 * BE + 2 * BE * w;      // 2 temporaries of this wave
-    C* red = x1 + 4 * BE + 8 * BE;         // 8 reduction slots
+    C* xd = a + TS * LDA;                  // X_II = L_II^-1, I = 0..3
+    C* zd = xd + 4 * BE;                   // Z_II = U_II^-1
+    C* nm = zd + 4 * BE;                   // 12 blocks: N_IJ / X_IJ and M_IJ / Z_IJ
+    C* red = nm + 12 * BE;                 // 8 reduction slots
     C* dsc = red + 8;                      // CHOL: d_j = sqrt(u_jj), then 1 / d_j
+    enum { N10, N20, N30, N21, N31, N32, M01, M02, M03, M12, M13, M23 };
+    T* Lg = pool + (size_t)outl * TE;
+    T* Ug = (!CHOL && outu >= 0) ? pool + (size_t)outu * TE : nullptr;
+    T* Xg = outli >= 0 ? pool + (size_t)outli * TE : nullptr;
+    T* Zg = (!CHOL && outui >= 0) ? pool + (size_t)outui * TE : nullptr;
+    auto ablk = [&](int I, int J) { return a + 16 * I * LDA + 16 * J; };
+    auto slot = [&](int s) { return nm + s * BE; };
+    auto ldacc = [&](const C* p) {
+        acc_t acc;
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[r] = p[MfmaOp<C>::row(blk, r) * BS + bli];
+        return acc;
+    };
+    // N_s (+)= L_IK X and M_s (+)= Z U_KJ, X and Z blocks of row stride BS
+    auto n_add = [&](int s, bool first, int I, int K, const C* X) {
+        acc_t acc = first ? acc_t{0, 0, 0, 0} : ldacc(slot(s));
+        mm16<C, 4>(acc, ablk(I, K), LDA, X, BS, bli, blk);
+        st16(acc, slot(s), BS, bli, blk);
+    };
+    auto m_add = [&](int s, bool first, const C* Z, int K, int J) {
+        acc_t acc = first ? acc_t{0, 0, 0, 0} : ldacc(slot(s));
+        mm16<C, 4>(acc, Z, BS, ablk(K, J), LDA, bli, blk);
+        st16(acc, slot(s), BS, bli, blk);
+    };
+    // X_IJ = -X_II N_IJ and Z_IJ = -M_IJ Z_JJ to HBM, and in place over N / M (keep)
+    auto x_fin = [&](int s, int I, int J, bool keep) {
+        acc_t acc = acc_t{0, 0, 0, 0};
+        mm16<C, 4>(acc, xd + I * BE, BS, slot(s), BS, bli, blk);
+        if (keep) st16(acc, slot(s), BS, bli, blk, C(-1));
+        if constexpr (CHOL) st16_global_rs(acc, Xg, I, J, bli, blk, C(-1), (const C*)(dsc + TS));
+        else st16_global(acc, Xg, I, J, bli, blk, C(-1));
+    };
+    auto z_fin = [&](int s, int I, int J, bool keep) {
+        acc_t acc = acc_t{0, 0, 0, 0};
+        mm16<C, 4>(acc, slot(s), BS, zd + J * BE, BS, bli, blk);
+        if (keep) st16(acc, slot(s), BS, bli, blk, C(-1));
+        st16_global(acc, Zg, I, J, bli, blk, C(-1));
+    };
+    // block column K of L, block row K of U, the diagonal blocks of X and Z, and the
+    // zero halves of each
+    auto l_col = [&](int K) {
+        for (int I = K; I < 4; I++) l_blk_to_global<T, C, CHOL>(a, Lg, I, K, bl, dsc);
+        for (int J = K + 1; J < 4; J++) zero_blk_global(Lg, K, J, bl);
+    };
+    C um = 0;
+    auto u_row = [&](int K) {
+        for (int J = K; J < 4; J++) u_blk_to_global<T, C>(a, Ug, K, J, bl, um);
+        for (int I = K + 1; I < 4; I++) zero_blk_global(Ug, I, K, bl);
+    };
+    auto inv_diag = [&](int K) {
+        if (Xg) {
+            if constexpr (CHOL) blk_to_global_rs(xd + K * BE, BS, Xg, K, K, bl, (const C*)(dsc + TS));
+            else blk_to_global(xd + K * BE, BS, Xg, K, K, bl);
+            for (int J = K + 1; J < 4; J++) zero_blk_global(Xg, K, J, bl);
+        }
+        if (Zg) {
+            blk_to_global(zd + K * BE, BS, Zg, K, K, bl);
+            for (int I = K + 1; I < 4; I++) zero_blk_global(Zg, I, K, bl);
+        }
+    };
     {
         const T* A = pool + (size_t)src * TE;
         const int r = tid >> 2, c0 = (tid & 3) * 16;
@@ -845,9 +949,91 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
 #pragma unroll 1
     for (int K = 0; K < 4; K++) {
         const int r0 = 16 * K;
+        if (w != 0 && K > 0) {
+            asm volatile("" : "+v"(bl));
+            bli = bl & 15;
+            blk = bl >> 4;
+        }
         if (w == 0) {
             if constexpr (sizeof(C) == 8) diag16_split<C, CHOL>(a, r0, xd + K * BE, zd + K * BE, lane);
             else diag16<C, CHOL>(a, r0, xd + K * BE, zd + K * BE, lane);
+            if constexpr (CHOL) {
+                if (lane < 16) {  // this lane stored the pivot of row r0 + lane
+                    const C d = sqrt(a[(r0 + lane) * LDA + r0 + lane]);
+                    dsc[r0 + lane] = d;
+                    dsc[TS + r0 + lane] = C(1) / d;
+                }
+            }
+        } else if (K == 1) {  // block row / column 0
+            if (w == 1) {
+                l_col(0);
+                if (Xg) {
+                    n_add(N10, true, 1, 0, xd);
+                    n_add(N20, true, 2, 0, xd);
+                    n_add(N30, true, 3, 0, xd);
+                }
+            } else if (w == 2) {
+                if (Ug) u_row(0);
+                if (Zg) {
+                    m_add(M01, true, zd, 0, 1);
+                    m_add(M02, true, zd, 0, 2);
+                    m_add(M03, true, zd, 0, 3);
+                }
+            } else {
+                inv_diag(0);
+            }
+        } else if (K == 2) {  // block row / column 1
+            if (w == 1) {
+                l_col(1);
+                if (Xg) {
+                    x_fin(N10, 1, 0, true);  // X10
+                    __builtin_amdgcn_wave_barrier();
+                    n_add(N20, false, 2, 1, slot(N10));
+                    n_add(N30, false, 3, 1, slot(N10));
+                }
+            } else if (w == 2) {
+                if (Ug) u_row(1);
+                if (Zg) {
+                    z_fin(M01, 0, 1, true);  // Z01
+                    __builtin_amdgcn_wave_barrier();
+                    m_add(M02, false, slot(M01), 1, 2);
+                    m_add(M03, false, slot(M01), 1, 3);
+                }
+            } else {
+                inv_diag(1);
+                if (Xg) {
+                    n_add(N21, true, 2, 1, xd + BE);
+                    n_add(N31, true, 3, 1, xd + BE);
+                }
+                if (Zg) {
+                    m_add(M12, true, zd + BE, 1, 2);
+                    m_add(M13, true, zd + BE, 1, 3);
+                }
+            }
+        } else if (K == 3) {  // block row / column 2
+            if (w == 1) {
+                l_col(2);
+                if (Xg) {
+                    x_fin(N20, 2, 0, true);  // X20
+                    x_fin(N21, 2, 1, true);  // X21
+                    __builtin_amdgcn_wave_barrier();
+                    n_add(N30, false, 3, 2, slot(N20));
+                    n_add(N31, false, 3, 2, slot(N21));
+                }
+            } else if (w == 2) {
+                if (Ug) u_row(2);
+                if (Zg) {
+                    z_fin(M02, 0, 2, true);  // Z02
+                    z_fin(M12, 1, 2, true);  // Z12
+                    __builtin_amdgcn_wave_barrier();
+                    m_add(M03, false, slot(M02), 2, 3);
+                    m_add(M13, false, slot(M12), 2, 3);
+                }
+            } else {
+                inv_diag(2);
+                if (Xg) n_add(N32, true, 3, 2, xd + 2 * BE);
+                if (Zg) m_add(M23, true, zd + 2 * BE, 2, 3);
+            }
         }
         __syncthreads();
         if (K == 3) break;
@@ -876,138 +1062,16 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
         }
         __syncthreads();
     }
-    if constexpr (CHOL) {
-        // L = L_u diag(d) to HBM; the row scales of L^-1 stay in LDS
-        if (tid < TS) {
-            const C d = sqrt(a[tid * LDA + tid]);
-            dsc[tid] = d;
-            dsc[TS + tid] = C(1) / d;
-        }
-        __syncthreads();
-        const int r = tid >> 2, c0 = (tid & 3) * 16;
-        T* Lg = pool + (size_t)outl * TE + r * TS + c0;
-#pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int j = c0 + q;
-            Lg[q] = T(r > j ? a[r * LDA + j] * dsc[j] : (r == j ? dsc[j] : C(0)));
-        }
-        if (outli < 0) return;
-        // L_u^-1 is assembled below as for the LU; its rows are stored scaled by 1 / d
+    // block row / column 3: waves 0-2 finish X_3w and Z_w3, wave 3 stores the rest
+    if (w == 3) {
+        l_col(3);
+        if (Ug) u_row(3);
+        inv_diag(3);
     } else {
-        // L (unit lower) and U to HBM
-        const int r = tid >> 2, c0 = (tid & 3) * 16;
-        T* Lg = pool + (size_t)outl * TE + r * TS + c0;
-        T* Ug = pool + (size_t)outu * TE + r * TS + c0;
-        C um = 0;
-#pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int j = c0 + q;
-            const C x = a[r * LDA + j];
-            Lg[q] = T(r > j ? x : (r == j ? C(1) : C(0)));
-            const C u = j >= r ? x : C(0);
-            Ug[q] = T(u);
-            um = fabs(u) > um ? fabs(u) : um;
-        }
-        if (umax_out) *umax_out = (float)block_absmax(um, red + 4, tid);
-        if (outli < 0 && outui < 0) return;
+        if (Xg) x_fin(w == 0 ? N30 : (w == 1 ? N31 : N32), 3, w, false);
+        if (Zg) z_fin(w == 0 ? M03 : (w == 1 ? M13 : M23), w, 3, false);
     }
-    // X = L^-1 (lower) and Z = U^-1 (upper) from the diagonal-block inverses.
-    // First level, one chain per wave: X10 = -X11 L10 X00, X32 = -X33 L32 X22,
-    // Z01 = -Z00 U01 Z11, Z23 = -Z22 U23 Z33.
-    T* Xg = outli >= 0 ? pool + (size_t)outli * TE : nullptr;
-    T* Zg = outui >= 0 ? pool + (size_t)outui * TE : nullptr;
-    {
-        const bool lo = w < 2;  // waves 0, 1: X; waves 2, 3: Z
-        const int b = 2 * (w & 1);  // first block index of the 2x2 diagonal pair
-        acc_t acc = acc_t{0, 0, 0, 0};
-        if (lo) mm16<C, 4>(acc, a + 16 * (b + 1) * LDA + 16 * b, LDA, xd + b * BE, BS, li, lk);  // L_{b+1,b} X_bb
-        else mm16<C, 4>(acc, a + 16 * b * LDA + 16 * (b + 1), LDA, zd + (b + 1) * BE, BS, li, lk);  // U_{b,b+1} Z_{b+1,b+1}
-        st16(acc, tw, BS, li, lk);
-        __builtin_amdgcn_wave_barrier();
-        acc = acc_t{0, 0, 0, 0};
-        if (lo) mm16<C, 4>(acc, xd + (b + 1) * BE, BS, tw, BS, li, lk);
-        else mm16<C, 4>(acc, zd + b * BE, BS, tw, BS, li, lk);
-        st16(acc, x1 + w * BE, BS, li, lk, C(-1));
-        if constexpr (CHOL) {
-            if (lo && Xg) st16_global_rs(acc, Xg, b + 1, b, li, lk, C(-1), (const C*)(dsc + TS));
-        } else {
-            if (lo && Xg) st16_global(acc, Xg, b + 1, b, li, lk, C(-1));
-        }
-        if (!lo && Zg) st16_global(acc, Zg, b, b + 1, li, lk, C(-1));
-        // diagonal blocks and the zero half of each inverse
-        if (Xg) {
-            if constexpr (CHOL) blk_to_global_rs(xd + w * BE, BS, Xg, w, w, lane, (const C*)(dsc + TS));
-            else blk_to_global(xd + w * BE, BS, Xg, w, w, lane);
-            for (int J = w + 1; J < 4; J++) zero_blk_global(Xg, w, J, lane);
-        }
-        if (Zg) {
-            blk_to_global(zd + w * BE, BS, Zg, w, w, lane);
-            for (int I = w + 1; I < 4; I++) zero_blk_global(Zg, I, w, lane);
-        }
-    }
-    __syncthreads();
-    // Second level.  X: with T = [L20 L21; L30 L31] [X00 0; X10 X11],
-    // [X20 X21; X30 X31] = -[X22 0; X32 X33] T; wave 0 takes T's first block column,
-    // wave 1 the second.  Z: with T = [U02 U03; U12 U13] [Z22 Z23; 0 Z33],
-    // [Z02 Z03; Z12 Z13] = -[Z00 Z01; 0 Z11] T; waves 2 and 3.
-    {
-        const int col = w & 1;
-        C* t0 = tw;       // T block (0, col)
-        C* t1 = tw + BE;  // T block (1, col)
-        const C* X10 = x1;
-        const C* X32 = x1 + BE;
-        const C* Z01 = x1 + 2 * BE;
-        const C* Z23 = x1 + 3 * BE;
-        acc_t c0 = acc_t{0, 0, 0, 0}, c1 = acc_t{0, 0, 0, 0};
-        if (w < 2) {
-            if (col == 0) {  // T00 = L20 X00 + L21 X10, T10 = L30 X00 + L31 X10
-                mm16<C, 4>(c0, a + 32 * LDA, LDA, xd, BS, li, lk);
-                mm16<C, 4>(c0, a + 32 * LDA + 16, LDA, X10, BS, li, lk);
-                mm16<C, 4>(c1, a + 48 * LDA, LDA, xd, BS, li, lk);
-                mm16<C, 4>(c1, a + 48 * LDA + 16, LDA, X10, BS, li, lk);
-            } else {  // T01 = L21 X11, T11 = L31 X11
-                mm16<C, 4>(c0, a + 32 * LDA + 16, LDA, xd + BE, BS, li, lk);
-                mm16<C, 4>(c1, a + 48 * LDA + 16, LDA, xd + BE, BS, li, lk);
-            }
-        } else {
-            if (col == 0) {  // T00 = U02 Z22, T10 = U12 Z22
-                mm16<C, 4>(c0, a + 32, LDA, zd + 2 * BE, BS, li, lk);
-                mm16<C, 4>(c1, a + 16 * LDA + 32, LDA, zd + 2 * BE, BS, li, lk);
-            } else {  // T01 = U02 Z23 + U03 Z33, T11 = U12 Z23 + U13 Z33
-                mm16<C, 4>(c0, a + 32, LDA, Z23, BS, li, lk);
-                mm16<C, 4>(c0, a + 48, LDA, zd + 3 * BE, BS, li, lk);
-                mm16<C, 4>(c1, a + 16 * LDA + 32, LDA, Z23, BS, li, lk);
-                mm16<C, 4>(c1, a + 16 * LDA + 48, LDA, zd + 3 * BE, BS, li, lk);
-            }
-        }
-        st16(c0, t0, BS, li, lk);
-        st16(c1, t1, BS, li, lk);
-        __builtin_amdgcn_wave_barrier();
-        c0 = acc_t{0, 0, 0, 0};
-        c1 = acc_t{0, 0, 0, 0};
-        if (w < 2) {  // X2c = -X22 T0c, X3c = -(X32 T0c + X33 T1c)
-            mm16<C, 4>(c0, xd + 2 * BE, BS, t0, BS, li, lk);
-            mm16<C, 4>(c1, X32, BS, t0, BS, li, lk);
-            mm16<C, 4>(c1, xd + 3 * BE, BS, t1, BS, li, lk);
-            if (Xg) {
-                if constexpr (CHOL) {
-                    st16_global_rs(c0, Xg, 2, col, li, lk, C(-1), (const C*)(dsc + TS));
-                    st16_global_rs(c1, Xg, 3, col, li, lk, C(-1), (const C*)(dsc + TS));
-                } else {
-                    st16_global(c0, Xg, 2, col, li, lk, C(-1));
-                    st16_global(c1, Xg, 3, col, li, lk, C(-1));
-                }
-            }
-        } else {  // Z0c' = -(Z00 T0c + Z01 T1c), Z1c' = -Z11 T1c  (c' = 2 + col)
-            mm16<C, 4>(c0, zd, BS, t0, BS, li, lk);
-            mm16<C, 4>(c0, Z01, BS, t1, BS, li, lk);
-            mm16<C, 4>(c1, zd + BE, BS, t1, BS, li, lk);
-            if (Zg) {
-                st16_global(c0, Zg, 0, 2 + col, li, lk, C(-1));
-                st16_global(c1, Zg, 1, 2 + col, li, lk, C(-1));
-            }
-        }
-    }
+    if (umax_out) *umax_out = (float)block_absmax(um, red + 4, tid);
 }
 
 // FP64 refactorisation of a low-precision tile whose pivot-free growth is too large.

@@ -77,6 +77,36 @@ def test_lu_tile_clamp(dev):
     assert u[5, 5] == 1e-9 and u[9, 9] == -1e-9
 
 
+@pytest.mark.parametrize("mode", [sogl.EXEC_PERSISTENT, sogl.EXEC_LEVELS])
+@pytest.mark.parametrize("precision,tol", [(sogl.FP64, 1e-13), (sogl.FP32, 2e-5)])
+def test_lu_tile_fused_inverses(dev, precision, tol, mode):
+    """lu followed by lowerInv / upperInv of its outputs: the inverses are made inside
+    the lu task from the inverses of its 16 x 16 diagonal blocks; every block of both
+    inverses against the oracle's inv_lower / inv_upper of the oracle's factors.
+    X and Z are read back as extra factor tiles of a 2 x 2 block map (an identity input
+    tile fills its second diagonal block)."""
+    a = dd_tile()
+    ops = [op(1, 0, 1, 2, 3, stage=2, seq=0), op(2, 0, 2, 4, stage=3, seq=1), op(3, 0, 3, 5, stage=3, seq=2)]
+    fid = dev.load_ops(7, np.array(ops, np.int32), np.array([1, 6], np.int32), np.stack([a, np.eye(64)]),
+                       np.array([[0, 0, 2], [1, 0, 4], [1, 1, 6]], np.int32),
+                       np.array([[0, 0, 3], [0, 1, 5], [1, 1, 6]], np.int32), 2, precision)
+    dev.factor(fid, mode)
+    l, x, _ = dev.factor_tiles(fid, 0, 3)
+    u, z, _ = dev.factor_tiles(fid, 1, 3)
+    dev.release(fid)
+    lo, uo = oracle.lu(a)
+    xo, zo = oracle.inv_lower(lo), oracle.inv_upper(uo)
+    assert np.abs(l - lo).max() < tol * 10
+    assert np.abs(u - uo).max() / np.abs(uo).max() < tol
+    for name, got, want in (("L^-1", x, xo), ("U^-1", z, zo)):
+        for bi in range(4):
+            for bj in range(4):
+                g = got[16 * bi:16 * bi + 16, 16 * bj:16 * bj + 16]
+                w = want[16 * bi:16 * bi + 16, 16 * bj:16 * bj + 16]
+                assert np.abs(g - w).max() <= tol * np.abs(want).max(), (name, bi, bj)
+    assert np.all(np.triu(x, 1) == 0) and np.all(np.tril(z, -1) == 0)
+
+
 @pytest.mark.parametrize("lower", [True, False])
 @pytest.mark.parametrize("precision,tol", [(sogl.FP64, 1e-13), (sogl.FP32, 2e-5)])
 def test_triangular_inverse_tile(dev, lower, precision, tol):
@@ -134,6 +164,28 @@ def test_llt_tile(dev, precision, tol):
     assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 50 * tol
     yl = np.linalg.solve(want, b)
     assert np.linalg.norm(y - np.diag(want) * yl) / np.linalg.norm(yl) < 50 * tol
+
+
+@pytest.mark.parametrize("precision,tol", [(sogl.FP64, 1e-13), (sogl.FP32, 2e-5)])
+def test_llt_tile_fused_inverse(dev, precision, tol):
+    """llt followed by lowerInv of its factor: L^-1 made inside the llt task (rows of
+    L_u^-1 scaled by 1 / d), every block against the inverse of the oracle's factor."""
+    a = spd_tile()
+    ops = [op(1, 0, LLT, 2, stage=2, seq=0), op(2, 0, LINV, 4, stage=3, seq=1)]
+    fid = dev.load_ops(6, np.array(ops, np.int32), np.array([1, 5], np.int32), np.stack([a, np.eye(64)]),
+                       np.array([[0, 0, 2], [1, 0, 4], [1, 1, 5]], np.int32), np.zeros((0, 3), np.int32), 2,
+                       precision)
+    dev.factor(fid, sogl.EXEC_PERSISTENT)
+    l, x, _ = dev.factor_tiles(fid, 0, 3)
+    dev.release(fid)
+    want = oracle.llt(a)
+    xo = oracle.inv_lower(want)
+    assert np.abs(l - want).max() / np.abs(want).max() < tol
+    for bi in range(4):
+        for bj in range(4):
+            g = x[16 * bi:16 * bi + 16, 16 * bj:16 * bj + 16]
+            w = xo[16 * bi:16 * bi + 16, 16 * bj:16 * bj + 16]
+            assert np.abs(g - w).max() <= tol * np.abs(xo).max(), (bi, bj)
 
 
 def test_llt_tile_clamp(dev):
