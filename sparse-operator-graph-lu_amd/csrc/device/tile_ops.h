@@ -524,20 +524,16 @@ __device__ inline void inv_upper_regs(const C* us, C* y, C* bro, int lane, int w
     }
 }
 
-// block-wide max of |x|; red has >= 4 slots
-template <typename C>
-__device__ inline C block_absmax(C x, C* red, int tid) {
-    x = x < 0 ? -x : x;
-    for (int o = 32; o > 0; o >>= 1) {
-        C y = __shfl_xor(x, o);
-        x = y > x ? y : x;
-    }
-    __syncthreads();
-    if ((tid & 63) == 0) red[tid >> 6] = x;
-    __syncthreads();
-    C m = red[0];
-    for (int q = 1; q < NT / 64; q++) m = red[q] > m ? red[q] : m;
-    return m;
+// max |x| over the 64 lanes of a wave, in every lane: a DPP butterfly inside each
+// 16-lane row (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror, row_mirror), then the
+// four row maxima by readlane.  A DPP source lane is always active here (full EXEC).
+__device__ __forceinline__ float wave_absmax(float x) {
+    x = fabsf(x);
+    x = fmaxf(x, __builtin_amdgcn_update_dpp(0.f, x, 0xB1, 0xF, 0xF, false));
+    x = fmaxf(x, __builtin_amdgcn_update_dpp(0.f, x, 0x4E, 0xF, 0xF, false));
+    x = fmaxf(x, __builtin_amdgcn_update_dpp(0.f, x, 0x141, 0xF, 0xF, false));
+    x = fmaxf(x, __builtin_amdgcn_update_dpp(0.f, x, 0x140, 0xF, 0xF, false));
+    return fmaxf(fmaxf(readlane(x, 0), readlane(x, 16)), fmaxf(readlane(x, 32), readlane(x, 48)));
 }
 
 // ------------------------------------------------------------------ blocked tile LU
@@ -943,8 +939,11 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
                 if (c > r) a[r * LDA + c] = a[c * LDA + r];
             }
         }
-        if (amax_out) *amax_out = (float)block_absmax(m, red, tid);
-        else __syncthreads();
+        if (amax_out) {  // the wave's max |a| to red[w]; max over the waves at the end
+            const float wm = wave_absmax((float)m);
+            if (lane == 0) red[w] = C(wm);
+        }
+        __syncthreads();
     }
 #pragma unroll 1
     for (int K = 0; K < 4; K++) {
@@ -1071,7 +1070,13 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
         if (Xg) x_fin(w == 0 ? N30 : (w == 1 ? N31 : N32), 3, w, false);
         if (Zg) z_fin(w == 0 ? M03 : (w == 1 ? M13 : M23), w, 3, false);
     }
-    if (umax_out) *umax_out = (float)block_absmax(um, red + 4, tid);
+    if (amax_out || umax_out) {  // the growth check's max |a| and max |u| over the four waves
+        const float wu = wave_absmax((float)um);
+        if (lane == 0) red[4 + w] = C(wu);
+        __syncthreads();
+        if (amax_out) *amax_out = (float)fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        if (umax_out) *umax_out = (float)fmax(fmax(red[4], red[5]), fmax(red[6], red[7]));
+    }
 }
 
 // FP64 refactorisation of a low-precision tile whose pivot-free growth is too large.

@@ -77,6 +77,39 @@ def test_lu_tile_clamp(dev):
     assert u[5, 5] == 1e-9 and u[9, 9] == -1e-9
 
 
+def test_lu_tile_fp32_growth_check(dev):
+    """FP32 lu tiles report the pivot-free growth max|U| / max|A|; above the limit
+    (1e4) the tile is factored again in FP64 (L, U and both inverses)."""
+    a = dd_tile()
+    fid = dev.load_ops(4, np.array([op(1, 0, 1, 2, 3)], np.int32), np.array([1], np.int32), np.stack([a]),
+                       np.array([[0, 0, 2]], np.int32), np.array([[0, 0, 3]], np.int32), 1, sogl.FP32)
+    dev.factor(fid, sogl.EXEC_PERSISTENT)
+    st = dev.stats(fid)
+    dev.release(fid)
+    lo, uo = oracle.lu(a)
+    assert st.growth_tiles == 0
+    assert abs(st.max_growth - np.abs(uo).max() / np.abs(a).max()) < 1e-5 * st.max_growth
+    # a 1e-6 leading pivot (above the 1e-9 clamp): u ~ 1e6 against max |a| ~ 64
+    g = dd_tile()
+    g[0, 0] = 1e-6
+    g[0, 1:] = 1.0
+    g[1:, 0] = 1.0
+    ops = [op(1, 0, 1, 2, 3, stage=2, seq=0), op(2, 0, 2, 4, stage=3, seq=1), op(3, 0, 3, 5, stage=3, seq=2)]
+    fid = dev.load_ops(7, np.array(ops, np.int32), np.array([1, 6], np.int32), np.stack([g, np.eye(64)]),
+                       np.array([[0, 0, 2], [1, 0, 4], [1, 1, 6]], np.int32),
+                       np.array([[0, 0, 3], [0, 1, 5], [1, 1, 6]], np.int32), 2, sogl.FP32)
+    dev.factor(fid, sogl.EXEC_PERSISTENT)
+    st = dev.stats(fid)
+    l, x, _ = dev.factor_tiles(fid, 0, 3)
+    u, z, _ = dev.factor_tiles(fid, 1, 3)
+    dev.release(fid)
+    lo, uo = oracle.lu(g)
+    go = np.abs(uo).max() / np.abs(g).max()
+    assert go > 1e4 and st.growth_tiles == 1 and abs(st.max_growth - go) < 1e-4 * go
+    for got, want in ((l, lo), (u, uo), (x, oracle.inv_lower(lo)), (z, oracle.inv_upper(uo))):
+        assert np.abs(got - want).max() <= 1e-6 * np.abs(want).max()
+
+
 @pytest.mark.parametrize("mode", [sogl.EXEC_PERSISTENT, sogl.EXEC_LEVELS])
 @pytest.mark.parametrize("precision,tol", [(sogl.FP64, 1e-13), (sogl.FP32, 2e-5)])
 def test_lu_tile_fused_inverses(dev, precision, tol, mode):
