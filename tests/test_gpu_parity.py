@@ -379,8 +379,9 @@ def test_executor_modes(dev, name):
 @pytest.mark.parametrize("name", ["c1", "convdiff2d_128", "random_band_2048"])
 def test_block_masks(dev, name, golden):
     """16x16 block masks: a masked factor (zero blocks skipped in products and
-    substitutions) against a dense one and the reference x; the automatic policy keeps
-    the masks after the first run only when a quarter of the factor's blocks are zero."""
+    substitutions) against a dense one and the reference x.  The automatic policy keeps
+    the masks after the first run for FP64 factors, and for FP32 factors only when a
+    quarter of the factor's blocks are zero."""
     p = CASES[name]()
     b = p.b if p.b.ndim == 1 else p.b[:, 0]
     pl = sogl.Plan(p.n, p.row, p.col, p.val, b)
@@ -404,11 +405,16 @@ def test_block_masks(dev, name, golden):
     fid = dev.load(pl, sogl.FP64)  # automatic (the default)
     dev.factor(fid)
     st = dev.stats(fid)
-    assert abs(st.zero_blocks - zs) < 1e-12 and st.block_masks == (1 if zs >= 0.25 else 0)
+    assert abs(st.zero_blocks - zs) < 1e-12 and st.block_masks == 1
     dev.factor(fid)
     x2 = pl.unpermute(dev.solve(fid, pl.permute_rhs(b)))
     dev.release(fid)
     assert np.linalg.norm(x2 - ref) / np.linalg.norm(ref) < 1e-12
+    fid = dev.load(pl, sogl.FP32)
+    dev.factor(fid)
+    st = dev.stats(fid)
+    dev.release(fid)
+    assert 0.0 <= st.zero_blocks <= 1.0 and st.block_masks == (1 if st.zero_blocks >= 0.25 else 0)
 
 
 def _factor_tiles(dev, pl, run):
