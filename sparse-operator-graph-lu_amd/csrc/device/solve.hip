@@ -8,10 +8,12 @@
 // backward,
 //     x_I = Uinv_II (y_I - sum_{J>I} U_IJ x_J)
 // with the inverses of the diagonal tiles made once per factor.  Block rows
-// are taken in dependency order (ascending / descending), a row waits on the
-// ready flags of the rows it reads, so independent rows overlap and the
-// chain only serialises where the factor's block structure does.  Sums run
-// in FP64 for every tile precision.
+// are taken in dependency order (ascending / descending) and a row waits only
+// for the rows it reads, so independent rows overlap and the chain only
+// serialises where the factor's block structure does.  With 1-15 right-hand
+// sides a row waits on the stored values themselves (a pending marker until
+// stored); the MFMA form for more right-hand sides waits on per-row ready flags.
+// Sums run in FP64 for every tile precision.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -67,21 +69,71 @@ __device__ inline void spin_row(const SolveArgs& a, int J) {
     }
 }
 
-// One block row per task.  The rows a row reads are consumed oldest first (the
-// order they complete in), each after its own ready flag, with the tile row
-// already in registers; so once the newest dependency lands only one tile
-// product and the diagonal product remain on the chain.  CW right-hand sides
-// are handled per pass over the row.
+// Rows of the output that a one-to-15-RHS pass has not stored yet hold this signalling
+// NaN, written by k_fill_pending before the pass.  FP arithmetic never produces a
+// signalling NaN (a NaN result is quiet), so a stored value is never mistaken for it.
+// A row task sorts the rows it reads into those already published when it starts (an
+// oldest-first prefix: one check of their ready flags, one acquire, then ordinary
+// cached loads) and the rest, the newest, usually the one on the chain: for those it
+// polls the values themselves with agent-scope atomic loads until none of the 16 it
+// needs is the marker.  The task stores its row with agent-scope atomic stores (they
+// reach the coherence point without waiting for the release), then releases and
+// raises its ready flag for the tasks that start later.
+constexpr unsigned long long SOLVE_PENDING = 0x7FF4C0DE5EED0001ull;
+
+__global__ void k_fill_pending(double* out, int64_t ld, int64_t rows, int nrhs) {
+    const int64_t n = rows * nrhs;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = i / rows, r = i - c * rows;
+        reinterpret_cast<unsigned long long*>(out)[c * ld + r] = SOLVE_PENDING;
+    }
+}
+
+// The 64 values of one column of row J, which another task stores, into this wave's
+// LDS slot ls once all are stored: lane l polls value l (one coalesced load per spin
+// for the wave), the wave's ballot says when none is pending.  On the wait bound (or
+// an abort elsewhere) it raises the abort flag and leaves zeros.  The caller reads ls
+// after it (LDS accesses of one wave complete in order).
+__device__ inline void load_row_stored(const double* col, double* ls, int lane, const SolveArgs& a) {
+    const unsigned long long* q = reinterpret_cast<const unsigned long long*>(col) + lane;
+    long long t0 = 0;
+    unsigned long long u;
+    for (int spin = 0;; spin++) {
+        u = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!__ballot(u == SOLVE_PENDING)) break;
+        if (spin == 0) {
+            t0 = wall_clock64();
+        } else if (__hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                   wall_clock64() - t0 > SOLVE_SPIN_LIMIT) {
+            __hip_atomic_store(a.ctrl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            u = 0;  // the bit pattern of 0.0
+            break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    ls[lane] = __longlong_as_double((long long)u);
+    __builtin_amdgcn_wave_barrier();
+}
+
+// One block row per task, rows taken in dependency order.  The rows a row reads are
+// consumed oldest first (the order they complete in), so once the newest dependency
+// lands only one tile product and the diagonal product remain on the chain.  A thread
+// (r, kp) of a late row waits only for the 16 values of x_J its 16x16 block
+// multiplies, and every thread skips blocks the factor's masks mark zero (a zero block
+// never reads x_J: a pending marker times zero would be NaN).  CW right-hand sides per
+// pass over the row.
 template <typename T, int CW>
 __global__ __launch_bounds__(NT) void k_trsv_persistent(SolveArgs a) {
     __shared__ double vec[TS * CW];
-    __shared__ int s_row, s_abort;
-    const int tid = threadIdx.x, r = tid >> 2, kp = tid & 3;
+    __shared__ double rowls[NT];  // per wave: one column of a polled row
+    __shared__ int s_row;
+    const int tid = threadIdx.x, r = tid >> 2, kp = tid & 3, lane = tid & 63;
+    double* ls = rowls + (tid & ~63);
     const T* pool = reinterpret_cast<const T*>(a.pool);
     for (;;) {
         if (tid == 0) {
-            s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_row = s_abort ? a.nbr : atomicAdd(a.ctrl, 1);
+            const int abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_row = abort ? a.nbr : atomicAdd(a.ctrl, 1);
         }
         __syncthreads();
         const int q = __builtin_amdgcn_readfirstlane(s_row);  // wave-uniform loop control
@@ -99,31 +151,23 @@ __global__ __launch_bounds__(NT) void k_trsv_persistent(SolveArgs a) {
                 acc[c] = (kp == 0 && c < cw) ? a.in[rowI + r + (int64_t)(c0 + c) * a.ld] : 0.0;
             for (int i = 0; i < cnt; i++) {
                 const int e = a.forward ? e0 + i : e1 - 1 - i;
-                const int J = a.col[e];
-                // a tile whose blocks all count as zero adds nothing: no wait for row J
                 const uint32_t tm = a.mask[a.slot[e]];
-                if (tm == 0) continue;
+                if (tm == 0) continue;  // a zero tile: nothing to add, no wait for its row
+                const int J = a.col[e];
+                const bool blk = (tm >> (4 * (r >> 4) + kp)) & 1u;  // this thread's block is nonzero
                 double seg[16];
-                if ((tm >> (4 * (r >> 4) + kp)) & 1u) {
-                    load_seg(pool + (size_t)a.slot[e] * TE, r, kp, seg);
-                } else {
-#pragma unroll
-                    for (int k = 0; k < 16; k++) seg[k] = 0.0;
-                }
-                if (c0 == 0) {
-                    // one thread waits and acquires for the workgroup; the barrier
-                    // orders the other waves after it
-                    if (tid == 0) {
-                        spin_row(a, J);
-                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                    }
-                    __syncthreads();
-                }
-                const double* xj = a.out + (int64_t)J * TS + kp * 16;
+                if (blk) load_seg(pool + (size_t)a.slot[e] * TE, r, kp, seg);
+                const double* xj = a.out + (int64_t)J * TS;
 #pragma unroll
                 for (int c = 0; c < CW; c++) {
                     if (c < cw) {
-                        const double* xc = xj + (int64_t)(c0 + c) * a.ld;
+                        double xc[16];
+                        const double* px = xj + (int64_t)(c0 + c) * a.ld;
+                        load_row_stored(px, ls, lane, a);  // the whole wave, also lanes whose block is zero
+#pragma unroll
+                        for (int k = 0; k < 16; k++) xc[k] = ls[kp * 16 + k];
+                        __builtin_amdgcn_wave_barrier();  // ls is rewritten for the next column
+                        if (!blk) continue;
                         double s = 0;
 #pragma unroll
                         for (int k = 0; k < 16; k++) s += seg[k] * xc[k];
@@ -155,16 +199,13 @@ __global__ __launch_bounds__(NT) void k_trsv_persistent(SolveArgs a) {
                 o[c] += __shfl_xor(o[c], 2);
             }
             if (kp == 0) {
-                for (int c = 0; c < cw; c++) a.out[rowI + r + (int64_t)(c0 + c) * a.ld] = o[c];
+                for (int c = 0; c < cw; c++)
+                    __hip_atomic_store(reinterpret_cast<unsigned long long*>(a.out) + rowI + r + (int64_t)(c0 + c) * a.ld,
+                                       (unsigned long long)__double_as_longlong(o[c]), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
             }
             __syncthreads();  // vec is reused by the next chunk
         }
-        __syncthreads();  // every wave's stores are complete (workgroup release)
-        if (tid == 0) {   // one device-scope release for the workgroup, then the flag
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __hip_atomic_store(a.ready + I, a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        __syncthreads();  // separates the publish branch from the next fetch branch
     }
 }
 
@@ -355,6 +396,11 @@ int launch_pass(Context* c, Factor* f, bool forward, const double* in, double* o
     hipLaunchKernelGGL(k_solve_ctrl_reset, dim3(1), dim3(64), 0, c->stream, a.ctrl);
     int per_cu = 0;
     const bool mfma = nrhs >= TRSM_MIN_RHS;
+    if (!mfma) {  // the rows of out are pending until their task stores them
+        const int64_t n = a.ld * nrhs;
+        const unsigned fg = (unsigned)std::min<int64_t>((n + 255) / 256, 2048);
+        hipLaunchKernelGGL(k_fill_pending, dim3(fg), dim3(256), 0, c->stream, out, a.ld, a.ld, nrhs);
+    }
     const int np = (nrhs + TS - 1) / TS;  // panels of 64 columns (at most TRSM_MAX_RHS / 64)
     const void* kern = !mfma ? (nrhs == 1 ? (const void*)k_trsv_persistent<T, 1> : (const void*)k_trsv_persistent<T, 8>)
                        : np == 1 ? (const void*)k_trsm_persistent<T, 1>
