@@ -551,9 +551,9 @@ constexpr int BS = 17;        // LDS row stride of a 16x16 block
 constexpr int BE = 16 * BS;   // LDS elements of a 16x16 block
 
 // LDS elements (in the compute type) of lu_inv_blocked: the tile, 8 diagonal-block
-// inverses, 12 blocks of partial inverse products, 8 reduction slots, and the 2 x 64
-// column scales of the Cholesky form
-constexpr int lu_lds_elems() { return TS * LDA + 8 * BE + 12 * BE + 8 + 2 * TS; }
+// inverses, 12 blocks of partial inverse products, 8 reduction slots, the 2 x 64
+// column scales of the Cholesky form, and one slot for the look-ahead's panel counter
+constexpr int lu_lds_elems() { return TS * LDA + 8 * BE + 12 * BE + 8 + 2 * TS + 1; }
 
 // Lane k of each 16-lane row to every lane of that row (DPP row_newbcast).  A DPP
 // read of a lane that EXEC disables returns 0, so callers evaluate it where every
@@ -860,6 +860,7 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
     C* nm = zd + 4 * BE;                   // 12 blocks: N_IJ / X_IJ and M_IJ / Z_IJ
     C* red = nm + 12 * BE;                 // 8 reduction slots
     C* dsc = red + 8;                      // CHOL: d_j = sqrt(u_jj), then 1 / d_j
+    int* pcnt = reinterpret_cast<int*>(dsc + 2 * TS);  // panel blocks written (look-ahead)
     enum { N10, N20, N30, N21, N31, N32, M01, M02, M03, M12, M13, M23 };
     T* Lg = pool + (size_t)outl * TE;
     T* Ug = (!CHOL && outu >= 0) ? pool + (size_t)outu * TE : nullptr;
@@ -923,6 +924,7 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
     {
         const T* A = pool + (size_t)src * TE;
         const int r = tid >> 2, c0 = (tid & 3) * 16;
+        if (tid == 0) *pcnt = 0;
         C m = 0;
 #pragma unroll
         for (int q = 0; q < 16; q++) {
@@ -945,119 +947,158 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
         }
         __syncthreads();
     }
-#pragma unroll 1
-    for (int K = 0; K < 4; K++) {
+    // diagonal block K on wave 0 (and, CHOL, the scales of its columns)
+    auto diag_step = [&](int K) {
         const int r0 = 16 * K;
-        if (w != 0 && K > 0) {
+        if constexpr (sizeof(C) == 8) diag16_split<C, CHOL>(a, r0, xd + K * BE, zd + K * BE, lane);
+        else diag16<C, CHOL>(a, r0, xd + K * BE, zd + K * BE, lane);
+        if constexpr (CHOL) {
+            if (lane < 16) {  // this lane stored the pivot of row r0 + lane
+                const C d = sqrt(a[(r0 + lane) * LDA + r0 + lane]);
+                dsc[r0 + lane] = d;
+                dsc[TS + r0 + lane] = C(1) / d;
+            }
+        }
+    };
+    // panel blocks, in place: U_KJ = L_KK^-1 A_KJ and L_IK = A_IK U_KK^-1; trailing update
+    // A_IJ -= L_IK U_KJ
+    auto u_panel = [&](int K, int J, int ci, int ck) {
+        acc_t acc = acc_t{0, 0, 0, 0};
+        C* d = ablk(K, J);
+        mm16<C, 4>(acc, xd + K * BE, BS, d, LDA, ci, ck);
+        st16(acc, d, LDA, ci, ck);
+    };
+    auto l_panel = [&](int I, int K, int ci, int ck) {
+        acc_t acc = acc_t{0, 0, 0, 0};
+        C* d = ablk(I, K);
+        mm16<C, 4>(acc, d, LDA, zd + K * BE, BS, ci, ck);
+        st16(acc, d, LDA, ci, ck);
+    };
+    auto trail = [&](int I, int J, int K, int ci, int ck) {
+        C* d = ablk(I, J);
+        acc_t acc;
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[r] = d[MfmaOp<C>::row(ck, r) * LDA + ci];
+        mm16<C, 4>(acc, ablk(I, K), LDA, ablk(K, J), LDA, ci, ck, C(-1));
+        st16(acc, d, LDA, ci, ck);
+    };
+    // Panel blocks of step K written by all four waves: each wave adds 1 to the LDS
+    // counter after its blocks; waves 1-3 wait for 4(K+1) before the trailing update.
+    // Wave 0 never waits on it, so the wait always ends; the spin is bounded all the same.
+    auto panel_done = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (lane == 0) __hip_atomic_fetch_add(pcnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+    auto panel_wait = [&](int K) {
+        if (lane == 0) {
+            for (int spin = 0; spin < (1 << 22); spin++) {
+                if (__hip_atomic_load(pcnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= 4 * (K + 1)) break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    };
+    if (w == 0) diag_step(0);
+    __syncthreads();
+#pragma unroll 1
+    for (int K = 0; K < 3; K++) {
+        const int nb = 3 - K;  // blocks right of / below diagonal block K
+        if (w == 0) {
+            // look-ahead: the two panel blocks and the update of diagonal block K+1 first,
+            // then its elimination, while waves 1-3 finish step K
+            u_panel(K, K + 1, li, lk);
+            l_panel(K + 1, K, li, lk);
+            panel_done();
+            __builtin_amdgcn_wave_barrier();
+            trail(K + 1, K + 1, K, li, lk);
+            __builtin_amdgcn_wave_barrier();
+            diag_step(K + 1);
+        } else {
             asm volatile("" : "+v"(bl));
             bli = bl & 15;
             blk = bl >> 4;
-        }
-        if (w == 0) {
-            if constexpr (sizeof(C) == 8) diag16_split<C, CHOL>(a, r0, xd + K * BE, zd + K * BE, lane);
-            else diag16<C, CHOL>(a, r0, xd + K * BE, zd + K * BE, lane);
-            if constexpr (CHOL) {
-                if (lane < 16) {  // this lane stored the pivot of row r0 + lane
-                    const C d = sqrt(a[(r0 + lane) * LDA + r0 + lane]);
-                    dsc[r0 + lane] = d;
-                    dsc[TS + r0 + lane] = C(1) / d;
-                }
+            for (int p = w - 1; p < 2 * (nb - 1); p += 3) {  // the other panel blocks
+                if (p < nb - 1) u_panel(K, K + 2 + p, bli, blk);
+                else l_panel(K + 2 + p - (nb - 1), K, bli, blk);
             }
-        } else if (K == 1) {  // block row / column 0
-            if (w == 1) {
-                l_col(0);
-                if (Xg) {
-                    n_add(N10, true, 1, 0, xd);
-                    n_add(N20, true, 2, 0, xd);
-                    n_add(N30, true, 3, 0, xd);
+            panel_done();
+            panel_wait(K);
+            for (int q = w; q < nb * nb; q += 3) {  // the other trailing blocks (q = 0 is wave 0's)
+                trail(K + 1 + q / nb, K + 1 + q % nb, K, bli, blk);
+            }
+            // block row / column K: stores and the inverse products it allows
+            if (K == 0) {
+                if (w == 1) {
+                    l_col(0);
+                    if (Xg) {
+                        n_add(N10, true, 1, 0, xd);
+                        n_add(N20, true, 2, 0, xd);
+                        n_add(N30, true, 3, 0, xd);
+                    }
+                } else if (w == 2) {
+                    if (Ug) u_row(0);
+                    if (Zg) {
+                        m_add(M01, true, zd, 0, 1);
+                        m_add(M02, true, zd, 0, 2);
+                        m_add(M03, true, zd, 0, 3);
+                    }
+                } else {
+                    inv_diag(0);
                 }
-            } else if (w == 2) {
-                if (Ug) u_row(0);
-                if (Zg) {
-                    m_add(M01, true, zd, 0, 1);
-                    m_add(M02, true, zd, 0, 2);
-                    m_add(M03, true, zd, 0, 3);
+            } else if (K == 1) {
+                if (w == 1) {
+                    l_col(1);
+                    if (Xg) {
+                        x_fin(N10, 1, 0, true);  // X10
+                        __builtin_amdgcn_wave_barrier();
+                        n_add(N20, false, 2, 1, slot(N10));
+                        n_add(N30, false, 3, 1, slot(N10));
+                    }
+                } else if (w == 2) {
+                    if (Ug) u_row(1);
+                    if (Zg) {
+                        z_fin(M01, 0, 1, true);  // Z01
+                        __builtin_amdgcn_wave_barrier();
+                        m_add(M02, false, slot(M01), 1, 2);
+                        m_add(M03, false, slot(M01), 1, 3);
+                    }
+                } else {
+                    inv_diag(1);
+                    if (Xg) {
+                        n_add(N21, true, 2, 1, xd + BE);
+                        n_add(N31, true, 3, 1, xd + BE);
+                    }
+                    if (Zg) {
+                        m_add(M12, true, zd + BE, 1, 2);
+                        m_add(M13, true, zd + BE, 1, 3);
+                    }
                 }
             } else {
-                inv_diag(0);
-            }
-        } else if (K == 2) {  // block row / column 1
-            if (w == 1) {
-                l_col(1);
-                if (Xg) {
-                    x_fin(N10, 1, 0, true);  // X10
-                    __builtin_amdgcn_wave_barrier();
-                    n_add(N20, false, 2, 1, slot(N10));
-                    n_add(N30, false, 3, 1, slot(N10));
-                }
-            } else if (w == 2) {
-                if (Ug) u_row(1);
-                if (Zg) {
-                    z_fin(M01, 0, 1, true);  // Z01
-                    __builtin_amdgcn_wave_barrier();
-                    m_add(M02, false, slot(M01), 1, 2);
-                    m_add(M03, false, slot(M01), 1, 3);
-                }
-            } else {
-                inv_diag(1);
-                if (Xg) {
-                    n_add(N21, true, 2, 1, xd + BE);
-                    n_add(N31, true, 3, 1, xd + BE);
-                }
-                if (Zg) {
-                    m_add(M12, true, zd + BE, 1, 2);
-                    m_add(M13, true, zd + BE, 1, 3);
+                if (w == 1) {
+                    l_col(2);
+                    if (Xg) {
+                        x_fin(N20, 2, 0, true);  // X20
+                        x_fin(N21, 2, 1, true);  // X21
+                        __builtin_amdgcn_wave_barrier();
+                        n_add(N30, false, 3, 2, slot(N20));
+                        n_add(N31, false, 3, 2, slot(N21));
+                    }
+                } else if (w == 2) {
+                    if (Ug) u_row(2);
+                    if (Zg) {
+                        z_fin(M02, 0, 2, true);  // Z02
+                        z_fin(M12, 1, 2, true);  // Z12
+                        __builtin_amdgcn_wave_barrier();
+                        m_add(M03, false, slot(M02), 2, 3);
+                        m_add(M13, false, slot(M12), 2, 3);
+                    }
+                } else {
+                    inv_diag(2);
+                    if (Xg) n_add(N32, true, 3, 2, xd + 2 * BE);
+                    if (Zg) m_add(M23, true, zd + 2 * BE, 2, 3);
                 }
             }
-        } else if (K == 3) {  // block row / column 2
-            if (w == 1) {
-                l_col(2);
-                if (Xg) {
-                    x_fin(N20, 2, 0, true);  // X20
-                    x_fin(N21, 2, 1, true);  // X21
-                    __builtin_amdgcn_wave_barrier();
-                    n_add(N30, false, 3, 2, slot(N20));
-                    n_add(N31, false, 3, 2, slot(N21));
-                }
-            } else if (w == 2) {
-                if (Ug) u_row(2);
-                if (Zg) {
-                    z_fin(M02, 0, 2, true);  // Z02
-                    z_fin(M12, 1, 2, true);  // Z12
-                    __builtin_amdgcn_wave_barrier();
-                    m_add(M03, false, slot(M02), 2, 3);
-                    m_add(M13, false, slot(M12), 2, 3);
-                }
-            } else {
-                inv_diag(2);
-                if (Xg) n_add(N32, true, 3, 2, xd + 2 * BE);
-                if (Zg) m_add(M23, true, zd + 2 * BE, 2, 3);
-            }
-        }
-        __syncthreads();
-        if (K == 3) break;
-        const int nb = 3 - K;  // blocks right of / below the diagonal block
-        for (int p = w; p < 2 * nb; p += 4) {
-            acc_t acc = acc_t{0, 0, 0, 0};
-            if (p < nb) {  // U_KJ = L_KK^-1 A_KJ
-                C* d = a + r0 * LDA + 16 * (K + 1 + p);
-                mm16<C, 4>(acc, xd + K * BE, BS, d, LDA, li, lk);
-                st16(acc, d, LDA, li, lk);
-            } else {  // L_IK = A_IK U_KK^-1
-                C* d = a + 16 * (K + 1 + p - nb) * LDA + r0;
-                mm16<C, 4>(acc, d, LDA, zd + K * BE, BS, li, lk);
-                st16(acc, d, LDA, li, lk);
-            }
-        }
-        __syncthreads();
-        for (int q = w; q < nb * nb; q += 4) {  // A_IJ -= L_IK U_KJ
-            const int I = K + 1 + q / nb, J = K + 1 + q % nb;
-            C* d = a + 16 * I * LDA + 16 * J;
-            acc_t acc;
-#pragma unroll
-            for (int r = 0; r < 4; r++) acc[r] = d[MfmaOp<C>::row(lk, r) * LDA + li];
-            mm16<C, 4>(acc, a + 16 * I * LDA + r0, LDA, a + r0 * LDA + 16 * J, LDA, li, lk, C(-1));
-            st16(acc, d, LDA, li, lk);
         }
         __syncthreads();
     }
