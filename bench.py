@@ -43,7 +43,7 @@ HBM_PEAK = 8000.0  # GB/s
 # --pmc runs (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the scales a plain 1 GiB copy
 # reads): profiles/k_factor_persistent_c2_pmc_traffic.md (dense tiles) and
 # profiles/k_factor_persistent_c5_pmc_traffic.md (16x16 block masks on)
-TRAFFIC_FP32_FACTOR = {"c2": 23.135e9, "c5": 24.08e9}
+TRAFFIC_FP32_FACTOR = {"c2": 22.738e9, "c5": 23.9e9}
 
 WORKLOADS = {
     "c1": ("2D 5-point Poisson 64x64 (n=4,096), 1 RHS U[-1,1] seed 0", 1e-12),
@@ -58,6 +58,9 @@ WORKLOADS = {
             "log-normal (sigma 1) node coefficients seed 23, lower triangle in the file, 1 RHS U[-1,1] "
             "seed 1023", 1e-12),
 }
+# --exec choices: (executor mode, resident workgroups per CU; 0 = as many as fit)
+EXECUTORS = {"auto": (sogl.EXEC_AUTO, 0), "persistent": (sogl.EXEC_PERSISTENT, 0),
+             "persistent-1wg": (sogl.EXEC_PERSISTENT, 1), "levels": (sogl.EXEC_LEVELS, 0)}
 # workloads beyond the named configurations, and the fixture holding their reference x
 EXTRA = {"sym": (lambda: W.symmetric_lower(W.diffusion_varcoef(48, 3, 23)), "sym_diffusion3d_48")}
 
@@ -128,7 +131,7 @@ def physical_cores() -> int:
         return os.cpu_count() or 1
 
 
-def run_path(dev, plan, prob, precision, tol, max_iter, steps, warmup, mode):
+def run_path(dev, plan, prob, precision, tol, max_iter, steps, warmup, mode, wg_per_cu=0):
     fid = dev.load(plan, precision)
     dev.attach_refine(fid, plan)
     n = prob.n
@@ -138,13 +141,17 @@ def run_path(dev, plan, prob, precision, tol, max_iter, steps, warmup, mode):
     d_x = dev.alloc(b.nbytes)
     dev.upload(d_b, b.T.copy())  # column-major n x nrhs
     fac, exe, sol, its, res = [], [], [], [], []
+    tuned = None
+    if mode == sogl.EXEC_AUTO:  # tune the executor on this factor before warmup, untimed
+        best, _ = dev.autotune(fid, 2)
+        tuned = sogl.TUNE_CANDIDATES[best]
     for _ in range(warmup):
-        dev.factor(fid, mode)
+        dev.factor_config(fid, mode, 1, wg_per_cu)
         dev.solve_refined_device(fid, d_b, d_x, nrhs, tol, max_iter)
     dev.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
-        dev.factor(fid, mode)
+        dev.factor_config(fid, mode, 1, wg_per_cu)
         st = dev.stats(fid)
         fac.append(st.factor_ms)
         exe.append(st.exec_ms)
@@ -161,7 +168,7 @@ def run_path(dev, plan, prob, precision, tol, max_iter, steps, warmup, mode):
     dev.release(fid)
     return dict(step_s=wall, factor_s=float(np.mean(fac)) / 1e3, exec_s=float(np.mean(exe)) / 1e3,
                 solve_s=float(np.mean(sol)) / 1e3, ir_iters=int(max(its)), residual=float(max(res)),
-                pool_gb=st.pool_bytes / 1e9, n_tasks=int(st.n_tasks), n_levels=int(st.n_levels),
+                pool_gb=st.pool_bytes / 1e9, n_tasks=int(st.n_tasks), n_levels=int(st.n_levels), executor=tuned,
                 growth_fp64_tiles=int(st.growth_tiles), x=x[:, 0] if nrhs == 1 else x)
 
 
@@ -214,7 +221,10 @@ def main():
     ap.add_argument("--requests", type=int, default=200, help="c4: requests in the serving stream")
     ap.add_argument("--rate", type=float, default=20.0, help="c4: mean arrivals per second")
     ap.add_argument("--slots", type=int, default=4, help="c4: concurrent requests on the GPU")
-    ap.add_argument("--exec", default="persistent", choices=["persistent", "levels"])
+    ap.add_argument("--exec", default="auto", choices=sorted(EXECUTORS),
+                    help="auto: the executor configuration the autotuner picks on each factor "
+                         "(tuned before the warmup steps, outside the timed region); "
+                         "persistent-1wg: the persistent executor with one workgroup per CU")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--bf16", action="store_true",
                     help="also run the BF16 experiment (bf16 MFMA operands, FP32 tiles, FP64 refinement)")
@@ -233,7 +243,7 @@ def main():
     if args.config == "c4":
         run_serving(args)
         return
-    mode = sogl.EXEC_PERSISTENT if args.exec == "persistent" else sogl.EXEC_LEVELS
+    mode = EXECUTORS[args.exec]
     desc, tol = WORKLOADS[args.config]
     prob = EXTRA[args.config][0]() if args.config in EXTRA else W.config(args.config)
     t0 = time.perf_counter()
@@ -247,10 +257,10 @@ def main():
                    for k, s in (("fp64", 8), ("fp32", 4))}
     dev = sogl.Device(0)
     paths = {}
-    paths["fp32_ir"] = run_path(dev, plan, prob, sogl.FP32, tol, 30, args.steps, args.warmup, mode)
-    paths["fp64"] = run_path(dev, plan, prob, sogl.FP64, tol, 0, args.steps, args.warmup, mode)
+    paths["fp32_ir"] = run_path(dev, plan, prob, sogl.FP32, tol, 30, args.steps, args.warmup, *mode)
+    paths["fp64"] = run_path(dev, plan, prob, sogl.FP64, tol, 0, args.steps, args.warmup, *mode)
     if args.bf16:
-        paths["bf16_ir"] = run_path(dev, plan, prob, sogl.BF16, tol, 50, args.steps, args.warmup, mode)
+        paths["bf16_ir"] = run_path(dev, plan, prob, sogl.BF16, tol, 50, args.steps, args.warmup, *mode)
     executors = None
     if args.compare_executors or args.config == "c3":
         executors = compare_executors(dev, plan, per_op=True)
@@ -259,7 +269,7 @@ def main():
         r, c, v = W.full_matrix(prob)
         full = W.Problem(prob.name + "_full", prob.n, r, c, v, prob.b)
         lu_plan = sogl.Plan(full.n, full.row, full.col, full.val, full.b)
-        lu_same = run_path(dev, lu_plan, full, sogl.FP64, tol, 0, args.steps, args.warmup, mode)
+        lu_same = run_path(dev, lu_plan, full, sogl.FP64, tol, 0, args.steps, args.warmup, *mode)
         lu_same.pop("x")
         lu_same = {k: (float(f"{v:.6g}") if isinstance(v, float) else v) for k, v in lu_same.items()}
         lu_same["unit_ops"] = int(lu_plan.info.n_ops)
@@ -323,7 +333,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 factor + f64 refinement",
         "data": "synthetic",
-        "config": {"workload": f"{args.config}: {desc}", "tol": tol, "executor": args.exec,
+        "config": {"workload": f"{args.config}: {desc}", "tol": tol,
+                   "executor": args.exec if head["executor"] is None else f"autotuned: {head['executor']}",
                    "n": prob.n, "nnz": prob.nnz, "unit_ops": int(info.n_ops),
                    "gemm_ops": int(info.n_mul + info.n_mulneg + info.n_mult), "stages": int(info.n_stages),
                    "gemm_gflop": round(gemm_flops / 1e9, 3), "host_plan_s": round(t_plan, 3)},

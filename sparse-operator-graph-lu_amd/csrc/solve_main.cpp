@@ -10,7 +10,8 @@
 // Options after the file name (none in the reference):
 //   --fp32 | --bf16        low-precision factor with FP64 iterative refinement
 //   --tol T                refinement tolerance (default 1e-12)
-//   --exec persistent|levels|perop
+//   --exec persistent|levels|perop|auto   (auto: tune the executor configuration
+//                          on the factor first; 'kernel time' is the tuned factor)
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -49,7 +50,10 @@ int main(int argc, char** argv) {
         else if (a == "--tol" && i + 1 < argc) tol = std::atof(argv[++i]);
         else if (a == "--exec" && i + 1 < argc) {
             std::string m = argv[++i];
-            mode = m == "levels" ? SOGL_EXEC_LEVELS : m == "perop" ? SOGL_EXEC_PER_OP : SOGL_EXEC_PERSISTENT;
+            mode = m == "levels" ? SOGL_EXEC_LEVELS
+                   : m == "perop" ? SOGL_EXEC_PER_OP
+                   : m == "auto"  ? SOGL_EXEC_AUTO
+                                   : SOGL_EXEC_PERSISTENT;
         }
     }
     const std::string fname = argv[1];
@@ -78,6 +82,8 @@ int main(int argc, char** argv) {
     if ((rc = sogl_ctx_create(0, &ctx))) return die("device", rc, sogl_last_error());
     int32_t fid = 0;
     if ((rc = sogl_factor_load_plan(ctx, plan, precision, &fid))) return die("load", rc, sogl_ctx_error(ctx));
+    if (mode == SOGL_EXEC_AUTO && (rc = sogl_factor_autotune(ctx, fid, 1, nullptr, nullptr)))
+        return die("autotune", rc, sogl_ctx_error(ctx));
     auto tc = std::chrono::steady_clock::now();
     if ((rc = sogl_factor_run(ctx, fid, mode))) return die("factor", rc, sogl_ctx_error(ctx));
     double t_calc = std::chrono::duration<double>(std::chrono::steady_clock::now() - tc).count();
