@@ -140,7 +140,6 @@ __device__ inline unsigned long long spin_ready(const Args& a, int s) {
     }
 }
 
-// Per-pair wait of a GEMM task inside the persistent executor
 // Device-scope ordering for a whole workgroup, done by one thread.  A workgroup
 // barrier orders the other waves with that thread (it is a workgroup-scope release
 // and acquire), and the device-scope fences are cumulative, so one L2 write-back
@@ -149,15 +148,83 @@ __device__ inline unsigned long long spin_ready(const Args& a, int s) {
 __device__ __forceinline__ void acquire_for_workgroup() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 __device__ __forceinline__ void release_for_workgroup() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
 
-// The masks come with the flags: thread 0 leaves them in LDS, in one of two slots by
-// the pair's parity (a wave may still be reading the previous pair's pair of masks when
-// thread 0 writes the next one; every other pair has a barrier in between).
+// The masks come with the flags.  A window of pairs: wave 0 loads the flags of up to
+// PAIR_WINDOW pairs at once (lane 2j / 2j+1: pair s+j's A / B), polls until pair s is
+// ready, counts the leading pairs whose flags are both raised and leaves their masks in
+// an LDS ring of PAIR_RING pairs (pair j at 2 (j mod PAIR_RING)) and the count in one of
+// two words by the window's parity; one acquire serves the whole window.  Zero products
+// are skipped without a barrier, so a wave may still be reading window k's masks and
+// count while wave 0 writes window k + 1's; it cannot be further behind (window k + 1
+// starts after the barrier that ends the wait for window k), and with windows of at
+// most half the ring the two never share an entry.  Single waits (the add tile, the
+// bf16 loop): thread 0 leaves the masks in one of two slots by the parity of s (a wave
+// may still be reading the previous pair's masks when thread 0 writes the next one;
+// every other pair has a barrier in between).
+constexpr int PAIR_WINDOW = 32;
+constexpr int PAIR_RING = 2 * PAIR_WINDOW;
+constexpr int PM_LEN = 2 * PAIR_RING;          // two window counts
+constexpr int PM_SINGLE = PM_LEN + 2;          // two single-wait slots of two words
+constexpr int PM_WORDS = PM_SINGLE + 4;
 struct PairWait {
     const Args* a;
     long long* t_wait;  // tracing: wall clock when the latest wait ended (thread 0)
-    uint32_t* s_pm;     // LDS, 4 words
+    uint32_t* s_pm;     // LDS, PM_WORDS words
+    __device__ inline int extend(const Pair* pairs, int s, int npairs, int window) const {
+        uint32_t* ring = s_pm;
+        int* s_len = reinterpret_cast<int*>(s_pm + PM_LEN + (window & 1));
+        if (threadIdx.x < 64) {  // wave 0
+            const int lane = threadIdx.x;
+            const int cnt = min(npairs - s, PAIR_WINDOW);
+            const int j = s + (lane >> 1);
+            int slot = -1;
+            if (lane < 2 * cnt) {
+                const Pair p = pairs[j];
+                slot = (lane & 1) ? pair_b(p) : pair_a(p);
+            }
+            const unsigned long long want = (unsigned long long)(unsigned)a->epoch << 32;
+            auto poll = [&](bool on, unsigned long long& v) {  // leading pairs with both flags raised
+                if (on) v = __hip_atomic_load(a->ready + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned long long late = __ballot(on && v < want);
+                return (late ? __builtin_ctzll(late) : 2 * cnt) >> 1;
+            };
+            unsigned long long v = 0;
+            int len = poll(slot >= 0, v);
+            if (len == 0) {
+                // spin on pair s alone (two flags, not the window's), then read the window
+                const long long t0 = wall_clock64();
+                for (;;) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (poll(lane < 2, v) > 0) {
+                        len = poll(slot >= 0, v);
+                        break;
+                    }
+                    // as spin_ready: on abort or timeout the window is taken as ready with
+                    // zero masks (the run's results are discarded)
+                    const int ab = __hip_atomic_load(a->ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (__builtin_amdgcn_readfirstlane(ab) || wall_clock64() - t0 > SPIN_LIMIT_TICKS) {  // wave-uniform
+                        if (lane == 0) __hip_atomic_store(a->ctrl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        v = 0;
+                        len = cnt;
+                        break;
+                    }
+                }
+            }
+            if (lane < 2 * len) ring[2 * (j % PAIR_RING) + (lane & 1)] = (uint32_t)v;
+            acquire_for_workgroup();
+            if (lane == 0) {
+                *s_len = len;
+                if (a->trace) *t_wait = wall_clock64();
+            }
+        }
+        __syncthreads();
+        return s + __builtin_amdgcn_readfirstlane(*s_len);
+    }
+    __device__ inline void pair_masks(int j, const Pair&, uint32_t& ma, uint32_t& mb) const {
+        ma = s_pm[2 * (j % PAIR_RING)];
+        mb = s_pm[2 * (j % PAIR_RING) + 1];
+    }
     __device__ inline void operator()(int s, int sa, int sb, uint32_t& ma, uint32_t& mb) const {
-        uint32_t* pm = s_pm + 2 * (s & 1);
+        uint32_t* pm = s_pm + PM_SINGLE + 2 * (s & 1);
         if (threadIdx.x == 0) {
             pm[0] = (uint32_t)spin_ready(*a, sa);
             pm[1] = (uint32_t)spin_ready(*a, sb);
@@ -230,15 +297,16 @@ __device__ inline void write_masks(const DevTask& t, const Args& a, uint32_t m) 
 }
 
 // Persistent executor: every workgroup loops taking the next task index.  GEMM
-// tasks wait pair by pair (pairs are ordered by when their operands become
-// ready); other tasks wait for all their sources first.
+// tasks wait for windows of pairs (pairs are ordered by when their operands become
+// ready) and multiply each ready pair while loading the next; other tasks wait for
+// all their sources first.
 template <typename T, bool BF16, bool LLT, bool MASKS>
 __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
     __shared__ __attribute__((aligned(16))) char lds[lds_bytes<T>()];
     __shared__ int s_task;
     __shared__ int s_abort;
     __shared__ uint32_t s_mask[NT / 64];
-    __shared__ uint32_t s_pm[4];
+    __shared__ uint32_t s_pm[PM_WORDS];
     const int tid = threadIdx.x;
     bool aborted = false;
     for (;;) {

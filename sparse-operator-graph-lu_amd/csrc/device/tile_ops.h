@@ -81,6 +81,58 @@ __device__ inline void tile_to_lds(const T* __restrict__ g, T* __restrict__ s, i
     }
 }
 
+// One thread's segment of a tile (the same row segment as tile_to_lds) held in
+// registers: the next pair's operands are loaded into it while the current pair
+// multiplies from LDS, then written to LDS.
+template <typename T>
+struct Seg;
+template <>
+struct Seg<float> {
+    float4 v[4];
+};
+template <>
+struct Seg<double> {
+    double2 v[8];
+};
+// on = false: the segment's 16x16 block is zero by the tile's mask and is not loaded
+template <typename T>
+__device__ __forceinline__ void seg_load(const T* __restrict__ g, Seg<T>& s, int tid, bool on) {
+    if (!on) return;
+    const int r = tid >> 2, c0 = (tid & 3) * 16;
+    typedef typename std::remove_reference<decltype(s.v[0])>::type V;
+    const V* src = reinterpret_cast<const V*>(g + r * TS + c0);
+#pragma unroll
+    for (int q = 0; q < (int)(sizeof(s.v) / sizeof(V)); q++) s.v[q] = src[q];
+}
+// on = false: write zeros (the block is zero by the tile's mask)
+template <typename T>
+__device__ __forceinline__ void seg_store(const Seg<T>& s, T* __restrict__ lds, int ld, int tid, bool on) {
+    const int r = tid >> 2, c0 = (tid & 3) * 16;
+    T* dst = lds + r * ld + c0;
+    if (on) {
+        if constexpr (sizeof(T) == 8) {
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                dst[2 * q] = s.v[q].x;
+                dst[2 * q + 1] = s.v[q].y;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                dst[4 * q] = s.v[q].x;
+                dst[4 * q + 1] = s.v[q].y;
+                dst[4 * q + 2] = s.v[q].z;
+                dst[4 * q + 3] = s.v[q].w;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; q++) dst[q] = T(0);
+    }
+}
+// whether thread tid's segment lies in a block whose bit is set in mask m
+__device__ __forceinline__ bool seg_on(uint32_t m, int tid) { return (m >> (4 * (tid >> 6) + (tid & 3))) & 1u; }
+
 // The same with the blocks whose mask bit is clear written as zeros, not loaded.
 template <typename T>
 __device__ inline void tile_to_lds_masked(const T* __restrict__ g, T* __restrict__ s, int ld, int tid, uint32_t m) {
@@ -203,12 +255,23 @@ __device__ __forceinline__ uint32_t quad_mask(const bool (&nz)[2][2], int I0, in
 // The wait also hands every thread the block masks of both tiles (`ma`, `mb`); `s` is
 // the pair's index in the task (the add tile comes last).  NoWait: all operands exist
 // already (level / per-op launches), the masks come from the mask array.
+// The pair loop asks in windows: `extend(pairs, s, npairs, window)` (every thread, a
+// barrier inside; window counts the calls) returns e > s such that pairs s .. e-1 are
+// all ready, and `pair_masks(j, p, ...)` then gives pair j's masks for j < e.
 struct NoWait {
     const uint32_t* mask;
     __device__ inline void operator()(int, int sa, int sb, uint32_t& ma, uint32_t& mb) const {
         __syncthreads();
         ma = mask[sa];
         mb = mask[sb];
+    }
+    __device__ inline int extend(const Pair*, int, int npairs, int) const {
+        __syncthreads();
+        return npairs;
+    }
+    __device__ inline void pair_masks(int, const Pair& p, uint32_t& ma, uint32_t& mb) const {
+        ma = mask[pair_a(p)];
+        mb = mask[pair_b(p)];
     }
 };
 
@@ -257,24 +320,58 @@ __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, c
                     acc[m][n][r] = C0[(wr + m * 16 + MfmaOp<T>::row(lk, r)) * TS + wc + n * 16 + li];
             }
         }
+    // A product is zero when either operand's mask is (B taken in (k, j) orientation).
+    auto zero_product = [&](const Pair& q, uint32_t qa, uint32_t qb) {
+        if constexpr (!MASKS) return false;
+        const uint32_t qbk = (TBOK && q.a < 0) ? mask_transpose(qb) : qb;
+        return qa == 0 || qbk == 0;
+    };
+    // Pairs [0, ready) are known to be ready.  The operands of `staged` (the next
+    // nonzero pair inside that window) are loaded into ga / gb while the current pair
+    // multiplies; a pair outside the window is waited for (and the window extended)
+    // when the loop reaches it.  FP64 tiles are not staged: 64 more live registers
+    // across the MFMA loop make the FP64 kernels spill.
+    constexpr bool STAGE = sizeof(T) == 4;
+    Seg<T> ga, gb;
+    int ready = 0, staged = -1, window = 0;
+    uint32_t sma = 0, smb = 0;
     for (int s = 0; s < npairs; s++) {
         const Pair p = pairs[s];
         const bool neg = p.b < 0, tb = TBOK && p.a < 0;
-        const int sa = pair_a(p), sb = pair_b(p);
-        uint32_t ma, mbs;                                       // the tiles' masks as stored
-        wait(s, sa, sb, ma, mbs);
-        uint32_t kuse = 0xFu;
-        if constexpr (MASKS) {
-            const uint32_t mb = tb ? mask_transpose(mbs) : mbs;  // B (or B^T) in (k, j) orientation
-            if (ma == 0 || mb == 0) continue;                    // a zero product: every thread skips
-            tile_to_lds_masked(pool + (size_t)sa * TE, la, LDA, tid, ma);
-            tile_to_lds_masked(pool + (size_t)sb * TE, lb, tb ? LDA : LDB, tid, mbs);
-            kuse = k_blocks_used(ma, mb, wr >> 4, wc >> 4);
+        uint32_t ma, mbs;  // the tiles' masks as stored
+        bool synced = false;
+        if (s != staged) {
+            if (s == ready) {
+                ready = wait.extend(pairs, s, npairs, window++);  // its barrier also frees LDS
+                synced = true;
+            }
+            wait.pair_masks(s, p, ma, mbs);
+            if (zero_product(p, ma, mbs)) continue;  // every thread skips
+            seg_load(pool + (size_t)pair_a(p) * TE, ga, tid, !MASKS || seg_on(ma, tid));
+            seg_load(pool + (size_t)pair_b(p) * TE, gb, tid, !MASKS || seg_on(mbs, tid));
         } else {
-            tile_to_lds(pool + (size_t)sa * TE, la, LDA, tid);
-            tile_to_lds(pool + (size_t)sb * TE, lb, tb ? LDA : LDB, tid);
+            ma = sma;
+            mbs = smb;
+        }
+        if (!synced) __syncthreads();  // every wave is done reading the previous pair from LDS
+        seg_store(ga, la, LDA, tid, !MASKS || seg_on(ma, tid));
+        seg_store(gb, lb, tb ? LDA : LDB, tid, !MASKS || seg_on(mbs, tid));
+        staged = -1;
+        for (int n = s + 1; STAGE && n < ready; n++) {
+            const Pair q = pairs[n];
+            uint32_t qa, qb;
+            wait.pair_masks(n, q, qa, qb);
+            if (zero_product(q, qa, qb)) continue;
+            seg_load(pool + (size_t)pair_a(q) * TE, ga, tid, !MASKS || seg_on(qa, tid));
+            seg_load(pool + (size_t)pair_b(q) * TE, gb, tid, !MASKS || seg_on(qb, tid));
+            staged = n;
+            sma = qa;
+            smb = qb;
+            break;
         }
         __syncthreads();
+        uint32_t kuse = 0xFu;
+        if constexpr (MASKS) kuse = k_blocks_used(ma, tb ? mask_transpose(mbs) : mbs, wr >> 4, wc >> 4);
         const T sg = neg ? T(-1) : T(1);
         if (tb) gemm_lds<T, TBOK>(acc, la, lb, sg, wr, wc, li, lk, kuse);
         else gemm_lds<T, false>(acc, la, lb, sg, wr, wc, li, lk, kuse);
