@@ -339,7 +339,7 @@ __global__ __launch_bounds__(NT, 2) void k_trsm_persistent(SolveArgs a) {
         }
         __syncthreads();  // every wave's stores are complete (workgroup release)
         if (tid == 0) {   // one device-scope release for the workgroup, then the flag
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            release_agent();
             __hip_atomic_store(a.ready + I, a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();  // separates the publish branch from the next fetch branch

@@ -52,6 +52,17 @@ constexpr int lds_bytes() {
                 sizeof(T) == 8 ? 0 : (TS * LDA + 2 * TS) * 8);
 }
 
+// ------------------------------------------------------------------ publishing
+// Device-scope release: write the L2 back, then wait until the write-back is done, so
+// a flag stored next is seen only after every earlier store of the workgroup (the
+// caller's barrier orders the other waves' stores before it).  Written as asm: the
+// compiler's wait-count pass does not count buffer_wbl2 as an outstanding vector
+// memory operation and drops the wait that should follow it when no store of the wave
+// is pending, and the flag can then overtake the write-back (consumers on other XCDs
+// read stale tiles).  vmcnt(0) also waits for every earlier vector memory operation of
+// the wave.
+__device__ __forceinline__ void release_agent() { asm volatile("buffer_wbl2 sc1\n\ts_waitcnt vmcnt(0)" ::: "memory"); }
+
 // ------------------------------------------------------------------ loads
 // thread t moves row t/4, columns (t%4)*16 .. +15 of a tile between global and LDS
 // (that segment lies in 16x16 block (t/64, t%4))

@@ -1,4 +1,5 @@
-"""Wait states around the hand-issued DPP updates of the tile LU.
+"""Wait states around the hand-issued DPP updates of the tile LU, and the order of
+the device-scope releases that publish tiles and rows.
 
 The diagonal-block elimination issues its row-broadcast FMAs as asm statements
 (v_fmac_*_dpp row_newbcast), which the compiler's hazard recognizer does not see
@@ -8,6 +9,13 @@ instruction in the five before.  This test compiles the factor kernels to gfx950
 assembly and checks every DPP instruction in it against those two rules, counting
 one wait state per instruction and N + 1 per s_nop N, and treating a label inside
 the window as a violation (the branch into it could come from anywhere).
+
+A release writes the L2 back (buffer_wbl2) and must wait for that write-back
+(s_waitcnt vmcnt(0)) before the flag store that follows it; the compiler's own
+release omitted that wait where no store of the wave was pending, so the flag
+could be seen before the tiles.  The second check compiles the factor and solve
+kernels and requires such a wait after every buffer_wbl2, before the next vector
+store or atomic and before any label.
 """
 import os
 import re
@@ -36,9 +44,12 @@ def instructions(asm):
     out = []
     for line in asm.splitlines():
         s = line.split(";")[0].strip()
-        if not s or s.startswith("."):
+        if not s:
             continue
-        out.append(("label", s) if s.endswith(":") else ("inst", s))
+        if s.endswith(":"):  # block labels (.LBB...) start with a dot too
+            out.append(("label", s))
+        elif not s.startswith("."):
+            out.append(("inst", s))
     return out
 
 
@@ -96,15 +107,65 @@ def test_checker_finds_planted_hazards():
     assert "v27" in bad[0][1] and "v[4:5]" in bad[1][1] and "EXEC" in bad[2][1]
 
 
-@pytest.mark.skipif(not (os.path.exists(HIPCC) or shutil.which("hipcc")), reason="hipcc not installed")
-def test_factor_kernels_keep_dpp_wait_states(tmp_path):
-    out = tmp_path / "factor.s"
+def release_violations(asm):
+    """Every buffer_wbl2 must be followed by s_waitcnt vmcnt(0) (or a wait covering it)
+    before the next vector store / atomic and before any label."""
+    insts = instructions(asm)
+    bad, n = [], 0
+    for idx, (kind, s) in enumerate(insts):
+        if kind != "inst" or not s.startswith("buffer_wbl2"):
+            continue
+        n += 1
+        for k2, t in insts[idx + 1:]:
+            if k2 == "label":
+                bad.append((s, "label before the wait: " + t))
+                break
+            if t.startswith("s_waitcnt") and re.search(r"vmcnt\(0\)", t):
+                break
+            if re.match(r"(global|buffer|flat)_(store|atomic)", t):
+                bad.append((s, "store before the wait: " + t))
+                break
+    return n, bad
+
+
+def test_release_checker_finds_planted_mistakes():
+    asm = "\n".join([
+        "f:", "\tbuffer_wbl2 sc1", "\ts_waitcnt vmcnt(0)", "\tglobal_store_dwordx2 v[6:7], v[4:5], off sc1",
+        "\tbuffer_wbl2 sc1", "\tglobal_store_dwordx2 v[6:7], v[4:5], off sc1",          # flag first
+        "\tbuffer_wbl2 sc1", "\ts_waitcnt lgkmcnt(0)", ".L1:", "\ts_waitcnt vmcnt(0)",  # label first
+        "\tbuffer_wbl2 sc1", "\ts_waitcnt vmcnt(0) lgkmcnt(0)", "\tglobal_atomic_add v1, v2, s[0:1]",
+    ])
+    n, bad = release_violations(asm)
+    assert n == 4 and len(bad) == 2
+    assert "store" in bad[0][1] and "label" in bad[1][1]
+
+
+def _compile(tmp_path, name):
+    out = tmp_path / (name + ".s")
     cmd = [HIPCC, "-std=c++17", "-O3", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
            "-packed-fp32-ops", "-Wno-unused-result", "-I" + os.path.join(ROOT, "include"),
-           "--cuda-device-only", "-S", "-o", str(out), os.path.join(PKG, "csrc", "device", "factor.hip")]
+           "--cuda-device-only", "-S", "-o", str(out), os.path.join(PKG, "csrc", "device", name + ".hip")]
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-2000:]
-    asm = out.read_text()
+    return out.read_text()
+
+
+@pytest.fixture(scope="module")
+def factor_asm(tmp_path_factory):
+    return _compile(tmp_path_factory.mktemp("asm"), "factor")
+
+
+@pytest.mark.skipif(not (os.path.exists(HIPCC) or shutil.which("hipcc")), reason="hipcc not installed")
+def test_releases_wait_for_the_write_back(factor_asm, tmp_path):
+    for name, asm in (("factor", factor_asm), ("solve", _compile(tmp_path, "solve"))):
+        n, bad = release_violations(asm)
+        assert n > 0, name
+        assert not bad, (name, bad[:5])
+
+
+@pytest.mark.skipif(not (os.path.exists(HIPCC) or shutil.which("hipcc")), reason="hipcc not installed")
+def test_factor_kernels_keep_dpp_wait_states(factor_asm):
+    asm = factor_asm
     n, bad = dpp_violations(asm)
     assert asm.count("v_fmac_f32_dpp") > 0 and asm.count("v_fmac_f64_dpp") > 0
     assert n > 1000
