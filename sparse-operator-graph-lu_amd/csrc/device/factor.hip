@@ -148,7 +148,7 @@ __device__ inline unsigned long long spin_ready(const Args& a, int s) {
 // and acquire), and the device-scope fences are cumulative, so one L2 write-back
 // (release, after the barrier) or one L1/L2 invalidate (acquire, before the barrier)
 // per task serves all four waves; per-wave fences would repeat them four times.
-__device__ __forceinline__ void acquire_for_workgroup() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+__device__ __forceinline__ void acquire_for_workgroup() { acquire_agent(); }
 __device__ __forceinline__ void release_for_workgroup() { release_agent(); }
 
 // The masks come with the flags.  A window of pairs: wave 0 loads the flags of up to

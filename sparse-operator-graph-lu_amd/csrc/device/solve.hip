@@ -295,7 +295,7 @@ __global__ __launch_bounds__(NT, 2) void k_trsm_persistent(SolveArgs a) {
             if (tm == 0) continue;  // a zero tile: no product, no wait for row J
             if (tid == 0) {
                 spin_row(a, J);
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                acquire_agent();
             }
             __syncthreads();  // also: the previous products are done with la / lb
             tile_to_lds_f64(pool + (size_t)a.slot[e] * TE, la, LDA, tid, tm);
