@@ -60,6 +60,57 @@ def test_gemm_tile(dev, precision, tol, npairs):
 
 
 @pytest.mark.parametrize("precision,tol", [(sogl.FP64, 1e-13), (sogl.FP32, 2e-5)])
+def test_gemm_task_windows(dev, precision, tol):
+    """One GEMM task of 70 pairs: more than the 64-pair mask ring and three flag
+    windows of at most 32 pairs.  Eight of its first operands are made by other GEMM
+    tasks of the same run, so the task waits for them after its ready windows; every
+    fifth second operand is a zero tile (a product the block masks skip, with no
+    barrier) and others have zero 16x16 blocks (loaded as zeros).  Two runs must agree
+    bit for bit and match the oracle's sum."""
+    g = np.random.default_rng(77)
+    ins = []
+
+    def add(t):
+        ins.append(t)
+        return len(ins)  # input ids are 1..len(ins)
+
+    made, pairs = {}, []
+    for k in range(70):
+        y = g.standard_normal((64, 64))
+        if k % 5 == 3:
+            y = np.zeros((64, 64))
+        elif k % 4 == 1:
+            y[16:32, :] = 0
+            y[:, 48:] = 0
+        if k % 9 == 4:
+            made[k] = (add(g.standard_normal((64, 64))), add(g.standard_normal((64, 64))))
+            xid, x = None, None
+        else:
+            x = g.standard_normal((64, 64))
+            xid = add(x)
+        pairs.append((k, xid, add(y), x, y))
+    nid = len(ins) + 1
+    ops, xs = [], {}
+    for k, (uid, vid) in made.items():
+        ops.append(op(uid, vid, 8, nid, stage=1))
+        xs[k] = (nid, oracle.gemm(ins[uid - 1], ins[vid - 1], np.zeros((64, 64)), 1))
+        nid += 1
+    r = nid
+    want = np.zeros((64, 64))
+    for k, xid, yid, x, y in pairs:
+        if xid is None:
+            xid, x = xs[k]
+        sign = 1 if k % 2 == 0 else -1
+        ops.append(op(xid, yid, 8 if sign > 0 else 9, r, stage=2))
+        want = oracle.gemm(x, y, want, sign)
+    c1, _ = run_ops(dev, r + 1, ops, ins, r, precision)
+    c2, _ = run_ops(dev, r + 1, ops, ins, r, precision)
+    assert np.array_equal(c1, c2)
+    err = np.abs(c1 - want).max() / np.abs(want).max()
+    assert err < tol, err
+
+
+@pytest.mark.parametrize("precision,tol", [(sogl.FP64, 1e-13), (sogl.FP32, 2e-5)])
 def test_lu_tile(dev, precision, tol):
     a = dd_tile()
     l, u = run_ops(dev, 4, [op(1, 0, 1, 2, 3)], [a], 2, precision, out2=3)
