@@ -98,9 +98,6 @@ __global__ void k_fill(T* p, size_t n, T v) {
     for (; i < n; i += stride) p[i] = v;
 }
 
-// s_waitcnt immediate (gfx9 layout) waiting for vmcnt = 0 only: vmcnt bits 3:0 and
-// 15:14 zero, expcnt (6:4) and lgkmcnt (11:8) at their maxima
-constexpr int VMCNT0 = 0x0F70;
 constexpr long long SPIN_LIMIT_TICKS = 20LL * 100 * 1000 * 1000;  // ~20 s of the 100 MHz wall clock
 
 struct Args {
@@ -351,14 +348,12 @@ __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
             break;
         }
         run_task<T, BF16, false, LLT, MASKS>(t, a, lds, s_mask, PairWait{&a, &t_wait, s_pm});
-        // publish: every wave waits until its own stores are done (vmcnt 0; the
-        // barrier's workgroup release waits for LDS only), so the device-scope release
-        // that thread 0 then makes covers all four waves' tiles, and a chained task's
-        // loads after the barrier read the finished sum.  Thread 0 writes the output
-        // masks, releases once and raises the flags, which carry the masks to the
-        // consumers.  A sum that goes to the chained task is read by nobody else: no
-        // release, no flag.
-        __builtin_amdgcn_s_waitcnt(VMCNT0);
+        // publish: the barrier (a workgroup release: the waves' stores go through the
+        // CU's memory pipeline in order, ahead of anything issued after it), then
+        // thread 0 writes the output masks, makes everything visible at device scope
+        // once (the write-back covers all four waves' tiles) and raises the flags,
+        // which carry the masks to the consumers.  A sum that goes to the chained task
+        // is read by nobody else: no release, no flag.
         __syncthreads();
         if (a.trace && tid == 0) t_ready = wall_clock64();  // work done, publish starts
         if (tid == 0) {
