@@ -239,7 +239,7 @@ struct PairWait {
 
 // LLT: the kernel of an L L^T factor (llt, transpose and A B^T tasks, no lu); the LU
 // kernels are built without those, which keeps their register use as it was.
-template <typename T, bool BF16, bool ACC, bool LLT, bool MASKS, typename Wait>
+template <typename T, bool BF16, bool ACC, bool LLT, bool MASKS, bool ONE = false, typename Wait>
 __device__ __attribute__((always_inline)) void run_task(const DevTask& t, const Args& a, char* lds, uint32_t* s_mask,
                                                         const Wait& wait) {
     T* pool = reinterpret_cast<T*>(a.pool);
@@ -249,8 +249,8 @@ __device__ __attribute__((always_inline)) void run_task(const DevTask& t, const 
                 gemm_task_bf16<LLT, MASKS>(reinterpret_cast<float*>(a.pool), a.pairs + t.a, t.b, t.out0, lds, s_mask,
                                     wait, t.add, t.add_neg != 0);
             } else {
-                gemm_task<T, ACC, LLT, MASKS>(pool, a.pairs + t.a, t.b, t.out0, lds, s_mask, wait, t.add,
-                                       t.add_neg != 0);
+                gemm_task<T, ACC, LLT, MASKS, Wait, ONE>(pool, a.pairs + t.a, t.b, t.out0, lds, s_mask, wait, t.add,
+                                                  t.add_neg != 0);
             }
             break;
         case TK_SUB:
@@ -300,8 +300,10 @@ __device__ inline void write_masks(const DevTask& t, const Args& a, uint32_t m) 
 // tasks wait for windows of pairs (pairs are ordered by when their operands become
 // ready) and multiply each ready pair while loading the next; other tasks wait for
 // all their sources first.
-template <typename T, bool BF16, bool LLT, bool MASKS>
-__global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
+// ONE: built for one resident workgroup per CU (twice the registers; FP64 GEMMs stage
+// their next pair too), launched when the executor asks for one per CU.
+template <typename T, bool BF16, bool LLT, bool MASKS, bool ONE>
+__global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
     __shared__ __attribute__((aligned(16))) char lds[lds_bytes<T>()];
     __shared__ int s_task;
     __shared__ int s_abort;
@@ -347,7 +349,7 @@ __global__ __launch_bounds__(NT, 2) void k_factor_persistent(Args a) {
             aborted = true;
             break;
         }
-        run_task<T, BF16, false, LLT, MASKS>(t, a, lds, s_mask, PairWait{&a, &t_wait, s_pm});
+        run_task<T, BF16, false, LLT, MASKS, ONE>(t, a, lds, s_mask, PairWait{&a, &t_wait, s_pm});
         // publish: the barrier (a workgroup release: the waves' stores go through the
         // CU's memory pipeline in order, ahead of anything issued after it), then
         // thread 0 writes the output masks, makes everything visible at device scope
@@ -1257,12 +1259,15 @@ int launch_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
         a.ntasks = (int32_t)f->tasks.size();
         a.queue = (const int32_t*)f->queue_d.p;
         a.nqueue = (int32_t)f->queue.size();
+        // one workgroup per CU: the kernel built for that occupancy
+        const bool one = wg_per_cu == 1;
+        auto kern = one ? k_factor_persistent<T, BF16, LLT, MASKS, true> : k_factor_persistent<T, BF16, LLT, MASKS, false>;
         int per_cu = 0;
-        HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_persistent<T, BF16, LLT, MASKS>, NT, 0));
+        HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, 0));
         if (per_cu < 1) per_cu = 1;
         if (wg_per_cu > 0) per_cu = std::min(per_cu, wg_per_cu);  // never more than fit: all must be resident
         int grid = std::max(1, std::min<int>(a.nqueue, c->num_cu * per_cu));
-        if (a.nqueue > 0) hipLaunchKernelGGL((k_factor_persistent<T, BF16, LLT, MASKS>), dim3(grid), dim3(NT), 0, st, a);
+        if (a.nqueue > 0) hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, st, a);
     } else if (mode == SOGL_EXEC_LEVELS) {
         launch_levels();
     } else if (mode == SOGL_EXEC_GRAPH) {

@@ -314,7 +314,10 @@ __device__ __forceinline__ void add_tile(acc_t (&acc)[2][2], const T* __restrict
 // MASKS: skip zero products and zero k blocks by the operands' masks, and leave the
 // mask bits of wave w's quadrant of the result in out_mask[w]; without, every pair is
 // multiplied in full and nothing is left (the result counts as full).
-template <typename T, bool ACC = false, bool TBOK = false, bool MASKS = true, typename Wait = NoWait>
+// STAGE64: stage FP64 operands too (the kernel built for one workgroup per CU has the
+// registers for it).
+template <typename T, bool ACC = false, bool TBOK = false, bool MASKS = true, typename Wait = NoWait,
+          bool STAGE64 = false>
 __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, const Pair* __restrict__ pairs, int npairs,
                           int out, char* lds_raw, uint32_t* out_mask, const Wait& wait, int add = -1,
                           bool add_neg = false) {
@@ -346,9 +349,10 @@ __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, c
     // Pairs [0, ready) are known to be ready.  The operands of `staged` (the next
     // nonzero pair inside that window) are loaded into ga / gb while the current pair
     // multiplies; a pair outside the window is waited for (and the window extended)
-    // when the loop reaches it.  FP64 tiles are not staged: 64 more live registers
-    // across the MFMA loop make the FP64 kernels spill.
-    constexpr bool STAGE = sizeof(T) == 4;
+    // when the loop reaches it.  FP64 tiles are staged only in the kernel built for one
+    // workgroup per CU: 64 more live registers across the MFMA loop make the FP64
+    // kernels built for two spill.
+    constexpr bool STAGE = sizeof(T) == 4 || STAGE64;
     Seg<T> ga, gb;
     int ready = 0, staged = -1, window = 0;
     uint32_t sma = 0, smb = 0;

@@ -540,6 +540,24 @@ def test_autotune_and_auto_mode(dev):
     dev.release(fid)
 
 
+@pytest.mark.parametrize("precision", [sogl.FP64, sogl.FP32])
+def test_one_workgroup_per_cu_kernel(dev, precision):
+    """The persistent kernel built for one workgroup per CU (its FP64 GEMMs stage the
+    next pair as well) gives the factor bits of the levels executor, masks on."""
+    p = W.convdiff2d(128)
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
+    out = []
+    for run in (lambda f: dev.factor(f, sogl.EXEC_LEVELS),
+                lambda f: dev.factor_config(f, sogl.EXEC_PERSISTENT, 1, 1)):
+        fid = dev.load(pl, precision)
+        dev.set_masks(fid, sogl.MASKS_ON)
+        run(fid)
+        run(fid)
+        out.append((dev.factor_tiles(fid, 0, pl.info.n_l_tiles), dev.factor_tiles(fid, 1, pl.info.n_u_tiles)))
+        dev.release(fid)
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 def test_resident_factor_repeat_solves(dev):
     p = W.convdiff2d(128)
     pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
