@@ -43,7 +43,7 @@ HBM_PEAK = 8000.0  # GB/s
 # --pmc runs (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the scales a plain 1 GiB copy
 # reads): profiles/k_factor_persistent_c2_pmc_traffic.md (dense tiles) and
 # profiles/k_factor_persistent_c5_pmc_traffic.md (16x16 block masks on)
-TRAFFIC_FP32_FACTOR = {"c2": 22.65e9, "c5": 23.85e9}
+TRAFFIC_FP32_FACTOR = {"c2": 22.61e9, "c5": 23.85e9}
 
 WORKLOADS = {
     "c1": ("2D 5-point Poisson 64x64 (n=4,096), 1 RHS U[-1,1] seed 0", 1e-12),
