@@ -229,6 +229,20 @@ int sogl_solve_device(sogl_ctx* ctx, int32_t factor_id, const double* d_bp, doub
                       int32_t nrhs);
 int sogl_factor_release(sogl_ctx* ctx, int32_t factor_id);
 
+/* No counterpart in the reference.  sogl_solve_device for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same kernels, but the solve's intermediates (the forward
+ * result y, per-block-row flags and control words) live in a fixed per-GPU scratch
+ * array of SOGL_NOALLOC_SCRATCH_BYTES instead of buffers the factor allocates, and the
+ * call allocates nothing.  It needs 256 + 4*block_rows (rounded up to a multiple of
+ * 256) + 512*block_rows*nrhs bytes of the array; a larger solve returns SOGL_ERR_ARG.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU (several
+ * host threads or contexts) race.  The factor must be loaded and factored (loading
+ * allocates). */
+#define SOGL_NOALLOC_SCRATCH_BYTES (64u << 20)
+int sogl_solve_device_noalloc(sogl_ctx* ctx, int32_t factor_id, const double* d_bp, double* d_xp,
+                              int32_t nrhs);
+
 typedef struct sogl_factor_stats {
     double factor_ms;       /* last sogl_factor_run, device time (executor + diagonal inverses) */
     double exec_ms;         /* last sogl_factor_run, executor kernel(s) alone */

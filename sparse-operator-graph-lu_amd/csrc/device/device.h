@@ -111,6 +111,9 @@ int check_hip(Context* c, hipError_t e, const char* what);
 int dev_alloc(Context* c, DevBuf& b, size_t bytes, const char* what);
 void dev_free(DevBuf& b);
 Factor* find_factor(Context* c, int32_t id);
+// implemented in solve.hip: looks up the device address of the no-allocation solve's
+// scratch array on `device` (current), once per device
+void noalloc_scratch_init(int device);
 
 }  // namespace sogl_dev
 

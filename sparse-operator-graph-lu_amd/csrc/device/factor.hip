@@ -1327,6 +1327,7 @@ extern "C" int sogl_ctx_create(int32_t device, sogl_ctx** out) {
         delete x;
         return rc;
     }
+    noalloc_scratch_init(device);
     *out = x;
     return SOGL_OK;
 }

@@ -375,8 +375,18 @@ namespace {
         if (_rc != SOGL_OK) return _rc;                          \
     } while (0)
 
+// Buffers of one solve beyond its input and output: the forward result y (padded n x
+// nrhs), the per-block-row ready flags of the MFMA form (they hold the epoch of the pass
+// that stored the row, so a fresh set must start at zero) and the control words.
+struct SolveWork {
+    double* y;
+    int* ready;
+    int* ctrl;
+    int32_t* epoch;  // last pass's epoch of these flags; each pass takes the next one
+};
+
 template <typename T>
-int launch_pass(Context* c, Factor* f, bool forward, const double* in, double* out, int nrhs) {
+int launch_pass(Context* c, Factor* f, const SolveWork& w, bool forward, const double* in, double* out, int nrhs) {
     SolveArgs a{};
     a.pool = f->pool.p;
     a.rowptr = (const int32_t*)(forward ? f->l_rowptr.p : f->u_rowptr.p);
@@ -389,9 +399,9 @@ int launch_pass(Context* c, Factor* f, bool forward, const double* in, double* o
     a.nrhs = nrhs;
     a.nbr = f->block_rows;
     a.forward = forward ? 1 : 0;
-    a.epoch = ++f->solve_epoch;
-    a.ready = (int*)f->solve_ready.p;
-    a.ctrl = (int*)f->solve_ctrl.p;
+    a.epoch = ++*w.epoch;
+    a.ready = w.ready;
+    a.ctrl = w.ctrl;
     a.mask = (const uint32_t*)f->mask_d.p;
     hipLaunchKernelGGL(k_solve_ctrl_reset, dim3(1), dim3(64), 0, c->stream, a.ctrl);
     int per_cu = 0;
@@ -427,6 +437,36 @@ int launch_pass(Context* c, Factor* f, bool forward, const double* in, double* o
 
 }  // namespace
 
+// Both passes over nrhs columns on the context stream, timed; d_x may not alias d_b.
+int solve_core(Context* c, Factor* f, const SolveWork& w, const double* d_b, double* d_x, int nrhs) {
+    const int R = f->block_rows;
+    int rc;
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    const int64_t ld = (int64_t)R * TS;
+    for (int c0 = 0; c0 < nrhs; c0 += TRSM_MAX_RHS) {  // column groups of one MFMA solve launch
+        const int m = std::min(TRSM_MAX_RHS, nrhs - c0);
+        const int64_t off = (int64_t)c0 * ld;
+        if (f->elem == 8) {
+            if ((rc = launch_pass<double>(c, f, w, true, d_b + off, w.y + off, m))) return rc;
+            if ((rc = launch_pass<double>(c, f, w, false, w.y + off, d_x + off, m))) return rc;
+        } else {
+            if ((rc = launch_pass<float>(c, f, w, true, d_b + off, w.y + off, m))) return rc;
+            if ((rc = launch_pass<float>(c, f, w, false, w.y + off, d_x + off, m))) return rc;
+        }
+    }
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(c, hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    f->solve_ms = ms;
+    int ctrl[4];
+    HIP_TRY(c, hipMemcpy(ctrl, w.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
+    if (ctrl[1]) return set_error(c, SOGL_ERR_TIMEOUT, "solve: a dependency wait exceeded its bound");
+    return SOGL_OK;
+}
+
+// The factor's own work buffers (allocated on first use; the flags are zeroed once and
+// then ordered by the factor's epoch counter).
 int solve_device(Context* c, Factor* f, const double* d_b, double* d_x, double* d_y, int nrhs) {
     const int R = f->block_rows;
     int rc;
@@ -436,28 +476,8 @@ int solve_device(Context* c, Factor* f, const double* d_b, double* d_x, double* 
         hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, c->stream, (int*)f->solve_ready.p, R);
         HIP_TRY(c, hipGetLastError());
     }
-    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-    const int64_t ld = (int64_t)R * TS;
-    for (int c0 = 0; c0 < nrhs; c0 += TRSM_MAX_RHS) {  // column groups of one MFMA solve launch
-        const int m = std::min(TRSM_MAX_RHS, nrhs - c0);
-        const int64_t off = (int64_t)c0 * ld;
-        if (f->elem == 8) {
-            if ((rc = launch_pass<double>(c, f, true, d_b + off, d_y + off, m))) return rc;
-            if ((rc = launch_pass<double>(c, f, false, d_y + off, d_x + off, m))) return rc;
-        } else {
-            if ((rc = launch_pass<float>(c, f, true, d_b + off, d_y + off, m))) return rc;
-            if ((rc = launch_pass<float>(c, f, false, d_y + off, d_x + off, m))) return rc;
-        }
-    }
-    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
-    HIP_TRY(c, hipEventSynchronize(c->ev1));
-    float ms = 0;
-    HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    f->solve_ms = ms;
-    int ctrl[4];
-    HIP_TRY(c, hipMemcpy(ctrl, f->solve_ctrl.p, sizeof(ctrl), hipMemcpyDeviceToHost));
-    if (ctrl[1]) return set_error(c, SOGL_ERR_TIMEOUT, "solve: a dependency wait exceeded its bound");
-    return SOGL_OK;
+    const SolveWork w{d_y, (int*)f->solve_ready.p, (int*)f->solve_ctrl.p, &f->solve_epoch};
+    return solve_core(c, f, w, d_b, d_x, nrhs);
 }
 
 int ensure_vectors(Context* c, Factor* f, int nrhs) {
@@ -521,4 +541,92 @@ extern "C" int sogl_solve_device(sogl_ctx* x, int32_t factor_id, const double* d
     int rc;
     if ((rc = ensure_vectors(c, f, nrhs))) return rc;
     return solve_device(c, f, d_bp, d_xp, (double*)f->vec_y.p, nrhs);
+}
+
+// ------------------------------------------------------------------ solve without allocation
+// The solve's work buffers (y, flags, control) in one fixed array per GPU, for callers
+// that cannot allocate device memory.  In host code `scratch` names a host-side stand-in
+// for the array, never its device address: the address each GPU gives its own copy is
+// looked up once per device, when the first context on it is created, and only that
+// address is handed to kernels and copies.
+namespace sogl_dev {
+namespace {
+
+static __device__ __align__(256) unsigned char scratch[SOGL_NOALLOC_SCRATCH_BYTES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[SCRATCH_MAX_DEVICES];       // device address of `scratch` on each GPU
+hipError_t g_scratch_rc[SCRATCH_MAX_DEVICES];   // result of the lookup (valid when looked up)
+bool g_scratch_done[SCRATCH_MAX_DEVICES];
+
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Layout inside the array: control words, the flags of `rows` block rows, then y.
+struct ScratchLayout {
+    size_t ctrl, ready, y, end;
+};
+ScratchLayout scratch_layout(int32_t rows, int32_t nrhs) {
+    ScratchLayout l;
+    l.ctrl = 0;
+    l.ready = 256;
+    l.y = l.ready + align256((size_t)rows * 4);
+    l.end = l.y + (size_t)rows * TS * (size_t)nrhs * 8;
+    return l;
+}
+
+}  // namespace
+
+// Looks up this GPU's address of `scratch`, once per device (the device is current).
+void noalloc_scratch_init(int device) {
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_done[device]) return;
+    void* p = nullptr;
+    g_scratch_rc[device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (g_scratch_rc[device] != hipSuccess) {
+        (void)hipGetLastError();
+        p = nullptr;
+    }
+    g_scratch_ptr[device] = p;
+    g_scratch_done[device] = true;
+}
+
+}  // namespace sogl_dev
+
+extern "C" int sogl_solve_device_noalloc(sogl_ctx* x, int32_t factor_id, const double* d_bp, double* d_xp,
+                                         int32_t nrhs) {
+    if (!x || !d_bp || !d_xp || nrhs <= 0) return set_error(x ? &x->c : nullptr, SOGL_ERR_ARG, "bad argument");
+    Context* c = &x->c;
+    std::lock_guard<std::mutex> g(c->mu);
+    Factor* f = find_factor(c, factor_id);
+    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+    const int dev = c->device;
+    void* base = nullptr;
+    hipError_t lookup = hipErrorNotInitialized;
+    {
+        std::lock_guard<std::mutex> sg(g_scratch_mu);
+        if (dev >= 0 && dev < SCRATCH_MAX_DEVICES && g_scratch_done[dev]) {
+            base = g_scratch_ptr[dev];
+            lookup = g_scratch_rc[dev];
+        }
+    }
+    if (!base)
+        return set_error(c, SOGL_ERR_DEVICE, std::string("solve without allocation: no scratch address for this "
+                                                         "device: ") + hipGetErrorString(lookup));
+    const ScratchLayout l = scratch_layout(f->block_rows, nrhs);
+    if (l.end > (size_t)SOGL_NOALLOC_SCRATCH_BYTES)
+        return set_error(c, SOGL_ERR_ARG, "solve without allocation: " + std::to_string(f->block_rows) +
+                                              " block rows x " + std::to_string(nrhs) + " right-hand sides need " +
+                                              std::to_string(l.end) + " bytes of scratch, more than its " +
+                                              std::to_string((size_t)SOGL_NOALLOC_SCRATCH_BYTES));
+    HIP_TRY(c, hipSetDevice(dev));
+    char* s = (char*)base;
+    // the flags start at zero for every call (earlier calls, of any factor, left their
+    // epochs in them); a fill kernel, so the call stays capturable
+    hipLaunchKernelGGL(k_zero_i32, dim3(64), dim3(256), 0, c->stream, (int*)(s + l.ready), f->block_rows);
+    HIP_TRY(c, hipGetLastError());
+    int32_t epoch = 0;
+    const SolveWork w{(double*)(s + l.y), (int*)(s + l.ready), (int*)(s + l.ctrl), &epoch};
+    return solve_core(c, f, w, d_bp, d_xp, nrhs);
 }

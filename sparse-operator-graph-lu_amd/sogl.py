@@ -119,6 +119,7 @@ def load(name: str = "libsogl.so") -> C.CDLL:
             "sogl_factor_autotune": (C.c_int, [vp, i32, i32, P(dbl), P(i32)]),
             "sogl_solve": (C.c_int, [vp, i32, P(dbl), P(dbl), P(dbl), i32]),
             "sogl_solve_device": (C.c_int, [vp, i32, vp, vp, i32]),
+            "sogl_solve_device_noalloc": (C.c_int, [vp, i32, vp, vp, i32]),
             "sogl_factor_release": (C.c_int, [vp, i32]),
             "sogl_factor_get_stats": (C.c_int, [vp, i32, P(FactorStats)]),
             "sogl_factor_get_tiles": (C.c_int, [vp, i32, i32, P(dbl)]),
