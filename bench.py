@@ -3,18 +3,22 @@
 
 One step = one factorisation of the workload's matrix from its input tiles
 resident in HBM, plus the solve of its right-hand side(s) from b resident in
-HBM, to relative residual ||b-Ax||_2/(||A||_F ||x||_2 + ||b||_2) <= tol:
+HBM, to a normwise backward error <= tol in both forms
+||b-Ax||_inf/(||A||_inf ||x||_inf + ||b||_inf) and ||b-Ax||_2/(||A||_F ||x||_2 + ||b||_2):
 
   fp32_ir : FP32 tiles, FP32 MFMA block GEMMs, then FP64 iterative refinement
             (the headline `value`)
   fp64    : the plain FP64 drop-in (FP64 MFMA GEMMs, FP64 tiles)
+
+The default workload is the 1M-unknown 2D convection-diffusion problem (c5) at
+tol 1e-10, the configuration the metric is quoted on.
 
 Host work (Matrix Market parse, GPS ordering, planning, tile upload) happens
 before the timed region, like the reference's separate "re Order time" /
 "plan time" lines.  The reference's own CPU path (built from its sources by
 oracle/Makefile) is timed on the same host as the CPU baseline.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c5]
 """
 from __future__ import annotations
 
@@ -123,15 +127,49 @@ def cpu_reference(prob, threads: int, timeout: float = 600.0):
 
 
 def physical_cores() -> int:
+    """Physical cores this process may run on (its CPU affinity, counted once per core)."""
     try:
-        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True).stdout
-        cores = {ln for ln in out.splitlines() if ln and not ln.startswith("#")}
+        allowed = os.sched_getaffinity(0)
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True).stdout
+        cores = set()
+        for ln in out.splitlines():
+            if ln and not ln.startswith("#"):
+                cpu, core, sock = ln.split(",")[:3]
+                if int(cpu) in allowed:
+                    cores.add((sock, core))
         return max(1, len(cores))
-    except OSError:
+    except (OSError, ValueError):
         return os.cpu_count() or 1
 
 
-def run_path(dev, plan, prob, precision, tol, max_iter, steps, warmup, mode, wg_per_cu=0):
+def cpu_description() -> dict:
+    """CPU model, sockets, physical cores and logical CPUs of the host, and how many of
+    them this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    host_cores, sockets = None, None
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True).stdout
+        rows = [ln.split(",") for ln in out.splitlines() if ln and not ln.startswith("#")]
+        host_cores = len({(r[2], r[1]) for r in rows})
+        sockets = len({r[2] for r in rows})
+    except (OSError, IndexError):
+        pass
+    return {"cpu_model": model, "sockets": sockets, "host_physical_cores": host_cores,
+            "host_logical_cpus": os.cpu_count(), "usable_logical_cpus": len(os.sched_getaffinity(0)),
+            "usable_physical_cores": physical_cores()}
+
+
+def run_path(dev, plan, prob, precision, tol, max_iter, steps, warmup, mode, wg_per_cu=0, sweep=()):
+    """Timed factor + refined-solve steps of one precision.  sweep: numbers of refinement
+    corrections after which x is also kept (untimed, after the timed steps)."""
     fid = dev.load(plan, precision)
     dev.attach_refine(fid, plan)
     n = prob.n
@@ -163,13 +201,19 @@ def run_path(dev, plan, prob, precision, tol, max_iter, steps, warmup, mode, wg_
     wall = (time.perf_counter() - t0) / steps
     x = dev.download(d_x, np.empty((nrhs, n))).T
     st = dev.stats(fid)
+    swept = []
+    for k in sweep:  # tol 0: exactly k corrections
+        dev.solve_refined_device(fid, d_b, d_x, nrhs, 0.0, k)
+        xk = dev.download(d_x, np.empty((nrhs, n))).T
+        swept.append((k, xk[:, 0] if nrhs == 1 else xk))
     dev.free(d_b)
     dev.free(d_x)
     dev.release(fid)
     return dict(step_s=wall, factor_s=float(np.mean(fac)) / 1e3, exec_s=float(np.mean(exe)) / 1e3,
                 solve_s=float(np.mean(sol)) / 1e3, ir_iters=int(max(its)), residual=float(max(res)),
                 pool_gb=st.pool_bytes / 1e9, n_tasks=int(st.n_tasks), n_levels=int(st.n_levels), executor=tuned,
-                growth_fp64_tiles=int(st.growth_tiles), x=x[:, 0] if nrhs == 1 else x)
+                growth_fp64_tiles=int(st.growth_tiles), zero_block_share=float(st.zero_blocks),
+                block_masks=bool(st.block_masks), x=x[:, 0] if nrhs == 1 else x, swept=swept)
 
 
 def calibrate_copy(nbytes: int = 1 << 30, reps: int = 3):
@@ -216,7 +260,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS) + ["c4"],
+    ap.add_argument("--config", default="c5", choices=sorted(WORKLOADS) + ["c4"],
                     help="c1..c5: the named workloads; sym: a symmetric-input (L L^T) workload")
     ap.add_argument("--requests", type=int, default=200, help="c4: requests in the serving stream")
     ap.add_argument("--rate", type=float, default=20.0, help="c4: mean arrivals per second")
@@ -227,7 +271,11 @@ def main():
                          "persistent-1wg: the persistent executor with one workgroup per CU")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--bf16", action="store_true",
-                    help="also run the BF16 experiment (bf16 MFMA operands, FP32 tiles, FP64 refinement)")
+                    help="also run the BF16 experiment (bf16 MFMA operands, FP32 tiles, FP64 refinement); "
+                         "on by default for c5")
+    ap.add_argument("--no-bf16", action="store_true", help="skip the BF16 experiment leg of c5")
+    ap.add_argument("--no-ir-sweep", action="store_true",
+                    help="skip the refinement sweep (x error of FP32+IR after 0, 1, 2, ... corrections)")
     ap.add_argument("--compare-executors", action="store_true",
                     help="also time every executor (FP64) incl. one launch per unit op (default for c3)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -257,10 +305,12 @@ def main():
                    for k, s in (("fp64", 8), ("fp32", 4))}
     dev = sogl.Device(0)
     paths = {}
-    paths["fp32_ir"] = run_path(dev, plan, prob, sogl.FP32, tol, 30, args.steps, args.warmup, *mode)
+    sweep = () if args.no_ir_sweep or nrhs != 1 else tuple(range(0, 7))
+    paths["fp32_ir"] = run_path(dev, plan, prob, sogl.FP32, tol, 30, args.steps, args.warmup, *mode, sweep=sweep)
     paths["fp64"] = run_path(dev, plan, prob, sogl.FP64, tol, 0, args.steps, args.warmup, *mode)
-    if args.bf16:
-        paths["bf16_ir"] = run_path(dev, plan, prob, sogl.BF16, tol, 50, args.steps, args.warmup, *mode)
+    if args.bf16 or (args.config == "c5" and not args.no_bf16):
+        paths["bf16_ir"] = run_path(dev, plan, prob, sogl.BF16, tol, 50, min(args.steps, 3), min(args.warmup, 1),
+                                    *mode)
     executors = None
     if args.compare_executors or args.config == "c3":
         executors = compare_executors(dev, plan, per_op=True)
@@ -286,21 +336,30 @@ def main():
     gold = os.path.join(ROOT, "tests", "golden", f"{EXTRA[args.config][1] if args.config in EXTRA else args.config}.npz")
     mat = W.full_matrix(prob)  # the whole matrix (a symmetric input stores one triangle)
     g = np.load(gold, allow_pickle=False) if os.path.exists(gold) else None
+    def x_error(x):
+        x0 = x if x.ndim == 1 else x[:, 0]
+        if ref is not None and ref["xref"] is not None:
+            return float(np.linalg.norm(x0 - ref["xref"]) / np.linalg.norm(ref["xref"]))
+        if g is not None and "xorig" in g:
+            return float(np.linalg.norm(x0 - g["xorig"]) / np.linalg.norm(g["xorig"]))
+        if g is not None and "xorig_sample" in g:
+            idx = g["xorig_idx"]
+            return float(np.linalg.norm(x0[idx] - g["xorig_sample"]) / np.linalg.norm(g["xorig_sample"]))
+        return None
+
     out_paths = {}
+    sweeps = {}
     for k, p in paths.items():
         x = p.pop("x")
         resid = sogl.oracle_residual(prob.n, mat[0], mat[1], mat[2], x, prob.b)
-        err = None
-        x0 = x if x.ndim == 1 else x[:, 0]
-        if ref is not None and ref["xref"] is not None:
-            err = float(np.linalg.norm(x0 - ref["xref"]) / np.linalg.norm(ref["xref"]))
-        elif g is not None and "xorig" in g:
-            err = float(np.linalg.norm(x0 - g["xorig"]) / np.linalg.norm(g["xorig"]))
-        elif g is not None and "xorig_sample" in g:
-            idx = g["xorig_idx"]
-            err = float(np.linalg.norm(x0[idx] - g["xorig_sample"]) / np.linalg.norm(g["xorig_sample"]))
+        berr = sogl.backward_error_inf(prob.n, mat[0], mat[1], mat[2], x, prob.b)
+        err = x_error(x)
+        sweeps[k] = [{"corrections": it, "rel_err_vs_reference_x": x_error(xk),
+                      "backward_error_inf": sogl.backward_error_inf(prob.n, mat[0], mat[1], mat[2], xk, prob.b),
+                      "oracle_residual": sogl.oracle_residual(prob.n, mat[0], mat[1], mat[2], xk, prob.b)}
+                     for it, xk in p.pop("swept")]
         prec = "fp64" if k == "fp64" else "fp32"
-        p.update(oracle_residual=resid, rel_err_vs_reference_x=err,
+        p.update(oracle_residual=resid, backward_error_inf=berr, rel_err_vs_reference_x=err,
                  gemm_tflops=gemm_flops / p["exec_s"] / 1e12,
                  gemm_frac_of_peak=gemm_flops / p["exec_s"] / 1e12 / PEAK[prec],
                  solve_gb_s=solve_bytes[prec] * (1 + p["ir_iters"]) / p["solve_s"] / 1e9)
@@ -319,10 +378,25 @@ def main():
                          ("" if nrhs == 1 else f" for the first of the {nrhs} RHS (the CLI solves one)"),
                "factor_s": ref["factor_s"], "solve_s": ref["solve_s"],
                "reorder_s": ref["reorder_s"], "plan_s": ref["plan_s"],
-               "max_rhs_error": ref["max_rhs_error"]}
+               "max_rhs_error": ref["max_rhs_error"], "wall_s": round(ref["wall_s"], 3),
+               "all_cores": "the reference runs at most 16 OpenMP threads: its per-thread buffers are "
+                            "arrays of MAXTHREAD = 16 (const.h:23, memutil.cpp:33-47) indexed by the "
+                            "thread id, so more threads write past them; cores = min(16, usable "
+                            "physical cores)"}
+        cpu.update(cpu_description())
+    sweep = sweeps.get("fp32_ir") or []
+    fp64_err = out_paths["fp64"]["rel_err_vs_reference_x"]
+    reach = None
+    if sweep and fp64_err is not None:
+        for e in sweep:
+            if e["rel_err_vs_reference_x"] is not None and e["rel_err_vs_reference_x"] <= 10 * fp64_err:
+                reach = e["corrections"]
+                break
     line = {
         "metric": METRIC,
-        "value": round(head["factor_s"] + head["solve_s"], 6),
+        # the whole step: factor, then the refined solve (triangular solves, FP64 residuals
+        # and corrections, permutations) to the tolerance, inputs resident in HBM
+        "value": round(head["step_s"], 6),
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -347,6 +421,12 @@ def main():
         "fp32_ir": out_paths["fp32_ir"],
         "fp64": out_paths["fp64"],
     }
+    if sweep:
+        line["fp32_ir_sweep"] = {
+            "what": "FP32 factor + k FP64 corrections (untimed, after the timed steps): x error against "
+                    "the reference's FP64 x, and both backward-error forms",
+            "steps": [{kk: (float(f"{v:.4g}") if isinstance(v, float) else v) for kk, v in e.items()} for e in sweep],
+            "corrections_to_reach_10x_fp64_x_error": reach}
     if executors is not None:
         line["executors_fp64"] = executors
     if "bf16_ir" in out_paths:

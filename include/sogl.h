@@ -278,9 +278,11 @@ int sogl_factor_get_graph(sogl_ctx* ctx, int32_t factor_id, int64_t* n_deps, int
 /* ------------------------------------------------ mixed precision + refinement */
 /* Attach the original (unpermuted) matrix for the FP64 residual r = b - A x. */
 int sogl_refine_attach(sogl_ctx* ctx, int32_t factor_id, const sogl_plan* plan);
-/* Solve with FP64 iterative refinement in the original order:
- * x = A^-1 b to ||b-Ax||_2 / (||A||_F ||x||_2 + ||b||_2) <= tol, at most max_iter
- * corrections.  b, x: n x nrhs column-major host arrays. */
+/* Solve with FP64 iterative refinement in the original order: x = A^-1 b until the
+ * normwise backward error is <= tol in both forms, ||b-Ax||_inf / (||A||_inf ||x||_inf +
+ * ||b||_inf) and ||b-Ax||_2 / (||A||_F ||x||_2 + ||b||_2), at most max_iter corrections;
+ * *resid receives the larger of the two (the worst column).  b, x: n x nrhs column-major
+ * host arrays. */
 int sogl_solve_refined(sogl_ctx* ctx, int32_t factor_id, const double* b, double* x,
                        int32_t nrhs, double tol, int32_t max_iter, int32_t* iters,
                        double* resid);

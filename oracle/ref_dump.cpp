@@ -43,6 +43,13 @@
 
 using namespace SOGLU;
 
+// Built with SOGL_GPU_SITES (oracle/Makefile target `integration`): the two call
+// sites of INTEGRATION.md section 2, extracted from it verbatim, replace
+// BlockPlanner::calculate() and BlockPlanner::solve() below.
+#ifdef SOGL_GPU_SITES
+#include "integration_sites.inc"
+#endif
+
 static std::string g_out;
 
 template <typename T>
@@ -221,11 +228,27 @@ int main(int argc, char** argv) {
     double t_calc = 0, t_solve = 0;
     if (!plan_only) {
         auto tc = std::chrono::steady_clock::now();
+#ifdef SOGL_GPU_SITES
+        if (gpu_calculate(bl2, bu2) != 0) {
+            std::fprintf(stderr, "gpu_calculate: %s\n", sogl_last_error());
+            return 3;
+        }
+#else
         BlockPlanner::calculate();
+#endif
         t_calc = std::chrono::duration<double>(std::chrono::steady_clock::now() - tc).count();
         data::clearOperations();
         auto ts = std::chrono::steady_clock::now();
+#ifdef SOGL_GPU_SITES
+        std::vector<double> xs((size_t)N);
+        if (gpu_solve(data::b, xs.data()) != 0) {
+            std::fprintf(stderr, "gpu_solve: %s\n", sogl_last_error());
+            return 3;
+        }
+        data::x = xs.data();
+#else
         BlockPlanner::solve(bl2, bu2, data::b, data::blockRows * data::blockSize);
+#endif
         t_solve = std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
         dump("x.f64", data::x, dim);
         dump("y.f64", data::b, N);

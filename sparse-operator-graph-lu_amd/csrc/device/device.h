@@ -73,7 +73,7 @@ struct Factor {
     int32_t n = 0;                // unknowns of the original problem
     int64_t nnz = 0;
     std::vector<int32_t> new_order;  // new -> old
-    double a_fro = 0;
+    double a_fro = 0, a_inf = 0;  // ||A||_F and ||A||_inf of the original matrix
     DevBuf csr_ptr, csr_col, csr_val, vec_r, vec_d, red, perm_d, vec_io;
     // stats
     double factor_ms = 0, solve_ms = 0, exec_ms = 0;

@@ -140,13 +140,12 @@ __device__ inline unsigned long long spin_ready(const Args& a, int s) {
     }
 }
 
-// Device-scope ordering for a whole workgroup, done by one thread.  A workgroup
-// barrier orders the other waves with that thread (it is a workgroup-scope release
-// and acquire), and the device-scope fences are cumulative, so one L2 write-back
-// (release, after the barrier) or one L1/L2 invalidate (acquire, before the barrier)
-// per task serves all four waves; per-wave fences would repeat them four times.
+// Device-scope acquire for a whole workgroup, done by one thread: one L1/L2
+// invalidate before the barrier serves all four waves (the barrier orders it before
+// their later loads).  The release side needs no fence: tile stores are write-through
+// (gst), each wave waits for its own before the publish barrier, and the flag stores
+// follow the barrier.
 __device__ __forceinline__ void acquire_for_workgroup() { acquire_agent(); }
-__device__ __forceinline__ void release_for_workgroup() { release_agent(); }
 
 // The masks come with the flags.  A window of pairs: wave 0 loads the flags of up to
 // PAIR_WINDOW pairs at once (lane 2j / 2j+1: pair s+j's A / B), polls until pair s is
@@ -293,7 +292,7 @@ __device__ inline uint32_t task_mask(const DevTask& t, const uint32_t* s_mask) {
 }
 __device__ inline void write_masks(const DevTask& t, const Args& a, uint32_t m) {
     for (int32_t o : {t.out0, t.out1, t.out2, t.out3})
-        if (o >= 0) a.mask[o] = m;
+        if (o >= 0) gst(&a.mask[o], m);
 }
 
 // Persistent executor: every workgroup loops taking the next task index.  GEMM
@@ -350,18 +349,17 @@ __global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
             break;
         }
         run_task<T, BF16, false, LLT, MASKS, ONE>(t, a, lds, s_mask, PairWait{&a, &t_wait, s_pm});
-        // publish: the barrier (a workgroup release: the waves' stores go through the
-        // CU's memory pipeline in order, ahead of anything issued after it), then
-        // thread 0 writes the output masks, makes everything visible at device scope
-        // once (the write-back covers all four waves' tiles) and raises the flags,
-        // which carry the masks to the consumers.  A sum that goes to the chained task
-        // is read by nobody else: no release, no flag.
+        // publish: every wave waits until its write-through tile stores have reached the
+        // device coherence point, the barrier orders those waits before thread 0, and
+        // thread 0 raises the flags, which carry the output masks to the consumers.  A
+        // sum that goes to the chained task is read by nobody else: no wait, no flag
+        // (the chained task runs on this CU and reads it after the barrier).
+        if (nxt < 0) wave_stores_done();
         __syncthreads();
         if (a.trace && tid == 0) t_ready = wall_clock64();  // work done, publish starts
         if (tid == 0) {
             const uint32_t m = task_mask<MASKS>(t, s_mask);
             if (nxt < 0) {
-                release_for_workgroup();
                 const unsigned long long fv = ((unsigned long long)(unsigned)a.epoch << 32) | m;
                 __hip_atomic_store(a.ready + t.out0, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (t.out1 >= 0)
@@ -377,10 +375,10 @@ __global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
             write_masks(t, a, m);
             if (a.trace) {
                 long long* tr = a.trace + (size_t)ti * 4;
-                tr[0] = t_taken;
-                tr[1] = t_ready;
-                tr[2] = wall_clock64();
-                tr[3] = t_wait;
+                gst(&tr[0], t_taken);
+                gst(&tr[1], t_ready);
+                gst(&tr[2], (long long)wall_clock64());
+                gst(&tr[3], t_wait);
             }
         }
         // keeps the publishing branch and the next fetch branch apart, so the

@@ -1,7 +1,11 @@
 // Mixed precision: an FP32 (or bf16-GEMM) factor is corrected by FP64
 // iterative refinement against the original matrix,
 //     x_0 = solve(b);  repeat  r = b - A x (FP64 CSR SpMV);  x += solve(r)
-// until ||r||_2 / (||A||_F ||x||_2 + ||b||_2) <= tol.  FP64 lives only in the
+// until the normwise backward error is <= tol in both of its usual forms,
+//     ||r||_inf / (||A||_inf ||x||_inf + ||b||_inf)  and
+//     ||r||_2 / (||A||_F ||x||_2 + ||b||_2).
+// (The second alone is about sqrt(n) looser at large n, since ||A||_F grows with
+// sqrt(n) where ||A||_inf, a row sum, does not.)  FP64 lives only in the
 // residual, the correction accumulation and the solve's vector sums; the
 // factor tiles stay in their low precision.  The reference has no
 // counterpart (it factors in FP64 only); its residual check is the max-abs
@@ -69,30 +73,53 @@ __global__ void k_axpy(double* x, const double* d, int64_t n) {
     if (i < n) x[i] += d[i];
 }
 
-// partial sums of squares of r and x over the first n rows, per column:
-// out[(c*2 + {0,1}) * RED_BLOCKS + block]
+// Per column and block, over the first n rows: the sums of squares of r and x and
+// their largest magnitudes, at out[(c*4 + {0,1,2,3}) * RED_BLOCKS + block].  A NaN
+// reaches the maxima through the sums (fmax drops NaN; the host checks the sums).
+constexpr int RED_VALUES = 4;
 __global__ void k_norms(const double* r, const double* x, int64_t n, int64_t N, double* out) {
-    __shared__ double sr[256], sx[256];
+    __shared__ double sr[256], sx[256], mr[256], mx[256];
     const int c = blockIdx.y;
-    double ar = 0, ax = 0;
+    double ar = 0, ax = 0, br = 0, bx = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const double a = r[i + c * N], b = x[i + c * N];
         ar += a * a;
         ax += b * b;
+        br = fmax(br, fabs(a));
+        bx = fmax(bx, fabs(b));
     }
     sr[threadIdx.x] = ar;
     sx[threadIdx.x] = ax;
+    mr[threadIdx.x] = br;
+    mx[threadIdx.x] = bx;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if (threadIdx.x < o) {
             sr[threadIdx.x] += sr[threadIdx.x + o];
             sx[threadIdx.x] += sx[threadIdx.x + o];
+            mr[threadIdx.x] = fmax(mr[threadIdx.x], mr[threadIdx.x + o]);
+            mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + o]);
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        out[(c * 2 + 0) * RED_BLOCKS + blockIdx.x] = sr[0];
-        out[(c * 2 + 1) * RED_BLOCKS + blockIdx.x] = sx[0];
+        out[(c * RED_VALUES + 0) * RED_BLOCKS + blockIdx.x] = sr[0];
+        out[(c * RED_VALUES + 1) * RED_BLOCKS + blockIdx.x] = sx[0];
+        out[(c * RED_VALUES + 2) * RED_BLOCKS + blockIdx.x] = mr[0];
+        out[(c * RED_VALUES + 3) * RED_BLOCKS + blockIdx.x] = mx[0];
+    }
+}
+
+// Column c's reduced norms from the per-block partials: {||r||_2^2, ||x||_2^2,
+// ||r||_inf, ||x||_inf}.
+void reduce_norms(const std::vector<double>& part, int c, double out[RED_VALUES]) {
+    for (int v = 0; v < RED_VALUES; v++) {
+        double a = 0;
+        for (int q = 0; q < RED_BLOCKS; q++) {
+            const double p = part[(size_t)((c * RED_VALUES + v) * RED_BLOCKS + q)];
+            a = v < 2 ? a + p : std::max(a, p);
+        }
+        out[v] = a;
     }
 }
 
@@ -148,19 +175,25 @@ extern "C" int sogl_refine_attach(sogl_ctx* x, int32_t factor_id, const sogl_pla
             col[(size_t)p] = v.in_bj[t] * TS + (v.cell_pos[q] & 63);
             val[(size_t)p] = v.cell_val[q];
         }
-    // each row by column, and the Frobenius norm over the rows of the original matrix
-    double fro = 0;
+    // each row by column, and the Frobenius and infinity norms over the rows of the
+    // original matrix
+    double fro = 0, inf = 0;
     std::vector<std::pair<int32_t, double>> row;
     for (int64_t i = 0; i < v.N; i++) {
         const int64_t b = ptr[(size_t)i], e = ptr[(size_t)i + 1];
         row.clear();
         for (int64_t p = b; p < e; p++) row.emplace_back(col[(size_t)p], val[(size_t)p]);
         std::sort(row.begin(), row.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+        double rsum = 0;
         for (int64_t p = b; p < e; p++) {
             col[(size_t)p] = row[(size_t)(p - b)].first;
             val[(size_t)p] = row[(size_t)(p - b)].second;
-            if (i < v.n) fro += val[(size_t)p] * val[(size_t)p];
+            if (i < v.n) {
+                fro += val[(size_t)p] * val[(size_t)p];
+                rsum += std::fabs(val[(size_t)p]);
+            }
         }
+        inf = std::max(inf, rsum);
     }
     int rc;
     if ((rc = dev_alloc(c, f->csr_ptr, ptr.size() * 8, "csr ptr")) ||
@@ -178,6 +211,7 @@ extern "C" int sogl_refine_attach(sogl_ctx* x, int32_t factor_id, const sogl_pla
     if ((rc = dev_alloc(c, f->perm_d, (size_t)v.n * 4, "permutation"))) return rc;
     HIP_TRY(c, hipMemcpy(f->perm_d.p, v.new_order, (size_t)v.n * 4, hipMemcpyHostToDevice));
     f->a_fro = std::sqrt(fro);
+    f->a_inf = inf;
     f->has_refine = true;
     return SOGL_OK;
 }
@@ -190,7 +224,7 @@ static int refined_core(Context* c, Factor* f, const double* d_b, double* d_x, i
     if ((rc = ensure_vectors(c, f, nrhs))) return rc;
     if ((rc = dev_alloc(c, f->vec_r, (size_t)N * nrhs * 8, "residual")) ||
         (rc = dev_alloc(c, f->vec_d, (size_t)N * nrhs * 8, "correction")) ||
-        (rc = dev_alloc(c, f->red, (size_t)nrhs * 2 * RED_BLOCKS * 8, "norms")))
+        (rc = dev_alloc(c, f->red, (size_t)nrhs * RED_VALUES * RED_BLOCKS * 8, "norms")))
         return rc;
     double* db = (double*)f->vec_b.p;
     double* dx = (double*)f->vec_x.p;
@@ -201,17 +235,18 @@ static int refined_core(Context* c, Factor* f, const double* d_b, double* d_x, i
     hipLaunchKernelGGL(k_permute_in, dim3(gN), dim3(256), 0, c->stream, d_b, (const int32_t*)f->perm_d.p, db, n, N,
                        nrhs);
     HIP_TRY(c, hipGetLastError());
-    std::vector<double> part((size_t)nrhs * 2 * RED_BLOCKS);
-    // ||b|| over the real rows, once
+    std::vector<double> part((size_t)nrhs * RED_VALUES * RED_BLOCKS);
+    // ||b||_2 and ||b||_inf over the real rows, once
     hipLaunchKernelGGL(k_norms, dim3(RED_BLOCKS, nrhs), dim3(256), 0, c->stream, db, db, (int64_t)n, N,
                        (double*)f->red.p);
     HIP_TRY(c, hipMemcpyAsync(part.data(), f->red.p, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    std::vector<double> bnorm((size_t)nrhs, 0.0);
+    std::vector<double> bnorm((size_t)nrhs, 0.0), bmax((size_t)nrhs, 0.0);
     for (int32_t r = 0; r < nrhs; r++) {
-        double sb = 0;
-        for (int q = 0; q < RED_BLOCKS; q++) sb += part[(size_t)((r * 2 + 0) * RED_BLOCKS + q)];
-        bnorm[(size_t)r] = std::sqrt(sb);
+        double v[RED_VALUES];
+        reduce_norms(part, r, v);
+        bnorm[(size_t)r] = std::sqrt(v[0]);
+        bmax[(size_t)r] = v[2];
     }
     if ((rc = solve_device(c, f, db, dx, (double*)f->vec_y.p, nrhs))) return rc;
     double solve_ms = f->solve_ms;
@@ -227,13 +262,13 @@ static int refined_core(Context* c, Factor* f, const double* d_b, double* d_x, i
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         worst = 0;
         for (int32_t r = 0; r < nrhs; r++) {
-            double sr = 0, sx = 0;
-            for (int q = 0; q < RED_BLOCKS; q++) {
-                sr += part[(size_t)((r * 2 + 0) * RED_BLOCKS + q)];
-                sx += part[(size_t)((r * 2 + 1) * RED_BLOCKS + q)];
-            }
-            const double den = f->a_fro * std::sqrt(sx) + bnorm[(size_t)r];
-            const double rel = den > 0 ? std::sqrt(sr) / den : std::sqrt(sr);
+            double v[RED_VALUES];
+            reduce_norms(part, r, v);
+            const double den2 = f->a_fro * std::sqrt(v[1]) + bnorm[(size_t)r];
+            const double rel2 = den2 > 0 ? std::sqrt(v[0]) / den2 : std::sqrt(v[0]);
+            const double deni = f->a_inf * v[3] + bmax[(size_t)r];
+            const double reli = deni > 0 ? v[2] / deni : v[2];
+            const double rel = std::isnan(rel2) ? rel2 : std::max(rel2, reli);
             if (!(rel <= worst)) worst = rel;  // NaN propagates
         }
         if (worst <= tol || it >= max_iter || std::isnan(worst)) break;

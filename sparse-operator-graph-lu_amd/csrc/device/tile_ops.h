@@ -69,6 +69,22 @@ __device__ __forceinline__ void release_agent() { asm volatile("buffer_wbl2 sc1\
 // 1 us to every hand-off and 3% to the C5 factor.
 __device__ __forceinline__ void acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
+// Every tile store of the factor tasks is a relaxed agent-scope atomic store, which
+// gfx942 / gfx950 code emit as `global_store ... sc1`: written through the L2 to the
+// device coherence point, never left dirty in it.  Once a wave's s_waitcnt vmcnt(0)
+// has seen them complete they are visible to any CU that invalidates its caches
+// (the acquire above), so publishing a tile needs no L2 write-back (buffer_wbl2, which
+// writes back every dirty line of the XCD's L2 and was most of a task's publish).
+template <typename T>
+__device__ __forceinline__ void gst(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// The wave's own stores (and loads) have completed: with gst stores, the wave's part
+// of a workgroup release at agent scope (a barrier after it orders the flag store).
+// The comment marks the wait in the assembly, where a test checks that the publish
+// barrier follows it with no store in between.
+__device__ __forceinline__ void wave_stores_done() { asm volatile("s_waitcnt vmcnt(0) ; publish: stores done" ::: "memory"); }
+
 // ------------------------------------------------------------------ loads
 // thread t moves row t/4, columns (t%4)*16 .. +15 of a tile between global and LDS
 // (that segment lies in 16x16 block (t/64, t%4))
@@ -411,7 +427,7 @@ __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, c
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const T v = acc[m][n][r];
-                C[(wr + m * 16 + MfmaOp<T>::row(lk, r)) * TS + wc + n * 16 + li] = v;
+                gst(&C[(wr + m * 16 + MfmaOp<T>::row(lk, r)) * TS + wc + n * 16 + li], v);
                 if constexpr (MASKS) nz[m][n] |= fabs(v) > T(MASK_TINY);
             }
     if constexpr (MASKS) {
@@ -488,7 +504,7 @@ __device__ __attribute__((always_inline)) void gemm_task_bf16(float* __restrict_
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const float v = acc[m][n][r];
-                C[(wr + m * 16 + 4 * lk + r) * TS + wc + n * 16 + li] = v;
+                gst(&C[(wr + m * 16 + 4 * lk + r) * TS + wc + n * 16 + li], v);
                 if constexpr (MASKS) nz[m][n] |= fabsf(v) > float(MASK_TINY);
             }
     if constexpr (MASKS) {
@@ -540,7 +556,7 @@ __device__ __attribute__((always_inline)) void sub_task(T* __restrict__ pool, in
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         const T v = a[q] - b[q];
-        R[q * NT + tid] = v;
+        gst(&R[q * NT + tid], v);
         if constexpr (MASKS) {
             const unsigned long long bb = __ballot(fabs(v) > T(MASK_TINY));
 #pragma unroll
@@ -577,7 +593,7 @@ template <typename T, typename C>
 __device__ inline void store_rowseg(T* g, const C* v, int lane, int w) {
     T* p = g + lane * TS + 16 * w;
 #pragma unroll
-    for (int c = 0; c < 16; c++) p[c] = T(v[c]);
+    for (int c = 0; c < 16; c++) gst(&p[c], T(v[c]));
 }
 
 // Row k of the wave's 16 columns, from lane k to every lane of the wave through
@@ -762,7 +778,7 @@ __device__ __forceinline__ void st16_global(const typename MfmaOp<C>::acc_t& acc
                                             int li, int lk, C sg = C(1)) {
 #pragma unroll
     for (int r = 0; r < 4; r++)
-        tile[(16 * I + MfmaOp<C>::row(lk, r)) * TS + 16 * J + li] = T(sg * acc[r]);
+        gst(&tile[(16 * I + MfmaOp<C>::row(lk, r)) * TS + 16 * J + li], T(sg * acc[r]));
 }
 
 // 16x16 block of LDS (stride ld) to global block (I, J) of a tile; 4 elements per lane
@@ -771,7 +787,7 @@ __device__ __forceinline__ void blk_to_global(const C* pb, int ld, T* tile, int 
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int r = 4 * q + (lane >> 4), c = lane & 15;
-        tile[(16 * I + r) * TS + 16 * J + c] = T(pb[r * ld + c]);
+        gst(&tile[(16 * I + r) * TS + 16 * J + c], T(pb[r * ld + c]));
     }
 }
 
@@ -782,7 +798,7 @@ __device__ __forceinline__ void st16_global_rs(const typename MfmaOp<C>::acc_t& 
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         const int row = 16 * I + MfmaOp<C>::row(lk, r);
-        tile[row * TS + 16 * J + li] = T(sg * acc[r] * rs[row]);
+        gst(&tile[row * TS + 16 * J + li], T(sg * acc[r] * rs[row]));
     }
 }
 template <typename T, typename C>
@@ -790,14 +806,14 @@ __device__ __forceinline__ void blk_to_global_rs(const C* pb, int ld, T* tile, i
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int r = 4 * q + (lane >> 4), c = lane & 15;
-        tile[(16 * I + r) * TS + 16 * J + c] = T(pb[r * ld + c] * rs[16 * I + r]);
+        gst(&tile[(16 * I + r) * TS + 16 * J + c], T(pb[r * ld + c] * rs[16 * I + r]));
     }
 }
 
 template <typename T>
 __device__ __forceinline__ void zero_blk_global(T* tile, int I, int J, int lane) {
 #pragma unroll
-    for (int q = 0; q < 4; q++) tile[(16 * I + 4 * q + (lane >> 4)) * TS + 16 * J + (lane & 15)] = T(0);
+    for (int q = 0; q < 4; q++) gst(&tile[(16 * I + 4 * q + (lane >> 4)) * TS + 16 * J + (lane & 15)], T(0));
 }
 
 // One wave factors the 16x16 diagonal block at (r0, r0) of the LDS tile a in place
@@ -926,7 +942,7 @@ __device__ __forceinline__ void l_blk_to_global(const C* a, T* Lg, int I, int J,
         const C x = a[gr * LDA + gc];
         C v = (I > J || r > c) ? x : (r == c ? C(1) : C(0));
         if constexpr (CHOL) v = (I == J && r == c) ? dsc[gc] : v * dsc[gc];
-        Lg[gr * TS + gc] = T(v);
+        gst(&Lg[gr * TS + gc], T(v));
     }
 }
 template <typename T, typename C>
@@ -937,7 +953,7 @@ __device__ __forceinline__ void u_blk_to_global(const C* a, T* Ug, int I, int J,
         const int gr = 16 * I + r, gc = 16 * J + c;
         const C x = a[gr * LDA + gc];
         const C v = (I < J || c >= r) ? x : C(0);
-        Ug[gr * TS + gc] = T(v);
+        gst(&Ug[gr * TS + gc], T(v));
         um = fabs(v) > um ? fabs(v) : um;
     }
 }
@@ -1281,8 +1297,8 @@ __device__ inline void lu_inv_fp64_lds(T* __restrict__ pool, int src, int outl, 
     for (int q = 0; q < 16; q++) {
         const int j = j0 + q;
         const double v = a[i * LDA + j];
-        L[i * TS + j] = i > j ? T(v) : (i == j ? T(1) : T(0));
-        U[i * TS + j] = j >= i ? T(v) : T(0);
+        gst(&L[i * TS + j], i > j ? T(v) : (i == j ? T(1) : T(0)));
+        gst(&U[i * TS + j], j >= i ? T(v) : T(0));
     }
     double y[16];
     double* bro = lcol + TS + 16 * w;
@@ -1365,7 +1381,7 @@ __device__ __attribute__((always_inline)) void trans_task(T* __restrict__ pool, 
 #pragma unroll
     for (int q = 0; q < 16; q++) {
         const T v = s[(c0 + q) * LDA + r];
-        R[r * TS + c0 + q] = v;
+        gst(&R[r * TS + c0 + q], v);
         any |= fabs(v) > T(MASK_TINY);
     }
     if constexpr (!MASKS) return;

@@ -836,8 +836,10 @@ bool rebind_plan(const Plan& like, const Coo& a, const double* b, Plan& out) {
     auto t0 = clk::now();
     const int32_t n = a.n;
     const size_t nnz = a.val.size();
-    if (a.symmetric || n != like.cfg.mSize || like.blockSorted || (int64_t)nnz != like.nnzOrig ||
-        a.row.size() != nnz || a.col.size() != nnz)
+    // an L L^T plan factors one triangle: never the plan of a general matrix, whatever
+    // its pattern
+    if (a.symmetric || like.symmetric || n != like.cfg.mSize || like.blockSorted ||
+        (int64_t)nnz != like.nnzOrig || a.row.size() != nnz || a.col.size() != nnz)
         return false;
     // the same pattern: every entry lands where the earlier plan put it
     const int32_t* rev = like.reverseOrder.data();

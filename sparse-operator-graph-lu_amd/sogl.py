@@ -491,3 +491,13 @@ def oracle_residual(n, row, col, val, x, b) -> float:
         return max(oracle_residual(n, row, col, val, x[:, j], b[:, j]) for j in range(x.shape[1]))
     r = b - np.bincount(row, weights=val * x[col], minlength=n)
     return float(np.linalg.norm(r) / (np.linalg.norm(val) * np.linalg.norm(x) + np.linalg.norm(b)))
+
+
+def backward_error_inf(n, row, col, val, x, b) -> float:
+    """||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf), FP64 (||A||_inf: the largest
+    row sum of |a_ij|); the largest over columns when x and b are (n, k)."""
+    if np.ndim(x) == 2:
+        return max(backward_error_inf(n, row, col, val, x[:, j], b[:, j]) for j in range(x.shape[1]))
+    r = b - np.bincount(row, weights=val * x[col], minlength=n)
+    a_inf = np.bincount(row, weights=np.abs(val), minlength=n).max()
+    return float(np.abs(r).max() / (a_inf * np.abs(x).max() + np.abs(b).max()))

@@ -11,6 +11,10 @@ dumps into tests/golden/*.npz (allow_pickle=False) -- data only:
   large cases   SHA-256 of the canonical int32 little-endian dumps, counts,
                 and a strided sample of the reference x
 
+For the drop-in CLI it also runs the reference's own ./solve (oracle/_ref/solve) on
+the CLI_CASES and keeps its stdout and the _x.mtx it writes as text files
+(cli_<case>.stdout, cli_<case>_x.mtx.golden).
+
 Usage: python tests/golden/make_golden.py [case ...]
 """
 from __future__ import annotations
@@ -108,11 +112,34 @@ def run_case(name: str) -> None:
           f"({os.path.getsize(path) // 1024} KiB)")
 
 
+REF_CLI = os.path.join(ROOT, "oracle", "_ref", "solve")
+CLI_CASES = ("c1", "sym_poisson2d_64")
+
+
+def run_cli(name: str) -> None:
+    """The reference CLI on one case: its stdout and its _x.mtx, verbatim."""
+    prob = CASES[name][0]()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, f"{name}.mtx")
+        W.write_mtx(prob, path)
+        r = subprocess.run([REF_CLI, path], capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS="4"))
+        if r.returncode != 0:
+            raise RuntimeError(f"{name}: reference CLI failed: {r.stderr}")
+        with open(os.path.join(OUT, f"cli_{name}.stdout"), "w") as fh:
+            fh.write(r.stdout)
+        with open(os.path.join(d, f"{name}_x.mtx"), "rb") as src, open(os.path.join(OUT, f"cli_{name}_x.mtx.golden"), "wb") as dst:
+            dst.write(src.read())
+    print(f"cli {name}: stdout {len(r.stdout.splitlines())} lines")
+
+
 def main(argv):
     if not os.path.exists(REF_DUMP):
         raise SystemExit("oracle/_ref/ref_dump is not built: make -C oracle ref")
     for name in (argv or list(CASES)):
         run_case(name)
+    for name in CLI_CASES:
+        if not argv or name in argv:
+            run_cli(name)
 
 
 if __name__ == "__main__":

@@ -13,9 +13,18 @@ the window as a violation (the branch into it could come from anywhere).
 A release writes the L2 back (buffer_wbl2) and must wait for that write-back
 (s_waitcnt vmcnt(0)) before the flag store that follows it; the compiler's own
 release omitted that wait where no store of the wave was pending, so the flag
-could be seen before the tiles.  The second check compiles the factor and solve
-kernels and requires such a wait after every buffer_wbl2, before the next vector
-store or atomic and before any label.
+could be seen before the tiles.  The second check compiles the solve kernels and
+requires such a wait after every buffer_wbl2, before the next vector store or
+atomic and before any label.
+
+The persistent factor kernels publish without a write-back: every tile store is
+write-through (global_store ... sc1), each wave waits for its own stores
+(s_waitcnt vmcnt(0), marked "publish: stores done") before the barrier that comes
+ahead of the flag stores, and a consumer invalidates its caches (buffer_inv sc1)
+after its flag loads have returned.  The third check reads those rules off the
+compiled persistent kernels, and that no kernel is built for TgSplit mode (where a
+workgroup's waves may sit on different CUs and the barrier would not order their
+stores through one CU's memory pipeline).
 """
 import os
 import re
@@ -28,6 +37,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "sparse-operator-graph-lu_amd")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 REG = re.compile(r"\bv(\d+)\b|\bv\[(\d+):(\d+)\]")
+PUBLISH = "publish: stores done"  # the comment on the factor's per-wave publish wait
 
 
 def vregs(text):
@@ -40,11 +50,11 @@ def vregs(text):
     return out
 
 
-def instructions(asm):
+def instructions(asm, keep_comments=False):
     out = []
     for line in asm.splitlines():
-        s = line.split(";")[0].strip()
-        if not s:
+        s = line.strip() if keep_comments and PUBLISH in line else line.split(";")[0].strip()
+        if not s or s.startswith(";"):
             continue
         if s.endswith(":"):  # block labels (.LBB...) start with a dot too
             out.append(("label", s))
@@ -156,11 +166,83 @@ def factor_asm(tmp_path_factory):
 
 
 @pytest.mark.skipif(not (os.path.exists(HIPCC) or shutil.which("hipcc")), reason="hipcc not installed")
-def test_releases_wait_for_the_write_back(factor_asm, tmp_path):
-    for name, asm in (("factor", factor_asm), ("solve", _compile(tmp_path, "solve"))):
-        n, bad = release_violations(asm)
-        assert n > 0, name
+def test_releases_wait_for_the_write_back(tmp_path):
+    asm = _compile(tmp_path, "solve")
+    n, bad = release_violations(asm)
+    assert n > 0
+    assert not bad, bad[:5]
+
+
+def kernel_bodies(asm, pattern):
+    """{symbol: instruction list} of the functions whose symbol contains pattern."""
+    lines = asm.split("\n")
+    starts = [i for i, ln in enumerate(lines) if re.match(r"^_Z\S+:(\s|$)", ln)]
+    out = {}
+    for k, st in enumerate(starts):
+        name = lines[st].split(":")[0]
+        if pattern in name:
+            end = starts[k + 1] if k + 1 < len(starts) else len(lines)
+            out[name] = instructions("\n".join(lines[st + 1:end]), keep_comments=True)
+    return out
+
+
+def publish_violations(insts):
+    """Rules of the write-through publish on one kernel's instructions; returns
+    (number of publish waits, list of violations)."""
+    bad, n = [], 0
+    for idx, (kind, s) in enumerate(insts):
+        if kind != "inst":
+            continue
+        if s.startswith("buffer_wbl2"):
+            bad.append((s, "an L2 write-back in a write-through kernel"))
+        if re.match(r"(global|buffer|flat)_store", s) and not re.search(r"\bsc1\b", s):
+            bad.append((s, "a store that is not write-through"))
+        if s.startswith("buffer_inv"):
+            prev = [t for k2, t in insts[max(0, idx - 3):idx] if k2 == "inst"]
+            if not any(t.startswith("s_waitcnt") and "vmcnt(0)" in t for t in prev):
+                bad.append((s, "invalidate before the flag loads have returned"))
+        if PUBLISH in s:
+            n += 1
+            if not s.startswith("s_waitcnt") or "vmcnt(0)" not in s:
+                bad.append((s, "publish wait is not a full vmcnt wait"))
+            for k2, t in insts[idx + 1:]:
+                if k2 == "inst" and t.startswith("s_barrier"):
+                    break
+                if k2 == "inst" and re.match(r"(global|buffer|flat)_(store|atomic)", t):
+                    bad.append((s, "store between the publish wait and its barrier: " + t))
+                    break
+            else:
+                bad.append((s, "no barrier after the publish wait"))
+    return n, bad
+
+
+def test_publish_checker_finds_planted_mistakes():
+    insts = instructions("\n".join([
+        "\tglobal_store_dword v1, v2, s[0:1] sc1",
+        "\ts_waitcnt vmcnt(0) ; publish: stores done", "\ts_barrier",             # fine
+        "\tglobal_store_dword v1, v2, s[0:1]",                                    # not write-through
+        "\ts_waitcnt vmcnt(0) ; publish: stores done",
+        "\tglobal_store_dword v1, v2, s[0:1] sc1", "\ts_barrier",               # store after the wait
+        "\tglobal_load_dword v3, v1, s[0:1] sc1", "\tbuffer_inv sc1",           # no wait for the load
+        "\ts_waitcnt vmcnt(0) lgkmcnt(0)", "\tbuffer_inv sc1",                  # fine
+        "\tbuffer_wbl2 sc1",                                                     # write-back
+    ]), keep_comments=True)
+    n, bad = publish_violations(insts)
+    assert n == 2
+    assert [b[1][:12] for b in bad] == ["a store that", "store betwee", "invalidate b", "an L2 write-"]
+
+
+def test_factor_publish_is_write_through(factor_asm):
+    bodies = kernel_bodies(factor_asm, "k_factor_persistent")
+    assert len(bodies) >= 12
+    for name, insts in bodies.items():
+        n, bad = publish_violations(insts)
+        assert n >= 1, name
+        assert sum(1 for k, s in insts if k == "inst" and s.startswith("buffer_inv")) >= 2, name
         assert not bad, (name, bad[:5])
+    # no kernel of the library is built for TgSplit mode
+    modes = re.findall(r"\.amdhsa_tg_split\s+(\d+)", factor_asm)
+    assert modes and set(modes) == {"0"}
 
 
 @pytest.mark.skipif(not (os.path.exists(HIPCC) or shutil.which("hipcc")), reason="hipcc not installed")

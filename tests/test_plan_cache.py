@@ -127,3 +127,35 @@ def test_c2_pattern_reuse_matches():
     assert got.reused
     assert got.info.t_plan < 0.5 * (ref.info.t_plan + ref.info.t_reorder)
     assert_same(arrays(got, p2.b), arrays(ref, p2.b))
+
+
+def test_general_matrix_never_reuses_a_symmetric_plan():
+    """A general matrix whose entry sequence is exactly the mirrored COO of an L L^T
+    plan (each off-diagonal entry followed by its mirror, solver.cpp:136-149) has the
+    same pattern as that plan's internal matrix, but the L L^T factor ignores the
+    upper triangle: create_like must plan it from scratch as a general (LU) plan."""
+    import ctypes as C
+    lower = W.symmetric_lower(W.diffusion_varcoef(20, 2, seed=31))
+    like = sogl.Plan(lower.n, lower.row, lower.col, lower.val, lower.b, lib=LIB, symmetric=True)
+    assert like.info.symmetric == 1
+    rows, cols, vals = [], [], []
+    for r, c, v in zip(lower.row, lower.col, lower.val):
+        rows.append(r), cols.append(c), vals.append(v)
+        if r != c:  # the mirror, with a different value: the matrix is not symmetric
+            rows.append(c), cols.append(r), vals.append(0.5 * v)
+    row = np.array(rows, np.int32)
+    col = np.array(cols, np.int32)
+    val = np.array(vals)
+    lib = sogl.load(LIB)
+    h = C.c_void_p()
+    reused = C.c_int32(-1)
+    P = C.POINTER
+    rc = lib.sogl_plan_create_like(like.h, lower.n, val.size, row.ctypes.data_as(P(C.c_int32)),
+                                   col.ctypes.data_as(P(C.c_int32)), val.ctypes.data_as(P(C.c_double)),
+                                   lower.b.ctypes.data_as(P(C.c_double)), C.byref(h), C.byref(reused))
+    assert rc == 0
+    info = sogl.PlanInfo()
+    lib.sogl_plan_get_info(h, C.byref(info))
+    lib.sogl_plan_destroy(h)
+    assert reused.value == 0
+    assert info.symmetric == 0 and info.n_llt == 0 and info.n_lu > 0
