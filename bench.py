@@ -167,10 +167,12 @@ def cpu_description() -> dict:
             "usable_physical_cores": physical_cores()}
 
 
-def run_path(dev, plan, prob, precision, tol, max_iter, steps, warmup, mode, wg_per_cu=0, sweep=()):
+def run_path(dev, plan, prob, precision, tol, max_iter, steps, warmup, mode, wg_per_cu=0, sweep=(), chain=True):
     """Timed factor + refined-solve steps of one precision.  sweep: numbers of refinement
-    corrections after which x is also kept (untimed, after the timed steps)."""
+    corrections after which x is also kept (untimed, after the timed steps).  chain:
+    dynamic chaining of the persistent executor."""
     fid = dev.load(plan, precision)
+    dev.set_chaining(fid, chain)
     dev.attach_refine(fid, plan)
     n = prob.n
     b = np.asfortranarray(prob.b.reshape(n, -1))
@@ -213,6 +215,7 @@ def run_path(dev, plan, prob, precision, tol, max_iter, steps, warmup, mode, wg_
                 solve_s=float(np.mean(sol)) / 1e3, ir_iters=int(max(its)), residual=float(max(res)),
                 pool_gb=st.pool_bytes / 1e9, n_tasks=int(st.n_tasks), n_levels=int(st.n_levels), executor=tuned,
                 growth_fp64_tiles=int(st.growth_tiles), zero_block_share=float(st.zero_blocks),
+                claimed_tasks=int(st.claimed_tasks), tasks_run=int(st.tasks_run),
                 block_masks=bool(st.block_masks), x=x[:, 0] if nrhs == 1 else x, swept=swept)
 
 
@@ -270,6 +273,10 @@ def main():
                          "(tuned before the warmup steps, outside the timed region); "
                          "persistent-1wg: the persistent executor with one workgroup per CU")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-chain", action="store_true",
+                    help="persistent executor without dynamic chaining (for comparison)")
+    ap.add_argument("--chain-claims", type=int, default=0,
+                    help="dynamic chaining: tasks one finished task may claim for its workgroup")
     ap.add_argument("--bf16", action="store_true",
                     help="also run the BF16 experiment (bf16 MFMA operands, FP32 tiles, FP64 refinement); "
                          "on by default for c5")
@@ -306,11 +313,13 @@ def main():
     dev = sogl.Device(0)
     paths = {}
     sweep = () if args.no_ir_sweep or nrhs != 1 else tuple(range(0, 7))
-    paths["fp32_ir"] = run_path(dev, plan, prob, sogl.FP32, tol, 30, args.steps, args.warmup, *mode, sweep=sweep)
-    paths["fp64"] = run_path(dev, plan, prob, sogl.FP64, tol, 0, args.steps, args.warmup, *mode)
+    ch = 0 if args.no_chain else args.chain_claims
+    paths["fp32_ir"] = run_path(dev, plan, prob, sogl.FP32, tol, 30, args.steps, args.warmup, *mode, sweep=sweep,
+                                chain=ch)
+    paths["fp64"] = run_path(dev, plan, prob, sogl.FP64, tol, 0, args.steps, args.warmup, *mode, chain=ch)
     if args.bf16 or (args.config == "c5" and not args.no_bf16):
         paths["bf16_ir"] = run_path(dev, plan, prob, sogl.BF16, tol, 50, min(args.steps, 3), min(args.warmup, 1),
-                                    *mode)
+                                    *mode, chain=ch)
     executors = None
     if args.compare_executors or args.config == "c3":
         executors = compare_executors(dev, plan, per_op=True)

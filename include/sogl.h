@@ -254,9 +254,18 @@ typedef struct sogl_factor_stats {
     double max_growth;      /* largest pivot-free growth factor seen */
     double zero_blocks;     /* share of the factor tiles' 16x16 blocks that are zero (last masked run) */
     int32_t block_masks;    /* 1 when the next run uses block masks */
-    int32_t reserved;
+    int32_t claimed_tasks;  /* last persistent run: tasks run by the workgroup that made an operand */
+    int32_t dropped_tasks;  /* last persistent run: queued tasks given up because another claimed them */
+    int32_t tasks_run;      /* last run: tasks executed (n_tasks when each ran exactly once) */
 } sogl_factor_stats;
 int sogl_factor_get_stats(sogl_ctx* ctx, int32_t factor_id, sogl_factor_stats* st);
+/* Dynamic chaining of the persistent executor: a workgroup that has made a tile runs a
+ * near-critical GEMM task that reads it right away, when that task's other operands are
+ * already published, instead of handing it over through memory.  claims = the most such
+ * tasks one finished task hands to its own workgroup (0: off, the default; at most 8).  No
+ * reference counterpart (BlockPlanner::calculate runs stage by stage); the results are
+ * bit-identical either way. */
+int sogl_factor_set_chaining(sogl_ctx* ctx, int32_t factor_id, int32_t claims);
 /* Copy factor tiles back (FP64 view).  which 0 = L, 1 = U, in the order given at load. */
 int sogl_factor_get_tiles(sogl_ctx* ctx, int32_t factor_id, int32_t which, double* tiles);
 
@@ -265,7 +274,9 @@ int sogl_factor_get_tiles(sogl_ctx* ctx, int32_t factor_id, int32_t which, doubl
  * count, dependency level, dependency count; times4[4t..]: wall-clock ticks (100 MHz)
  * when the task was taken, when its work was done (before publishing), when its
  * outputs were published, and when its last dependency wait ended (GEMM tasks wait
- * pair by pair).  Arrays hold n_tasks * 4 entries. */
+ * pair by pair); bits 60-61 of that last entry say how the task was started (0 taken
+ * from the queue, 1 run after the GEMM that made its only source, 2 claimed by the
+ * workgroup that made one of its operands).  Arrays hold n_tasks * 4 entries. */
 int sogl_factor_set_trace(sogl_ctx* ctx, int32_t factor_id, int32_t enable);
 int sogl_factor_get_trace(sogl_ctx* ctx, int32_t factor_id, int64_t* tasks4, int64_t* times4);
 /* Diagnostics: the executor's task graph.  n_deps / n_slots receive the sizes; with

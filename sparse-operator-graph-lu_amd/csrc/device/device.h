@@ -31,7 +31,19 @@ struct DevTask {
     int32_t add_neg = 0;  // GEMM: the added tile is subtracted instead
     int32_t next = -1;    // persistent executor: the task the same workgroup runs right after
                           // this one (the lu / llt that alone reads this GEMM's sum), or -1
-    int32_t pad = 0;
+    // Dynamic chaining (persistent executor).  ctag: byte l (pair operand lane l = 2j or
+    // 2j+1 of pair j) holds 1 + this task's index in the candidate list of the task that
+    // makes that operand, 0 if none; chk_begin / chk_count: this task's own candidates,
+    // as check entries (ChainChk), one per candidate operand.
+    int32_t ctag = 0;
+    int32_t chk_begin = 0, chk_count = 0;
+};
+
+// One check entry of a task's chain candidates: candidate `task` (index `cand` in the
+// list) reads `slot` at operand position `pos` (0-3: pair lanes, 4: the added tile);
+// slot -1: an output of the checking task itself.
+struct ChainChk {
+    int32_t slot, task, cand, pos;
 };
 
 struct DevBuf {
@@ -56,7 +68,10 @@ struct Factor {
     int64_t n_folded = 0;       // copies, negations and subtractions folded into other tasks
     int64_t n_chained = 0;      // lu / llt tasks run by the workgroup that made their source
     // device arrays
-    DevBuf pool, tasks_d, pairs_d, deps_d, ready_d, ctrl_d, op_tasks_d, op_pairs_d, inputs_d, queue_d;
+    DevBuf pool, tasks_d, pairs_d, deps_d, ready_d, ctrl_d, op_tasks_d, op_pairs_d, inputs_d, queue_d, chk_d;
+    std::vector<ChainChk> chk;             // chain candidates' check entries (see DevTask)
+    int64_t n_chain_cands = 0;             // tasks that are some task's chain candidate
+    int32_t chain_claims = 0;              // persistent executor: claims per finished task (0: off)
     std::vector<int32_t> queue;            // persistent executor: task indices handed out, in order
     DevBuf mask_d;                         // per slot: 16x16 block mask (uint32)
     int32_t epoch = 0;
@@ -78,6 +93,9 @@ struct Factor {
     // stats
     double factor_ms = 0, solve_ms = 0, exec_ms = 0;
     int32_t growth_tiles = 0;
+    int32_t claimed = 0;  // last persistent run: tasks run by the workgroup that claimed them
+    int32_t dropped = 0;  // last persistent run: queued tasks dropped because claimed
+    int32_t ran = 0;      // last run: tasks run (the task count when each ran once)
     double max_growth = 0;
     double growth_limit = 1e4;
     bool trace_on = false;

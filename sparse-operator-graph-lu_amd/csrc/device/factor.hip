@@ -84,7 +84,10 @@ Factor* find_factor(Context* c, int32_t id) {
 
 // ------------------------------------------------------------------ control block
 // ctrl[0] task counter, ctrl[1] abort flag, ctrl[2] low-precision tiles refactored in
-// FP64, ctrl[3] max growth (float bits, non-negative so int order = float order)
+// FP64, ctrl[3] max growth (float bits, non-negative so int order = float order),
+// ctrl[4] tasks run by a workgroup that claimed them (dynamic chaining), ctrl[5] tasks a
+// workgroup took from the queue and dropped because another had claimed them, ctrl[6]
+// tasks run (each task exactly once per run)
 constexpr int CTRL_WORDS = 8;
 
 __global__ void k_ctrl_reset(int* ctrl) {
@@ -116,22 +119,30 @@ struct Args {
     double growth_limit;
     long long* trace;     // optional: per task {taken, work done, published, last wait ended}
     uint32_t* mask;       // per slot: 16x16 block mask of the tile (see tile_ops.h)
+    const ChainChk* chk;  // persistent executor: chain candidates' check entries
+    int32_t chain;        // claims per finished task at most (0: no dynamic chaining)
 };
 
 // Waits (thread 0 polls, relaxed) until the epoch in ready[s] reaches this run's and
 // returns the flag (its low word is the tile's block mask); on abort or timeout it
 // raises ctrl[1] and returns 0.  Relaxed polling keeps the spin loop free of cache
-// invalidations; the caller issues one acquire fence after the barrier.
+// invalidations, and the tiles are read with coherent loads afterwards (no acquire).
+// The abort word and the wall clock are looked at every SPIN_CHECK polls only: each poll
+// is a round trip to the device coherence point (~0.3 us measured between two
+// workgroups), and a second one per poll would double the delay before a raised flag is
+// seen.
+constexpr unsigned SPIN_CHECK = 64;
 __device__ inline unsigned long long spin_ready(const Args& a, int s) {
     unsigned long long* flag = a.ready + s;
     const unsigned long long want = (unsigned long long)(unsigned)a.epoch << 32;
     unsigned long long v = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (v >= want) return v;
     const long long t0 = wall_clock64();
-    for (;;) {
+    for (unsigned spins = 1;; spins++) {
         __builtin_amdgcn_s_sleep(1);
         v = __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (v >= want) return v;
+        if (spins % SPIN_CHECK) continue;
         if (__hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return 0;
         if (wall_clock64() - t0 > SPIN_LIMIT_TICKS) {
             __hip_atomic_store(a.ctrl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -140,18 +151,22 @@ __device__ inline unsigned long long spin_ready(const Args& a, int s) {
     }
 }
 
-// Device-scope acquire for a whole workgroup, done by one thread: one L1/L2
-// invalidate before the barrier serves all four waves (the barrier orders it before
-// their later loads).  The release side needs no fence: tile stores are write-through
-// (gst), each wave waits for its own before the publish barrier, and the flag stores
-// follow the barrier.
-__device__ __forceinline__ void acquire_for_workgroup() { acquire_agent(); }
+// No fence on either side of a hand-off.  Release: tile stores are write-through (gst),
+// each wave waits for its own before the publish barrier, and the flag stores follow
+// that barrier.  Acquire: within one run every pool slot is written once, and no CU
+// reads a slot before its flag is up (the loads are issued after the flag was seen),
+// so no cache can hold a copy of a tile made in this run that predates it; the only
+// stale lines a plain load could hit are left from earlier runs, and each workgroup
+// invalidates its CU's L1 and its XCD's L2 once, when it starts, before any tile load.
+// (A per-task invalidate measured ~0.45 us median, 2.3 us at p90, on every hand-off;
+// agent-coherent loads instead cost more than they saved: they are at most 8 bytes
+// wide and bypass the L2.)
 
 // The masks come with the flags.  A window of pairs: wave 0 loads the flags of up to
 // PAIR_WINDOW pairs at once (lane 2j / 2j+1: pair s+j's A / B), polls until pair s is
 // ready, counts the leading pairs whose flags are both raised and leaves their masks in
 // an LDS ring of PAIR_RING pairs (pair j at 2 (j mod PAIR_RING)) and the count in one of
-// two words by the window's parity; one acquire serves the whole window.  Zero products
+// two words by the window's parity.  Zero products
 // are skipped without a barrier, so a wave may still be reading window k's masks and
 // count while wave 0 writes window k + 1's; it cannot be further behind (window k + 1
 // starts after the barrier that ends the wait for window k), and with windows of at
@@ -164,13 +179,36 @@ constexpr int PAIR_RING = 2 * PAIR_WINDOW;
 constexpr int PM_LEN = 2 * PAIR_RING;          // two window counts
 constexpr int PM_SINGLE = PM_LEN + 2;          // two single-wait slots of two words
 constexpr int PM_WORDS = PM_SINGLE + 4;
+// A task claimed by the workgroup that made one of its operands (dynamic chaining) has
+// every operand published already: `pre` holds the operands' masks (pair lanes 0-3, the
+// added tile at 4), taken from the flags the claimer read, so nothing is polled.  A
+// queued task that is some task's chain candidate (`ctag` != 0) looks for its claim bit
+// in its operands' flags and, when another workgroup has claimed it, sets *s_lost and
+// gives up the task (extend returns -1; no operand is read after that point and
+// nothing has been stored).
 struct PairWait {
     const Args* a;
     long long* t_wait;  // tracing: wall clock when the latest wait ended (thread 0)
     uint32_t* s_pm;     // LDS, PM_WORDS words
+    const uint32_t* pre = nullptr;
+    int32_t ctag = 0;
+    int* s_lost = nullptr;
     __device__ inline int extend(const Pair* pairs, int s, int npairs, int window) const {
         uint32_t* ring = s_pm;
         int* s_len = reinterpret_cast<int*>(s_pm + PM_LEN + (window & 1));
+        if (pre) {  // a claimed task (at most CHAIN_MAX_PAIRS pairs: one window)
+            if (threadIdx.x < 64) {
+                const int lane = threadIdx.x;
+                const int cnt = min(npairs - s, PAIR_WINDOW);
+                if (lane < 2 * cnt) ring[2 * ((s + (lane >> 1)) % PAIR_RING) + (lane & 1)] = pre[2 * s + lane];
+                if (lane == 0) {
+                    *s_len = cnt;
+                    if (a->trace) *t_wait = wall_clock64();
+                }
+            }
+            __syncthreads();
+            return s + __builtin_amdgcn_readfirstlane(*s_len);
+        }
         if (threadIdx.x < 64) {  // wave 0
             const int lane = threadIdx.x;
             const int cnt = min(npairs - s, PAIR_WINDOW);
@@ -189,14 +227,22 @@ struct PairWait {
             unsigned long long v = 0;
             int len = poll(slot >= 0, v);
             if (len == 0) {
-                // spin on pair s alone (two flags, not the window's), then read the window
+                // spin until pair s is ready, then read the window.  A small window (at
+                // most SMALL_WINDOW pairs, the usual case on the critical path) is polled
+                // whole each time, so the leading ready pairs are known without a second
+                // round trip; a large one polls pair s alone (two flags, not the window's)
+                // and then reads the window once.
+                constexpr int SMALL_WINDOW = 4;
+                const bool small = cnt <= SMALL_WINDOW;
                 const long long t0 = wall_clock64();
-                for (;;) {
+                for (unsigned spins = 1;; spins++) {
                     __builtin_amdgcn_s_sleep(1);
-                    if (poll(lane < 2, v) > 0) {
-                        len = poll(slot >= 0, v);
+                    const int got = poll(small ? slot >= 0 : lane < 2, v);
+                    if (got > 0) {
+                        len = small ? got : poll(slot >= 0, v);
                         break;
                     }
+                    if (spins % SPIN_CHECK) continue;
                     // as spin_ready: on abort or timeout the window is taken as ready with
                     // zero masks (the run's results are discarded)
                     const int ab = __hip_atomic_load(a->ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -208,14 +254,21 @@ struct PairWait {
                     }
                 }
             }
-            if (lane < 2 * len) ring[2 * (j % PAIR_RING) + (lane & 1)] = (uint32_t)v;
-            acquire_for_workgroup();
+            if (lane < 2 * len) ring[2 * (j % PAIR_RING) + (lane & 1)] = (uint32_t)v & MASK_ALL;
+            if (ctag) {  // claimed by the workgroup that made one of these operands?
+                const int op = 2 * s + lane;  // operand position in the task
+                const int tg = op < 4 && lane < 2 * cnt ? (ctag >> (8 * op)) & 0xFF : 0;
+                const bool lost = tg != 0 && v >= want && ((v >> (15 + tg)) & 1ull);
+                const bool any_lost = __ballot(lost) != 0;  // every lane of the wave votes
+                if (lane == 0) *s_lost = any_lost;
+            }
             if (lane == 0) {
                 *s_len = len;
                 if (a->trace) *t_wait = wall_clock64();
             }
         }
         __syncthreads();
+        if (ctag && __builtin_amdgcn_readfirstlane(*s_lost)) return -1;
         return s + __builtin_amdgcn_readfirstlane(*s_len);
     }
     __device__ inline void pair_masks(int j, const Pair&, uint32_t& ma, uint32_t& mb) const {
@@ -225,9 +278,12 @@ struct PairWait {
     __device__ inline void operator()(int s, int sa, int sb, uint32_t& ma, uint32_t& mb) const {
         uint32_t* pm = s_pm + PM_SINGLE + 2 * (s & 1);
         if (threadIdx.x == 0) {
-            pm[0] = (uint32_t)spin_ready(*a, sa);
-            pm[1] = (uint32_t)spin_ready(*a, sb);
-            acquire_for_workgroup();
+            if (pre) {  // only the added tile is waited for this way in a claimed task
+                pm[0] = pm[1] = pre[4];
+            } else {
+                pm[0] = (uint32_t)spin_ready(*a, sa) & MASK_ALL;
+                pm[1] = (uint32_t)spin_ready(*a, sb) & MASK_ALL;
+            }
             if (a->trace) *t_wait = wall_clock64();
         }
         __syncthreads();
@@ -298,22 +354,41 @@ __device__ inline void write_masks(const DevTask& t, const Args& a, uint32_t m) 
 // Persistent executor: every workgroup loops taking the next task index.  GEMM
 // tasks wait for windows of pairs (pairs are ordered by when their operands become
 // ready) and multiply each ready pair while loading the next; other tasks wait for
-// all their sources first.
+// all their sources first.  After a task, the workgroup may go on with tasks it claims
+// (dynamic chaining, see build_chains): it reads the claim candidates' other operands'
+// flags when the task starts, and at its end claims every candidate whose operands were
+// all published by then, in candidate order, as far as the ring of CHAIN_RING claimed
+// tasks has room; it runs them one after the other, each with the operand masks noted
+// at the claim (a claimed task never waits).
 // ONE: built for one resident workgroup per CU (twice the registers; FP64 GEMMs stage
 // their next pair too), launched when the executor asks for one per CU.
+constexpr int CHAIN_RING = 8;          // claimed tasks a workgroup holds at once
+constexpr int CHAIN_MAX_PAIRS = 2;     // pairs of a chain candidate
+constexpr int CHAIN_MAX_CANDS = 16;    // claim bits 16..31 of a ready flag
+constexpr int CHAIN_MAX_CHECKS = 64;   // check entries per task: one lane of wave 0 each
 template <typename T, bool BF16, bool LLT, bool MASKS, bool ONE>
 __global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
     __shared__ __attribute__((aligned(16))) char lds[lds_bytes<T>()];
     __shared__ int s_task;
     __shared__ int s_abort;
+    __shared__ int s_lost;
     __shared__ uint32_t s_mask[NT / 64];
     __shared__ uint32_t s_pm[PM_WORDS];
+    __shared__ int s_ring_task[CHAIN_RING];
+    __shared__ uint32_t s_ring_mask[CHAIN_RING][5];
+    __shared__ int s_ring_head, s_ring_n;
+    __shared__ uint32_t s_pre[5];
+    __shared__ int s_next_ti, s_next_mode;
     const int tid = threadIdx.x;
+    const unsigned long long want = (unsigned long long)(unsigned)a.epoch << 32;
     bool aborted = false;
+    int n_ran = 0, n_claimed = 0, n_dropped = 0;  // thread 0: this workgroup's counts
+    acquire_agent();  // every wave: lines left from earlier runs (see above), before its first load
     for (;;) {
         if (tid == 0) {
             s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             s_task = s_abort ? a.nqueue : atomicAdd(a.ctrl, a.chunk);
+            s_ring_head = s_ring_n = 0;
         }
         __syncthreads();
         // wave-uniform loop control: the exit branch must not depend on the lane
@@ -321,46 +396,87 @@ __global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
         if (base >= a.nqueue || aborted) break;
         // a chunk is run in queue order; every dependency points to a task handed out
         // earlier (or to one chained to it), so with all workgroups resident no two
-        // chunks can wait on each other
+        // chunks can wait on each other; a claimed task waits for nothing
         const int end = min(base + a.chunk, a.nqueue);
       for (int qi = base; qi < end && !aborted; qi++) {
         // the task and its chain: a task with `next` hands its sum to the lu / llt that
         // alone reads it, which this workgroup runs at once, without a wait (the sum is
-        // its only source, and the barrier below makes it visible to the workgroup)
+        // its only source, and the barrier below makes it visible to the workgroup);
+        // then the tasks it claimed
         int ti = a.queue[qi];
-        bool chained_in = false;
+        int mode = 0;  // 0: from the queue, 1: the static chain's next task, 2: claimed
         for (;;) {
         const DevTask t = a.tasks[ti];
         const int nxt = __builtin_amdgcn_readfirstlane(t.next);
         long long t_taken = 0, t_ready = 0, t_wait = 0;
         if (a.trace && tid == 0) t_taken = wall_clock64();
-        if (t.kind != TK_GEMM && !chained_in) {
+        if (t.kind != TK_GEMM && mode == 0) {
             if (tid == 0) {
                 for (int d = 0; d < t.dep_count; d++) spin_ready(a, a.deps[t.dep_begin + d]);
-                acquire_for_workgroup();
             }
             __syncthreads();
         }
         if (a.trace && tid == 0) t_wait = wall_clock64();
-        if (tid == 0) s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tid == 0) {
+            s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_lost = 0;
+        }
         __syncthreads();
         if (__builtin_amdgcn_readfirstlane(s_abort)) {
             aborted = true;
             break;
         }
-        run_task<T, BF16, false, LLT, MASKS, ONE>(t, a, lds, s_mask, PairWait{&a, &t_wait, s_pm});
+        // the claim candidates' operand flags, read now (wave 0, one lane per entry) and
+        // looked at when the task is done
+        const int nchk = a.chain ? t.chk_count : 0;
+        unsigned long long cv = 0;
+        if (tid < nchk) {
+            const int32_t cs = a.chk[t.chk_begin + tid].slot;
+            if (cs >= 0) cv = __hip_atomic_load(a.ready + cs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        run_task<T, BF16, false, LLT, MASKS, ONE>(
+            t, a, lds, s_mask,
+            PairWait{&a, &t_wait, s_pm, mode == 2 ? s_pre : nullptr, mode == 0 ? t.ctag : 0, &s_lost});
+        if (__builtin_amdgcn_readfirstlane(s_lost)) {  // claimed by another workgroup: dropped
+            n_dropped++;
+            break;
+        }
         // publish: every wave waits until its write-through tile stores have reached the
         // device coherence point, the barrier orders those waits before thread 0, and
-        // thread 0 raises the flags, which carry the output masks to the consumers.  A
-        // sum that goes to the chained task is read by nobody else: no wait, no flag
-        // (the chained task runs on this CU and reads it after the barrier).
+        // thread 0 raises the flags, which carry the output masks to the consumers and the
+        // claims of this workgroup.  A sum that goes to the chained task is read by nobody
+        // else: no wait, no flag (the chained task runs on this CU and reads it after the
+        // barrier).
         if (nxt < 0) wave_stores_done();
         __syncthreads();
         if (a.trace && tid == 0) t_ready = wall_clock64();  // work done, publish starts
+        uint32_t claims = 0;
+        if (nchk > 0 && tid < 64) {  // wave 0: which candidates can be claimed
+            const uint32_t m_own = task_mask<MASKS>(t, s_mask);
+            const bool on = tid < nchk;
+            const ChainChk ce = on ? a.chk[t.chk_begin + tid] : ChainChk{-1, -1, -1, -1};
+            const bool rdy = on && (ce.slot < 0 || cv >= want);
+            uint32_t present = 0, blocked = 0;
+            for (int c = 0; c < CHAIN_MAX_CANDS; c++) {
+                if (__ballot(on && ce.cand == c)) present |= 1u << c;
+                if (__ballot(on && ce.cand == c && !rdy)) blocked |= 1u << c;
+            }
+            claims = present & ~blocked;
+            const int n0 = __builtin_amdgcn_readfirstlane(s_ring_n);
+            const int h0 = __builtin_amdgcn_readfirstlane(s_ring_head);
+            const int room = min(CHAIN_RING - n0, a.chain);
+            while (__popc(claims) > room) claims &= ~(1u << (31 - __clz(claims)));  // keep the first
+            if (on && ((claims >> ce.cand) & 1u)) {
+                const int r = (h0 + n0 + __popc(claims & ((1u << ce.cand) - 1u))) % CHAIN_RING;
+                s_ring_mask[r][ce.pos] = ce.slot < 0 ? m_own : ((uint32_t)cv & MASK_ALL);
+                if (ce.pos == 0) s_ring_task[r] = ce.task;
+            }
+            if (tid == 0) s_ring_n = n0 + __popc(claims);
+        }
         if (tid == 0) {
             const uint32_t m = task_mask<MASKS>(t, s_mask);
             if (nxt < 0) {
-                const unsigned long long fv = ((unsigned long long)(unsigned)a.epoch << 32) | m;
+                const unsigned long long fv = want | ((unsigned long long)claims << 16) | m;
                 __hip_atomic_store(a.ready + t.out0, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (t.out1 >= 0)
                     __hip_atomic_store(a.ready + t.out1, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -370,25 +486,47 @@ __global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
                     __hip_atomic_store(a.ready + t.out3, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             // consumers in this run read the masks from the flags; the mask array serves
-            // the solve and the other executors, after the kernel, so its stores stay off
-            // the release above
+            // the solve and the other executors, after the kernel
             write_masks(t, a, m);
             if (a.trace) {
                 long long* tr = a.trace + (size_t)ti * 4;
                 gst(&tr[0], t_taken);
                 gst(&tr[1], t_ready);
                 gst(&tr[2], (long long)wall_clock64());
-                gst(&tr[3], t_wait);
+                gst(&tr[3], t_wait | ((long long)mode << 60));  // bits 60-61: how the task started
+            }
+            n_claimed += __popc(claims);
+            n_ran++;
+            // the next task of this chain, decided here and read by every wave after the
+            // barrier below (nothing writes these words again before the next such
+            // barrier): the static chain's next task, else the ring's head with its masks
+            if (nxt >= 0) {
+                s_next_ti = nxt;
+                s_next_mode = 1;
+            } else if (s_ring_n > 0) {
+                const int h = s_ring_head;
+                s_next_ti = s_ring_task[h];
+                for (int q = 0; q < 5; q++) s_pre[q] = s_ring_mask[h][q];
+                s_ring_head = (h + 1) % CHAIN_RING;
+                s_ring_n--;
+                s_next_mode = 2;
+            } else {
+                s_next_mode = 0;
             }
         }
         // keeps the publishing branch and the next fetch branch apart, so the
         // back-edge cannot join them into a lane-dependent inner loop
         __syncthreads();
-        if (nxt < 0) break;
-        ti = nxt;
-        chained_in = true;
+        mode = __builtin_amdgcn_readfirstlane(s_next_mode);
+        if (mode == 0) break;
+        ti = __builtin_amdgcn_readfirstlane(s_next_ti);
         }
       }
+    }
+    if (tid == 0) {  // one update per workgroup, off the task path
+        if (n_claimed) atomicAdd(a.ctrl + 4, n_claimed);
+        if (n_dropped) atomicAdd(a.ctrl + 5, n_dropped);
+        atomicAdd(a.ctrl + 6, n_ran);
     }
 }
 
@@ -471,6 +609,123 @@ struct HostGraph {
     std::vector<Pair> pairs, op_pairs;
     std::vector<int32_t> deps, lp, lc, ls, up, uc, us;
 };
+
+// Dynamic chaining of the persistent executor.  A workgroup that has just made a tile
+// may run a task that reads it at once, without handing it to another workgroup
+// (flag store, poll, acquire, operand loads from memory): it claims the task when all
+// the task's other operands are already published, marking the claim in the ready
+// flag it then publishes (bit 16 + the task's index in its candidate list), and the
+// workgroup that took the task from the queue drops it when it sees that bit among its
+// operands' flags.  Candidates are GEMM tasks of at most CHAIN_MAX_PAIRS pairs on or
+// near the critical path (a longer GEMM's own workgroup has done most of its pairs by
+// the time the last operand lands), each listed with every task that makes one of its
+// pair operands.  The critical path comes from a cost model in microseconds (task
+// work plus a hand-off per edge, measured on the C2 / C5 factors): slack(t) = length
+// of the longest path - the longest path through t.
+constexpr double CHAIN_SLACK_US = 1.0;
+
+void build_chains(Factor* f, HostGraph& hg) {
+    const size_t T = f->tasks.size();
+    f->chk.clear();
+    f->n_chain_cands = 0;
+    for (DevTask& k : f->tasks) {
+        k.ctag = 0;
+        k.chk_begin = k.chk_count = 0;
+    }
+    if (T == 0 || f->precision == SOGL_BF16) return;  // the bf16 GEMM waits pair by pair
+    std::vector<int32_t> slot_task((size_t)f->n_slots, -1);
+    for (size_t t = 0; t < T; t++)
+        for (int32_t o : {f->tasks[t].out0, f->tasks[t].out1, f->tasks[t].out2, f->tasks[t].out3})
+            if (o >= 0) slot_task[(size_t)o] = (int32_t)t;
+    auto cost = [&](const DevTask& k) -> double {
+        switch (k.kind) {
+            case TK_LU: case TK_LLT: return 10.0;
+            case TK_LINV: case TK_UINV: return 5.0;
+            case TK_GEMM: return 0.8 + 0.5 * k.b + (k.add >= 0 ? 0.2 : 0.0);
+            default: return 1.0;
+        }
+    };
+    constexpr double HANDOFF = 2.5;  // publish + hand-off + operand loads from memory
+    auto edge = [&](size_t p, size_t t) { return f->tasks[p].next == (int32_t)t ? 0.0 : HANDOFF; };
+    std::vector<double> tl(T, 0.0), bl(T, 0.0);
+    std::vector<std::vector<int32_t>> succ(T);
+    for (size_t t = 0; t < T; t++) {  // tasks are in dependency-level order
+        const DevTask& k = f->tasks[t];
+        double st = 0;
+        for (int32_t q = k.dep_begin; q < k.dep_begin + k.dep_count; q++) {
+            const int32_t p = slot_task[(size_t)hg.deps[(size_t)q]];
+            if (p < 0) continue;
+            st = std::max(st, tl[(size_t)p] + cost(f->tasks[(size_t)p]) + edge((size_t)p, t));
+            succ[(size_t)p].push_back((int32_t)t);
+        }
+        tl[t] = st;
+    }
+    double cp = 0;
+    for (size_t t = T; t-- > 0;) {
+        double b = 0;
+        for (int32_t u : succ[t]) b = std::max(b, edge(t, (size_t)u) + bl[(size_t)u]);
+        bl[t] = cost(f->tasks[t]) + b;
+        cp = std::max(cp, tl[t] + bl[t]);
+    }
+    // operand slots of a GEMM task by position: pair lanes 0..2n-1, then the added tile
+    auto positions = [&](const DevTask& k, int32_t* sl) {
+        int n = 0;
+        for (int32_t j = 0; j < k.b; j++) {
+            const Pair& pr = hg.pairs[(size_t)(k.a + j)];
+            sl[n++] = pair_a(pr);
+            sl[n++] = pair_b(pr);
+        }
+        return n;
+    };
+    std::vector<std::vector<int32_t>> cand(T);
+    std::vector<int32_t> nchk(T, 0);
+    std::vector<int32_t> order;
+    for (size_t u = 0; u < T; u++) {
+        const DevTask& k = f->tasks[u];
+        if (k.kind == TK_GEMM && k.b >= 1 && k.b <= CHAIN_MAX_PAIRS && cp - (tl[u] + bl[u]) <= CHAIN_SLACK_US)
+            order.push_back((int32_t)u);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        return cp - (tl[(size_t)x] + bl[(size_t)x]) < cp - (tl[(size_t)y] + bl[(size_t)y]);
+    });
+    for (int32_t u : order) {  // most critical first: they get the low candidate indices
+        const DevTask& k = f->tasks[(size_t)u];
+        int32_t sl[2 * CHAIN_MAX_PAIRS];
+        const int n = positions(k, sl);
+        const int nent = n + (k.add >= 0 ? 1 : 0);
+        bool any = false;
+        for (int l = 0; l < n; l++) {
+            const int32_t p = slot_task[(size_t)sl[l]];
+            if (p < 0) continue;
+            auto& cl = cand[(size_t)p];
+            if (std::find(cl.begin(), cl.end(), u) != cl.end()) continue;
+            if ((int)cl.size() >= CHAIN_MAX_CANDS || nchk[(size_t)p] + nent > CHAIN_MAX_CHECKS) continue;
+            cl.push_back(u);
+            nchk[(size_t)p] += nent;
+            any = true;
+        }
+        f->n_chain_cands += any ? 1 : 0;
+    }
+    for (size_t p = 0; p < T; p++) {
+        DevTask& kp = f->tasks[p];
+        kp.chk_begin = (int32_t)f->chk.size();
+        const int32_t outs[4] = {kp.out0, kp.out1, kp.out2, kp.out3};
+        for (size_t i = 0; i < cand[p].size(); i++) {
+            const int32_t u = cand[p][i];
+            DevTask& ku = f->tasks[(size_t)u];
+            int32_t sl[2 * CHAIN_MAX_PAIRS + 1];
+            int n = positions(ku, sl);
+            for (int l = 0; l < n; l++)
+                if (slot_task[(size_t)sl[l]] == (int32_t)p) ku.ctag |= (int32_t)((i + 1) << (8 * l));
+            if (ku.add >= 0) sl[n++] = ku.add;
+            for (int l = 0; l < n; l++) {
+                const bool own = std::find(outs, outs + 4, sl[l]) != outs + 4;
+                f->chk.push_back(ChainChk{own ? -1 : sl[l], u, (int32_t)i, l < n - (ku.add >= 0 ? 1 : 0) ? l : 4});
+            }
+        }
+        kp.chk_count = (int32_t)f->chk.size() - kp.chk_begin;
+    }
+}
 
 // The task graph of a factor description, on the host only (no device calls).
 int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph& hg) {
@@ -898,6 +1153,7 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
         for (size_t u = 0; u < f->tasks.size(); u++)
             if (!chained[u]) f->queue.push_back((int32_t)u);
     }
+    build_chains(f, hg);
 
     // triangular-solve structure
     std::vector<std::vector<std::pair<int32_t, int32_t>>> lrow((size_t)R), urow((size_t)R);
@@ -1039,6 +1295,30 @@ int validate_graph(Context* c, const sogl_factor_desc* d, const Factor* f, const
         for (size_t t = 0; t < T; t++)
             if (!seen[t]) return bad("task " + std::to_string(t) + " is neither queued nor chained");
     }
+    // chain candidates: each check entry names a queued GEMM of at most CHAIN_MAX_PAIRS
+    // pairs, an operand position of it and that operand's slot (-1: an output of the
+    // checking task), and the candidate's claim tag for the checking task's operands
+    for (size_t t = 0; t < T; t++) {
+        const DevTask& k = f->tasks[t];
+        if (k.chk_count < 0 || k.chk_count > CHAIN_MAX_CHECKS || k.chk_begin < 0 ||
+            (size_t)k.chk_begin + (size_t)k.chk_count > f->chk.size())
+            return bad("chain checks of task " + std::to_string(t));
+        for (int32_t q = k.chk_begin; q < k.chk_begin + k.chk_count; q++) {
+            const ChainChk& e = f->chk[(size_t)q];
+            if (e.task < 0 || (size_t)e.task >= T || e.cand < 0 || e.cand >= CHAIN_MAX_CANDS || e.pos < 0 || e.pos > 4)
+                return bad("chain check " + std::to_string(q));
+            const DevTask& u = f->tasks[(size_t)e.task];
+            if (u.kind != TK_GEMM || u.b < 1 || u.b > CHAIN_MAX_PAIRS || (e.pos < 4 && e.pos >= 2 * u.b) ||
+                (e.pos == 4 && u.add < 0))
+                return bad("chain candidate " + std::to_string(e.task));
+            const Pair& pr = hg.pairs[(size_t)(u.a + (e.pos < 4 ? e.pos / 2 : 0))];
+            const int32_t operand = e.pos == 4 ? u.add : ((e.pos & 1) ? pair_b(pr) : pair_a(pr));
+            const bool own = operand == k.out0 || operand == k.out1 || operand == k.out2 || operand == k.out3;
+            if (e.slot != (own ? -1 : operand)) return bad("chain check slot " + std::to_string(q));
+            if (own && e.pos < 4 && ((u.ctag >> (8 * e.pos)) & 0xFF) != e.cand + 1)
+                return bad("claim tag of task " + std::to_string(e.task));
+        }
+    }
     for (const DevTask& k : f->op_tasks) {
         if (!ok_slot(k.out0) || !ok_opt(k.out1) || !ok_opt(k.out2) || !ok_opt(k.out3)) return bad("per-op output");
         if (k.kind == TK_GEMM) {
@@ -1088,6 +1368,8 @@ int upload_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells,
     if ((rc = dev_alloc(c, f->op_tasks_d, std::max<size_t>(f->op_tasks.size(), 1) * sizeof(DevTask), "op tasks")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->op_pairs_d, std::max<size_t>(op_pairs.size(), 1) * sizeof(Pair), "op pairs")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->queue_d, std::max<size_t>(f->queue.size(), 1) * 4, "task queue")) != SOGL_OK) return rc;
+    if ((rc = dev_alloc(c, f->chk_d, std::max<size_t>(f->chk.size(), 1) * sizeof(ChainChk), "chain checks")) != SOGL_OK)
+        return rc;
     auto up_i32 = [&](DevBuf& b, const std::vector<int32_t>& v, const char* what) -> int {
         int r2 = dev_alloc(c, b, std::max<size_t>(v.size(), 1) * 4, what);
         if (r2 != SOGL_OK) return r2;
@@ -1101,6 +1383,8 @@ int upload_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells,
     if (!f->op_tasks.empty()) HIP_TRY(c, hipMemcpy(f->op_tasks_d.p, f->op_tasks.data(), f->op_tasks.size() * sizeof(DevTask), hipMemcpyHostToDevice));
     if (!op_pairs.empty()) HIP_TRY(c, hipMemcpy(f->op_pairs_d.p, op_pairs.data(), op_pairs.size() * sizeof(Pair), hipMemcpyHostToDevice));
     if (!f->queue.empty()) HIP_TRY(c, hipMemcpy(f->queue_d.p, f->queue.data(), f->queue.size() * 4, hipMemcpyHostToDevice));
+    if (!f->chk.empty())
+        HIP_TRY(c, hipMemcpy(f->chk_d.p, f->chk.data(), f->chk.size() * sizeof(ChainChk), hipMemcpyHostToDevice));
     if ((rc = up_i32(f->l_rowptr, hg.lp, "L rows")) || (rc = up_i32(f->l_col, hg.lc, "L cols")) ||
         (rc = up_i32(f->l_slot, hg.ls, "L slots")) || (rc = up_i32(f->u_rowptr, hg.up, "U rows")) ||
         (rc = up_i32(f->u_col, hg.uc, "U cols")) || (rc = up_i32(f->u_slot, hg.us, "U slots")) ||
@@ -1204,7 +1488,7 @@ int build_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells, 
 void free_factor(Factor* f) {
     if (f->graph_exec) (void)hipGraphExecDestroy(f->graph_exec);
     if (f->graph) (void)hipGraphDestroy(f->graph);
-    for (DevBuf* b : {&f->pool, &f->mask_d, &f->queue_d, &f->tasks_d, &f->pairs_d, &f->deps_d, &f->ready_d, &f->ctrl_d,
+    for (DevBuf* b : {&f->pool, &f->mask_d, &f->queue_d, &f->chk_d, &f->tasks_d, &f->pairs_d, &f->deps_d, &f->ready_d, &f->ctrl_d,
                       &f->op_tasks_d, &f->op_pairs_d, &f->inputs_d, &f->l_rowptr, &f->l_col,
                       &f->l_slot, &f->u_rowptr, &f->u_col, &f->u_slot, &f->l_diag_d, &f->u_diag_d,
                       &f->linv_d, &f->uinv_d, &f->solve_ready, &f->solve_ctrl, &f->vec_b, &f->vec_x,
@@ -1226,6 +1510,8 @@ int launch_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
     a.growth_limit = f->growth_limit;
     a.deps = (const int32_t*)f->deps_d.p;
     a.trace = (mode == SOGL_EXEC_PERSISTENT && f->trace_on) ? (long long*)f->trace_d.p : nullptr;
+    a.chk = (const ChainChk*)f->chk_d.p;
+    a.chain = (mode == SOGL_EXEC_PERSISTENT && !f->chk.empty()) ? f->chain_claims : 0;
     auto launch_levels = [&]() {
         a.tasks = (const DevTask*)f->tasks_d.p;
         a.pairs = (const Pair*)f->pairs_d.p;
@@ -1472,6 +1758,9 @@ int run_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
     int ctrl[CTRL_WORDS];
     HIP_TRY(c, hipMemcpy(ctrl, f->ctrl_d.p, sizeof(ctrl), hipMemcpyDeviceToHost));
     f->growth_tiles = ctrl[2];
+    f->claimed = mode == SOGL_EXEC_PERSISTENT ? ctrl[4] : 0;
+    f->dropped = mode == SOGL_EXEC_PERSISTENT ? ctrl[5] : 0;
+    f->ran = mode == SOGL_EXEC_PERSISTENT ? ctrl[6] : (int32_t)f->tasks.size();
     float gm;
     std::memcpy(&gm, &ctrl[3], 4);
     f->max_growth = gm;
@@ -1603,7 +1892,9 @@ extern "C" int sogl_factor_get_stats(sogl_ctx* x, int32_t factor_id, sogl_factor
     st->max_growth = f->max_growth;
     st->zero_blocks = f->zero_blocks;
     st->block_masks = (f->mask_mode == SOGL_MASKS_ON || (f->mask_mode == SOGL_MASKS_AUTO && f->masks_on)) ? 1 : 0;
-    st->reserved = 0;
+    st->claimed_tasks = f->claimed;
+    st->dropped_tasks = f->dropped;
+    st->tasks_run = f->ran;
     return SOGL_OK;
 }
 
@@ -1620,6 +1911,17 @@ extern "C" int sogl_factor_set_masks(sogl_ctx* x, int32_t factor_id, int32_t mod
         f->masks_on = true;
         f->masks_decided = false;
     }
+    return SOGL_OK;
+}
+
+extern "C" int sogl_factor_set_chaining(sogl_ctx* x, int32_t factor_id, int32_t enable) {
+    if (!x) return set_error(nullptr, SOGL_ERR_ARG, "null argument");
+    Context* c = &x->c;
+    std::lock_guard<std::mutex> g(c->mu);
+    Factor* f = find_factor(c, factor_id);
+    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+    if (enable < 0 || enable > CHAIN_RING) return set_error(c, SOGL_ERR_ARG, "claims per task: 0 .. 8");
+    f->chain_claims = enable;
     return SOGL_OK;
 }
 

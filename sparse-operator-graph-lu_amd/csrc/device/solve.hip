@@ -54,13 +54,18 @@ __device__ inline void load_seg(const T* tile, int r, int kp, double* seg) {
     for (int k = 0; k < 16; k++) seg[k] = (double)p[k];
 }
 
+// The abort word and the wall clock are looked at every SOLVE_SPIN_CHECK polls only, so
+// that each poll costs one round trip to the device coherence point.
+constexpr int SOLVE_SPIN_CHECK = 64;
+
 __device__ inline void spin_row(const SolveArgs& a, int J) {
     int* flag = a.ready + J;
     if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= a.epoch) return;
     const long long t0 = wall_clock64();
-    for (;;) {
+    for (int spins = 1;; spins++) {
         __builtin_amdgcn_s_sleep(1);
         if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= a.epoch) return;
+        if (spins % SOLVE_SPIN_CHECK) continue;
         if (__hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
         if (wall_clock64() - t0 > SOLVE_SPIN_LIMIT) {
             __hip_atomic_store(a.ctrl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -103,8 +108,9 @@ __device__ inline void load_row_stored(const double* col, double* ls, int lane, 
         if (!__ballot(u == SOLVE_PENDING)) break;
         if (spin == 0) {
             t0 = wall_clock64();
-        } else if (__hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
-                   wall_clock64() - t0 > SOLVE_SPIN_LIMIT) {
+        } else if (spin % SOLVE_SPIN_CHECK == 0 &&
+                   (__hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                    wall_clock64() - t0 > SOLVE_SPIN_LIMIT)) {
             __hip_atomic_store(a.ctrl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             u = 0;  // the bit pattern of 0.0
             break;

@@ -380,6 +380,7 @@ __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, c
         if (s != staged) {
             if (s == ready) {
                 ready = wait.extend(pairs, s, npairs, window++);  // its barrier also frees LDS
+                if (ready < 0) return;  // another workgroup took the task over (wave-uniform)
                 synced = true;
             }
             wait.pair_masks(s, p, ma, mbs);

@@ -62,7 +62,7 @@ class FactorStats(C.Structure):
                 ("pool_bytes", C.c_int64),
                 ("n_tasks", C.c_int64), ("n_levels", C.c_int32), ("growth_tiles", C.c_int32),
                 ("max_growth", C.c_double), ("zero_blocks", C.c_double), ("block_masks", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("claimed_tasks", C.c_int32), ("dropped_tasks", C.c_int32), ("tasks_run", C.c_int32)]
 
 
 def _ptr(a: np.ndarray, ctype):
@@ -124,6 +124,7 @@ def load(name: str = "libsogl.so") -> C.CDLL:
             "sogl_factor_get_stats": (C.c_int, [vp, i32, P(FactorStats)]),
             "sogl_factor_get_tiles": (C.c_int, [vp, i32, i32, P(dbl)]),
         "sogl_factor_set_masks": (C.c_int, [vp, i32, i32]),
+            "sogl_factor_set_chaining": (C.c_int, [vp, i32, i32]),
             "sogl_refine_attach": (C.c_int, [vp, i32, vp]),
             "sogl_solve_refined": (C.c_int, [vp, i32, P(dbl), P(dbl), i32, dbl, i32, P(i32),
                                              P(dbl)]),
@@ -383,6 +384,12 @@ class Device:
 
     def factor(self, fid: int, mode: int = EXEC_PERSISTENT):
         self._check(self.lib.sogl_factor_run(self.h, fid, mode))
+
+    def set_chaining(self, fid: int, claims):
+        """Dynamic chaining of the persistent executor: at most `claims` tasks claimed per
+        finished task (0 / False: off; True: the default, 2)."""
+        n = 2 if claims is True else int(claims)
+        self._check(self.lib.sogl_factor_set_chaining(self.h, fid, n))
 
     def set_masks(self, fid: int, mode: int):
         """16x16 block-mask policy: MASKS_OFF, MASKS_ON or MASKS_AUTO (the default)."""
