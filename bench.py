@@ -69,10 +69,13 @@ EXECUTORS = {"auto": (sogl.EXEC_AUTO, 0), "persistent": (sogl.EXEC_PERSISTENT, 0
 EXTRA = {"sym": (lambda: W.symmetric_lower(W.diffusion_varcoef(48, 3, 23)), "sym_diffusion3d_48")}
 
 
-def compare_executors(dev, plan, per_op: bool):
+def compare_executors(dev, plan, per_op: bool, reuse_mode: int):
     """FP64 factor time of every executor on one resident factor (ms, best of 2), the
-    autotuner's choice, and the per-op executor (one launch per unit op) once."""
+    autotuner's choice, and the per-op executor (one launch per unit op) once (the
+    factor is loaded without slot reuse, which the per-op executor needs)."""
+    dev.set_pool_reuse(sogl.REUSE_OFF)
     fid = dev.load(plan, sogl.FP64)
+    dev.set_pool_reuse(reuse_mode)
     out = {}
     dev.factor(fid, sogl.EXEC_GRAPH)  # capture outside the timed runs
     for name, mode in (("persistent", sogl.EXEC_PERSISTENT), ("levels", sogl.EXEC_LEVELS),
@@ -273,6 +276,12 @@ def main():
                          "(tuned before the warmup steps, outside the timed region); "
                          "persistent-1wg: the persistent executor with one workgroup per CU")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--pool-reuse", type=int, default=sogl.REUSE_AUTO, choices=(0, 1, 2),
+                    help="1: factors reuse pool slots of values no longer read (smaller pool, coherent "
+                         "tile loads); 0: one slot per value; 2 (default): reuse when the values would "
+                         "take over a quarter of the device's HBM")
+    ap.add_argument("--no-reuse-leg", action="store_true",
+                    help="skip the FP64 leg that factors with pool slot reuse forced on")
     ap.add_argument("--no-chain", action="store_true",
                     help="persistent executor without dynamic chaining (for comparison)")
     ap.add_argument("--chain-claims", type=int, default=0,
@@ -311,6 +320,7 @@ def main():
     solve_bytes = {k: (info.n_l_tiles + info.n_u_tiles) * 64 * 64 * s + 2 * plan.N * 8 * nrhs
                    for k, s in (("fp64", 8), ("fp32", 4))}
     dev = sogl.Device(0)
+    dev.set_pool_reuse(args.pool_reuse)
     paths = {}
     sweep = () if args.no_ir_sweep or nrhs != 1 else tuple(range(0, 7))
     ch = 0 if args.no_chain else args.chain_claims
@@ -320,9 +330,20 @@ def main():
     if args.bf16 or (args.config == "c5" and not args.no_bf16):
         paths["bf16_ir"] = run_path(dev, plan, prob, sogl.BF16, tol, 50, min(args.steps, 3), min(args.warmup, 1),
                                     *mode, chain=ch)
+    reuse_leg = None
+    if not args.no_reuse_leg and args.pool_reuse != sogl.REUSE_ON:
+        # the same FP64 factor+solve with slot reuse forced on: its pool, its time, and
+        # its bits against the leg above
+        dev.set_pool_reuse(sogl.REUSE_ON)
+        r = run_path(dev, plan, prob, sogl.FP64, tol, 0, min(args.steps, 2), min(args.warmup, 1), *mode, chain=ch)
+        dev.set_pool_reuse(args.pool_reuse)
+        xr = r.pop("x")
+        reuse_leg = {"precision": "fp64", "exec_s": round(r["exec_s"], 6), "step_s": round(r["step_s"], 6),
+                     "pool_gb": round(r["pool_gb"], 3), "pool_gb_main_leg": round(paths["fp64"]["pool_gb"], 3),
+                     "x_bits_equal_main_leg": bool(np.array_equal(xr, paths["fp64"]["x"]))}
     executors = None
     if args.compare_executors or args.config == "c3":
-        executors = compare_executors(dev, plan, per_op=True)
+        executors = compare_executors(dev, plan, per_op=True, reuse_mode=args.pool_reuse)
     lu_same = None
     if prob.symmetric:  # the same matrix given in full (both triangles) takes the LU path
         r, c, v = W.full_matrix(prob)
@@ -438,6 +459,8 @@ def main():
             "corrections_to_reach_10x_fp64_x_error": reach}
     if executors is not None:
         line["executors_fp64"] = executors
+    if reuse_leg is not None:
+        line["pool_reuse_fp64"] = reuse_leg
     if "bf16_ir" in out_paths:
         line["bf16_ir_experiment"] = out_paths["bf16_ir"]
     if lu_same is not None:

@@ -184,13 +184,33 @@ typedef struct sogl_factor_desc {
 /* Build the device task graph and tile pool for one factorisation and copy the
  * input tiles to HBM.  The factor stays resident under *factor_id until released. */
 int sogl_factor_load(sogl_ctx* ctx, const sogl_factor_desc* desc, int32_t* factor_id);
+/* Pool slot reuse for the factors this context loads from now on: the reference's
+ * liveness-based tile recycling (BlockPlanner.cpp:597-634, memutil.cpp:74-113).  A
+ * result takes over the HBM tile of a value that only its own task reads (a GEMM's added
+ * tile or operand, a sub's operand, an FP64 lu / llt source), or, for a GEMM, a tile
+ * whose last value's readers all lie 1024 or more dependency levels earlier, after
+ * waiting for those readers' flags.  The factor tiles and diagonal inverses keep theirs for the
+ * solve.  A factor loaded from a plan keeps its input cells on the device and scatters
+ * them into the input tiles again before every run (inside factor_ms, outside exec_ms),
+ * so their tiles are reused as well.  Reuse reads tiles with device-coherent loads
+ * (2-7% slower factors); the factor's bits are the same either way.  ON / OFF, or AUTO
+ * (the default): reuse when the factor's values would take more than a quarter of the
+ * device's HBM.  The per-op executor accumulates unit ops in the result's tile and runs
+ * only without reuse. */
+#define SOGL_REUSE_OFF 0
+#define SOGL_REUSE_ON 1
+#define SOGL_REUSE_AUTO 2
+int sogl_ctx_set_pool_reuse(sogl_ctx* ctx, int32_t mode);
 /* Builds the executor's task graph of a description on the host only (no device is
  * touched) and checks it: every slot index inside the pool, one writer per tile,
- * every operand produced at an earlier dependency level and waited for.
- * sogl_factor_load runs the same check before any upload.  info (may be NULL)
- * receives 8 counts: tasks, pool tiles, dependency levels, GEMM pairs,
- * dependencies, folded copy/negate/subtract ops, aliased repeated inverses, and lu /
- * llt tasks chained to the GEMM that makes their source (same workgroup, no hand-off). */
+ * every operand produced at an earlier dependency level and waited for, and the pool
+ * slot reuse (see sogl_ctx_set_pool_reuse) handing a slot over only from a value to the
+ * task that alone reads it.  sogl_factor_load runs the same checks before any upload.
+ * info (may be NULL) receives 9 counts: tasks, value tiles (the pool without slot
+ * reuse), dependency levels, GEMM pairs, dependencies, folded copy/negate/subtract ops,
+ * aliased repeated inverses, lu / llt tasks chained to the GEMM that makes their source
+ * (same workgroup, no hand-off), and pool tiles with slot reuse (the _plan form's inputs
+ * can be scattered again, so their slots are reused too). */
 int sogl_graph_check(const sogl_factor_desc* desc, int64_t* info);
 int sogl_graph_check_plan(const sogl_plan* plan, int32_t precision, int64_t* info);
 /* Convenience: sogl_factor_load from a host plan. */
@@ -257,6 +277,8 @@ typedef struct sogl_factor_stats {
     int32_t claimed_tasks;  /* last persistent run: tasks run by the workgroup that made an operand */
     int32_t dropped_tasks;  /* last persistent run: queued tasks given up because another claimed them */
     int32_t tasks_run;      /* last run: tasks executed (n_tasks when each ran exactly once) */
+    int64_t pool_tiles;     /* 64x64 tiles in the pool (pool_bytes / tile bytes) */
+    int64_t value_tiles;    /* tiles the factor's values would need without slot reuse */
 } sogl_factor_stats;
 int sogl_factor_get_stats(sogl_ctx* ctx, int32_t factor_id, sogl_factor_stats* st);
 /* Dynamic chaining of the persistent executor: a workgroup that has made a tile runs a

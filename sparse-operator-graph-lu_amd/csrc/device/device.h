@@ -18,12 +18,15 @@ enum TaskKind : int32_t { TK_GEMM = 0, TK_SUB = 1, TK_LU = 2, TK_LINV = 3, TK_UI
 // One device task = every unit op that writes one result tile (an lu writes two).
 // Writers of one tile always share a stage in the reference's plan (the "fix"
 // pass, BlockPlanner.cpp:269-276), so a tile is produced by exactly one task.
+// Slots address tile data and block masks in the pool; value ids (f0..f3, fadd, the
+// dependency list, Pair::fa / fb) index the ready flags.  Without slot reuse the two
+// are the same numbers.
 struct DevTask {
     int32_t kind;
     int32_t out0, out1;   // result slots (out1: U of an lu, else -1)
     int32_t a, b;         // GEMM: pair begin / pair count; SUB: minuend / subtrahend slot
                           // (-1 = zero); LU, LINV, UINV, LLT, TRANS: source slot
-    int32_t dep_begin, dep_count;  // slots this task waits for (persistent executor)
+    int32_t dep_begin, dep_count;  // value ids this task waits for (persistent executor)
     int32_t level;        // dependency level (0-based)
     int32_t out2, out3;   // lu: slots of L^-1 and U^-1 made by the same task, llt: of L^-1
                           // (-1: none)
@@ -37,6 +40,14 @@ struct DevTask {
     // as check entries (ChainChk), one per candidate operand.
     int32_t ctag = 0;
     int32_t chk_begin = 0, chk_count = 0;
+    int32_t f0 = -1, f1 = -1, f2 = -1, f3 = -1;  // value ids of out0..out3 (-1: none)
+    int32_t fadd = -1;                            // value id of the added tile
+    int32_t fsa = -1, fsb = -1;                   // value ids of a / b (other than GEMM)
+    // slot reuse: before storing its result, a task whose result slot held another value
+    // waits for the tasks that read that value; their flags are war[war_begin ..
+    // war_begin + war_count) (value ids)
+    int32_t war_begin = 0, war_count = 0;
+    int32_t pad[3] = {0, 0, 0};
 };
 
 // One check entry of a task's chain candidates: candidate `task` (index `cand` in the
@@ -55,7 +66,15 @@ struct Factor {
     int32_t id = 0;
     int32_t precision = SOGL_FP64;
     int32_t block_rows = 0;
-    int64_t n_slots = 0;        // tiles in the pool (inputs + results + zero + diagonal inverses)
+    int64_t n_slots = 0;        // values: inputs + results + zero + diagonal inverses (one flag each)
+    int64_t n_locs = 0;         // tiles in the pool: n_slots, or fewer when slots are reused
+    bool reuse = false;         // a result may take over the slot of a value only its task reads
+    bool rescatter = false;     // input slots are taken over too: the inputs are scattered
+                                // again from the cells kept on the device before every run
+    int64_t n_inputs = 0;
+    DevBuf cell_begin, cell_pos, cell_val;  // rescatter: the input cells
+    std::vector<int32_t> war;   // write-after-read flags of reused result slots (DevTask)
+    DevBuf war_d;
     int32_t zero_slot = -1;
     size_t elem = 8;            // bytes per tile element in the pool
     // host copies
@@ -117,6 +136,8 @@ struct Context {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ex0 = nullptr, ex1 = nullptr;
     int num_cu = 0;
+    int32_t pool_reuse = SOGL_REUSE_AUTO;  // for the factors loaded from now on (sogl_ctx_set_pool_reuse)
+    size_t free_mem = 0, total_mem = 0;    // device HBM when the context was made
     std::string err;
     std::map<int32_t, Factor*> factors;
     int32_t next_id = 1;

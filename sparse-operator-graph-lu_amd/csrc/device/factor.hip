@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <string>
 #include <vector>
@@ -121,6 +122,7 @@ struct Args {
     uint32_t* mask;       // per slot: 16x16 block mask of the tile (see tile_ops.h)
     const ChainChk* chk;  // persistent executor: chain candidates' check entries
     int32_t chain;        // claims per finished task at most (0: no dynamic chaining)
+    const int32_t* war;   // value ids whose readers a reused result slot waits for (DevTask)
 };
 
 // Waits (thread 0 polls, relaxed) until the epoch in ready[s] reaches this run's and
@@ -178,7 +180,9 @@ constexpr int PAIR_WINDOW = 32;
 constexpr int PAIR_RING = 2 * PAIR_WINDOW;
 constexpr int PM_LEN = 2 * PAIR_RING;          // two window counts
 constexpr int PM_SINGLE = PM_LEN + 2;          // two single-wait slots of two words
-constexpr int PM_WORDS = PM_SINGLE + 4;
+constexpr int PM_ADD = PM_SINGLE + 4;          // the added tile: seen ready, its mask
+constexpr int PM_WAR = PM_ADD + 2;             // the result slot's readers all seen done
+constexpr int PM_WORDS = PM_WAR + 1;
 // A task claimed by the workgroup that made one of its operands (dynamic chaining) has
 // every operand published already: `pre` holds the operands' masks (pair lanes 0-3, the
 // added tile at 4), taken from the flags the claimer read, so nothing is polled.  A
@@ -193,7 +197,8 @@ struct PairWait {
     const uint32_t* pre = nullptr;
     int32_t ctag = 0;
     int* s_lost = nullptr;
-    __device__ inline int extend(const Pair* pairs, int s, int npairs, int window) const {
+    int32_t war_b = 0, war_n = 0;  // the task's write-after-read flags (Args::war)
+    __device__ inline int extend(const Pair* pairs, int s, int npairs, int window, int fadd = -1) const {
         uint32_t* ring = s_pm;
         int* s_len = reinterpret_cast<int*>(s_pm + PM_LEN + (window & 1));
         if (pre) {  // a claimed task (at most CHAIN_MAX_PAIRS pairs: one window)
@@ -203,6 +208,7 @@ struct PairWait {
                 if (lane < 2 * cnt) ring[2 * ((s + (lane >> 1)) % PAIR_RING) + (lane & 1)] = pre[2 * s + lane];
                 if (lane == 0) {
                     *s_len = cnt;
+                    if (window == 0) s_pm[PM_WAR] = 0;  // looked at before the stores (war_done)
                     if (a->trace) *t_wait = wall_clock64();
                 }
             }
@@ -214,14 +220,24 @@ struct PairWait {
             const int cnt = min(npairs - s, PAIR_WINDOW);
             const int j = s + (lane >> 1);
             int slot = -1;
-            if (lane < 2 * cnt) {
+            const bool pl = lane < 2 * cnt;  // a pair's lane
+            // lane 63 polls the added tile's flag in the first window when no pair has it,
+            // and the lanes between the pairs' and 63 the write-after-read flags
+            const bool al = fadd >= 0 && window == 0 && lane == 63 && !pl;
+            const int wj = lane - 2 * cnt;
+            const bool wl = window == 0 && !pl && lane < 63 && wj < war_n;
+            if (pl) {
                 const Pair p = pairs[j];
-                slot = (lane & 1) ? pair_b(p) : pair_a(p);
+                slot = (lane & 1) ? p.fb : p.fa;  // the flags are per value
+            } else if (al) {
+                slot = fadd;
+            } else if (wl) {
+                slot = a->war[war_b + wj];
             }
             const unsigned long long want = (unsigned long long)(unsigned)a->epoch << 32;
             auto poll = [&](bool on, unsigned long long& v) {  // leading pairs with both flags raised
                 if (on) v = __hip_atomic_load(a->ready + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const unsigned long long late = __ballot(on && v < want);
+                const unsigned long long late = __ballot(on && pl && v < want);
                 return (late ? __builtin_ctzll(late) : 2 * cnt) >> 1;
             };
             unsigned long long v = 0;
@@ -255,6 +271,14 @@ struct PairWait {
                 }
             }
             if (lane < 2 * len) ring[2 * (j % PAIR_RING) + (lane & 1)] = (uint32_t)v & MASK_ALL;
+            if (fadd >= 0 && window == 0 && lane == 63) {  // not polled (a full window): not seen
+                s_pm[PM_ADD] = al && v >= want;
+                s_pm[PM_ADD + 1] = (uint32_t)v & MASK_ALL;
+            }
+            if (window == 0 && war_n > 0) {  // wave-uniform
+                const bool late = __ballot(wl && v < want) != 0;
+                if (lane == 0) s_pm[PM_WAR] = !late && war_n <= 63 - 2 * cnt;
+            }
             if (ctag) {  // claimed by the workgroup that made one of these operands?
                 const int op = 2 * s + lane;  // operand position in the task
                 const int tg = op < 4 && lane < 2 * cnt ? (ctag >> (8 * op)) & 0xFF : 0;
@@ -275,14 +299,54 @@ struct PairWait {
         ma = s_pm[2 * (j % PAIR_RING)];
         mb = s_pm[2 * (j % PAIR_RING) + 1];
     }
-    __device__ inline void operator()(int s, int sa, int sb, uint32_t& ma, uint32_t& mb) const {
+    // Before the first result store of a task whose result slot held another value in
+    // this run: every task that read that value has published (its loads are done).
+    // Usually the first window saw all these flags up already (polled: extend ran for
+    // window 0 and left its verdict in PM_WAR); otherwise wave 0 polls them here, 64 at a
+    // time, and the barrier holds the other waves' stores back.
+    __device__ inline void war_done(bool polled) const {
+        if (war_n == 0) return;                 // wave-uniform (task field)
+        if (polled && s_pm[PM_WAR]) return;     // uniform: written before window 0's barrier
+        if (threadIdx.x < 64) {
+            const int lane = threadIdx.x;
+            const unsigned long long want = (unsigned long long)(unsigned)a->epoch << 32;
+            for (int base = 0; base < war_n; base += 64) {
+                const int fl = base + lane < war_n ? a->war[war_b + base + lane] : -1;
+                const long long t0 = wall_clock64();
+                for (unsigned spins = 1;; spins++) {
+                    const bool up = fl < 0 || __hip_atomic_load(a->ready + fl, __ATOMIC_RELAXED,
+                                                                 __HIP_MEMORY_SCOPE_AGENT) >= want;
+                    if (__ballot(!up) == 0) break;
+                    __builtin_amdgcn_s_sleep(1);
+                    if (spins % SPIN_CHECK) continue;
+                    const int ab = __hip_atomic_load(a->ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (__builtin_amdgcn_readfirstlane(ab) || wall_clock64() - t0 > SPIN_LIMIT_TICKS) {
+                        if (lane == 0) __hip_atomic_store(a->ctrl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // after the first window: was the added tile's flag up (written by that window, read
+    // before the next task's first window can write it again)
+    __device__ inline bool add_ready(int, uint32_t& m) const {
+        if (pre) {
+            m = pre[4];
+            return true;
+        }
+        m = s_pm[PM_ADD + 1];
+        return s_pm[PM_ADD] != 0;
+    }
+    __device__ inline void operator()(int s, int fa, int fb, int, int, uint32_t& ma, uint32_t& mb) const {
         uint32_t* pm = s_pm + PM_SINGLE + 2 * (s & 1);
         if (threadIdx.x == 0) {
             if (pre) {  // only the added tile is waited for this way in a claimed task
                 pm[0] = pm[1] = pre[4];
             } else {
-                pm[0] = (uint32_t)spin_ready(*a, sa) & MASK_ALL;
-                pm[1] = (uint32_t)spin_ready(*a, sb) & MASK_ALL;
+                pm[0] = (uint32_t)spin_ready(*a, fa) & MASK_ALL;
+                pm[1] = fb == fa ? pm[0] : (uint32_t)spin_ready(*a, fb) & MASK_ALL;
             }
             if (a->trace) *t_wait = wall_clock64();
         }
@@ -294,26 +358,32 @@ struct PairWait {
 
 // LLT: the kernel of an L L^T factor (llt, transpose and A B^T tasks, no lu); the LU
 // kernels are built without those, which keeps their register use as it was.
-template <typename T, bool BF16, bool ACC, bool LLT, bool MASKS, bool ONE = false, typename Wait>
+// COH: the factor reuses pool slots, so tiles are read with coherent loads -- except the
+// source of an lu / llt run right after the GEMM that made it (chained): that sum is its
+// slot's first value in the run (no cache holds an earlier one), and this CU's own
+// caches have its stores, which coherent loads could pass (the chain does not wait for
+// them).
+template <typename T, bool BF16, bool ACC, bool LLT, bool MASKS, bool ONE = false, bool COH = false, typename Wait>
 __device__ __attribute__((always_inline)) void run_task(const DevTask& t, const Args& a, char* lds, uint32_t* s_mask,
-                                                        const Wait& wait) {
+                                                        const Wait& wait, bool chained = false) {
     T* pool = reinterpret_cast<T*>(a.pool);
     switch (t.kind) {
         case TK_GEMM:
             if constexpr (BF16 && !ACC) {
-                gemm_task_bf16<LLT, MASKS>(reinterpret_cast<float*>(a.pool), a.pairs + t.a, t.b, t.out0, lds, s_mask,
-                                    wait, t.add, t.add_neg != 0);
+                gemm_task_bf16<LLT, MASKS, Wait, COH>(reinterpret_cast<float*>(a.pool), a.pairs + t.a, t.b, t.out0,
+                                                      lds, s_mask, wait, t.add, t.add_neg != 0, t.fadd);
             } else {
-                gemm_task<T, ACC, LLT, MASKS, Wait, ONE>(pool, a.pairs + t.a, t.b, t.out0, lds, s_mask, wait, t.add,
-                                                  t.add_neg != 0);
+                gemm_task<T, ACC, LLT, MASKS, Wait, ONE, COH>(pool, a.pairs + t.a, t.b, t.out0, lds, s_mask, wait,
+                                                              t.add, t.add_neg != 0, t.fadd);
             }
             break;
         case TK_SUB:
-            sub_task<T, MASKS>(pool, t.b, t.a, t.out0, s_mask);
+            sub_task<T, MASKS, COH>(pool, t.b, t.a, t.out0, s_mask);
             break;
         case TK_LU:
             if constexpr (!LLT) {
-                double g = lu_task<T>(pool, t.a, t.out0, t.out1, t.out2, t.out3, lds, a.growth_limit, a.ctrl + 2);
+                double g = lu_task<T>(pool, t.a, t.out0, t.out1, t.out2, t.out3, lds, a.growth_limit, a.ctrl + 2,
+                                      COH && !chained);
                 if (threadIdx.x == 0 && sizeof(T) == 4) {
                     float gf = (float)g;
                     atomicMax(a.ctrl + 3, __float_as_int(gf));
@@ -321,16 +391,16 @@ __device__ __attribute__((always_inline)) void run_task(const DevTask& t, const 
             }
             break;
         case TK_LINV:
-            tri_inv_task<T, true>(pool, t.a, t.out0, lds);
+            tri_inv_task<T, true, COH>(pool, t.a, t.out0, lds);
             break;
         case TK_UINV:
-            tri_inv_task<T, false>(pool, t.a, t.out0, lds);
+            tri_inv_task<T, false, COH>(pool, t.a, t.out0, lds);
             break;
         case TK_LLT:
-            if constexpr (LLT) llt_task<T>(pool, t.a, t.out0, t.out2, lds);
+            if constexpr (LLT) llt_task<T>(pool, t.a, t.out0, t.out2, lds, COH && !chained);
             break;
         case TK_TRANS:
-            if constexpr (LLT) trans_task<T, MASKS>(pool, t.a, t.out0, lds, s_mask);
+            if constexpr (LLT) trans_task<T, MASKS, COH>(pool, t.a, t.out0, lds, s_mask);
             break;
         default:
             break;
@@ -366,7 +436,7 @@ constexpr int CHAIN_RING = 8;          // claimed tasks a workgroup holds at onc
 constexpr int CHAIN_MAX_PAIRS = 2;     // pairs of a chain candidate
 constexpr int CHAIN_MAX_CANDS = 16;    // claim bits 16..31 of a ready flag
 constexpr int CHAIN_MAX_CHECKS = 64;   // check entries per task: one lane of wave 0 each
-template <typename T, bool BF16, bool LLT, bool MASKS, bool ONE>
+template <typename T, bool BF16, bool LLT, bool MASKS, bool ONE, bool COH>
 __global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
     __shared__ __attribute__((aligned(16))) char lds[lds_bytes<T>()];
     __shared__ int s_task;
@@ -434,9 +504,11 @@ __global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
             const int32_t cs = a.chk[t.chk_begin + tid].slot;
             if (cs >= 0) cv = __hip_atomic_load(a.ready + cs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        run_task<T, BF16, false, LLT, MASKS, ONE>(
+        run_task<T, BF16, false, LLT, MASKS, ONE, COH>(
             t, a, lds, s_mask,
-            PairWait{&a, &t_wait, s_pm, mode == 2 ? s_pre : nullptr, mode == 0 ? t.ctag : 0, &s_lost});
+            PairWait{&a, &t_wait, s_pm, mode == 2 ? s_pre : nullptr, mode == 0 ? t.ctag : 0, &s_lost, t.war_begin,
+                     t.war_count},
+            mode == 1);
         if (__builtin_amdgcn_readfirstlane(s_lost)) {  // claimed by another workgroup: dropped
             n_dropped++;
             break;
@@ -477,13 +549,13 @@ __global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
             const uint32_t m = task_mask<MASKS>(t, s_mask);
             if (nxt < 0) {
                 const unsigned long long fv = want | ((unsigned long long)claims << 16) | m;
-                __hip_atomic_store(a.ready + t.out0, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (t.out1 >= 0)
-                    __hip_atomic_store(a.ready + t.out1, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (t.out2 >= 0)
-                    __hip_atomic_store(a.ready + t.out2, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (t.out3 >= 0)
-                    __hip_atomic_store(a.ready + t.out3, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(a.ready + t.f0, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (t.f1 >= 0)
+                    __hip_atomic_store(a.ready + t.f1, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (t.f2 >= 0)
+                    __hip_atomic_store(a.ready + t.f2, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (t.f3 >= 0)
+                    __hip_atomic_store(a.ready + t.f3, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             // consumers in this run read the masks from the flags; the mask array serves
             // the solve and the other executors, after the kernel
@@ -531,12 +603,12 @@ __global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
 }
 
 // Levels / per-op executors: one workgroup per task of the launch.
-template <typename T, bool BF16, bool ACC, bool LLT, bool MASKS>
+template <typename T, bool BF16, bool ACC, bool LLT, bool MASKS, bool COH>
 __global__ __launch_bounds__(NT) void k_factor_batch(Args a) {
     __shared__ __attribute__((aligned(16))) char lds[lds_bytes<T>()];
     __shared__ uint32_t s_mask[NT / 64];
     const DevTask t = a.tasks[a.task_offset + blockIdx.x];
-    run_task<T, BF16, ACC, LLT, MASKS>(t, a, lds, s_mask, NoWait{a.mask});
+    run_task<T, BF16, ACC, LLT, MASKS, false, COH>(t, a, lds, s_mask, NoWait{a.mask});
     __syncthreads();
     if (threadIdx.x == 0) write_masks(t, a, task_mask<MASKS>(t, s_mask));
 }
@@ -608,6 +680,7 @@ struct InputCells {
 struct HostGraph {
     std::vector<Pair> pairs, op_pairs;
     std::vector<int32_t> deps, lp, lc, ls, up, uc, us;
+    std::vector<int32_t> kept;  // value ids that keep their slot to the end (slot reuse)
 };
 
 // Dynamic chaining of the persistent executor.  A workgroup that has just made a tile
@@ -1347,6 +1420,306 @@ int validate_graph(Context* c, const sogl_factor_desc* d, const Factor* f, const
     return SOGL_OK;
 }
 
+// Pool slots by liveness (the reference frees a tile after the last stage that reads it,
+// BlockPlanner.cpp:597-634, and recycles it through free lists, memutil.cpp:74-113).
+// Tasks are visited in their level order; a result slot is, in this order of choice:
+//  * the slot of an operand that the task alone reads and nothing needs later (a
+//    GEMM's added tile or operand, a sub's operand, an FP64 lu / llt source; an FP32 or
+//    bf16 lu may re-read its source for the FP64 refactorisation).  The task has read
+//    that operand in full before it stores any result element (GEMM: every operand load
+//    precedes the barrier before the last product, and the added tile is read by the
+//    lane that stores the same element; sub: the same thread reads then writes each
+//    element; lu / llt: the source is copied to LDS first), so no wait is needed;
+//  * for a GEMM, a slot freed REUSE_LEVEL_GAP or more levels earlier: the last value in
+//    it has no reader left at a later level.  The levels executors run those readers in
+//    earlier launches; the persistent executor makes the task wait for their flags
+//    before its first result store (Args::war, polled with the task's first window of
+//    pairs).  The gap keeps those waits from holding workgroups up: the persistent
+//    executor runs far ahead of the slowest chain on wide graphs (C2 FP32, 1 workgroup
+//    per CU: 82 ms with a gap of 2 levels, 64 with 1024, 63 without reuse), and costs
+//    little memory (C5 FP64: 13.34 GB at 2, 13.39 at 1024);
+//  * a new slot.
+// The factor tiles, the diagonal inverses and the zero tile keep their slots; so do the
+// inputs, unless they can be scattered again before each run.  The per-op executor
+// accumulates each unit op into the result's slot and runs only without reuse.
+// Values keep their own ready flags: the numbers used so far become value ids, and every
+// data and mask reference becomes a slot.  The executors of a factor with reuse read
+// tiles with coherent loads (COH in tile_ops.h): lines of a slot's earlier values may sit
+// in some CU's L1 or some XCD's L2.
+constexpr int REUSE_LEVEL_GAP = 1024;
+
+// value ids a task reads (deduplicated), from its slot fields before they become slots
+std::vector<int32_t> task_operands(const DevTask& k, const HostGraph& hg) {
+    std::vector<int32_t> u;
+    if (k.kind == TK_GEMM) {
+        if (k.add >= 0) u.push_back(k.add);
+        for (int32_t q = k.a; q < k.a + k.b; q++) {
+            u.push_back(pair_a(hg.pairs[(size_t)q]));
+            u.push_back(pair_b(hg.pairs[(size_t)q]));
+        }
+    } else if (k.kind == TK_SUB) {
+        if (k.a >= 0) u.push_back(k.a);
+        if (k.b >= 0) u.push_back(k.b);
+    } else if (k.a >= 0) {
+        u.push_back(k.a);
+    }
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    return u;
+}
+
+// the value id whose flag a task's publish raises last (a GEMM chained to its lu raises
+// none; the lu does, after it)
+int32_t done_flag(const Factor* f, int32_t t) {
+    const DevTask& k = f->tasks[(size_t)t];
+    return k.next >= 0 ? f->tasks[(size_t)k.next].f0 : k.f0;
+}
+
+void assign_slots(Factor* f, HostGraph& hg, bool reuse, bool donate_inputs) {
+    const int64_t V = f->n_slots;
+    const size_t T = f->tasks.size();
+    for (auto* pv : {&hg.pairs, &hg.op_pairs})
+        for (Pair& p : *pv) {
+            p.fa = pair_a(p);
+            p.fb = pair_b(p);
+        }
+    auto set_flags = [](DevTask& k) {
+        k.f0 = k.out0;
+        k.f1 = k.out1;
+        k.f2 = k.out2;
+        k.f3 = k.out3;
+        k.fadd = k.kind == TK_GEMM ? k.add : -1;
+        k.fsa = k.kind == TK_GEMM ? -1 : k.a;
+        k.fsb = k.kind == TK_SUB ? k.b : -1;
+        k.war_begin = k.war_count = 0;
+    };
+    for (DevTask& k : f->tasks) set_flags(k);
+    for (DevTask& k : f->op_tasks) set_flags(k);
+    f->war.clear();
+    f->reuse = reuse;
+    f->rescatter = reuse && donate_inputs && f->n_inputs > 0;
+    hg.kept.clear();
+    if (!reuse) {
+        f->n_locs = V;
+        return;
+    }
+    // readers of each value (tasks, in task order) and the values that keep their slot
+    std::vector<int64_t> rb((size_t)V + 1, 0);
+    std::vector<std::vector<int32_t>> ops(T);
+    for (size_t t = 0; t < T; t++) {
+        ops[t] = task_operands(f->tasks[t], hg);
+        for (int32_t v : ops[t]) rb[(size_t)v + 1]++;
+    }
+    for (int64_t v = 0; v < V; v++) rb[(size_t)v + 1] += rb[(size_t)v];
+    std::vector<int32_t> rl((size_t)rb[(size_t)V]);
+    {
+        std::vector<int64_t> fill(rb.begin(), rb.end() - 1);
+        for (size_t t = 0; t < T; t++)
+            for (int32_t v : ops[t]) rl[(size_t)fill[(size_t)v]++] = (int32_t)t;
+    }
+    auto nread = [&](int32_t v) { return rb[(size_t)v + 1] - rb[(size_t)v]; };
+    std::vector<char> keep((size_t)V, 0);
+    keep[(size_t)f->zero_slot] = 1;
+    for (const auto* pv : {&f->l_slots, &f->u_slots, &f->l_diag, &f->u_diag, &f->linv_slot, &f->uinv_slot, &hg.ls, &hg.us})
+        for (int32_t v : *pv) keep[(size_t)v] = 1;
+    if (!f->rescatter)
+        for (int32_t v : f->input_slots) keep[(size_t)v] = 1;
+    for (int64_t v = 0; v < V; v++)
+        if (keep[(size_t)v]) hg.kept.push_back((int32_t)v);
+    // inputs and the zero tile where they are, each the first value of its slot
+    std::vector<int32_t> loc((size_t)V, -1);
+    std::vector<char> first((size_t)V, 0), given((size_t)V, 0);
+    std::vector<int64_t> seen((size_t)V, 0);
+    int32_t next = 0;
+    for (int32_t v : f->input_slots) {
+        loc[(size_t)v] = v;
+        first[(size_t)v] = 1;
+        next = std::max(next, v + 1);
+    }
+    loc[(size_t)f->zero_slot] = f->zero_slot;
+    first[(size_t)f->zero_slot] = 1;
+    next = std::max(next, f->zero_slot + 1);
+    struct Freed {
+        int32_t slot, level, value;
+    };
+    std::deque<Freed> freed;  // in the order their last readers were visited (by level)
+    const bool src_ok = f->precision == SOGL_FP64;
+    std::vector<int32_t> war;
+    for (size_t t = 0; t < T; t++) {
+        DevTask& k = f->tasks[t];
+        // a GEMM chained to its lu / llt leaves its sum to that task without waiting for
+        // its stores: the sum must be its slot's first value, which the lu reads with
+        // plain loads through this CU's own caches (coherent loads could pass the stores,
+        // and no cache holds lines of an earlier value of a first value's slot)
+        const bool fresh = k.next >= 0;
+        const bool donates = !fresh && (k.kind == TK_GEMM || k.kind == TK_SUB ||
+                                        ((k.kind == TK_LU || k.kind == TK_LLT) && src_ok));
+        std::vector<int32_t> cand;
+        if (donates) {
+            if (k.kind == TK_GEMM && k.add >= 0) cand.push_back(k.add);  // update chains stay in one slot
+            for (int32_t v : ops[t]) cand.push_back(v);
+        }
+        war.clear();
+        for (int32_t o : {k.out0, k.out1, k.out2, k.out3}) {
+            if (o < 0 || loc[(size_t)o] >= 0) continue;
+            int32_t from = -1;
+            for (int32_t v : cand)
+                if (!keep[(size_t)v] && nread(v) == 1 && !given[(size_t)v] && loc[(size_t)v] >= 0) {
+                    from = v;
+                    break;
+                }
+            if (from >= 0) {
+                given[(size_t)from] = 1;
+                loc[(size_t)o] = loc[(size_t)from];
+            } else if (!fresh && k.kind == TK_GEMM && !freed.empty() &&
+                       freed.front().level <= k.level - REUSE_LEVEL_GAP) {
+                const Freed fr = freed.front();
+                freed.pop_front();
+                loc[(size_t)o] = fr.slot;
+                for (int64_t q = rb[(size_t)fr.value]; q < rb[(size_t)fr.value + 1]; q++)
+                    war.push_back(done_flag(f, rl[(size_t)q]));
+            } else {
+                loc[(size_t)o] = next++;
+                first[(size_t)o] = 1;
+            }
+        }
+        std::sort(war.begin(), war.end());
+        war.erase(std::unique(war.begin(), war.end()), war.end());
+        k.war_begin = (int32_t)f->war.size();
+        k.war_count = (int32_t)war.size();
+        f->war.insert(f->war.end(), war.begin(), war.end());
+        // operands whose last reader this is free their slot (unless kept or handed over)
+        for (int32_t v : ops[t])
+            if (++seen[(size_t)v] == nread(v) && !keep[(size_t)v] && !given[(size_t)v])
+                freed.push_back({loc[(size_t)v], k.level, v});
+    }
+    for (int64_t v = 0; v < V; v++)
+        if (loc[(size_t)v] < 0) loc[(size_t)v] = next++, first[(size_t)v] = 1;  // made and read by no task
+    f->n_locs = next;
+    // every data and mask reference to its slot
+    auto L = [&](int32_t v) { return v < 0 ? v : loc[(size_t)v]; };
+    for (size_t t = 0; t < T; t++) {
+        DevTask& k = f->tasks[t];
+        k.out0 = L(k.out0);
+        k.out1 = L(k.out1);
+        k.out2 = L(k.out2);
+        k.out3 = L(k.out3);
+        k.add = L(k.add);
+        if (k.kind != TK_GEMM) {
+            k.a = L(k.a);
+            k.b = L(k.b);
+        }
+    }
+    for (Pair& p : hg.pairs) {
+        p.a = p.a < 0 ? -L(p.fa) - 1 : L(p.fa);
+        p.b = p.b < 0 ? -L(p.fb) - 1 : L(p.fb);
+    }
+    for (auto* pv : {&f->l_slots, &f->u_slots, &f->l_diag, &f->u_diag, &f->linv_slot, &f->uinv_slot, &hg.ls, &hg.us})
+        for (int32_t& v : *pv) v = L(v);
+}
+
+// The slot assignment's invariants, checked on the host: consecutive values of one slot
+// either pass from a value to the task that alone reads it (and read it whole), or the
+// later value's task is a GEMM at a later level than every reader of the earlier value
+// and waits for all their flags; the kept values end their slot's sequence; a GEMM sum
+// that goes to a chained lu / llt is its slot's first value (the lu reads it with plain
+// loads, see run_task).
+int validate_slots(Context* c, const Factor* f, const HostGraph& hg) {
+    auto bad = [&](const std::string& m) { return set_error(c, SOGL_ERR_ARG, "slot reuse check: " + m); };
+    if (!f->reuse) return SOGL_OK;
+    const int64_t V = f->n_slots;
+    const size_t T = f->tasks.size();
+    std::vector<int32_t> loc((size_t)V, -1), prod((size_t)V, -1);
+    auto put = [&](int32_t v, int32_t l) {
+        if (v < 0) return true;
+        if (v >= V || l < 0 || l >= f->n_locs) return false;
+        if (loc[(size_t)v] >= 0 && loc[(size_t)v] != l) return false;
+        loc[(size_t)v] = l;
+        return true;
+    };
+    // each task's reads (value ids)
+    std::vector<std::vector<int32_t>> reads(T);
+    for (size_t t = 0; t < T; t++) {
+        const DevTask& k = f->tasks[t];
+        if (!put(k.f0, k.out0) || !put(k.f1, k.out1) || !put(k.f2, k.out2) || !put(k.f3, k.out3))
+            return bad("outputs of task " + std::to_string(t));
+        for (int32_t v : {k.f0, k.f1, k.f2, k.f3})
+            if (v >= 0) prod[(size_t)v] = (int32_t)t;
+        auto& r = reads[t];
+        if (k.kind == TK_GEMM) {
+            if (!put(k.fadd, k.add)) return bad("added tile of task " + std::to_string(t));
+            if (k.fadd >= 0) r.push_back(k.fadd);
+            for (int32_t q = k.a; q < k.a + k.b; q++) {
+                const Pair& p = hg.pairs[(size_t)q];
+                if (!put(p.fa, pair_a(p)) || !put(p.fb, pair_b(p)))
+                    return bad("pair of task " + std::to_string(t));
+                r.push_back(p.fa);
+                r.push_back(p.fb);
+            }
+        } else {
+            if (!put(k.fsa, k.a) || (k.kind == TK_SUB && !put(k.fsb, k.b)))
+                return bad("operands of task " + std::to_string(t));
+            if (k.fsa >= 0) r.push_back(k.fsa);
+            if (k.fsb >= 0) r.push_back(k.fsb);
+        }
+        if (k.war_begin < 0 || k.war_count < 0 || (size_t)k.war_begin + (size_t)k.war_count > f->war.size())
+            return bad("write-after-read range of task " + std::to_string(t));
+        for (int32_t q = k.war_begin; q < k.war_begin + k.war_count; q++)
+            if (f->war[(size_t)q] < 0 || f->war[(size_t)q] >= V) return bad("write-after-read flag of task " + std::to_string(t));
+    }
+    std::vector<std::vector<int32_t>> readers((size_t)V);
+    for (size_t t = 0; t < T; t++) {
+        std::vector<int32_t> u(reads[t]);
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        for (int32_t v : u) readers[(size_t)v].push_back((int32_t)t);
+    }
+    // occupants of each slot in visiting order (inputs and the zero tile first)
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> occ((size_t)f->n_locs);
+    for (int64_t v = 0; v < V; v++)
+        if (loc[(size_t)v] >= 0) occ[(size_t)loc[(size_t)v]].push_back({prod[(size_t)v], (int32_t)v});
+    std::vector<char> first((size_t)V, 1);
+    for (int64_t l = 0; l < f->n_locs; l++) {
+        auto& o = occ[(size_t)l];
+        std::sort(o.begin(), o.end());
+        for (size_t i = 0; i + 1 < o.size(); i++) {
+            const int32_t v = o[i].second, w = o[i + 1].second, pw = o[i + 1].first;
+            first[(size_t)w] = 0;
+            if (v == f->zero_slot) return bad("the zero tile's slot is taken over");
+            if (o[i].first < 0 && !f->rescatter) return bad("an input slot is taken over");
+            const auto& rv = readers[(size_t)v];
+            if (pw < 0) return bad("slot " + std::to_string(l) + " holds two values that no task makes");
+            const DevTask& kw = f->tasks[(size_t)pw];
+            if (rv.size() == 1 && rv[0] == pw) {  // handed over by its only reader
+                if (kw.kind == TK_GEMM || kw.kind == TK_SUB ||
+                    ((kw.kind == TK_LU || kw.kind == TK_LLT) && f->precision == SOGL_FP64))
+                    continue;
+                return bad("task " + std::to_string(pw) + " may read its source again after storing over it");
+            }
+            if (kw.kind != TK_GEMM)
+                return bad("slot " + std::to_string(l) + " passes to a non-GEMM task while another reads it");
+            std::vector<int32_t> wf(f->war.begin() + kw.war_begin, f->war.begin() + kw.war_begin + kw.war_count);
+            std::sort(wf.begin(), wf.end());
+            for (int32_t r : rv) {
+                if (f->tasks[(size_t)r].level >= kw.level)
+                    return bad("slot " + std::to_string(l) + " is reused at the level of a reader");
+                if (!std::binary_search(wf.begin(), wf.end(), done_flag(f, r)))
+                    return bad("task " + std::to_string(pw) + " does not wait for a reader of its slot");
+            }
+        }
+    }
+    for (int32_t v : hg.kept) {
+        const int32_t l = loc[(size_t)v];
+        if (l >= 0 && occ[(size_t)l].back().second != v) return bad("kept value " + std::to_string(v) + " is overwritten");
+    }
+    for (size_t t = 0; t < T; t++) {
+        const DevTask& k = f->tasks[t];
+        if (k.next >= 0 && !first[(size_t)k.f0])
+            return bad("the sum of GEMM task " + std::to_string(t) + " goes to its chained task through a reused slot");
+    }
+    return SOGL_OK;
+}
+
 // Device arrays of a checked graph: pool, tasks, pairs, dependencies, flags, solve
 // structure, and the input tiles copied in.
 int upload_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells, Factor* f,
@@ -1358,18 +1731,21 @@ int upload_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells,
     // device arrays
     int rc;
     const size_t tile_bytes = (size_t)TE * f->elem;
-    if ((rc = dev_alloc(c, f->pool, (size_t)f->n_slots * tile_bytes, "tile pool")) != SOGL_OK) return rc;
+    if ((rc = dev_alloc(c, f->pool, (size_t)f->n_locs * tile_bytes, "tile pool")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->tasks_d, std::max<size_t>(T, 1) * sizeof(DevTask), "tasks")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->pairs_d, std::max<size_t>(pairs.size(), 1) * sizeof(Pair), "pairs")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->deps_d, std::max<size_t>(deps.size(), 1) * 4, "deps")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->ready_d, (size_t)f->n_slots * 8, "ready flags")) != SOGL_OK) return rc;
-    if ((rc = dev_alloc(c, f->mask_d, (size_t)f->n_slots * 4, "block masks")) != SOGL_OK) return rc;
+    if ((rc = dev_alloc(c, f->mask_d, (size_t)f->n_locs * 4, "block masks")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->ctrl_d, CTRL_WORDS * 4, "control")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->op_tasks_d, std::max<size_t>(f->op_tasks.size(), 1) * sizeof(DevTask), "op tasks")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->op_pairs_d, std::max<size_t>(op_pairs.size(), 1) * sizeof(Pair), "op pairs")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->queue_d, std::max<size_t>(f->queue.size(), 1) * 4, "task queue")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->chk_d, std::max<size_t>(f->chk.size(), 1) * sizeof(ChainChk), "chain checks")) != SOGL_OK)
         return rc;
+    if ((rc = dev_alloc(c, f->war_d, std::max<size_t>(f->war.size(), 1) * 4, "write-after-read flags")) != SOGL_OK)
+        return rc;
+    if (!f->war.empty()) HIP_TRY(c, hipMemcpy(f->war_d.p, f->war.data(), f->war.size() * 4, hipMemcpyHostToDevice));
     auto up_i32 = [&](DevBuf& b, const std::vector<int32_t>& v, const char* what) -> int {
         int r2 = dev_alloc(c, b, std::max<size_t>(v.size(), 1) * 4, what);
         if (r2 != SOGL_OK) return r2;
@@ -1439,6 +1815,11 @@ int upload_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells,
         e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) return done(check_hip(c, e, "input cell scatter"));
+        if (f->rescatter) {  // kept for the scatter before every run
+            std::swap(f->cell_begin, cb);
+            std::swap(f->cell_pos, cp);
+            std::swap(f->cell_val, cv);
+        }
         done(SOGL_OK);
     } else if (nin) {
         if (f->elem == 8) {
@@ -1462,7 +1843,7 @@ int upload_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells,
     }
     // block masks: the inputs' from their values, every other slot (the zero tile
     // included) empty until its task writes it
-    hipLaunchKernelGGL(k_fill<int>, dim3(256), dim3(256), 0, st, (int*)f->mask_d.p, (size_t)f->n_slots, 0);
+    hipLaunchKernelGGL(k_fill<int>, dim3(256), dim3(256), 0, st, (int*)f->mask_d.p, (size_t)f->n_locs, 0);
     if (d->n_inputs > 0) {
         const unsigned grid = (unsigned)std::min<int64_t>(d->n_inputs, 65536);
         if (f->elem == 8)
@@ -1481,6 +1862,17 @@ int build_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells, 
     HostGraph hg;
     int rc = build_host_graph(c, d, f, hg);
     if (rc == SOGL_OK) rc = validate_graph(c, d, f, hg);
+    if (rc == SOGL_OK) {
+        f->n_inputs = d->n_inputs;
+        // auto: reuse when the values would fill more than a quarter of the device's HBM
+        // (reuse costs 2-7% of factor time in the coherent loads; C5 FP64, 35 GB of values,
+        // runs without it on a 288 GB device)
+        const double value_bytes = (double)f->n_slots * TE * f->elem;
+        const bool reuse = c->pool_reuse == SOGL_REUSE_ON ||
+                           (c->pool_reuse == SOGL_REUSE_AUTO && value_bytes > 0.25 * (double)c->total_mem);
+        assign_slots(f, hg, reuse, cells != nullptr);
+        rc = validate_slots(c, f, hg);
+    }
     if (rc == SOGL_OK) rc = upload_graph(c, d, cells, f, hg);
     return rc;
 }
@@ -1488,7 +1880,7 @@ int build_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells, 
 void free_factor(Factor* f) {
     if (f->graph_exec) (void)hipGraphExecDestroy(f->graph_exec);
     if (f->graph) (void)hipGraphDestroy(f->graph);
-    for (DevBuf* b : {&f->pool, &f->mask_d, &f->queue_d, &f->chk_d, &f->tasks_d, &f->pairs_d, &f->deps_d, &f->ready_d, &f->ctrl_d,
+    for (DevBuf* b : {&f->pool, &f->mask_d, &f->queue_d, &f->war_d, &f->cell_begin, &f->cell_pos, &f->cell_val, &f->chk_d, &f->tasks_d, &f->pairs_d, &f->deps_d, &f->ready_d, &f->ctrl_d,
                       &f->op_tasks_d, &f->op_pairs_d, &f->inputs_d, &f->l_rowptr, &f->l_col,
                       &f->l_slot, &f->u_rowptr, &f->u_col, &f->u_slot, &f->l_diag_d, &f->u_diag_d,
                       &f->linv_d, &f->uinv_d, &f->solve_ready, &f->solve_ctrl, &f->vec_b, &f->vec_x,
@@ -1497,7 +1889,7 @@ void free_factor(Factor* f) {
     delete f;
 }
 
-template <typename T, bool BF16, bool LLT, bool MASKS>
+template <typename T, bool BF16, bool LLT, bool MASKS, bool COH>
 int launch_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
     hipStream_t st = c->stream;
     Args a{};
@@ -1511,6 +1903,7 @@ int launch_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
     a.deps = (const int32_t*)f->deps_d.p;
     a.trace = (mode == SOGL_EXEC_PERSISTENT && f->trace_on) ? (long long*)f->trace_d.p : nullptr;
     a.chk = (const ChainChk*)f->chk_d.p;
+    a.war = (const int32_t*)f->war_d.p;
     a.chain = (mode == SOGL_EXEC_PERSISTENT && !f->chk.empty()) ? f->chain_claims : 0;
     auto launch_levels = [&]() {
         a.tasks = (const DevTask*)f->tasks_d.p;
@@ -1520,7 +1913,7 @@ int launch_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
             int cnt = f->level_begin[v + 1] - f->level_begin[v];
             if (cnt <= 0) continue;
             a.task_offset = f->level_begin[v];
-            hipLaunchKernelGGL((k_factor_batch<T, BF16, false, LLT, MASKS>), dim3(cnt), dim3(NT), 0, st, a);
+            hipLaunchKernelGGL((k_factor_batch<T, BF16, false, LLT, MASKS, COH>), dim3(cnt), dim3(NT), 0, st, a);
         }
     };
     if (mode == SOGL_EXEC_GRAPH && !f->graph_exec) {
@@ -1535,6 +1928,22 @@ int launch_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
         f->graph = g;
         HIP_TRY(c, hipGraphInstantiate(&f->graph_exec, g, nullptr, nullptr, 0));
     }
+    if (mode == SOGL_EXEC_PER_OP && f->reuse)
+        return set_error(c, SOGL_ERR_UNSUPPORTED,
+                         "the per-op executor needs a factor loaded with pool slot reuse off "
+                         "(sogl_ctx_set_pool_reuse)");
+    if (f->rescatter) {
+        // input slots were taken over by the last run: scatter the inputs again, with
+        // their masks and always-ready flags (the executors' own time excludes this)
+        const int64_t ntl = f->n_inputs;
+        const unsigned grid = (unsigned)std::min<int64_t>(ntl, 65536);
+        hipLaunchKernelGGL(k_fill<T>, dim3(4096), dim3(256), 0, st, (T*)f->pool.p, (size_t)ntl * TE, T(0));
+        hipLaunchKernelGGL(k_scatter_cells<T>, dim3(grid), dim3(256), 0, st, (T*)f->pool.p,
+                           (const int64_t*)f->cell_begin.p, (const uint16_t*)f->cell_pos.p,
+                           (const double*)f->cell_val.p, ntl);
+        hipLaunchKernelGGL(k_tile_masks<T>, dim3(grid), dim3(NT), 0, st, (const T*)f->pool.p,
+                           (uint32_t*)f->mask_d.p, (unsigned long long*)f->ready_d.p, ntl);
+    }
     if (mode != SOGL_EXEC_GRAPH) hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, st, a.ctrl);
     HIP_TRY(c, hipEventRecord(c->ex0, st));  // executor alone, for kernel-level rates
     if (mode == SOGL_EXEC_PERSISTENT) {
@@ -1545,7 +1954,8 @@ int launch_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
         a.nqueue = (int32_t)f->queue.size();
         // one workgroup per CU: the kernel built for that occupancy
         const bool one = wg_per_cu == 1;
-        auto kern = one ? k_factor_persistent<T, BF16, LLT, MASKS, true> : k_factor_persistent<T, BF16, LLT, MASKS, false>;
+        auto kern = one ? k_factor_persistent<T, BF16, LLT, MASKS, true, COH>
+                        : k_factor_persistent<T, BF16, LLT, MASKS, false, COH>;
         int per_cu = 0;
         HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, 0));
         if (per_cu < 1) per_cu = 1;
@@ -1569,7 +1979,8 @@ int launch_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
         a.ntasks = (int32_t)f->op_tasks.size();
         for (int32_t k = 0; k < a.ntasks; k++) {
             a.task_offset = k;
-            hipLaunchKernelGGL((k_factor_batch<T, false, true, LLT, MASKS>), dim3(1), dim3(NT), 0, st, a);
+            if constexpr (!COH)
+                hipLaunchKernelGGL((k_factor_batch<T, false, true, LLT, MASKS, false>), dim3(1), dim3(NT), 0, st, a);
         }
     } else {
         return set_error(c, SOGL_ERR_ARG, "unknown executor mode");
@@ -1607,7 +2018,8 @@ extern "C" int sogl_ctx_create(int32_t device, sogl_ctx** out) {
         (rc = check_hip(c, hipEventCreate(&c->ev1), "event")) ||
         (rc = check_hip(c, hipEventCreate(&c->ex0), "event")) ||
         (rc = check_hip(c, hipEventCreate(&c->ex1), "event")) ||
-        (rc = check_hip(c, hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device), "attr"))) {
+        (rc = check_hip(c, hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device), "attr")) ||
+        (rc = check_hip(c, hipMemGetInfo(&c->free_mem, &c->total_mem), "memory info"))) {
         delete x;
         return rc;
     }
@@ -1681,33 +2093,55 @@ extern "C" int sogl_factor_load(sogl_ctx* x, const sogl_factor_desc* d, int32_t*
     return load_factor(x, d, nullptr, factor_id);
 }
 
-extern "C" int sogl_graph_check(const sogl_factor_desc* d, int64_t* info) {
+namespace {
+// cells: the inputs come as the plan's cells (they can be scattered again before each
+// run, so their slots may be taken over)
+int graph_check(const sogl_factor_desc* d, bool cells, int64_t* info) {
     if (!d || d->n_ids <= 0 || d->n_ops < 0 || d->n_inputs < 0 || d->block_rows <= 0 ||
         (d->n_ops && !d->ops) || (d->n_inputs && !d->input_ids))
         return set_error(nullptr, SOGL_ERR_ARG, "bad factor description");
     Factor f;
     f.precision = d->precision;
     f.elem = d->precision == SOGL_FP64 ? 8 : 4;
+    f.n_inputs = d->n_inputs;
     HostGraph hg;
     int rc = build_host_graph(nullptr, d, &f, hg);
     if (rc == SOGL_OK) rc = validate_graph(nullptr, d, &f, hg);
+    const int64_t values = f.n_slots;
+    if (rc == SOGL_OK) {
+        assign_slots(&f, hg, true, cells);
+        rc = validate_slots(nullptr, &f, hg);
+    }
     if (rc == SOGL_OK && info) {
         info[0] = (int64_t)f.tasks.size();
-        info[1] = f.n_slots;
+        info[1] = values;
         info[2] = (int64_t)f.level_begin.size() - 1;
         info[3] = (int64_t)hg.pairs.size();
         info[4] = (int64_t)hg.deps.size();
         info[5] = f.n_folded;
         info[6] = f.n_inv_aliased;
         info[7] = f.n_chained;
+        info[8] = f.n_locs;
     }
     return rc;
 }
+}  // namespace
+
+extern "C" int sogl_graph_check(const sogl_factor_desc* d, int64_t* info) { return graph_check(d, false, info); }
 
 extern "C" int sogl_graph_check_plan(const sogl_plan* plan, int32_t precision, int64_t* info) {
     sogl_factor_desc d;
     if (!sogl::plan_factor_desc(plan, precision, &d)) return set_error(nullptr, SOGL_ERR_ARG, "null plan");
-    return sogl_graph_check(&d, info);
+    return graph_check(&d, true, info);
+}
+
+extern "C" int sogl_ctx_set_pool_reuse(sogl_ctx* x, int32_t mode) {
+    if (!x) return set_error(nullptr, SOGL_ERR_ARG, "null context");
+    if (mode != SOGL_REUSE_OFF && mode != SOGL_REUSE_ON && mode != SOGL_REUSE_AUTO)
+        return set_error(&x->c, SOGL_ERR_ARG, "unknown pool reuse mode");
+    std::lock_guard<std::mutex> g(x->c.mu);
+    x->c.pool_reuse = mode;
+    return SOGL_OK;
 }
 
 extern "C" int sogl_factor_load_plan(sogl_ctx* x, const sogl_plan* plan, int32_t precision,
@@ -1737,11 +2171,14 @@ int run_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
         f->graph = nullptr;
     }
     f->graph_masks = masks;
-    auto go = [&](auto llt_c, auto masks_c) {
-        constexpr bool L = decltype(llt_c)::value, M = decltype(masks_c)::value;
-        if (f->precision == SOGL_FP64) return launch_factor<double, false, L, M>(c, f, mode, chunk, wg_per_cu);
-        if (f->precision == SOGL_FP32) return launch_factor<float, false, L, M>(c, f, mode, chunk, wg_per_cu);
-        return launch_factor<float, true, L, M>(c, f, mode, chunk, wg_per_cu);
+    auto go2 = [&](auto llt_c, auto masks_c, auto coh_c) {
+        constexpr bool L = decltype(llt_c)::value, M = decltype(masks_c)::value, H = decltype(coh_c)::value;
+        if (f->precision == SOGL_FP64) return launch_factor<double, false, L, M, H>(c, f, mode, chunk, wg_per_cu);
+        if (f->precision == SOGL_FP32) return launch_factor<float, false, L, M, H>(c, f, mode, chunk, wg_per_cu);
+        return launch_factor<float, true, L, M, H>(c, f, mode, chunk, wg_per_cu);
+    };
+    auto go = [&](auto llt_c, auto masks_c) {  // factors with slot reuse: the coherent-load kernels
+        return f->reuse ? go2(llt_c, masks_c, std::true_type{}) : go2(llt_c, masks_c, std::false_type{});
     };
     using TT = std::true_type;
     using FF = std::false_type;
@@ -1767,7 +2204,7 @@ int run_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
     if (ctrl[1]) return set_error(c, SOGL_ERR_TIMEOUT, "factor: a dependency wait exceeded its bound");
     if (masks) {
         // the share of the factor's 16x16 blocks that count as zero
-        std::vector<uint32_t> m((size_t)f->n_slots);
+        std::vector<uint32_t> m((size_t)f->n_locs);
         HIP_TRY(c, hipMemcpy(m.data(), f->mask_d.p, m.size() * 4, hipMemcpyDeviceToHost));
         int64_t zero = 0, total = 0;
         for (const auto* v : {&f->l_slots, &f->u_slots})
@@ -1895,6 +2332,8 @@ extern "C" int sogl_factor_get_stats(sogl_ctx* x, int32_t factor_id, sogl_factor
     st->claimed_tasks = f->claimed;
     st->dropped_tasks = f->dropped;
     st->tasks_run = f->ran;
+    st->pool_tiles = f->n_locs;
+    st->value_tiles = f->n_slots;
     return SOGL_OK;
 }
 
@@ -1998,7 +2437,7 @@ extern "C" int sogl_factor_get_graph(sogl_ctx* x, int32_t factor_id, int64_t* n_
         std::fill(slot_task, slot_task + f->n_slots, -1);
         for (size_t t = 0; t < f->tasks.size(); t++) {
             const DevTask& k = f->tasks[t];
-            for (int32_t o : {k.out0, k.out1, k.out2, k.out3})
+            for (int32_t o : {k.f0, k.f1, k.f2, k.f3})
                 if (o >= 0 && o < f->n_slots) slot_task[o] = (int32_t)t;
         }
     }

@@ -85,33 +85,79 @@ __device__ __forceinline__ void gst(T* p, T v) {
 // barrier follows it with no store in between.
 __device__ __forceinline__ void wave_stores_done() { asm volatile("s_waitcnt vmcnt(0) ; publish: stores done" ::: "memory"); }
 
+// Tile reads in a factor that reuses pool slots (COH): a CU's L1 or an XCD's L2 may
+// still hold lines of a slot's earlier value from the same run, so every tile read goes
+// to the device coherence point -- sc1 buffer loads, the form an agent-scope relaxed
+// atomic load takes, but 16 bytes per lane.  Without reuse every slot is written once
+// per run and read only after its flag, and each kernel starts with its caches
+// invalidated, so plain loads read the right bytes (and hit in L2 when they can).  The
+// form is fixed per kernel: a choice per operand inside the pair loop cost registers
+// (measured: spills in the kernels built for two workgroups per CU).
+constexpr int CPOL_SC1 = 16;  // the sc1 bit of a buffer instruction's cache policy
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const void* tile) {
+    // raw buffer over one tile (at most 64 x 64 doubles); dword 3: 32-bit data format
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(tile), 0, TE * 8, 0x00020000);
+}
+template <bool COH, typename T>
+__device__ __forceinline__ T ld1c(const T* tile, int i) {  // element i
+    if constexpr (!COH) {
+        return tile[i];
+    } else if constexpr (sizeof(T) == 8) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(tile_rsrc(tile), i * 8, 0, CPOL_SC1));
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(tile_rsrc(tile), i * 4, 0, CPOL_SC1));
+    }
+}
+// 16-byte vector at element i (16-byte aligned)
+template <bool COH, typename V, typename T>
+__device__ __forceinline__ V ld16c(const T* tile, int i) {
+    static_assert(sizeof(V) == 16, "16-byte vectors");
+    if constexpr (!COH)
+        return *reinterpret_cast<const V*>(tile + i);
+    else
+        return __builtin_bit_cast(
+            V, __builtin_amdgcn_raw_buffer_load_b128(tile_rsrc(tile), i * (int)sizeof(T), 0, CPOL_SC1));
+}
+// 16 consecutive elements from element i (a multiple of 4), converted to C
+template <bool COH, typename T, typename C>
+__device__ __forceinline__ void load16c(const T* tile, int i, C (&v)[16]) {
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float4 x = ld16c<COH, float4>(tile, i + 4 * q);
+            v[4 * q] = C(x.x);
+            v[4 * q + 1] = C(x.y);
+            v[4 * q + 2] = C(x.z);
+            v[4 * q + 3] = C(x.w);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const double2 x = ld16c<COH, double2>(tile, i + 2 * q);
+            v[2 * q] = C(x.x);
+            v[2 * q + 1] = C(x.y);
+        }
+    }
+}
+template <typename T, typename C>
+__device__ __forceinline__ void load16(const T* tile, int i, bool coh, C (&v)[16]) {
+    if (coh)
+        load16c<true>(tile, i, v);
+    else
+        load16c<false>(tile, i, v);
+}
+
 // ------------------------------------------------------------------ loads
 // thread t moves row t/4, columns (t%4)*16 .. +15 of a tile between global and LDS
 // (that segment lies in 16x16 block (t/64, t%4))
-template <typename T>
+template <bool COH = false, typename T>
 __device__ inline void tile_to_lds(const T* __restrict__ g, T* __restrict__ s, int ld, int tid) {
     const int r = tid >> 2, c0 = (tid & 3) * 16;
-    const T* src = g + r * TS + c0;
+    T v[16];
+    load16c<COH>(g, r * TS + c0, v);
     T* dst = s + r * ld + c0;
-    if constexpr (sizeof(T) == 8) {
-        const double2* v = reinterpret_cast<const double2*>(src);
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            double2 x = v[q];
-            dst[2 * q] = x.x;
-            dst[2 * q + 1] = x.y;
-        }
-    } else {
-        const float4* v = reinterpret_cast<const float4*>(src);
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            float4 x = v[q];
-            dst[4 * q] = x.x;
-            dst[4 * q + 1] = x.y;
-            dst[4 * q + 2] = x.z;
-            dst[4 * q + 3] = x.w;
-        }
-    }
+    for (int q = 0; q < 16; q++) dst[q] = v[q];
 }
 
 // One thread's segment of a tile (the same row segment as tile_to_lds) held in
@@ -128,14 +174,14 @@ struct Seg<double> {
     double2 v[8];
 };
 // on = false: the segment's 16x16 block is zero by the tile's mask and is not loaded
-template <typename T>
+template <bool COH = false, typename T>
 __device__ __forceinline__ void seg_load(const T* __restrict__ g, Seg<T>& s, int tid, bool on) {
     if (!on) return;
     const int r = tid >> 2, c0 = (tid & 3) * 16;
     typedef typename std::remove_reference<decltype(s.v[0])>::type V;
-    const V* src = reinterpret_cast<const V*>(g + r * TS + c0);
+    constexpr int NV = (int)(sizeof(s.v) / sizeof(V));
 #pragma unroll
-    for (int q = 0; q < (int)(sizeof(s.v) / sizeof(V)); q++) s.v[q] = src[q];
+    for (int q = 0; q < NV; q++) s.v[q] = ld16c<COH, V>(g, r * TS + c0 + q * (16 / NV));
 }
 // on = false: write zeros (the block is zero by the tile's mask)
 template <typename T>
@@ -167,11 +213,11 @@ __device__ __forceinline__ void seg_store(const Seg<T>& s, T* __restrict__ lds, 
 __device__ __forceinline__ bool seg_on(uint32_t m, int tid) { return (m >> (4 * (tid >> 6) + (tid & 3))) & 1u; }
 
 // The same with the blocks whose mask bit is clear written as zeros, not loaded.
-template <typename T>
+template <bool COH = false, typename T>
 __device__ inline void tile_to_lds_masked(const T* __restrict__ g, T* __restrict__ s, int ld, int tid, uint32_t m) {
     const int r = tid >> 2, c0 = (tid & 3) * 16;
     if ((m >> (4 * (r >> 4) + (tid & 3))) & 1u) {
-        tile_to_lds(g, s, ld, tid);
+        tile_to_lds<COH>(g, s, ld, tid);
     } else {
         T* dst = s + r * ld + c0;
 #pragma unroll
@@ -223,10 +269,15 @@ struct MfmaOp<float> {
     static __device__ inline int row(int g, int r) { return 4 * g + r; }
 };
 
-// One (A,B) pair of a GEMM task.  b < 0 encodes a subtracted product (mulneg):
-// slot = -b - 1; a < 0 encodes a transposed second operand (A B^T): slot = -a - 1.
+// One (A,B) pair of a GEMM task.  a, b: the operands' pool slots (tile data and block
+// masks); b < 0 encodes a subtracted product (mulneg): slot = -b - 1; a < 0 encodes a
+// transposed second operand (A B^T): slot = -a - 1.  fa, fb: the operands' value ids,
+// which index the ready flags.  A slot holds several values over a factor run when the
+// pool reuses slots (a value's slot taken over by the task that alone reads it); each
+// value keeps a flag of its own.
 struct Pair {
     int32_t a, b;
+    int32_t fa, fb;
 };
 __host__ __device__ inline int pair_a(const Pair& p) { return p.a < 0 ? -p.a - 1 : p.a; }
 __host__ __device__ inline int pair_b(const Pair& p) { return p.b < 0 ? -p.b - 1 : p.b; }
@@ -282,23 +333,27 @@ __device__ __forceinline__ uint32_t quad_mask(const bool (&nz)[2][2], int I0, in
 // Wave w computes the 32x32 quadrant (w/2, w%2) as 2x2 blocks of 16x16 MFMA
 // outputs; v_mfma_*_16x16x4 lane layout: A[i][k] in lane i + 16k, B[k][j] in
 // lane j + 16k, D[i][j] in lane j + 16g with (g, r) given by MfmaOp<T>::row.
-// A GEMM task may start before all of its pairs' operands exist: `wait(s, a, b, ...)` is
-// called by every thread before pair (a, b) is read and returns once both tiles
-// are ready (it includes the barrier that precedes the operand loads).
+// A GEMM task may start before all of its pairs' operands exist: `wait(s, fa, fb, a, b,
+// ...)` is called by every thread before the tiles in slots a, b (value ids fa, fb) are
+// read and returns once both are ready (it includes the barrier that precedes the
+// operand loads).
 // The wait also hands every thread the block masks of both tiles (`ma`, `mb`); `s` is
 // the pair's index in the task (the add tile comes last).  NoWait: all operands exist
 // already (level / per-op launches), the masks come from the mask array.
-// The pair loop asks in windows: `extend(pairs, s, npairs, window)` (every thread, a
-// barrier inside; window counts the calls) returns e > s such that pairs s .. e-1 are
-// all ready, and `pair_masks(j, p, ...)` then gives pair j's masks for j < e.
+// The pair loop asks in windows: `extend(pairs, s, npairs, window, fadd)` (every thread,
+// a barrier inside; window counts the calls) returns e > s such that pairs s .. e-1 are
+// all ready, and `pair_masks(j, p, ...)` then gives pair j's masks for j < e.  The first
+// window also looks at the flag of the added tile (value id fadd): after it,
+// `add_ready(add, m)` says whether that tile (slot add) could be read from then on
+// (every thread gets the same answer).
 struct NoWait {
     const uint32_t* mask;
-    __device__ inline void operator()(int, int sa, int sb, uint32_t& ma, uint32_t& mb) const {
+    __device__ inline void operator()(int, int, int, int sa, int sb, uint32_t& ma, uint32_t& mb) const {
         __syncthreads();
         ma = mask[sa];
         mb = mask[sb];
     }
-    __device__ inline int extend(const Pair*, int, int npairs, int) const {
+    __device__ inline int extend(const Pair*, int, int npairs, int, int = -1) const {
         __syncthreads();
         return npairs;
     }
@@ -306,12 +361,22 @@ struct NoWait {
         ma = mask[pair_a(p)];
         mb = mask[pair_b(p)];
     }
+    __device__ inline bool add_ready(int add, uint32_t& m) const {
+        m = mask[add];
+        return true;
+    }
+    // a result slot that held another value: its readers ran at earlier levels, i.e. in
+    // earlier launches
+    __device__ inline void war_done(bool) const {}
 };
 
 // A task made by folding a later sub/neg into the GEMM adds (or subtracts) one more
-// tile, `add`, to the sum: it is waited for after the pairs and read straight into
-// the accumulators.
-template <typename T, typename acc_t>
+// tile, `add`, to the sum, after the last product (the same order in every executor).
+// Its flag is polled with the first window of pairs; a tile that is ready by then is
+// never waited for again (STAGE64 loads it at once, into registers held across the pair
+// loop; the other kernels, which lack the registers, read it after the loop); one that
+// is not is waited for after the pairs.
+template <bool COH, typename T, typename acc_t>
 __device__ __forceinline__ void add_tile(acc_t (&acc)[2][2], const T* __restrict__ X, bool neg, int wr, int wc,
                                          int li, int lk) {
 #pragma unroll
@@ -320,7 +385,7 @@ __device__ __forceinline__ void add_tile(acc_t (&acc)[2][2], const T* __restrict
         for (int n = 0; n < 2; n++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const T x = X[(wr + m * 16 + MfmaOp<T>::row(lk, r)) * TS + wc + n * 16 + li];
+                const T x = ld1c<COH>(X, (wr + m * 16 + MfmaOp<T>::row(lk, r)) * TS + wc + n * 16 + li);
                 acc[m][n][r] += neg ? -x : x;
             }
 }
@@ -330,13 +395,13 @@ __device__ __forceinline__ void add_tile(acc_t (&acc)[2][2], const T* __restrict
 // MASKS: skip zero products and zero k blocks by the operands' masks, and leave the
 // mask bits of wave w's quadrant of the result in out_mask[w]; without, every pair is
 // multiplied in full and nothing is left (the result counts as full).
-// STAGE64: stage FP64 operands too (the kernel built for one workgroup per CU has the
-// registers for it).
+// STAGE64: stage FP64 operands too, and hold an added tile that is ready early in
+// registers (the kernel built for one workgroup per CU has the registers for it).
 template <typename T, bool ACC = false, bool TBOK = false, bool MASKS = true, typename Wait = NoWait,
-          bool STAGE64 = false>
+          bool STAGE64 = false, bool COH = false>
 __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, const Pair* __restrict__ pairs, int npairs,
                           int out, char* lds_raw, uint32_t* out_mask, const Wait& wait, int add = -1,
-                          bool add_neg = false) {
+                          bool add_neg = false, int fadd = -1) {
     typedef typename MfmaOp<T>::acc_t acc_t;
     T* la = reinterpret_cast<T*>(lds_raw);
     T* lb = la + TS * LDA;
@@ -372,6 +437,10 @@ __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, c
     Seg<T> ga, gb;
     int ready = 0, staged = -1, window = 0;
     uint32_t sma = 0, smb = 0;
+    // the added tile: 0 none or not seen yet, 1 ready (read after the loop), 2 in xr
+    int add_state = 0;
+    T xr[2][2][4];
+    const T* X = pool + (size_t)(add >= 0 ? add : 0) * TE;
     for (int s = 0; s < npairs; s++) {
         const Pair p = pairs[s];
         const bool neg = p.b < 0, tb = TBOK && p.a < 0;
@@ -379,14 +448,29 @@ __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, c
         bool synced = false;
         if (s != staged) {
             if (s == ready) {
-                ready = wait.extend(pairs, s, npairs, window++);  // its barrier also frees LDS
+                ready = wait.extend(pairs, s, npairs, window++, fadd);  // its barrier also frees LDS
                 if (ready < 0) return;  // another workgroup took the task over (wave-uniform)
                 synced = true;
+                uint32_t mx;
+                if (s == 0 && add >= 0 && wait.add_ready(add, mx)) {
+                    add_state = 1;
+                    if constexpr (STAGE64) {  // 16 (FP32) or 32 (FP64) more live registers
+#pragma unroll
+                        for (int m = 0; m < 2; m++)
+#pragma unroll
+                            for (int n = 0; n < 2; n++)
+#pragma unroll
+                                for (int r = 0; r < 4; r++)
+                                    xr[m][n][r] =
+                                        ld1c<COH>(X, (wr + m * 16 + MfmaOp<T>::row(lk, r)) * TS + wc + n * 16 + li);
+                        add_state = 2;
+                    }
+                }
             }
             wait.pair_masks(s, p, ma, mbs);
             if (zero_product(p, ma, mbs)) continue;  // every thread skips
-            seg_load(pool + (size_t)pair_a(p) * TE, ga, tid, !MASKS || seg_on(ma, tid));
-            seg_load(pool + (size_t)pair_b(p) * TE, gb, tid, !MASKS || seg_on(mbs, tid));
+            seg_load<COH>(pool + (size_t)pair_a(p) * TE, ga, tid, !MASKS || seg_on(ma, tid));
+            seg_load<COH>(pool + (size_t)pair_b(p) * TE, gb, tid, !MASKS || seg_on(mbs, tid));
         } else {
             ma = sma;
             mbs = smb;
@@ -400,8 +484,8 @@ __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, c
             uint32_t qa, qb;
             wait.pair_masks(n, q, qa, qb);
             if (zero_product(q, qa, qb)) continue;
-            seg_load(pool + (size_t)pair_a(q) * TE, ga, tid, !MASKS || seg_on(qa, tid));
-            seg_load(pool + (size_t)pair_b(q) * TE, gb, tid, !MASKS || seg_on(qb, tid));
+            seg_load<COH>(pool + (size_t)pair_a(q) * TE, ga, tid, !MASKS || seg_on(qa, tid));
+            seg_load<COH>(pool + (size_t)pair_b(q) * TE, gb, tid, !MASKS || seg_on(qb, tid));
             staged = n;
             sma = qa;
             smb = qb;
@@ -414,11 +498,21 @@ __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, c
         if (tb) gemm_lds<T, TBOK>(acc, la, lb, sg, wr, wc, li, lk, kuse);
         else gemm_lds<T, false>(acc, la, lb, sg, wr, wc, li, lk, kuse);
     }
-    if (add >= 0) {
-        uint32_t m0, m1;
-        wait(npairs, add, add, m0, m1);
-        add_tile<T>(acc, pool + (size_t)add * TE, add_neg, wr, wc, li, lk);
+    if (add_state == 2) {
+#pragma unroll
+        for (int m = 0; m < 2; m++)
+#pragma unroll
+            for (int n = 0; n < 2; n++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) acc[m][n][r] += add_neg ? -xr[m][n][r] : xr[m][n][r];
+    } else if (add >= 0) {
+        if (add_state == 0) {
+            uint32_t m0, m1;
+            wait(npairs, fadd, fadd, add, add, m0, m1);
+        }
+        add_tile<COH, T>(acc, X, add_neg, wr, wc, li, lk);
     }
+    wait.war_done(true);
     T* C = pool + (size_t)out * TE;
     bool nz[2][2] = {{false, false}, {false, false}};
 #pragma unroll
@@ -439,10 +533,10 @@ __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, c
 
 // bf16 experiment: FP32 tiles, operands rounded to bf16, FP32 accumulation
 // (v_mfma_f32_16x16x32_bf16: A[i][8q..8q+7] in lane i + 16q, same D layout).
-template <bool TBOK = false, bool MASKS = true, typename Wait = NoWait>
+template <bool TBOK = false, bool MASKS = true, typename Wait = NoWait, bool COH = false>
 __device__ __attribute__((always_inline)) void gemm_task_bf16(float* __restrict__ pool, const Pair* __restrict__ pairs,
                                       int npairs, int out, char* lds_raw, uint32_t* out_mask, const Wait& wait,
-                                      int add = -1, bool add_neg = false) {
+                                      int add = -1, bool add_neg = false, int fadd = -1) {
     float* la = reinterpret_cast<float*>(lds_raw);
     float* lb = la + TS * LDA;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -458,14 +552,14 @@ __device__ __attribute__((always_inline)) void gemm_task_bf16(float* __restrict_
         const bool neg = p.b < 0, tb = p.a < 0;
         const int sa = pair_a(p), sb = pair_b(p);
         uint32_t ma, mbs;
-        wait(s, sa, sb, ma, mbs);
+        wait(s, p.fa, p.fb, sa, sb, ma, mbs);
         if constexpr (MASKS) {
             if (ma == 0 || mbs == 0) continue;  // a zero product: every thread skips
-            tile_to_lds_masked(pool + (size_t)sa * TE, la, LDA, tid, ma);
-            tile_to_lds_masked(pool + (size_t)sb * TE, lb, TBOK && tb ? LDA : LDB, tid, mbs);
+            tile_to_lds_masked<COH>(pool + (size_t)sa * TE, la, LDA, tid, ma);
+            tile_to_lds_masked<COH>(pool + (size_t)sb * TE, lb, TBOK && tb ? LDA : LDB, tid, mbs);
         } else {
-            tile_to_lds(pool + (size_t)sa * TE, la, LDA, tid);
-            tile_to_lds(pool + (size_t)sb * TE, lb, TBOK && tb ? LDA : LDB, tid);
+            tile_to_lds<COH>(pool + (size_t)sa * TE, la, LDA, tid);
+            tile_to_lds<COH>(pool + (size_t)sb * TE, lb, TBOK && tb ? LDA : LDB, tid);
         }
         __syncthreads();
         const float sg = neg ? -1.f : 1.f;
@@ -493,9 +587,10 @@ __device__ __attribute__((always_inline)) void gemm_task_bf16(float* __restrict_
     }
     if (add >= 0) {  // the bf16 kernel's accumulator rows follow the f32 layout
         uint32_t m0, m1;
-        wait(npairs, add, add, m0, m1);
-        add_tile<float>(acc, pool + (size_t)add * TE, add_neg, wr, wc, li, lk);
+        wait(npairs, fadd, fadd, add, add, m0, m1);
+        add_tile<COH, float>(acc, pool + (size_t)add * TE, add_neg, wr, wc, li, lk);
     }
+    wait.war_done(false);
     float* C = pool + (size_t)out * TE;
     bool nz[2][2] = {{false, false}, {false, false}};
 #pragma unroll
@@ -539,7 +634,7 @@ __device__ inline uint32_t tile_mask(const T* __restrict__ g, int tid) {
 // but the compiler cannot know that).
 // Wave w leaves the mask bits of the rows it wrote in out_mask[w] (element q*256 + tid
 // is row 4q + w, column tid % 64).
-template <typename T, bool MASKS = true>
+template <typename T, bool MASKS = true, bool COH = false>
 __device__ __attribute__((always_inline)) void sub_task(T* __restrict__ pool, int src, int src2, int out,
                                                         uint32_t* out_mask) {
     const int tid = threadIdx.x, lane = tid & 63;
@@ -549,8 +644,8 @@ __device__ __attribute__((always_inline)) void sub_task(T* __restrict__ pool, in
     const T* B = src >= 0 ? pool + (size_t)src * TE : nullptr;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-        a[q] = A ? A[q * NT + tid] : T(0);
-        b[q] = B ? B[q * NT + tid] : T(0);
+        a[q] = A ? ld1c<COH>(A, q * NT + tid) : T(0);
+        b[q] = B ? ld1c<COH>(B, q * NT + tid) : T(0);
     }
     T* R = pool + (size_t)out * TE;
     uint32_t bits = 0;
@@ -581,13 +676,6 @@ __device__ inline double readlane(double x, int l) {
 }
 __device__ inline float readlane(float x, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
-}
-
-template <typename T, typename C>
-__device__ inline void load_rowseg(const T* g, C* v, int lane, int w) {
-    const T* p = g + lane * TS + 16 * w;
-#pragma unroll
-    for (int c = 0; c < 16; c++) v[c] = C(p[c]);
 }
 
 template <typename T, typename C>
@@ -979,7 +1067,7 @@ __device__ __forceinline__ void u_blk_to_global(const C* a, T* Ug, int I, int J,
 template <typename T, typename C, bool CHOL = false>
 __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ pool, int src, int outl, int outu,
                                                               int outli, int outui, char* lds_raw,
-                                                              float* amax_out, float* umax_out) {
+                                                              float* amax_out, float* umax_out, bool coh = false) {
     typedef typename MfmaOp<C>::acc_t acc_t;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
@@ -1061,10 +1149,12 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
         const int r = tid >> 2, c0 = (tid & 3) * 16;
         if (tid == 0) *pcnt = 0;
         C m = 0;
+        C av[16];
+        load16(A, r * TS + c0, coh, av);
 #pragma unroll
         for (int q = 0; q < 16; q++) {
             const int c = c0 + q;
-            const C x = C(A[r * TS + c]);
+            const C x = av[q];
             a[r * LDA + c] = x;
             m = fabs(x) > m ? fabs(x) : m;
         }
@@ -1262,15 +1352,19 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
 // with unit diagonal, upper part as U).
 template <typename T>
 __device__ inline void lu_inv_fp64_lds(T* __restrict__ pool, int src, int outl, int outu, int outli,
-                                       int outui, char* lds_raw) {
+                                       int outui, char* lds_raw, bool coh) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     double* a = reinterpret_cast<double*>(lds_raw);
     double* lcol = a + TS * LDA;
     const T* A = pool + (size_t)src * TE;
     const int i = tid >> 2, j0 = (tid & 3) * 16;
+    {
+        double av[16];
+        load16(A, i * TS + j0, coh, av);
 #pragma unroll
-    for (int q = 0; q < 16; q++) a[i * LDA + j0 + q] = (double)A[i * TS + j0 + q];
+        for (int q = 0; q < 16; q++) a[i * LDA + j0 + q] = av[q];
+    }
     __syncthreads();
     const double TINY2 = 1e-9;
 #pragma unroll 1
@@ -1319,17 +1413,17 @@ __device__ inline void lu_inv_fp64_lds(T* __restrict__ pool, int src, int outl, 
 // in FP64 (counted in n_fp64).  Returns the growth seen.
 template <typename T>
 __device__ __attribute__((always_inline)) double lu_task(T* __restrict__ pool, int src, int outl, int outu, int outli, int outui,
-                          char* lds_raw, double growth_limit, int* n_fp64) {
+                          char* lds_raw, double growth_limit, int* n_fp64, bool coh = false) {
     if constexpr (sizeof(T) == 8) {
-        lu_inv_blocked<T, double>(pool, src, outl, outu, outli, outui, lds_raw, nullptr, nullptr);
+        lu_inv_blocked<T, double>(pool, src, outl, outu, outli, outui, lds_raw, nullptr, nullptr, coh);
         return 1.0;
     } else {
         float amax = 0, umax = 0;
-        lu_inv_blocked<T, float>(pool, src, outl, outu, outli, outui, lds_raw, &amax, &umax);
+        lu_inv_blocked<T, float>(pool, src, outl, outu, outli, outui, lds_raw, &amax, &umax, coh);
         const double growth = amax > 0 ? (double)umax / (double)amax : 1.0;
         if (!(growth <= growth_limit)) {
             __syncthreads();
-            lu_inv_fp64_lds<T>(pool, src, outl, outu, outli, outui, lds_raw);
+            lu_inv_fp64_lds<T>(pool, src, outl, outu, outli, outui, lds_raw, coh);
             if (threadIdx.x == 0) atomicAdd(n_fp64, 1);
         }
         return growth;
@@ -1337,7 +1431,7 @@ __device__ __attribute__((always_inline)) double lu_task(T* __restrict__ pool, i
 }
 
 // Standalone inverse of a lower (general diagonal) or upper tile.
-template <typename T, bool LOWER>
+template <typename T, bool LOWER, bool COH = false>
 __device__ __attribute__((always_inline)) void tri_inv_task(T* __restrict__ pool, int src, int out, char* lds_raw) {
     typedef typename std::conditional<sizeof(T) == 8, double, float>::type C;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -1345,7 +1439,7 @@ __device__ __attribute__((always_inline)) void tri_inv_task(T* __restrict__ pool
     C* ts = reinterpret_cast<C*>(lds_raw);
     C* bro = ts + TS * LDA + 16 * w;
     C v[16];
-    load_rowseg(pool + (size_t)src * TE, v, lane, w);
+    load16c<COH>(pool + (size_t)src * TE, lane * TS + 16 * w, v);
 #pragma unroll
     for (int c = 0; c < 16; c++) ts[lane * LDA + 16 * w + c] = v[c];
     __syncthreads();
@@ -1360,22 +1454,20 @@ __device__ __attribute__((always_inline)) void tri_inv_task(T* __restrict__ pool
 // to 1e-20 before the square root as it does (2612).  FP32 tiles factor in FP32.
 template <typename T>
 __device__ __attribute__((always_inline)) void llt_task(T* __restrict__ pool, int src, int out, int outinv,
-                                                        char* lds_raw) {
+                                                        char* lds_raw, bool coh = false) {
     typedef typename std::conditional<sizeof(T) == 8, double, float>::type C;
-    lu_inv_blocked<T, C, true>(pool, src, out, -1, outinv, -1, lds_raw, nullptr, nullptr);
+    lu_inv_blocked<T, C, true>(pool, src, out, -1, outinv, -1, lds_raw, nullptr, nullptr, coh);
 }
 
 // R = A^T through LDS (odd row stride: the column reads hit distinct banks); wave w
 // leaves the mask bits of block row w of R in out_mask[w].
-template <typename T, bool MASKS = true>
+template <typename T, bool MASKS = true, bool COH = false>
 __device__ __attribute__((always_inline)) void trans_task(T* __restrict__ pool, int src, int out, char* lds_raw,
                                                           uint32_t* out_mask) {
     T* s = reinterpret_cast<T*>(lds_raw);
     const int tid = threadIdx.x, lane = tid & 63;
     const int r = tid >> 2, c0 = (tid & 3) * 16;
-    const T* A = pool + (size_t)src * TE;
-#pragma unroll
-    for (int q = 0; q < 16; q++) s[r * LDA + c0 + q] = A[r * TS + c0 + q];
+    tile_to_lds<COH>(pool + (size_t)src * TE, s, LDA, tid);
     __syncthreads();
     T* R = pool + (size_t)out * TE;
     bool any = false;

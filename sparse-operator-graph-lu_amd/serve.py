@@ -137,6 +137,8 @@ class Server:
     def __init__(self, slots: int = 4, precision: int = sogl.FP64, reserve_gb: float = 8.0,
                  exec_mode: int = sogl.EXEC_PERSISTENT, planners: int = 6, plan_cache_ops: int = 150_000_000):
         self.devs = [sogl.Device(0) for _ in range(slots)]
+        for d in self.devs:  # many resident factors share the HBM: pools hold live tiles only
+            d.set_pool_reuse(sogl.REUSE_ON)
         # host plans: made ahead by a planner pool as requests arrive, kept (least
         # recently used dropped first, by op count) so an evicted factor is not replanned
         self.planner = cf.ThreadPoolExecutor(max_workers=planners)
@@ -194,7 +196,14 @@ class Server:
     # ---- memory
     def _need_bytes(self, plan: sogl.Plan, nrhs: int) -> int:
         el = 8 if self.precision == sogl.FP64 else 4
-        tiles = plan.info.storage_count + 2 * plan.info.block_rows + 1
+        info = plan.info
+        # with pool slot reuse the pool holds the factor tiles, the diagonal inverses and
+        # the zero tile, and a few intermediates live at once (C2: 9, C5: 7 beyond these)
+        if info.symmetric:  # L, its transposes, L^-1 and theirs
+            kept = 2 * info.n_l_tiles + 2 * info.block_rows + 1
+        else:
+            kept = info.n_l_tiles + info.n_u_tiles + 2 * info.block_rows + 1
+        tiles = min(int(kept * 1.05) + 256, info.storage_count + 2 * info.block_rows + 1 + (kept if info.symmetric else 0))
         graph = plan.info.n_ops * 64
         vecs = 3 * plan.N * max(nrhs, 16) * 8 + 4 * plan.N * 8
         return tiles * TILE * el + graph + vecs

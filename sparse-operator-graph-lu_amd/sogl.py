@@ -18,6 +18,7 @@ BUILD = os.path.join(HERE, "build")
 OK = 0
 FP64, FP32, BF16 = 0, 1, 2
 EXEC_PERSISTENT, EXEC_LEVELS, EXEC_PER_OP, EXEC_GRAPH, EXEC_AUTO = 0, 1, 2, 3, 4
+REUSE_OFF, REUSE_ON, REUSE_AUTO = 0, 1, 2
 MASKS_OFF, MASKS_ON, MASKS_AUTO = 0, 1, 2
 TUNE_CANDIDATES = ("persistent chunk 1", "persistent chunk 2", "persistent chunk 4",
                    "persistent chunk 1, 1 workgroup/CU", "levels", "level graph")
@@ -62,7 +63,8 @@ class FactorStats(C.Structure):
                 ("pool_bytes", C.c_int64),
                 ("n_tasks", C.c_int64), ("n_levels", C.c_int32), ("growth_tiles", C.c_int32),
                 ("max_growth", C.c_double), ("zero_blocks", C.c_double), ("block_masks", C.c_int32),
-                ("claimed_tasks", C.c_int32), ("dropped_tasks", C.c_int32), ("tasks_run", C.c_int32)]
+                ("claimed_tasks", C.c_int32), ("dropped_tasks", C.c_int32), ("tasks_run", C.c_int32),
+                ("pool_tiles", C.c_int64), ("value_tiles", C.c_int64)]
 
 
 def _ptr(a: np.ndarray, ctype):
@@ -125,6 +127,7 @@ def load(name: str = "libsogl.so") -> C.CDLL:
             "sogl_factor_get_tiles": (C.c_int, [vp, i32, i32, P(dbl)]),
         "sogl_factor_set_masks": (C.c_int, [vp, i32, i32]),
             "sogl_factor_set_chaining": (C.c_int, [vp, i32, i32]),
+            "sogl_ctx_set_pool_reuse": (C.c_int, [vp, i32]),
             "sogl_refine_attach": (C.c_int, [vp, i32, vp]),
             "sogl_solve_refined": (C.c_int, [vp, i32, P(dbl), P(dbl), i32, dbl, i32, P(i32),
                                              P(dbl)]),
@@ -298,7 +301,7 @@ def make_desc(n_ids: int, ops, input_ids, input_tiles, ltiles, utiles, block_row
     return d, keep
 
 
-GRAPH_INFO = ("tasks", "pool_tiles", "levels", "pairs", "deps", "folded", "inv_aliased", "chained")
+GRAPH_INFO = ("tasks", "value_tiles", "levels", "pairs", "deps", "folded", "inv_aliased", "chained", "pool_tiles")
 
 
 def graph_check_ops(n_ids: int, ops, input_ids, input_tiles, ltiles, utiles, block_rows: int,
@@ -390,6 +393,12 @@ class Device:
         finished task (0 / False: off; True: the default, 2)."""
         n = 2 if claims is True else int(claims)
         self._check(self.lib.sogl_factor_set_chaining(self.h, fid, n))
+
+    def set_pool_reuse(self, mode):
+        """Pool slot reuse for the factors loaded from now on: True / REUSE_ON, False /
+        REUSE_OFF (the per-op executor needs it off), or REUSE_AUTO (the default)."""
+        m = REUSE_ON if mode is True else REUSE_OFF if mode is False else int(mode)
+        self._check(self.lib.sogl_ctx_set_pool_reuse(self.h, m))
 
     def set_masks(self, fid: int, mode: int):
         """16x16 block-mask policy: MASKS_OFF, MASKS_ON or MASKS_AUTO (the default)."""

@@ -334,7 +334,9 @@ def test_llt_executor_modes_and_refinement(dev, golden):
     pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b, symmetric=True)
     xs = []
     for mode in (sogl.EXEC_PERSISTENT, sogl.EXEC_LEVELS, sogl.EXEC_GRAPH, sogl.EXEC_PER_OP):
+        dev.set_pool_reuse(sogl.REUSE_OFF if mode == sogl.EXEC_PER_OP else sogl.REUSE_ON)
         fid = dev.load(pl, sogl.FP64)
+        dev.set_pool_reuse(sogl.REUSE_AUTO)
         dev.factor(fid, mode)
         xs.append(dev.solve(fid, pl.permute_rhs(p.b)))
         dev.release(fid)
@@ -393,7 +395,9 @@ CASES = {
 def solve_fp64(dev, p, mode=sogl.EXEC_PERSISTENT, b=None):
     b = (p.b if p.b.ndim == 1 else p.b[:, 0]) if b is None else b
     pl = sogl.Plan(p.n, p.row, p.col, p.val, b)
+    dev.set_pool_reuse(sogl.REUSE_OFF if mode == sogl.EXEC_PER_OP else sogl.REUSE_ON)
     fid = dev.load(pl, sogl.FP64)
+    dev.set_pool_reuse(sogl.REUSE_AUTO)
     dev.factor(fid, mode)
     bp = pl.permute_rhs(b)
     xp, yp = dev.solve(fid, bp, want_y=True)
@@ -537,6 +541,65 @@ def test_autotune_and_auto_mode(dev):
     dev.factor(fid, sogl.EXEC_AUTO)
     assert np.array_equal(dev.factor_tiles(fid, 0, pl.info.n_l_tiles), l0)
     assert np.array_equal(dev.factor_tiles(fid, 1, pl.info.n_u_tiles), u0)
+    dev.release(fid)
+
+
+REUSE_CASES = {
+    "lu": lambda: W.convdiff2d(128),
+    "llt": lambda: W.symmetric_lower(W.diffusion_varcoef(16, 3, 21)),
+    "c1": lambda: W.config("c1"),
+    "c2": lambda: W.config("c2"),
+}
+
+
+@pytest.mark.parametrize("precision", [sogl.FP64, sogl.FP32, sogl.BF16])
+@pytest.mark.parametrize("case", sorted(REUSE_CASES))
+def test_pool_slot_reuse_keeps_bits(dev, precision, case):
+    """Pool slot reuse (a result takes over the slot of a value whose readers are done,
+    after waiting for them, and reads a slot's later values with coherent loads) leaves
+    the factor bit for bit as a factor loaded without reuse gives it: the persistent
+    executor at one and two workgroups per CU, the levels executor and its captured
+    graph, each run twice (the inputs scattered into their reused slots again before
+    every run), and the solve after them.  The pool is smaller; the per-op executor,
+    which accumulates in the results' slots, is refused."""
+    p = REUSE_CASES[case]()
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b, symmetric=p.symmetric)
+    n_u = 0 if p.symmetric else pl.info.n_u_tiles
+    bp = pl.permute_rhs(p.b if p.b.ndim == 1 else p.b[:, 0])
+
+    def tiles(fid):
+        return dev.factor_tiles(fid, 0, pl.info.n_l_tiles), dev.factor_tiles(fid, 1, n_u) if n_u else None
+
+    dev.set_pool_reuse(False)
+    try:
+        f0 = dev.load(pl, precision)
+        dev.set_pool_reuse(True)
+        fid = dev.load(pl, precision)
+    finally:
+        dev.set_pool_reuse(sogl.REUSE_AUTO)
+    dev.set_masks(f0, sogl.MASKS_ON)
+    dev.factor(f0, sogl.EXEC_LEVELS)
+    l0, u0 = tiles(f0)
+    x0 = dev.solve(f0, bp)
+    s0 = dev.stats(f0)
+    dev.release(f0)
+    assert s0.pool_tiles == s0.value_tiles
+    dev.set_masks(fid, sogl.MASKS_ON)
+    st = dev.stats(fid)
+    assert st.value_tiles == s0.value_tiles and st.pool_tiles < st.value_tiles, (st.pool_tiles, st.value_tiles)
+    assert st.pool_bytes < s0.pool_bytes
+    for mode, wg in ((sogl.EXEC_PERSISTENT, 1), (sogl.EXEC_PERSISTENT, 0), (sogl.EXEC_LEVELS, 0),
+                     (sogl.EXEC_GRAPH, 0)):
+        for rep in range(2):
+            print(f"reuse {case} prec {precision} mode {mode} wg {wg} run {rep}", flush=True)
+            dev.factor_config(fid, mode, 1, wg)
+            l1, u1 = tiles(fid)
+            assert np.array_equal(l1, l0), (mode, wg, rep)
+            if n_u:
+                assert np.array_equal(u1, u0), (mode, wg, rep)
+    assert np.array_equal(dev.solve(fid, bp), x0)
+    with pytest.raises(sogl.SoglError):
+        dev.factor(fid, sogl.EXEC_PER_OP)
     dev.release(fid)
 
 

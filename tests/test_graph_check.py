@@ -40,7 +40,11 @@ def test_plan_graphs_pass(name):
         extra = info.n_l_tiles + info.block_rows if info.symmetric else 0
         # every unit op is a task, one more pair of a GEMM task, folded or aliased
         assert g["tasks"] <= info.n_ops - g["folded"] - g["inv_aliased"] + extra
-        assert g["levels"] > 0 and g["pool_tiles"] <= info.storage_count + 2 * info.block_rows + 1 + extra
+        assert g["levels"] > 0 and g["value_tiles"] <= info.storage_count + 2 * info.block_rows + 1 + extra
+        # slot reuse (checked inside graph_check): never more tiles than values, and at
+        # least the factor tiles, their diagonal inverses and the zero tile
+        kept = (2 * info.n_l_tiles if info.symmetric else info.n_l_tiles + info.n_u_tiles) + 2 * info.block_rows + 1
+        assert kept - extra <= g["pool_tiles"] <= g["value_tiles"]
 
 
 def test_c2_graph_passes():
@@ -51,6 +55,20 @@ def test_c2_graph_passes():
     assert g["levels"] < pl.info.n_stages  # fused tasks: fewer dependency levels than stages
     # most lu tasks follow the Schur-complement GEMM of their tile in the same workgroup
     assert 0 < g["chained"] <= pl.info.n_lu
+    # slot reuse: 220,885 values in 88,381 tiles (the factor tiles, their inverses and
+    # the zero tile are 78,961; a slot freed fewer than 1024 levels back is not reused)
+    assert g["value_tiles"] == 220_885 and 78_961 < g["pool_tiles"] < 90_000
+
+
+def test_c5_pool_fits_in_15_gb():
+    """The 1M-unknown convection-diffusion factor in FP64: 1,078,138 values of 32 KiB
+    (35.3 GB without slot reuse) in a pool of at most 15 GB; the reference peaks at
+    13.4 GB of host memory on the same matrix."""
+    p = W.config("c5")
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
+    g = sogl.graph_check(pl, sogl.FP64)
+    assert g["value_tiles"] == 1_078_138
+    assert g["pool_tiles"] * 4096 * 8 <= 15e9, g["pool_tiles"]
 
 
 # ---- planted mistakes in hand-built op lists (ids 1, 2 are inputs)
