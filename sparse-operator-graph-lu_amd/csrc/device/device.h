@@ -13,6 +13,8 @@
 
 namespace sogl_dev {
 
+struct Pair;  // tile_ops.h
+
 enum TaskKind : int32_t { TK_GEMM = 0, TK_SUB = 1, TK_LU = 2, TK_LINV = 3, TK_UINV = 4, TK_LLT = 5, TK_TRANS = 6 };
 
 // One device task = every unit op that writes one result tile (an lu writes two).
@@ -81,6 +83,7 @@ struct Factor {
     std::vector<DevTask> tasks;            // topological order (by level, then stage)
     std::vector<int32_t> level_begin;      // tasks of level v: [level_begin[v], level_begin[v+1])
     std::vector<DevTask> op_tasks;         // per-op executor: one entry per unit op, list order
+    std::vector<Pair> op_pairs;            // and their pairs (uploaded on the first per-op run)
     std::vector<int32_t> input_slots;
     int64_t n_pairs = 0, n_deps = 0;
     int64_t n_inv_aliased = 0;  // repeated inverses of one factor tile that share its slot

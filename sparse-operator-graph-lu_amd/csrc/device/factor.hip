@@ -1738,8 +1738,6 @@ int upload_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells,
     if ((rc = dev_alloc(c, f->ready_d, (size_t)f->n_slots * 8, "ready flags")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->mask_d, (size_t)f->n_locs * 4, "block masks")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->ctrl_d, CTRL_WORDS * 4, "control")) != SOGL_OK) return rc;
-    if ((rc = dev_alloc(c, f->op_tasks_d, std::max<size_t>(f->op_tasks.size(), 1) * sizeof(DevTask), "op tasks")) != SOGL_OK) return rc;
-    if ((rc = dev_alloc(c, f->op_pairs_d, std::max<size_t>(op_pairs.size(), 1) * sizeof(Pair), "op pairs")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->queue_d, std::max<size_t>(f->queue.size(), 1) * 4, "task queue")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->chk_d, std::max<size_t>(f->chk.size(), 1) * sizeof(ChainChk), "chain checks")) != SOGL_OK)
         return rc;
@@ -1756,8 +1754,13 @@ int upload_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells,
     if (T) HIP_TRY(c, hipMemcpy(f->tasks_d.p, f->tasks.data(), T * sizeof(DevTask), hipMemcpyHostToDevice));
     if (!pairs.empty()) HIP_TRY(c, hipMemcpy(f->pairs_d.p, pairs.data(), pairs.size() * sizeof(Pair), hipMemcpyHostToDevice));
     if (!deps.empty()) HIP_TRY(c, hipMemcpy(f->deps_d.p, deps.data(), deps.size() * 4, hipMemcpyHostToDevice));
-    if (!f->op_tasks.empty()) HIP_TRY(c, hipMemcpy(f->op_tasks_d.p, f->op_tasks.data(), f->op_tasks.size() * sizeof(DevTask), hipMemcpyHostToDevice));
-    if (!op_pairs.empty()) HIP_TRY(c, hipMemcpy(f->op_pairs_d.p, op_pairs.data(), op_pairs.size() * sizeof(Pair), hipMemcpyHostToDevice));
+    // the per-op executor's tables (one entry per unit op: 680 MB for a 3D 64^3 plan) go to
+    // the device on its first run only; a factor with slot reuse cannot use them
+    if (f->reuse) {
+        std::vector<DevTask>().swap(f->op_tasks);
+    } else {
+        f->op_pairs = op_pairs;
+    }
     if (!f->queue.empty()) HIP_TRY(c, hipMemcpy(f->queue_d.p, f->queue.data(), f->queue.size() * 4, hipMemcpyHostToDevice));
     if (!f->chk.empty())
         HIP_TRY(c, hipMemcpy(f->chk_d.p, f->chk.data(), f->chk.size() * sizeof(ChainChk), hipMemcpyHostToDevice));
@@ -1974,6 +1977,18 @@ int launch_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
         hipLaunchKernelGGL(k_fill<T>, dim3(4096), dim3(256), 0, st,
                            (T*)f->pool.p + first_result * TE, nres * TE, T(0));
         hipLaunchKernelGGL(k_fill<int>, dim3(256), dim3(256), 0, st, (int*)f->mask_d.p + first_result, nres, 0);
+        if (!f->op_tasks_d.p) {
+            int rc;
+            if ((rc = dev_alloc(c, f->op_tasks_d, std::max<size_t>(f->op_tasks.size(), 1) * sizeof(DevTask), "op tasks")) ||
+                (rc = dev_alloc(c, f->op_pairs_d, std::max<size_t>(f->op_pairs.size(), 1) * sizeof(Pair), "op pairs")))
+                return rc;
+            if (!f->op_tasks.empty())
+                HIP_TRY(c, hipMemcpy(f->op_tasks_d.p, f->op_tasks.data(), f->op_tasks.size() * sizeof(DevTask),
+                                     hipMemcpyHostToDevice));
+            if (!f->op_pairs.empty())
+                HIP_TRY(c, hipMemcpy(f->op_pairs_d.p, f->op_pairs.data(), f->op_pairs.size() * sizeof(Pair),
+                                     hipMemcpyHostToDevice));
+        }
         a.tasks = (const DevTask*)f->op_tasks_d.p;
         a.pairs = (const Pair*)f->op_pairs_d.p;
         a.ntasks = (int32_t)f->op_tasks.size();

@@ -10,6 +10,8 @@
 // once per coarse op re-created by the L2 expansion (BlockPlanner.cpp:1362).
 // The counter starts at 0 for every plan, which is the reference's state in
 // its one-solve-per-process use.
+#include <omp.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -55,6 +57,7 @@ struct Planner {
     std::vector<int32_t> stage, laststage;
     int32_t maxStage = 0;
     std::string* log = nullptr;
+    int threads = 1;        // OpenMP threads for the per-stage op sort
 
     Node* node(int32_t blockrows2, int32_t level) {  // memutil::newmatrix
         arena.emplace_back();
@@ -467,14 +470,30 @@ struct Planner {
         }
         // (vi) order by (stage, group, result, src, seq).  The reference buckets by
         // stage%700, sorts each bucket and merges by stage (:278-333); seq is unique,
-        // so this is a total order and one global sort gives the same sequence.
-        std::stable_sort(graph.begin(), graph.end(), [](const sogl_op& x, const sogl_op& y) {
-            if (x.stage != y.stage) return x.stage < y.stage;
-            if (x.group != y.group) return x.group < y.group;
-            if (x.result != y.result) return x.result < y.result;
-            if (x.src != y.src) return x.src < y.src;
-            return x.seq < y.seq;
-        });
+        // so this is a total order and any sort gives the same sequence.  Here: a
+        // counting pass by stage, then each stage's ops sorted on their own, the stages
+        // spread over threads (3D 64^3: 7.1M ops, 1.3 s in one global sort).
+        {
+            std::vector<int64_t> at((size_t)maxstg + 2, 0);
+            for (const sogl_op& o : graph) at[(size_t)o.stage + 1]++;
+            for (size_t v = 1; v < at.size(); v++) at[v] += at[v - 1];
+            std::vector<sogl_op> by((size_t)graph.size());
+            {
+                std::vector<int64_t> fill(at.begin(), at.end() - 1);
+                for (const sogl_op& o : graph) by[(size_t)fill[(size_t)o.stage]++] = o;
+            }
+            const int64_t nst = (int64_t)at.size() - 1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+            for (int64_t v = 0; v < nst; v++)
+                std::sort(by.begin() + at[(size_t)v], by.begin() + at[(size_t)v + 1],
+                          [](const sogl_op& x, const sogl_op& y) {
+                              if (x.group != y.group) return x.group < y.group;
+                              if (x.result != y.result) return x.result < y.result;
+                              if (x.src != y.src) return x.src < y.src;
+                              return x.seq < y.seq;
+                          });
+            graph.swap(by);
+        }
         // (vii) band split: after every 8001 ops of one stage, later ops move one stage on
         int32_t jump = 0, band = 0, stgc = 0;
         for (sogl_op& o : graph) {
@@ -620,6 +639,7 @@ void build_plan(const Coo& a, const double* b, const PlanOptions& opt, Plan& pla
     mark = log.size();
     Planner P;
     P.log = &log;
+    P.threads = opt.threads > 0 ? opt.threads : std::max(1, omp_get_max_threads());
     // coarse level: ids in COO order of first touch, no tile memory (BlockPlanner.cpp:1480-1490)
     P.planL2 = true;
     int32_t blockRows = cfg.blockRowsL2, blockSize = cfg.blockSizeL2;

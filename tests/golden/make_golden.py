@@ -34,7 +34,7 @@ import workloads as W  # noqa: E402
 OUT = os.path.join(ROOT, "tests", "golden")
 REF_DUMP = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
 
-# name -> (generator, full arrays?, plan only?)
+# name -> (generator, full arrays? ("x": the solution in full, the rest as hashes), plan only?)
 CASES = {
     "c1": (lambda: W.config("c1"), True, False),
     "random_band_2048": (lambda: W.random_band(2048, nrhs=1), True, False),
@@ -48,6 +48,9 @@ CASES = {
     "sym_diffusion3d_16": (lambda: W.symmetric_lower(W.diffusion_varcoef(16, 3, 21)), True, False),
     "sym_diffusion2d_100": (lambda: W.symmetric_lower(W.diffusion_varcoef(100, 2, 22)), True, False),
     "sym_diffusion3d_48": (lambda: W.symmetric_lower(W.diffusion_varcoef(48, 3, 23)), False, False),
+    # a serving-stream request (serve.matrix_problem((2, 128, 117))): 2D variable-coefficient
+    # diffusion, log-normal coefficients
+    "c4_2d_128": (lambda: W.diffusion_varcoef(128, 2, 117), "x", False),
     "c2": (lambda: W.config("c2"), False, False),
     "c5": (lambda: W.config("c5"), False, False),
 }
@@ -93,15 +96,17 @@ def run_case(name: str) -> None:
                    ref_t_factor=np.float64(float(meta["t_factor"])),
                    ref_t_solve=np.float64(float(meta["t_solve"])),
                    ref_stdout=np.str_(r.stdout))
-        if full:
+        if full is True:
             out.update(perm=perm, ops=ops, coarse_ops=cops, tiles=tiles, inputs=inputs, ltiles=lt,
                        utiles=ut, bperm=rd("bperm.f64", np.float64))
         if not plan_only:
             x = rd("x.f64", np.float64)
             y = rd("y.f64", np.float64)
             xo = rd("xorig.f64", np.float64)
-            if full:
+            if full is True:
                 out.update(x=x, y=y, xorig=xo)
+            elif full == "x":
+                out.update(xorig=xo)
             else:
                 idx = np.unique(np.linspace(0, n - 1, 4096).astype(np.int64))
                 out.update(xorig_idx=idx, xorig_sample=xo[idx], xorig_norm=np.float64(np.linalg.norm(xo)),

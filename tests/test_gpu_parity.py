@@ -840,6 +840,51 @@ def test_serving_mix_matches_alone():
         assert sogl.oracle_residual(prob.n, prob.row, prob.col, prob.val, r.x, r.b) < 1e-14
 
 
+def test_c4_request_vs_reference(dev, golden):
+    """One request of the serving stream (2D variable-coefficient diffusion, side 128,
+    log-normal coefficients; its plan is pinned by hash in test_plan_golden) against the
+    reference's own FP64 solution: the FP64 path to 1e-12, and the FP32 factor with FP64
+    refinement to a 1e-12 backward error in both forms."""
+    import serve
+    p = serve.matrix_problem((2, 128, 117))
+    ref = golden("c4_2d_128")["xorig"]
+    pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
+    fid = dev.load(pl, sogl.FP64)
+    dev.factor(fid)
+    x = pl.unpermute(dev.solve(fid, pl.permute_rhs(p.b)))
+    dev.release(fid)
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-12
+    fid = dev.load(pl, sogl.FP32)
+    dev.factor(fid)
+    dev.attach_refine(fid, pl)
+    x, it, res, rc = dev.solve_refined(fid, p.b, tol=1e-12, max_iter=30)
+    dev.release(fid)
+    assert rc == 0 and res <= 1e-12, (rc, it, res)
+    assert sogl.backward_error_inf(p.n, p.row, p.col, p.val, x, p.b) <= 1e-12
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-9
+
+
+@pytest.mark.slow
+def test_serving_at_full_request_sizes():
+    """The largest requests of the serving stream -- a 2D 512^2 and a 3D 64^3 matrix
+    (262,144 unknowns each), and the 512^2 matrix again as a resident hit -- served
+    concurrently with smaller ones (3 slots, pool slot reuse on, multi-RHS batching),
+    give the x each gives served alone, and every x meets the FP64 residual bar."""
+    import serve
+    keys = [(2, 512, 201), (3, 64, 202), (2, 96, 203), (3, 24, 204), (2, 512, 201)]
+    def stream():
+        return [serve.Request(i, 0.02 * i, k, 1 + 3 * i % 8, 60_000 + i) for i, k in enumerate(keys)]
+    print("serving: mixed stream", flush=True)
+    m, mixed = serve.serve(stream(), slots=3)
+    assert m["requests"] == len(keys) and m["resident_hits"] >= 1, m
+    for r in mixed:
+        print(f"serving: request {r.rid} {r.matrix} alone", flush=True)
+        _, (a,) = serve.serve([stream()[r.rid]], slots=1)
+        assert np.linalg.norm(a.x - r.x) <= 1e-13 * np.linalg.norm(a.x), r.rid
+        prob = serve.matrix_problem(r.matrix)
+        assert sogl.oracle_residual(prob.n, prob.row, prob.col, prob.val, r.x, r.b) < 1e-14, r.rid
+
+
 @pytest.mark.parametrize("precision", [sogl.FP64, sogl.FP32])
 def test_c3_shape_many_rhs(dev, precision):
     """The C3 shape (random band, half-width 32 + 4 random entries per row, N(0,1)

@@ -58,7 +58,8 @@ def test_small_bit_exact(name, golden):
 
 
 LARGE = {"c2": lambda: W.config("c2"), "c5": lambda: W.config("c5"),
-         "sym_diffusion3d_48": lambda: W.symmetric_lower(W.diffusion_varcoef(48, 3, 23))}
+         "sym_diffusion3d_48": lambda: W.symmetric_lower(W.diffusion_varcoef(48, 3, 23)),
+         "c4_2d_128": lambda: W.diffusion_varcoef(128, 2, 117)}
 
 
 @pytest.mark.parametrize("name", list(LARGE))
