@@ -270,7 +270,7 @@ def main():
                     help="c1..c5: the named workloads; sym: a symmetric-input (L L^T) workload")
     ap.add_argument("--requests", type=int, default=200, help="c4: requests in the serving stream")
     ap.add_argument("--rate", type=float, default=20.0, help="c4: mean arrivals per second")
-    ap.add_argument("--slots", type=int, default=4, help="c4: concurrent requests on the GPU")
+    ap.add_argument("--slots", type=int, default=8, help="c4: concurrent requests on the GPU")
     ap.add_argument("--exec", default="auto", choices=sorted(EXECUTORS),
                     help="auto: the executor configuration the autotuner picks on each factor "
                          "(tuned before the warmup steps, outside the timed region); "

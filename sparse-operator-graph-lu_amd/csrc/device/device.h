@@ -84,6 +84,7 @@ struct Factor {
     std::vector<int32_t> level_begin;      // tasks of level v: [level_begin[v], level_begin[v+1])
     std::vector<DevTask> op_tasks;         // per-op executor: one entry per unit op, list order
     std::vector<Pair> op_pairs;            // and their pairs (uploaded on the first per-op run)
+    bool keep_ops = true;                  // build those tables (not when reuse is forced on)
     std::vector<int32_t> input_slots;
     int64_t n_pairs = 0, n_deps = 0;
     int64_t n_inv_aliased = 0;  // repeated inverses of one factor tile that share its slot

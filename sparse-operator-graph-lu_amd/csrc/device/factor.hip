@@ -866,6 +866,9 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
     std::vector<sogl_op> opl(d->ops, d->ops + d->n_ops);
     std::vector<Pair>& op_pairs = hg.op_pairs;
     f->op_tasks.clear();
+    // the per-op executor's tables (one entry per unit op): only for factors that may
+    // run without slot reuse
+    const bool keep_ops = f->keep_ops;
     auto src_slot = [&](int32_t id) -> int32_t { return id > 0 ? slot[(size_t)id] : -1; };
     for (int64_t k = 0; k < d->n_ops; k++) {
         const sogl_op& o = opl[(size_t)k];
@@ -906,7 +909,7 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
             pr = Pair{o.op == SOGL_OP_MULT ? -s1 - 1 : s1, o.op == SOGL_OP_MULNEG ? -s2 - 1 : s2};
             ot.a = (int32_t)op_pairs.size();
             ot.b = 1;
-            op_pairs.push_back(pr);
+            if (keep_ops) op_pairs.push_back(pr);
         } else if (kind == TK_SUB) {
             ot.a = s2;
             ot.b = s1;
@@ -915,7 +918,7 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
             ot.b = 0;
         }
         ot.out2 = ot.out3 = -1;
-        f->op_tasks.push_back(ot);
+        if (keep_ops) f->op_tasks.push_back(ot);
         auto seal_sources = [&]() {
             if (s1 >= 0) sealed[(size_t)s1] = 1;
             if (s2 >= 0) sealed[(size_t)s2] = 1;
@@ -924,7 +927,7 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
         // in place on that slot
         auto alias = [&](int32_t to) {
             slot[(size_t)o.result] = to;
-            f->op_tasks.back().out0 = to;
+            if (keep_ops) f->op_tasks.back().out0 = to;
             seal_sources();
             f->n_folded++;
         };
@@ -932,7 +935,7 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
             if (rs >= 0) return;
             rs = slot[(size_t)o.result] = new_slot();
             ot.out0 = rs;
-            f->op_tasks.back().out0 = rs;
+            if (keep_ops) f->op_tasks.back().out0 = rs;
         };
         if (kind == TK_SUB && rs < 0) {
             // copy: the result is its source (exact)
@@ -987,7 +990,7 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
                     prod[(size_t)rs] = pt;
                 } else {
                     slot[(size_t)o.result] = inv;
-                    f->op_tasks.back().out0 = inv;
+                    if (keep_ops) f->op_tasks.back().out0 = inv;
                     f->n_inv_aliased++;
                 }
                 seal_sources();
@@ -1005,7 +1008,7 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
             tasks.push_back(nt);
             tpairs.emplace_back();
             tstage.push_back(o.stage);
-            lu_op_task.push_back((int32_t)f->op_tasks.size() - 1);
+            lu_op_task.push_back(keep_ops ? (int32_t)f->op_tasks.size() - 1 : -1);
             prod[(size_t)rs] = t;
             if (ot.out1 >= 0) prod[(size_t)ot.out1] = t;
         } else {
@@ -1048,8 +1051,10 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
                     inv = new_slot();
                     prod[(size_t)inv] = pt;
                     (which == 0 ? tasks[(size_t)pt].out2 : tasks[(size_t)pt].out3) = inv;
-                    DevTask& ot = f->op_tasks[(size_t)lu_op_task[(size_t)pt]];
-                    (which == 0 ? ot.out2 : ot.out3) = inv;
+                    if (keep_ops) {
+                        DevTask& ot = f->op_tasks[(size_t)lu_op_task[(size_t)pt]];
+                        (which == 0 ? ot.out2 : ot.out3) = inv;
+                    }
                 }
             } else {
                 // a diagonal tile that no lu produced (hand-built descriptions): its own
@@ -1065,7 +1070,7 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
                 tstage.push_back(0);
                 lu_op_task.push_back(-1);
                 prod[(size_t)inv] = (int32_t)tasks.size() - 1;
-                f->op_tasks.push_back(nt);
+                if (keep_ops) f->op_tasks.push_back(nt);
             }
             (which == 0 ? f->linv_slot : f->uinv_slot)[(size_t)i] = inv;
         }
@@ -1087,7 +1092,7 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
         tstage.push_back(0);
         lu_op_task.push_back(-1);
         prod[(size_t)ts] = (int32_t)tasks.size() - 1;
-        f->op_tasks.push_back(nt);
+        if (keep_ops) f->op_tasks.push_back(nt);
         trans_of[src] = ts;
         return ts;
     };
@@ -1637,60 +1642,78 @@ int validate_slots(Context* c, const Factor* f, const HostGraph& hg) {
         loc[(size_t)v] = l;
         return true;
     };
-    // each task's reads (value ids)
-    std::vector<std::vector<int32_t>> reads(T);
+    // each task's reads (value ids, deduplicated), as a CSR over tasks
+    std::vector<int64_t> rb(T + 1, 0);
+    std::vector<int32_t> rv;
+    rv.reserve(hg.pairs.size() * 2 + T);
     for (size_t t = 0; t < T; t++) {
         const DevTask& k = f->tasks[t];
         if (!put(k.f0, k.out0) || !put(k.f1, k.out1) || !put(k.f2, k.out2) || !put(k.f3, k.out3))
             return bad("outputs of task " + std::to_string(t));
         for (int32_t v : {k.f0, k.f1, k.f2, k.f3})
             if (v >= 0) prod[(size_t)v] = (int32_t)t;
-        auto& r = reads[t];
+        const size_t b0 = rv.size();
         if (k.kind == TK_GEMM) {
             if (!put(k.fadd, k.add)) return bad("added tile of task " + std::to_string(t));
-            if (k.fadd >= 0) r.push_back(k.fadd);
+            if (k.fadd >= 0) rv.push_back(k.fadd);
             for (int32_t q = k.a; q < k.a + k.b; q++) {
                 const Pair& p = hg.pairs[(size_t)q];
                 if (!put(p.fa, pair_a(p)) || !put(p.fb, pair_b(p)))
                     return bad("pair of task " + std::to_string(t));
-                r.push_back(p.fa);
-                r.push_back(p.fb);
+                rv.push_back(p.fa);
+                rv.push_back(p.fb);
             }
         } else {
             if (!put(k.fsa, k.a) || (k.kind == TK_SUB && !put(k.fsb, k.b)))
                 return bad("operands of task " + std::to_string(t));
-            if (k.fsa >= 0) r.push_back(k.fsa);
-            if (k.fsb >= 0) r.push_back(k.fsb);
+            if (k.fsa >= 0) rv.push_back(k.fsa);
+            if (k.fsb >= 0) rv.push_back(k.fsb);
         }
+        std::sort(rv.begin() + (int64_t)b0, rv.end());
+        rv.erase(std::unique(rv.begin() + (int64_t)b0, rv.end()), rv.end());
+        rb[t + 1] = (int64_t)rv.size();
         if (k.war_begin < 0 || k.war_count < 0 || (size_t)k.war_begin + (size_t)k.war_count > f->war.size())
             return bad("write-after-read range of task " + std::to_string(t));
         for (int32_t q = k.war_begin; q < k.war_begin + k.war_count; q++)
             if (f->war[(size_t)q] < 0 || f->war[(size_t)q] >= V) return bad("write-after-read flag of task " + std::to_string(t));
     }
-    std::vector<std::vector<int32_t>> readers((size_t)V);
-    for (size_t t = 0; t < T; t++) {
-        std::vector<int32_t> u(reads[t]);
-        std::sort(u.begin(), u.end());
-        u.erase(std::unique(u.begin(), u.end()), u.end());
-        for (int32_t v : u) readers[(size_t)v].push_back((int32_t)t);
+    // readers of each value (CSR over values, tasks ascending)
+    std::vector<int64_t> qb((size_t)V + 1, 0);
+    for (int32_t v : rv) qb[(size_t)v + 1]++;
+    for (int64_t v = 0; v < V; v++) qb[(size_t)v + 1] += qb[(size_t)v];
+    std::vector<int32_t> ql(rv.size());
+    {
+        std::vector<int64_t> fill(qb.begin(), qb.end() - 1);
+        for (size_t t = 0; t < T; t++)
+            for (int64_t q = rb[t]; q < rb[t + 1]; q++) ql[(size_t)fill[(size_t)rv[(size_t)q]]++] = (int32_t)t;
     }
-    // occupants of each slot in visiting order (inputs and the zero tile first)
-    std::vector<std::vector<std::pair<int32_t, int32_t>>> occ((size_t)f->n_locs);
+    // occupants of each slot in visiting order (inputs and the zero tile first), CSR
+    const int64_t NL = f->n_locs;
+    std::vector<int64_t> ob((size_t)NL + 1, 0);
     for (int64_t v = 0; v < V; v++)
-        if (loc[(size_t)v] >= 0) occ[(size_t)loc[(size_t)v]].push_back({prod[(size_t)v], (int32_t)v});
+        if (loc[(size_t)v] >= 0) ob[(size_t)loc[(size_t)v] + 1]++;
+    for (int64_t l = 0; l < NL; l++) ob[(size_t)l + 1] += ob[(size_t)l];
+    std::vector<std::pair<int32_t, int32_t>> occ((size_t)ob[(size_t)NL]);
+    {
+        std::vector<int64_t> fill(ob.begin(), ob.end() - 1);
+        for (int64_t v = 0; v < V; v++)
+            if (loc[(size_t)v] >= 0) occ[(size_t)fill[(size_t)loc[(size_t)v]]++] = {prod[(size_t)v], (int32_t)v};
+    }
     std::vector<char> first((size_t)V, 1);
-    for (int64_t l = 0; l < f->n_locs; l++) {
-        auto& o = occ[(size_t)l];
-        std::sort(o.begin(), o.end());
-        for (size_t i = 0; i + 1 < o.size(); i++) {
+    std::vector<int32_t> wf;
+    for (int64_t l = 0; l < NL; l++) {
+        auto* o = occ.data() + ob[(size_t)l];
+        const int64_t no = ob[(size_t)l + 1] - ob[(size_t)l];
+        std::sort(o, o + no);
+        for (int64_t i = 0; i + 1 < no; i++) {
             const int32_t v = o[i].second, w = o[i + 1].second, pw = o[i + 1].first;
             first[(size_t)w] = 0;
             if (v == f->zero_slot) return bad("the zero tile's slot is taken over");
             if (o[i].first < 0 && !f->rescatter) return bad("an input slot is taken over");
-            const auto& rv = readers[(size_t)v];
             if (pw < 0) return bad("slot " + std::to_string(l) + " holds two values that no task makes");
             const DevTask& kw = f->tasks[(size_t)pw];
-            if (rv.size() == 1 && rv[0] == pw) {  // handed over by its only reader
+            const int64_t r0 = qb[(size_t)v], r1 = qb[(size_t)v + 1];
+            if (r1 - r0 == 1 && ql[(size_t)r0] == pw) {  // handed over by its only reader
                 if (kw.kind == TK_GEMM || kw.kind == TK_SUB ||
                     ((kw.kind == TK_LU || kw.kind == TK_LLT) && f->precision == SOGL_FP64))
                     continue;
@@ -1698,9 +1721,10 @@ int validate_slots(Context* c, const Factor* f, const HostGraph& hg) {
             }
             if (kw.kind != TK_GEMM)
                 return bad("slot " + std::to_string(l) + " passes to a non-GEMM task while another reads it");
-            std::vector<int32_t> wf(f->war.begin() + kw.war_begin, f->war.begin() + kw.war_begin + kw.war_count);
+            wf.assign(f->war.begin() + kw.war_begin, f->war.begin() + kw.war_begin + kw.war_count);
             std::sort(wf.begin(), wf.end());
-            for (int32_t r : rv) {
+            for (int64_t q = r0; q < r1; q++) {
+                const int32_t r = ql[(size_t)q];
                 if (f->tasks[(size_t)r].level >= kw.level)
                     return bad("slot " + std::to_string(l) + " is reused at the level of a reader");
                 if (!std::binary_search(wf.begin(), wf.end(), done_flag(f, r)))
@@ -1710,7 +1734,8 @@ int validate_slots(Context* c, const Factor* f, const HostGraph& hg) {
     }
     for (int32_t v : hg.kept) {
         const int32_t l = loc[(size_t)v];
-        if (l >= 0 && occ[(size_t)l].back().second != v) return bad("kept value " + std::to_string(v) + " is overwritten");
+        if (l >= 0 && occ[(size_t)ob[(size_t)l + 1] - 1].second != v)
+            return bad("kept value " + std::to_string(v) + " is overwritten");
     }
     for (size_t t = 0; t < T; t++) {
         const DevTask& k = f->tasks[t];
@@ -1863,6 +1888,7 @@ int upload_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells,
 
 int build_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells, Factor* f) {
     HostGraph hg;
+    f->keep_ops = c->pool_reuse != SOGL_REUSE_ON;  // a factor with reuse cannot run per op
     int rc = build_host_graph(c, d, f, hg);
     if (rc == SOGL_OK) rc = validate_graph(c, d, f, hg);
     if (rc == SOGL_OK) {
