@@ -44,10 +44,15 @@ METRIC = "factor+solve s & block-GEMM TFLOP/s at rel. residual ≤1e-10, FP32+IR
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense MFMA TFLOP/s
 HBM_PEAK = 8000.0  # GB/s
 # HBM bytes per launch of the FP32 factor kernel in the timed runs, from two rocprofv3
-# --pmc runs (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, the scales a plain 1 GiB copy
-# reads): profiles/k_factor_persistent_c2_pmc_traffic.md (dense tiles) and
-# profiles/k_factor_persistent_c5_pmc_traffic.md (16x16 block masks on)
-TRAFFIC_FP32_FACTOR = {"c2": 22.61e9, "c5": 23.85e9}
+# --pmc runs (FETCH_SIZE and WRITE_SIZE, scaled by what they read for a plain 1 GiB
+# copy): profiles/k_factor_persistent_c2_pmc_traffic.md (the timed run: dense tiles)
+# and profiles/k_factor_persistent_c5_pmc_traffic.md (16x16 block masks on)
+TRAFFIC_FP32_FACTOR = {"c2": 22.51e9, "c5": 23.87e9}
+# MFMA busy share of the FP32 factor kernel (SQ_VALU_MFMA_BUSY_CYCLES over the kernel's
+# SIMD cycles) and the MFMA flops it executes, from profiles/k_factor_persistent_*_mfma_busy.md:
+# at C5 the block masks skip 97% of the nominal products (84% of the 16x16 blocks are zero)
+MFMA_FP32_FACTOR = {"c2": {"busy_frac": 0.0543, "executed_gflop": 525.5},
+                    "c5": {"busy_frac": 0.0008, "executed_gflop": 43.5}}
 
 WORKLOADS = {
     "c1": ("2D 5-point Poisson 64x64 (n=4,096), 1 RHS U[-1,1] seed 0", 1e-12),
@@ -445,6 +450,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(head["gemm_tflops"], 4), "peak": PEAK["fp32"],
                      "unit": "TFLOP/s", "frac": round(head["gemm_tflops"] / PEAK["fp32"], 6),
                      "traffic": TRAFFIC_FP32_FACTOR.get(args.config),
+                     "mfma_pmc": MFMA_FP32_FACTOR.get(args.config),
                      "kernel": "k_factor_persistent<float> (FP32 factor executor); achieved = "
                                "(mul+mulneg+mult ops) x 2*64^3 flop / executor time from HIP events"},
         "cpu_baseline": cpu,
