@@ -279,8 +279,18 @@ typedef struct sogl_factor_stats {
     int32_t tasks_run;      /* last run: tasks executed (n_tasks when each ran exactly once) */
     int64_t pool_tiles;     /* 64x64 tiles in the pool (pool_bytes / tile bytes) */
     int64_t value_tiles;    /* tiles the factor's values would need without slot reuse */
+    int32_t uinv_out_of_tol; /* last run: diagonal U tiles whose inverse fails the check below */
 } sogl_factor_stats;
 int sogl_factor_get_stats(sogl_ctx* ctx, int32_t factor_id, sogl_factor_stats* st);
+/* The reference's check of every upper inverse (inv_check_diag, MatrixStdDouble.cpp:
+ * 2871-2937, run after each upperInv at BlockPlanner.cpp:573-577, which prints
+ * "diag min:..  diag max:.." or "middle min:..  middle max:.." and " upper out of
+ * tolerance " and goes on): after each LU factor run the diagonal of U_ii U_ii^-1 must
+ * lie within 1e-3 of 1 and its entries (i, i/2) within 1e-3 of 0 (a NaN counts as 10).
+ * n_failed receives the number of block rows that fail; detail (may be NULL; up to 16 x
+ * 4 doubles) receives block row, 0 (diagonal) / 1 (middle), min and max of each of the
+ * first 16 recorded.  L L^T factors make no upper inverses and report 0. */
+int sogl_factor_get_uinv_check(sogl_ctx* ctx, int32_t factor_id, int32_t* n_failed, double* detail);
 /* Dynamic chaining of the persistent executor: a workgroup that has made a tile runs a
  * near-critical GEMM task that reads it right away, when that task's other operands are
  * already published, instead of handing it over through memory.  claims = the most such

@@ -77,6 +77,8 @@ struct Factor {
     DevBuf cell_begin, cell_pos, cell_val;  // rescatter: the input cells
     std::vector<int32_t> war;   // write-after-read flags of reused result slots (DevTask)
     DevBuf war_d;
+    DevBuf uinv_chk_d;                // the upper inverses' check of the last run (k_check_uinv)
+    std::vector<double> uinv_check;   // its host copy
     int32_t zero_slot = -1;
     size_t elem = 8;            // bytes per tile element in the pool
     // host copies

@@ -95,6 +95,59 @@ __global__ void k_ctrl_reset(int* ctrl) {
     if (threadIdx.x < CTRL_WORDS) ctrl[threadIdx.x] = 0;
 }
 
+// The reference's check of every upper inverse it makes (inv_check_diag,
+// MatrixStdDouble.cpp:2871-2937, after each upperInv, BlockPlanner.cpp:573-577): the
+// diagonal of U U^-1 must lie within 1e-3 of 1 (its min and max start at 1), and the
+// entries (i, i/2), i >= 2, within 1e-3 of 0 (min and max start at 0); a NaN sum sets
+// the max to 10; the middle entries are looked at only when the diagonal passes.  Each
+// sum runs over k = 0..63 in FP64, in the reference's order.  One workgroup per block
+// row's U_ii and U_ii^-1, after the executor; out[0] counts the tiles out of tolerance
+// and out[1 + 4 j] .. holds (block row, 0 diagonal / 1 middle, min, max) of the first
+// UINV_KEEP of them (in no particular order).
+constexpr int UINV_KEEP = 16;
+template <typename T>
+__global__ __launch_bounds__(NT) void k_check_uinv(const T* __restrict__ pool, const int32_t* __restrict__ u_diag,
+                                                   const int32_t* __restrict__ uinv, double* out) {
+    __shared__ double s_v[2][TS];
+    __shared__ int s_nan[2][TS];
+    const int blk = blockIdx.x, tid = threadIdx.x;
+    const T* U = pool + (size_t)u_diag[blk] * TE;
+    const T* Z = pool + (size_t)uinv[blk] * TE;
+    if (tid < 2 * TS) {
+        const int which = tid / TS, i = tid % TS, j = which == 0 ? i : i / 2;
+        double sum = 0.0;
+        if (which == 0 || i >= 2)
+            for (int k = 0; k < TS; k++) sum += (double)U[i * TS + k] * (double)Z[k * TS + j];
+        s_v[which][i] = sum;
+        s_nan[which][i] = sum != sum;
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    const double tor = 1e-3;
+    for (int which = 0; which < 2; which++) {
+        const double c = which == 0 ? 1.0 : 0.0;
+        double mn = c, mx = c;
+        for (int i = which == 0 ? 0 : 2; i < TS; i++) {
+            if (s_nan[which][i]) {
+                mx = 10;
+                continue;
+            }
+            mn = fmin(mn, s_v[which][i]);
+            mx = fmax(mx, s_v[which][i]);
+        }
+        if (mx > c + tor || mx < c - tor || mn > c + tor || mn < c - tor) {
+            const int q = atomicAdd(reinterpret_cast<int*>(out), 1);
+            if (q < UINV_KEEP) {
+                out[1 + 4 * q] = blk;
+                out[2 + 4 * q] = which;
+                out[3 + 4 * q] = mn;
+                out[4 + 4 * q] = mx;
+            }
+            return;  // as the reference: a failed diagonal ends the check
+        }
+    }
+}
+
 template <typename T>
 __global__ void k_fill(T* p, size_t n, T v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1763,6 +1816,7 @@ int upload_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells,
     if ((rc = dev_alloc(c, f->ready_d, (size_t)f->n_slots * 8, "ready flags")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->mask_d, (size_t)f->n_locs * 4, "block masks")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->ctrl_d, CTRL_WORDS * 4, "control")) != SOGL_OK) return rc;
+    if ((rc = dev_alloc(c, f->uinv_chk_d, (1 + 4 * UINV_KEEP) * 8, "upper inverse check")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->queue_d, std::max<size_t>(f->queue.size(), 1) * 4, "task queue")) != SOGL_OK) return rc;
     if ((rc = dev_alloc(c, f->chk_d, std::max<size_t>(f->chk.size(), 1) * sizeof(ChainChk), "chain checks")) != SOGL_OK)
         return rc;
@@ -1909,7 +1963,7 @@ int build_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells, 
 void free_factor(Factor* f) {
     if (f->graph_exec) (void)hipGraphExecDestroy(f->graph_exec);
     if (f->graph) (void)hipGraphDestroy(f->graph);
-    for (DevBuf* b : {&f->pool, &f->mask_d, &f->queue_d, &f->war_d, &f->cell_begin, &f->cell_pos, &f->cell_val, &f->chk_d, &f->tasks_d, &f->pairs_d, &f->deps_d, &f->ready_d, &f->ctrl_d,
+    for (DevBuf* b : {&f->pool, &f->mask_d, &f->queue_d, &f->war_d, &f->uinv_chk_d, &f->cell_begin, &f->cell_pos, &f->cell_val, &f->chk_d, &f->tasks_d, &f->pairs_d, &f->deps_d, &f->ready_d, &f->ctrl_d,
                       &f->op_tasks_d, &f->op_pairs_d, &f->inputs_d, &f->l_rowptr, &f->l_col,
                       &f->l_slot, &f->u_rowptr, &f->u_col, &f->u_slot, &f->l_diag_d, &f->u_diag_d,
                       &f->linv_d, &f->uinv_d, &f->solve_ready, &f->solve_ctrl, &f->vec_b, &f->vec_x,
@@ -2226,6 +2280,18 @@ int run_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
     if (f->llt) rc = masks ? go(TT{}, TT{}) : go(TT{}, FF{});
     else rc = masks ? go(FF{}, TT{}) : go(FF{}, FF{});
     if (rc != SOGL_OK) return rc;
+    if (!f->llt && f->block_rows > 0) {  // the upper inverses' check (L L^T factors make none)
+        const int R = f->block_rows;
+        hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(256), 0, c->stream, (double*)f->uinv_chk_d.p,
+                           (size_t)(1 + 4 * UINV_KEEP), 0.0);
+        if (f->elem == 8)
+            hipLaunchKernelGGL(k_check_uinv<double>, dim3(R), dim3(NT), 0, c->stream, (const double*)f->pool.p,
+                               (const int32_t*)f->u_diag_d.p, (const int32_t*)f->uinv_d.p, (double*)f->uinv_chk_d.p);
+        else
+            hipLaunchKernelGGL(k_check_uinv<float>, dim3(R), dim3(NT), 0, c->stream, (const float*)f->pool.p,
+                               (const int32_t*)f->u_diag_d.p, (const int32_t*)f->uinv_d.p, (double*)f->uinv_chk_d.p);
+        HIP_TRY(c, hipGetLastError());
+    }
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
     HIP_TRY(c, hipEventSynchronize(c->ev1));
     float ms = 0;
@@ -2235,6 +2301,9 @@ int run_factor(Context* c, Factor* f, int mode, int chunk, int wg_per_cu) {
     f->exec_ms = ms;
     int ctrl[CTRL_WORDS];
     HIP_TRY(c, hipMemcpy(ctrl, f->ctrl_d.p, sizeof(ctrl), hipMemcpyDeviceToHost));
+    f->uinv_check.assign((size_t)(1 + 4 * UINV_KEEP), 0.0);
+    if (!f->llt && f->block_rows > 0)
+        HIP_TRY(c, hipMemcpy(f->uinv_check.data(), f->uinv_chk_d.p, f->uinv_check.size() * 8, hipMemcpyDeviceToHost));
     f->growth_tiles = ctrl[2];
     f->claimed = mode == SOGL_EXEC_PERSISTENT ? ctrl[4] : 0;
     f->dropped = mode == SOGL_EXEC_PERSISTENT ? ctrl[5] : 0;
@@ -2375,6 +2444,24 @@ extern "C" int sogl_factor_get_stats(sogl_ctx* x, int32_t factor_id, sogl_factor
     st->tasks_run = f->ran;
     st->pool_tiles = f->n_locs;
     st->value_tiles = f->n_slots;
+    int32_t nbad = 0;
+    if (!f->uinv_check.empty()) std::memcpy(&nbad, f->uinv_check.data(), 4);
+    st->uinv_out_of_tol = nbad;
+    return SOGL_OK;
+}
+
+extern "C" int sogl_factor_get_uinv_check(sogl_ctx* x, int32_t factor_id, int32_t* n_failed, double* detail) {
+    if (!x || !n_failed) return set_error(nullptr, SOGL_ERR_ARG, "null argument");
+    Context* c = &x->c;
+    std::lock_guard<std::mutex> g(c->mu);
+    Factor* f = find_factor(c, factor_id);
+    if (!f) return set_error(c, SOGL_ERR_ARG, "unknown factor id");
+    int32_t nbad = 0;
+    if (!f->uinv_check.empty()) std::memcpy(&nbad, f->uinv_check.data(), 4);
+    *n_failed = nbad;
+    if (detail)
+        for (int q = 0; q < std::min<int32_t>(nbad, UINV_KEEP); q++)
+            for (int e = 0; e < 4; e++) detail[4 * q + e] = f->uinv_check[(size_t)(1 + 4 * q + e)];
     return SOGL_OK;
 }
 

@@ -12,6 +12,7 @@
 //   --tol T                refinement tolerance (default 1e-12)
 //   --exec persistent|levels|perop|auto   (auto: tune the executor configuration
 //                          on the factor first; 'kernel time' is the tuned factor)
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -87,6 +88,18 @@ int main(int argc, char** argv) {
     auto tc = std::chrono::steady_clock::now();
     if ((rc = sogl_factor_run(ctx, fid, mode))) return die("factor", rc, sogl_ctx_error(ctx));
     double t_calc = std::chrono::duration<double>(std::chrono::steady_clock::now() - tc).count();
+    {  // the reference prints these while it factors (BlockPlanner.cpp:573-577, MatrixStdDouble.cpp:2908-2934)
+        int32_t nbad = 0;
+        double det[64];
+        if ((rc = sogl_factor_get_uinv_check(ctx, fid, &nbad, det))) return die("check", rc, sogl_ctx_error(ctx));
+        for (int q = 0; q < std::min<int32_t>(nbad, 16); q++) {
+            const char* what = det[4 * q + 1] == 0 ? "diag" : "middle";
+            std::cout << std::string(what) + " min:" + std::to_string(det[4 * q + 2]) + "  " + what +
+                             " max:" + std::to_string(det[4 * q + 3])
+                      << std::endl;
+            std::cout << " upper out of tolerance " << std::endl;
+        }
+    }
     std::cout << "kernel time: " << t_calc << '\n';
 
     std::vector<double> bp((size_t)N), xp((size_t)N), y((size_t)N), x((size_t)a.n);

@@ -64,7 +64,7 @@ class FactorStats(C.Structure):
                 ("n_tasks", C.c_int64), ("n_levels", C.c_int32), ("growth_tiles", C.c_int32),
                 ("max_growth", C.c_double), ("zero_blocks", C.c_double), ("block_masks", C.c_int32),
                 ("claimed_tasks", C.c_int32), ("dropped_tasks", C.c_int32), ("tasks_run", C.c_int32),
-                ("pool_tiles", C.c_int64), ("value_tiles", C.c_int64)]
+                ("pool_tiles", C.c_int64), ("value_tiles", C.c_int64), ("uinv_out_of_tol", C.c_int32)]
 
 
 def _ptr(a: np.ndarray, ctype):
@@ -128,6 +128,7 @@ def load(name: str = "libsogl.so") -> C.CDLL:
         "sogl_factor_set_masks": (C.c_int, [vp, i32, i32]),
             "sogl_factor_set_chaining": (C.c_int, [vp, i32, i32]),
             "sogl_ctx_set_pool_reuse": (C.c_int, [vp, i32]),
+            "sogl_factor_get_uinv_check": (C.c_int, [vp, i32, P(i32), P(dbl)]),
             "sogl_refine_attach": (C.c_int, [vp, i32, vp]),
             "sogl_solve_refined": (C.c_int, [vp, i32, P(dbl), P(dbl), i32, dbl, i32, P(i32),
                                              P(dbl)]),
@@ -399,6 +400,16 @@ class Device:
         REUSE_OFF (the per-op executor needs it off), or REUSE_AUTO (the default)."""
         m = REUSE_ON if mode is True else REUSE_OFF if mode is False else int(mode)
         self._check(self.lib.sogl_ctx_set_pool_reuse(self.h, m))
+
+    def uinv_check(self, fid: int):
+        """The last run's upper-inverse check: (number of block rows out of tolerance,
+        [(block row, 'diag' | 'middle', min, max), ...] for up to 16 of them)."""
+        n = C.c_int32()
+        d = np.zeros(64)
+        self._check(self.lib.sogl_factor_get_uinv_check(self.h, fid, C.byref(n), _ptr(d, C.c_double)))
+        rows = [(int(d[4 * q]), "diag" if d[4 * q + 1] == 0 else "middle", float(d[4 * q + 2]), float(d[4 * q + 3]))
+                for q in range(min(n.value, 16))]
+        return n.value, rows
 
     def set_masks(self, fid: int, mode: int):
         """16x16 block-mask policy: MASKS_OFF, MASKS_ON or MASKS_AUTO (the default)."""

@@ -22,7 +22,9 @@ per-task invalidate: every tile store is write-through (global_store ... sc1), e
 wave waits for its own stores (s_waitcnt vmcnt(0), marked "publish: stores done")
 before the barrier that comes ahead of the flag stores, and the only cache invalidate
 is the one each workgroup makes when it starts (no tile of the run can be cached
-before it is made, since every slot is written once and read only after its flag).
+before it is made, since every slot is written once and read only after its flag; a
+factor that reuses pool slots runs the kernels built to read tiles with coherent
+loads instead).
 The third check reads those rules off the compiled persistent kernels (no
 buffer_wbl2, one buffer_inv before the first barrier, stores write-through), and that
 no kernel is built for TgSplit mode (where a workgroup's waves may sit on different CUs

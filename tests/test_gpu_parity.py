@@ -161,6 +161,30 @@ def test_lu_tile_fp32_growth_check(dev):
         assert np.abs(got - want).max() <= 1e-6 * np.abs(want).max()
 
 
+@pytest.mark.parametrize("precision", [sogl.FP64, sogl.FP32])
+def test_upper_inverse_check(dev, precision):
+    """The reference's check of each upper inverse (inv_check_diag): a well-conditioned
+    tile passes; a tile holding a NaN fails on the diagonal of U U^-1 with the
+    reference's max of 10, at its block row, and the factor run still completes."""
+    ops = [op(1, 0, 1, 2, 3, stage=2, seq=0), op(3, 0, 3, 5, stage=3, seq=1)]
+    out = []
+    for bad in (False, True):
+        a = dd_tile()
+        if bad:
+            a[5, 9] = np.nan
+        fid = dev.load_ops(6, np.array(ops, np.int32), np.array([1], np.int32), a[None],
+                           np.array([[0, 0, 2]], np.int32), np.array([[0, 0, 3]], np.int32), 1, precision)
+        dev.factor(fid)
+        out.append((dev.uinv_check(fid), dev.stats(fid).uinv_out_of_tol))
+        dev.release(fid)
+    (n0, rows0), s0 = out[0]
+    (n1, rows1), s1 = out[1]
+    assert n0 == 0 and rows0 == [] and s0 == 0
+    assert n1 == 1 and s1 == 1
+    blk, kind, mn, mx = rows1[0]
+    assert blk == 0 and kind == "diag" and mx == 10.0 and mn <= 1.0
+
+
 @pytest.mark.parametrize("mode", [sogl.EXEC_PERSISTENT, sogl.EXEC_LEVELS])
 @pytest.mark.parametrize("precision,tol", [(sogl.FP64, 1e-13), (sogl.FP32, 2e-5)])
 def test_lu_tile_fused_inverses(dev, precision, tol, mode):
