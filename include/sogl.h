@@ -206,11 +206,13 @@ int sogl_ctx_set_pool_reuse(sogl_ctx* ctx, int32_t mode);
  * every operand produced at an earlier dependency level and waited for, and the pool
  * slot reuse (see sogl_ctx_set_pool_reuse) handing a slot over only from a value to the
  * task that alone reads it.  sogl_factor_load runs the same checks before any upload.
- * info (may be NULL) receives 9 counts: tasks, value tiles (the pool without slot
+ * info (may be NULL) receives 10 counts: tasks, value tiles (the pool without slot
  * reuse), dependency levels, GEMM pairs, dependencies, folded copy/negate/subtract ops,
  * aliased repeated inverses, lu / llt tasks chained to the GEMM that makes their source
- * (same workgroup, no hand-off), and pool tiles with slot reuse (the _plan form's inputs
- * can be scattered again, so their slots are reused too). */
+ * (same workgroup, no hand-off), pool tiles with slot reuse (the _plan form's inputs
+ * can be scattered again, so their slots are reused too), and the GEMM tasks (one pair
+ * each) that make the solve's two-row blocks: per pair of block rows 2p, 2p+1, two for
+ * the L block when L_2p+1,2p is present and two for the U block when U_2p,2p+1 is. */
 int sogl_graph_check(const sogl_factor_desc* desc, int64_t* info);
 int sogl_graph_check_plan(const sogl_plan* plan, int32_t precision, int64_t* info);
 /* Convenience: sogl_factor_load from a host plan. */

@@ -92,6 +92,7 @@ struct Factor {
     int64_t n_inv_aliased = 0;  // repeated inverses of one factor tile that share its slot
     int64_t n_folded = 0;       // copies, negations and subtractions folded into other tasks
     int64_t n_chained = 0;      // lu / llt tasks run by the workgroup that made their source
+    int64_t n_pair_tasks = 0;   // GEMM tasks making the solve's two-row blocks X_p / Y_p
     // device arrays
     DevBuf pool, tasks_d, pairs_d, deps_d, ready_d, ctrl_d, op_tasks_d, op_pairs_d, inputs_d, queue_d, chk_d;
     std::vector<ChainChk> chk;             // chain candidates' check entries (see DevTask)
@@ -103,6 +104,7 @@ struct Factor {
     // triangular solve structure (block CSR by block row, off-diagonal only)
     std::vector<int32_t> l_diag, u_diag, linv_slot, uinv_slot;  // per block row
     DevBuf l_rowptr, l_col, l_slot, u_rowptr, u_col, u_slot, l_diag_d, u_diag_d, linv_d, uinv_d;
+    DevBuf pairx_d, pairy_d;  // per row pair: slot of X_p / Y_p (the two-row solve steps), -1: zero
     DevBuf solve_ready, solve_ctrl, vec_b, vec_x, vec_y;
     size_t vec_cap = 0;  // doubles per vector buffer
     int32_t solve_epoch = 0;

@@ -734,6 +734,7 @@ struct HostGraph {
     std::vector<Pair> pairs, op_pairs;
     std::vector<int32_t> deps, lp, lc, ls, up, uc, us;
     std::vector<int32_t> kept;  // value ids that keep their slot to the end (slot reuse)
+    std::vector<int32_t> pair_x, pair_y;  // the solve's two-row inverse blocks per row pair (-1: zero)
 };
 
 // Dynamic chaining of the persistent executor.  A workgroup that has just made a tile
@@ -1152,6 +1153,7 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
     std::vector<int32_t> l_trans;
     if (llt) {
         for (int64_t k = 0; k < d->n_l; k++) {
+            if (d->l_ids[k] <= 0 || d->l_ids[k] >= S) return set_error(c, SOGL_ERR_ARG, "factor tile out of range");
             const int32_t s0 = slot[(size_t)d->l_ids[k]];
             if (s0 < 0) return set_error(c, SOGL_ERR_ARG, "factor tile never produced");
             l_trans.push_back(transposed(s0));
@@ -1159,6 +1161,58 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
         for (int32_t i = 0; i < R; i++) {
             f->u_diag[(size_t)i] = transposed(f->l_diag[(size_t)i]);
             f->uinv_slot[(size_t)i] = transposed(f->linv_slot[(size_t)i]);
+        }
+    }
+    // The solve's two-row steps (block rows I = 2p, I + 1 together): the off-diagonal
+    // block of the inverse of [L_II 0; L_I+1,I L_I+1,I+1],
+    //     X_p = -L_I+1,I+1^-1 (L_I+1,I L_II^-1),
+    // and of [U_II U_I,I+1; 0 U_I+1,I+1],
+    //     Y_p = -U_II^-1 (U_I,I+1 U_I+1,I+1^-1),
+    // each as two GEMM tasks of the factor over final factor tiles (nothing in the factor
+    // reads them; the solve does).  Without the sub-diagonal tile the block is zero.
+    {
+        std::map<std::pair<int32_t, int32_t>, int32_t> lt, ut;  // (block row, block col) -> slot
+        // (ids out of range or never produced are reported by the solve structure below)
+        auto ok_id = [&](int32_t id) { return id > 0 && id < S && slot[(size_t)id] >= 0; };
+        for (int64_t k = 0; k < d->n_l; k++) {
+            if (!ok_id(d->l_ids[k])) continue;
+            lt[{d->l_bi[k], d->l_bj[k]}] = slot[(size_t)d->l_ids[k]];
+            if (llt) ut[{d->l_bj[k], d->l_bi[k]}] = l_trans[(size_t)k];  // U_JI = L_IJ^T
+        }
+        for (int64_t k = 0; k < d->n_u && !llt; k++)
+            if (ok_id(d->u_ids[k])) ut[{d->u_bi[k], d->u_bj[k]}] = slot[(size_t)d->u_ids[k]];
+        f->n_pair_tasks = 0;
+        auto gemm = [&](int32_t x, int32_t y, bool neg) -> int32_t {  // a new slot = (neg ? -1 : 1) x y
+            const int32_t out = new_slot();
+            f->n_pair_tasks++;
+            const Pair pr{x, neg ? -y - 1 : y};
+            DevTask nt{};
+            nt.kind = TK_GEMM;
+            nt.out0 = out;
+            nt.out1 = nt.out2 = nt.out3 = -1;
+            tasks.push_back(nt);
+            tpairs.push_back({pr});
+            tstage.push_back(0);
+            lu_op_task.push_back(-1);
+            prod[(size_t)out] = (int32_t)tasks.size() - 1;
+            if (keep_ops) {
+                nt.a = (int32_t)op_pairs.size();
+                nt.b = 1;
+                op_pairs.push_back(pr);
+                f->op_tasks.push_back(nt);
+            }
+            return out;
+        };
+        hg.pair_x.assign((size_t)(R / 2), -1);
+        hg.pair_y.assign((size_t)(R / 2), -1);
+        for (int32_t q = 0; q < R / 2; q++) {
+            const int32_t I = 2 * q;
+            auto lsub = lt.find({I + 1, I});
+            if (lsub != lt.end())
+                hg.pair_x[(size_t)q] = gemm(f->linv_slot[(size_t)I + 1], gemm(lsub->second, f->linv_slot[(size_t)I], false), true);
+            auto usub = ut.find({I, I + 1});
+            if (usub != ut.end())
+                hg.pair_y[(size_t)q] = gemm(f->uinv_slot[(size_t)I], gemm(usub->second, f->uinv_slot[(size_t)I + 1], false), true);
         }
     }
     f->n_slots = next;
@@ -1495,9 +1549,10 @@ int validate_graph(Context* c, const sogl_factor_desc* d, const Factor* f, const
 //    pairs).  The gap keeps those waits from holding workgroups up: the persistent
 //    executor runs far ahead of the slowest chain on wide graphs (C2 FP32, 1 workgroup
 //    per CU: 82 ms with a gap of 2 levels, 64 with 1024, 63 without reuse), and costs
-//    little memory (C5 FP64: 13.34 GB at 2, 13.39 at 1024);
+//    little memory (C5 FP64 before the two-row blocks: 13.34 GB at 2, 13.39 at 1024);
 //  * a new slot.
-// The factor tiles, the diagonal inverses and the zero tile keep their slots; so do the
+// The factor tiles, the diagonal inverses, the solve's two-row blocks and the zero tile
+// keep their slots; so do the
 // inputs, unless they can be scattered again before each run.  The per-op executor
 // accumulates each unit op into the result's slot and runs only without reuse.
 // Values keep their own ready flags: the numbers used so far become value ids, and every
@@ -1578,8 +1633,10 @@ void assign_slots(Factor* f, HostGraph& hg, bool reuse, bool donate_inputs) {
     auto nread = [&](int32_t v) { return rb[(size_t)v + 1] - rb[(size_t)v]; };
     std::vector<char> keep((size_t)V, 0);
     keep[(size_t)f->zero_slot] = 1;
-    for (const auto* pv : {&f->l_slots, &f->u_slots, &f->l_diag, &f->u_diag, &f->linv_slot, &f->uinv_slot, &hg.ls, &hg.us})
-        for (int32_t v : *pv) keep[(size_t)v] = 1;
+    for (const auto* pv : {&f->l_slots, &f->u_slots, &f->l_diag, &f->u_diag, &f->linv_slot, &f->uinv_slot, &hg.ls, &hg.us,
+                           &hg.pair_x, &hg.pair_y})
+        for (int32_t v : *pv)
+            if (v >= 0) keep[(size_t)v] = 1;
     if (!f->rescatter)
         for (int32_t v : f->input_slots) keep[(size_t)v] = 1;
     for (int64_t v = 0; v < V; v++)
@@ -1672,7 +1729,8 @@ void assign_slots(Factor* f, HostGraph& hg, bool reuse, bool donate_inputs) {
         p.a = p.a < 0 ? -L(p.fa) - 1 : L(p.fa);
         p.b = p.b < 0 ? -L(p.fb) - 1 : L(p.fb);
     }
-    for (auto* pv : {&f->l_slots, &f->u_slots, &f->l_diag, &f->u_diag, &f->linv_slot, &f->uinv_slot, &hg.ls, &hg.us})
+    for (auto* pv : {&f->l_slots, &f->u_slots, &f->l_diag, &f->u_diag, &f->linv_slot, &f->uinv_slot, &hg.ls, &hg.us,
+                     &hg.pair_x, &hg.pair_y})
         for (int32_t& v : *pv) v = L(v);
 }
 
@@ -1848,6 +1906,7 @@ int upload_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells,
         (rc = up_i32(f->u_col, hg.uc, "U cols")) || (rc = up_i32(f->u_slot, hg.us, "U slots")) ||
         (rc = up_i32(f->l_diag_d, f->l_diag, "L diag")) || (rc = up_i32(f->u_diag_d, f->u_diag, "U diag")) ||
         (rc = up_i32(f->linv_d, f->linv_slot, "L inv")) || (rc = up_i32(f->uinv_d, f->uinv_slot, "U inv")) ||
+        (rc = up_i32(f->pairx_d, hg.pair_x, "row pair L blocks")) || (rc = up_i32(f->pairy_d, hg.pair_y, "row pair U blocks")) ||
         (rc = up_i32(f->inputs_d, f->input_slots, "input slots")))
         return rc;
     hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(256), dim3(256), 0, st, (unsigned long long*)f->ready_d.p,
@@ -1963,7 +2022,7 @@ int build_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells, 
 void free_factor(Factor* f) {
     if (f->graph_exec) (void)hipGraphExecDestroy(f->graph_exec);
     if (f->graph) (void)hipGraphDestroy(f->graph);
-    for (DevBuf* b : {&f->pool, &f->mask_d, &f->queue_d, &f->war_d, &f->uinv_chk_d, &f->cell_begin, &f->cell_pos, &f->cell_val, &f->chk_d, &f->tasks_d, &f->pairs_d, &f->deps_d, &f->ready_d, &f->ctrl_d,
+    for (DevBuf* b : {&f->pool, &f->mask_d, &f->queue_d, &f->war_d, &f->uinv_chk_d, &f->pairx_d, &f->pairy_d, &f->cell_begin, &f->cell_pos, &f->cell_val, &f->chk_d, &f->tasks_d, &f->pairs_d, &f->deps_d, &f->ready_d, &f->ctrl_d,
                       &f->op_tasks_d, &f->op_pairs_d, &f->inputs_d, &f->l_rowptr, &f->l_col,
                       &f->l_slot, &f->u_rowptr, &f->u_col, &f->u_slot, &f->l_diag_d, &f->u_diag_d,
                       &f->linv_d, &f->uinv_d, &f->solve_ready, &f->solve_ctrl, &f->vec_b, &f->vec_x,
@@ -2217,6 +2276,7 @@ int graph_check(const sogl_factor_desc* d, bool cells, int64_t* info) {
         info[6] = f.n_inv_aliased;
         info[7] = f.n_chained;
         info[8] = f.n_locs;
+        info[9] = f.n_pair_tasks;
     }
     return rc;
 }

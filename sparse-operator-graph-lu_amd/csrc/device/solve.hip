@@ -3,17 +3,18 @@
 // (BlockPlanner.cpp:669-862), which recurse over the L and U quadtrees with
 // one right-hand side, serially.
 //
-// Here each block row is a task of a persistent kernel: forward,
+// Here each block row (with 1-15 right-hand sides: each pair of block rows) is a task
+// of a persistent kernel: forward,
 //     y_I = Linv_II (b_I - sum_{J<I} L_IJ y_J)
 // backward,
 //     x_I = Uinv_II (y_I - sum_{J>I} U_IJ x_J)
-// with the inverses of the diagonal tiles made once per factor.  Block rows
-// are taken in dependency order (ascending / descending) and a row waits only
-// for the rows it reads, so independent rows overlap and the chain only
-// serialises where the factor's block structure does.  With 1-15 right-hand
-// sides a row waits on the stored values themselves (a pending marker until
-// stored); the MFMA form for more right-hand sides waits on per-row ready flags.
-// Sums run in FP64 for every tile precision.
+// with the inverses of the diagonal tiles (and, for row pairs, the off-diagonal block
+// of the pair's 2x2 diagonal block's inverse) made once per factor.  Tasks are taken in
+// dependency order (ascending / descending) and a task waits only for the rows it
+// reads, so independent rows overlap and the chain only serialises where the factor's
+// block structure does.  With 1-15 right-hand sides a task waits on the stored values
+// themselves (a pending marker until stored); the MFMA form for more right-hand sides
+// waits on per-row ready flags.  Sums run in FP64 for every tile precision.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -44,15 +45,9 @@ struct SolveArgs {
     int* ready;
     int* ctrl;            // [0] row counter, [1] abort
     const uint32_t* mask; // per slot: the factor's 16x16 block masks (a zero tile is skipped)
+    const int32_t* pairinv;  // per row pair: slot of the 2x2 diagonal block's inverse's
+                             // off-diagonal block (the one-to-15-RHS form), -1: zero
 };
-
-// thread (r, kp) of a row task owns tile row r, columns 16kp .. 16kp+15
-template <typename T>
-__device__ inline void load_seg(const T* tile, int r, int kp, double* seg) {
-    const T* p = tile + r * TS + kp * 16;
-#pragma unroll
-    for (int k = 0; k < 16; k++) seg[k] = (double)p[k];
-}
 
 // The abort word and the wall clock are looked at every SOLVE_SPIN_CHECK polls only, so
 // that each poll costs one round trip to the device coherence point.
@@ -96,115 +91,228 @@ __global__ void k_fill_pending(double* out, int64_t ld, int64_t rows, int nrhs) 
 
 // The 64 values of one column of row J, which another task stores, into this wave's
 // LDS slot ls once all are stored: lane l polls value l (one coalesced load per spin
-// for the wave), the wave's ballot says when none is pending.  On the wait bound (or
-// an abort elsewhere) it raises the abort flag and leaves zeros.  The caller reads ls
-// after it (LDS accesses of one wave complete in order).
-__device__ inline void load_row_stored(const double* col, double* ls, int lane, const SolveArgs& a) {
+// for the wave), the wave's ballot says when none is pending.  NC = 2 polls the same
+// column of a second row (col + step) in the same spin, into ls[64 ..]: the two rows a
+// row-pair task stores together cost one wait.  On the wait bound (or an abort
+// elsewhere) it raises the abort flag and leaves zeros.  The caller reads ls after it
+// (LDS accesses of one wave complete in order).
+template <int NC = 1>
+__device__ inline void load_row_stored(const double* col, double* ls, int lane, const SolveArgs& a,
+                                       int64_t step = 0) {
     const unsigned long long* q = reinterpret_cast<const unsigned long long*>(col) + lane;
     long long t0 = 0;
-    unsigned long long u;
+    unsigned long long u[NC];
     for (int spin = 0;; spin++) {
-        u = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (!__ballot(u == SOLVE_PENDING)) break;
+        bool pend = false;
+#pragma unroll
+        for (int i = 0; i < NC; i++) {
+            u[i] = __hip_atomic_load(q + i * step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            pend |= u[i] == SOLVE_PENDING;
+        }
+        if (!__ballot(pend)) break;
         if (spin == 0) {
             t0 = wall_clock64();
         } else if (spin % SOLVE_SPIN_CHECK == 0 &&
                    (__hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
                     wall_clock64() - t0 > SOLVE_SPIN_LIMIT)) {
             __hip_atomic_store(a.ctrl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            u = 0;  // the bit pattern of 0.0
+#pragma unroll
+            for (int i = 0; i < NC; i++) u[i] = 0;  // the bit pattern of 0.0
             break;
         }
         __builtin_amdgcn_s_sleep(2);
     }
-    ls[lane] = __longlong_as_double((long long)u);
+#pragma unroll
+    for (int i = 0; i < NC; i++) ls[i * TS + lane] = __longlong_as_double((long long)u[i]);
     __builtin_amdgcn_wave_barrier();
 }
 
-// One block row per task, rows taken in dependency order.  The rows a row reads are
-// consumed oldest first (the order they complete in), so once the newest dependency
-// lands only one tile product and the diagonal product remain on the chain.  A thread
-// (r, kp) of a late row waits only for the 16 values of x_J its 16x16 block
-// multiplies, and every thread skips blocks the factor's masks mark zero (a zero block
-// never reads x_J: a pending marker times zero would be NaN).  CW right-hand sides per
-// pass over the row.
+// Two block rows per task (a row pair p: rows 2p, 2p+1), pairs taken in dependency
+// order, so each hand-off along the chain serves two rows.  With A the pair's row solved
+// first (forward 2p, backward 2p+1) and B the other, and r_A, r_B the right-hand sides
+// less the products of every earlier row (B's entry in column A left out):
+//     x_A = D_A^-1 r_A,   x_B = D_B^-1 r_B + C_p r_A,
+// where C_p is the off-diagonal block of the inverse of the pair's 2x2 diagonal block
+// (forward X_p = -L_BB^-1 L_BA L_AA^-1, backward Y_p = -U_BB^-1 U_BA U_AA^-1; pairinv,
+// -1 when L_BA / U_BA is absent and the block is zero), made by two GEMM tasks of the
+// factor.  Waves 0-3 (group 0) sum row A, waves 4-7 (group 1) row B, each over its own
+// row's entries in dependency order (oldest first), independently up to one barrier
+// before the diagonal products.  The entries' columns, slots and masks are read 64 at a
+// time, one per lane, and taken from their lane (readlane), so no load chain runs
+// between two waits; the next nonzero tile is loaded while the current one's column is
+// polled; two consecutive entries stored by the same pair task (the usual last two) are
+// polled in one spin.  So once the newest dependency lands only its tile products and
+// the diagonal products remain on the chain.  A tile whose masks are all zero is
+// skipped with no wait, and every thread skips the 16x16 blocks the factor's masks mark
+// zero.  CW right-hand sides per pass over the pair.  An odd last row is a pair of its
+// own (B absent).
+constexpr int NTP = 2 * NT;  // threads of a row-pair task
+
+// one thread's 16 elements of a tile row.  Callers branch around it rather than
+// select per element: a select makes the compiler wait for each load in turn.
+template <typename T>
+__device__ inline void seg16(const T* p, T* v) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = p[k];
+}
+
+// Sum over the 4 lanes of a quad (lanes 4i .. 4i+3) in every lane, by two DPP quad
+// permutes (lane ^ 1, then lane ^ 2); every lane of the wave must be active.
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ inline double quad_sum(double v) {
+    v += dpp_f64<0xB1>(v);
+    return v + dpp_f64<0x4E>(v);
+}
+
 template <typename T, int CW>
-__global__ __launch_bounds__(NT) void k_trsv_persistent(SolveArgs a) {
-    __shared__ double vec[TS * CW];
-    __shared__ double rowls[NT];  // per wave: one column of a polled row
+__global__ __launch_bounds__(NTP) void k_trsv_persistent(SolveArgs a) {
+    __shared__ double vec[2][TS * CW];
+    __shared__ double rowls[NTP / 64][2 * TS];  // per wave: one column of one or two polled rows
     __shared__ int s_row;
-    const int tid = threadIdx.x, r = tid >> 2, kp = tid & 3, lane = tid & 63;
-    double* ls = rowls + (tid & ~63);
+    const int tid = threadIdx.x, g = tid / NT, tg = tid % NT;
+    const int r = tg >> 2, kp = tg & 3, lane = tid & 63;
+    const int bsh = 4 * (r >> 4) + kp;  // this thread's 16x16 block in a tile's mask
+    double* ls = rowls[tid >> 6];
     const T* pool = reinterpret_cast<const T*>(a.pool);
+    const int ntask = (a.nbr + 1) / 2;
+    const int rowoff = r * TS + kp * 16;
     for (;;) {
         if (tid == 0) {
             const int abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_row = abort ? a.nbr : atomicAdd(a.ctrl, 1);
+            s_row = abort ? ntask : atomicAdd(a.ctrl, 1);
         }
         __syncthreads();
         const int q = __builtin_amdgcn_readfirstlane(s_row);  // wave-uniform loop control
-        if (q >= a.nbr) break;
-        const int I = a.forward ? q : a.nbr - 1 - q;
-        const int e0 = a.rowptr[I], e1 = a.rowptr[I + 1], cnt = e1 - e0;
-        const int64_t rowI = (int64_t)I * TS;
-        double dseg[16];
-        load_seg(pool + (size_t)a.dinv[I] * TE, r, kp, dseg);
+        if (q >= ntask) break;
+        const int p = a.forward ? q : ntask - 1 - q;
+        const int I0 = 2 * p, I1 = 2 * p + 1 < a.nbr ? 2 * p + 1 : -1;
+        const int IA = a.forward || I1 < 0 ? I0 : I1;
+        const int IB = I1 < 0 ? -1 : (a.forward ? I1 : I0);
+        const int I = g == 0 ? IA : IB;  // this group's row, -1: none
+        const int cpl = g == 1 && IB >= 0 ? a.pairinv[p] : -1;
+        // the diagonal inverse (and for row B the coupling block), loaded while the rows
+        // wait; a group without a row or coupling block loads A's inverse and ignores it
+        T dg[16], cb[16];
+        const bool hcb = cpl >= 0;
+        seg16(pool + (size_t)a.dinv[I >= 0 ? I : IA] * TE + rowoff, dg);
+        seg16(pool + (size_t)(hcb ? cpl : a.dinv[IA]) * TE + rowoff, cb);
+        const int e0 = I >= 0 ? a.rowptr[I] : 0, e1 = I >= 0 ? a.rowptr[I + 1] : 0;
+        // entries in dependency order: forward e0, e0+1, ..; backward e1-1, e1-2, ..; row
+        // B's last one (column IA: forward the largest column, backward the smallest) is
+        // the coupling block's
+        int n = e1 - e0;
+        if (g == 1 && n > 0 && a.col[a.forward ? e1 - 1 : e0] == IA) n--;
+        const int64_t rowI = (int64_t)(I >= 0 ? I : 0) * TS;
         for (int c0 = 0; c0 < a.nrhs; c0 += CW) {
             const int cw = min(CW, a.nrhs - c0);
             double acc[CW];
 #pragma unroll
             for (int c = 0; c < CW; c++)
-                acc[c] = (kp == 0 && c < cw) ? a.in[rowI + r + (int64_t)(c0 + c) * a.ld] : 0.0;
-            for (int i = 0; i < cnt; i++) {
-                const int e = a.forward ? e0 + i : e1 - 1 - i;
-                const uint32_t tm = a.mask[a.slot[e]];
-                if (tm == 0) continue;  // a zero tile: nothing to add, no wait for its row
-                const int J = a.col[e];
-                const bool blk = (tm >> (4 * (r >> 4) + kp)) & 1u;  // this thread's block is nonzero
-                double seg[16];
-                if (blk) load_seg(pool + (size_t)a.slot[e] * TE, r, kp, seg);
-                const double* xj = a.out + (int64_t)J * TS;
+                acc[c] = (I >= 0 && kp == 0 && c < cw) ? a.in[rowI + r + (int64_t)(c0 + c) * a.ld] : 0.0;
+            for (int base = 0; base < n; base += 64) {  // group-uniform
+                const int m = min(64, n - base);
+                int Jv = 0, sv = 0;
+                uint32_t mv = 0;
+                if (lane < m) {
+                    const int e = a.forward ? e0 + base + lane : e1 - 1 - base - lane;
+                    Jv = a.col[e];
+                    sv = a.slot[e];
+                    mv = a.mask[sv];
+                }
+                uint64_t rem = __ballot(mv != 0u);  // entries with a nonzero tile, in order
+                if (!rem) continue;
+                int j = __builtin_ctzll(rem);
+                rem &= rem - 1;
+                T t0[16], t1[16];  // a tile's elements are loaded only where its block is nonzero
+                uint32_t m0 = __builtin_amdgcn_readlane(mv, j);
+                if ((m0 >> bsh) & 1u) seg16(pool + (size_t)__builtin_amdgcn_readlane(sv, j) * TE + rowoff, t0);
+                for (;;) {
+                    const int J = __builtin_amdgcn_readlane(Jv, j);
+                    const bool has1 = rem != 0;
+                    const int j1 = has1 ? __builtin_ctzll(rem) : 0;
+                    const int J1 = __builtin_amdgcn_readlane(Jv, j1);
+                    const uint32_t m1 = has1 ? __builtin_amdgcn_readlane(mv, j1) : 0u;
+                    // the next tile, loaded while J's column is polled
+                    if ((m1 >> bsh) & 1u) seg16(pool + (size_t)__builtin_amdgcn_readlane(sv, j1) * TE + rowoff, t1);
+                    const bool both = has1 && (J1 >> 1) == (J >> 1);  // stored by one pair task
+                    const bool b0 = (m0 >> bsh) & 1u, b1 = both && ((m1 >> bsh) & 1u);
+                    const double* xj = a.out + (int64_t)J * TS;
 #pragma unroll
-                for (int c = 0; c < CW; c++) {
-                    if (c < cw) {
-                        double xc[16];
-                        const double* px = xj + (int64_t)(c0 + c) * a.ld;
-                        load_row_stored(px, ls, lane, a);  // the whole wave, also lanes whose block is zero
+                    for (int c = 0; c < CW; c++) {
+                        if (c < cw) {
+                            if (both)  // the whole wave
+                                load_row_stored<2>(xj + (int64_t)(c0 + c) * a.ld, ls, lane, a, (int64_t)(J1 - J) * TS);
+                            else
+                                load_row_stored<1>(xj + (int64_t)(c0 + c) * a.ld, ls, lane, a);
+                            // the column into registers first (one batch of LDS reads),
+                            // then one branch per tile
+                            double xc[16], s = 0;
 #pragma unroll
-                        for (int k = 0; k < 16; k++) xc[k] = ls[kp * 16 + k];
-                        __builtin_amdgcn_wave_barrier();  // ls is rewritten for the next column
-                        if (!blk) continue;
-                        double s = 0;
+                            for (int k = 0; k < 16; k++) xc[k] = ls[kp * 16 + k];
+                            if (b0) {
 #pragma unroll
-                        for (int k = 0; k < 16; k++) s += seg[k] * xc[k];
-                        acc[c] -= s;
+                                for (int k = 0; k < 16; k++) s += (double)t0[k] * xc[k];
+                            }
+                            if (both) {
+#pragma unroll
+                                for (int k = 0; k < 16; k++) xc[k] = ls[TS + kp * 16 + k];
+                                if (b1) {
+#pragma unroll
+                                    for (int k = 0; k < 16; k++) s += (double)t1[k] * xc[k];
+                                }
+                            }
+                            __builtin_amdgcn_wave_barrier();  // ls is rewritten for the next column
+                            acc[c] -= s;
+                        }
+                    }
+                    if (!has1) break;
+                    rem &= rem - 1;
+                    if (both) {  // both consumed: the one after is loaded now
+                        if (!rem) break;
+                        j = __builtin_ctzll(rem);
+                        rem &= rem - 1;
+                        m0 = __builtin_amdgcn_readlane(mv, j);
+                        if ((m0 >> bsh) & 1u) seg16(pool + (size_t)__builtin_amdgcn_readlane(sv, j) * TE + rowoff, t0);
+                    } else {
+                        j = j1;
+                        m0 = m1;
+#pragma unroll
+                        for (int k = 0; k < 16; k++) t0[k] = t1[k];
                     }
                 }
             }
 #pragma unroll
-            for (int c = 0; c < CW; c++) {
-                acc[c] += __shfl_xor(acc[c], 1);
-                acc[c] += __shfl_xor(acc[c], 2);
-            }
+            for (int c = 0; c < CW; c++) acc[c] = quad_sum(acc[c]);
             if (kp == 0) {
 #pragma unroll
-                for (int c = 0; c < CW; c++) vec[r * CW + c] = acc[c];
+                for (int c = 0; c < CW; c++) vec[g][r * CW + c] = acc[c];
             }
             __syncthreads();
             double o[CW];
 #pragma unroll
             for (int c = 0; c < CW; c++) {
-                double s = 0;
+                double v[16], s = 0;
 #pragma unroll
-                for (int k = 0; k < 16; k++) s += dseg[k] * vec[(kp * 16 + k) * CW + c];
+                for (int k = 0; k < 16; k++) v[k] = vec[g][(kp * 16 + k) * CW + c];
+#pragma unroll
+                for (int k = 0; k < 16; k++) s += (double)dg[k] * v[k];
+                if (g == 1 && hcb) {
+#pragma unroll
+                    for (int k = 0; k < 16; k++) v[k] = vec[0][(kp * 16 + k) * CW + c];
+#pragma unroll
+                    for (int k = 0; k < 16; k++) s += (double)cb[k] * v[k];
+                }
                 o[c] = s;
             }
 #pragma unroll
-            for (int c = 0; c < CW; c++) {
-                o[c] += __shfl_xor(o[c], 1);
-                o[c] += __shfl_xor(o[c], 2);
-            }
-            if (kp == 0) {
+            for (int c = 0; c < CW; c++) o[c] = quad_sum(o[c]);
+            if (kp == 0 && I >= 0) {
                 for (int c = 0; c < cw; c++)
                     __hip_atomic_store(reinterpret_cast<unsigned long long*>(a.out) + rowI + r + (int64_t)(c0 + c) * a.ld,
                                        (unsigned long long)__double_as_longlong(o[c]), __ATOMIC_RELAXED,
@@ -409,6 +517,7 @@ int launch_pass(Context* c, Factor* f, const SolveWork& w, bool forward, const d
     a.ready = w.ready;
     a.ctrl = w.ctrl;
     a.mask = (const uint32_t*)f->mask_d.p;
+    a.pairinv = (const int32_t*)(forward ? f->pairx_d.p : f->pairy_d.p);
     hipLaunchKernelGGL(k_solve_ctrl_reset, dim3(1), dim3(64), 0, c->stream, a.ctrl);
     int per_cu = 0;
     const bool mfma = nrhs >= TRSM_MIN_RHS;
@@ -422,12 +531,12 @@ int launch_pass(Context* c, Factor* f, const SolveWork& w, bool forward, const d
                        : np == 1 ? (const void*)k_trsm_persistent<T, 1>
                        : np == 2 ? (const void*)k_trsm_persistent<T, 2>
                        : np == 3 ? (const void*)k_trsm_persistent<T, 3> : (const void*)k_trsm_persistent<T, 4>;
-    HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, 0));
+    HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, mfma ? NT : NTP, 0));
     if (per_cu < 1) per_cu = 1;
-    const int grid = std::max(1, std::min(a.nbr, c->num_cu * per_cu));
+    const int grid = std::max(1, std::min(mfma ? a.nbr : (a.nbr + 1) / 2, c->num_cu * per_cu));
     if (!mfma) {
-        if (nrhs == 1) hipLaunchKernelGGL((k_trsv_persistent<T, 1>), dim3(grid), dim3(NT), 0, c->stream, a);
-        else hipLaunchKernelGGL((k_trsv_persistent<T, 8>), dim3(grid), dim3(NT), 0, c->stream, a);
+        if (nrhs == 1) hipLaunchKernelGGL((k_trsv_persistent<T, 1>), dim3(grid), dim3(NTP), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_trsv_persistent<T, 8>), dim3(grid), dim3(NTP), 0, c->stream, a);
     } else if (np == 1) {
         hipLaunchKernelGGL((k_trsm_persistent<T, 1>), dim3(grid), dim3(NT), 0, c->stream, a);
     } else if (np == 2) {

@@ -302,7 +302,8 @@ def make_desc(n_ids: int, ops, input_ids, input_tiles, ltiles, utiles, block_row
     return d, keep
 
 
-GRAPH_INFO = ("tasks", "value_tiles", "levels", "pairs", "deps", "folded", "inv_aliased", "chained", "pool_tiles")
+GRAPH_INFO = ("tasks", "value_tiles", "levels", "pairs", "deps", "folded", "inv_aliased", "chained", "pool_tiles",
+              "pair_tasks")
 
 
 def graph_check_ops(n_ids: int, ops, input_ids, input_tiles, ltiles, utiles, block_rows: int,

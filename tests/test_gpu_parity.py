@@ -682,6 +682,55 @@ def test_resident_factor_repeat_solves(dev):
     assert sogl.oracle_residual(p.n, p.row, p.col, p.val, pl.unpermute(x2), b2) < 1e-15
 
 
+def block_factor(nbr, missing):
+    """A block factor given directly as tiles: L block lower triangular with lower
+    triangular diagonal tiles, U block upper triangular with upper triangular ones, every
+    off-diagonal tile present except the two of row pair `missing` (L_2m+1,2m and
+    U_2m,2m+1)."""
+    lt, ut, tiles = [], [], []
+    Lf = np.zeros((64 * nbr, 64 * nbr))
+    Uf = np.zeros((64 * nbr, 64 * nbr))
+    for i in range(nbr):
+        for j in range(nbr):
+            for which, full in ((0, Lf), (1, Uf)):
+                if (which == 0 and j > i) or (which == 1 and j < i):
+                    continue
+                if missing is not None and i != j and min(i, j) == 2 * missing and max(i, j) == 2 * missing + 1:
+                    continue
+                t = rng.standard_normal((64, 64)) / 64
+                if i == j:
+                    t = np.tril(t) if which == 0 else np.triu(t)
+                    t[np.diag_indices(64)] = 1.0 + rng.random(64)
+                tiles.append(t)
+                (lt if which == 0 else ut).append([i, j, len(tiles)])
+                full[64 * i:64 * i + 64, 64 * j:64 * j + 64] = t
+    return np.array(lt, np.int32), np.array(ut, np.int32), np.stack(tiles), Lf, Uf
+
+
+@pytest.mark.parametrize("precision,tol", [(sogl.FP64, 1e-12), (sogl.FP32, 1e-5)])
+@pytest.mark.parametrize("nbr,missing", [(3, None), (5, 1), (4, 0), (1, None)])
+def test_solve_row_pairs(dev, precision, tol, nbr, missing):
+    """The triangular solve takes block rows two at a time, with the off-diagonal block of
+    each pair's 2 x 2 diagonal block inverse made by the factor.  On hand-built factors
+    (no ops: the L and U tiles are inputs, their diagonal inverses made by inverse
+    tasks): odd block-row counts (the last row alone), a pair without sub-diagonal tiles
+    (no coupling block), one and 9 right-hand sides; x against dense triangular solves of
+    the same (rounded) tiles."""
+    import scipy.linalg as sl
+    lt, ut, tiles, Lf, Uf = block_factor(nbr, missing)
+    ids = np.arange(1, len(tiles) + 1, dtype=np.int32)
+    fid = dev.load_ops(len(tiles) + 1, np.zeros((0, 8), np.int32), ids, tiles, lt, ut, nbr, precision)
+    dev.factor(fid, sogl.EXEC_PERSISTENT)
+    if precision == sogl.FP32:
+        Lf, Uf = (m.astype(np.float32).astype(np.float64) for m in (Lf, Uf))
+    for nrhs in (1, 9):
+        b = rng.standard_normal((64 * nbr, nrhs))
+        x = dev.solve(fid, b)
+        want = sl.solve_triangular(Uf, sl.solve_triangular(Lf, b, lower=True), lower=False)
+        assert np.abs(x - want).max() / np.abs(want).max() < tol, (nrhs, np.abs(x - want).max())
+    dev.release(fid)
+
+
 @pytest.mark.parametrize("nrhs", [3, 20, 70, 300])
 def test_multi_rhs(dev, nrhs):
     """Few right-hand sides: the per-row dot-product kernel; from 16 on: block-row

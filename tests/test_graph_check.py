@@ -25,49 +25,69 @@ CASES = {
 }
 
 
+def pair_blocks(pl):
+    """Row pairs (2p, 2p+1) whose sub-diagonal L tile L_2p+1,2p is present, and those
+    whose U tile U_2p,2p+1 is (an L L^T factor: U = L^T, so the L tile's)."""
+    bi, bj, _ = pl.factor_tiles(0)
+    n_l = int(np.sum((bi % 2 == 1) & (bj == bi - 1)))
+    if pl.info.symmetric:
+        return n_l, n_l
+    ui, uj, _ = pl.factor_tiles(1)
+    return n_l, int(np.sum((ui % 2 == 0) & (uj == ui + 1)))
+
+
 @pytest.mark.parametrize("name", list(CASES))
 def test_plan_graphs_pass(name):
     p = CASES[name]()
     pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b, symmetric=p.symmetric)
     info = pl.info
+    n_xl, n_yu = pair_blocks(pl)
     for prec in (sogl.FP64, sogl.FP32):
         g = sogl.graph_check(pl, prec)
-        assert g["pairs"] == info.n_mul + info.n_mulneg + info.n_mult
+        # the solve's two-row blocks: two GEMM tasks (one pair each) per present
+        # sub-diagonal tile of a row pair, for L and for U
+        assert g["pair_tasks"] == 2 * (n_xl + n_yu)
+        assert g["pairs"] == info.n_mul + info.n_mulneg + info.n_mult + g["pair_tasks"]
         # folded ops are sub ops (copy / neg / sub), never more than the plan has
         assert 0 <= g["folded"] <= info.n_sub
         assert g["inv_aliased"] <= info.n_lower_inv + info.n_upper_inv
         # L L^T: one transpose task per L tile and per diagonal inverse (U = L^T)
         extra = info.n_l_tiles + info.block_rows if info.symmetric else 0
         # every unit op is a task, one more pair of a GEMM task, folded or aliased
-        assert g["tasks"] <= info.n_ops - g["folded"] - g["inv_aliased"] + extra
-        assert g["levels"] > 0 and g["value_tiles"] <= info.storage_count + 2 * info.block_rows + 1 + extra
+        assert g["tasks"] <= info.n_ops - g["folded"] - g["inv_aliased"] + extra + g["pair_tasks"]
+        assert g["levels"] > 0
+        assert g["value_tiles"] <= info.storage_count + 2 * info.block_rows + 1 + extra + g["pair_tasks"]
         # slot reuse (checked inside graph_check): never more tiles than values, and at
-        # least the factor tiles, their diagonal inverses and the zero tile
+        # least the factor tiles, their diagonal inverses, the two-row blocks and the
+        # zero tile
         kept = (2 * info.n_l_tiles if info.symmetric else info.n_l_tiles + info.n_u_tiles) + 2 * info.block_rows + 1
-        assert kept - extra <= g["pool_tiles"] <= g["value_tiles"]
+        assert kept - extra + n_xl + n_yu <= g["pool_tiles"] <= g["value_tiles"]
 
 
 def test_c2_graph_passes():
     p = W.config("c2")
     pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
     g = sogl.graph_check(pl, sogl.FP32)
-    assert g["pairs"] == 1_000_386
+    # 864 of the 1024 row pairs have both sub-diagonal tiles: 4 GEMM tasks each
+    assert g["pair_tasks"] == 3456
+    assert g["pairs"] == 1_000_386 + g["pair_tasks"]
     assert g["levels"] < pl.info.n_stages  # fused tasks: fewer dependency levels than stages
     # most lu tasks follow the Schur-complement GEMM of their tile in the same workgroup
     assert 0 < g["chained"] <= pl.info.n_lu
-    # slot reuse: 220,885 values in 88,381 tiles (the factor tiles, their inverses and
-    # the zero tile are 78,961; a slot freed fewer than 1024 levels back is not reused)
-    assert g["value_tiles"] == 220_885 and 78_961 < g["pool_tiles"] < 90_000
+    # slot reuse: 224,341 values in 90,316 tiles (the factor tiles, their inverses, the
+    # two-row blocks and the zero tile are 80,689; a slot freed fewer than 1024 levels
+    # back is not reused)
+    assert g["value_tiles"] == 224_341 and 80_689 < g["pool_tiles"] < 92_000
 
 
 def test_c5_pool_fits_in_15_gb():
-    """The 1M-unknown convection-diffusion factor in FP64: 1,078,138 values of 32 KiB
-    (35.3 GB without slot reuse) in a pool of at most 15 GB; the reference peaks at
+    """The 1M-unknown convection-diffusion factor in FP64: 1,109,386 values of 32 KiB
+    (36.4 GB without slot reuse) in a pool of at most 15 GB; the reference peaks at
     13.4 GB of host memory on the same matrix."""
     p = W.config("c5")
     pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b)
     g = sogl.graph_check(pl, sogl.FP64)
-    assert g["value_tiles"] == 1_078_138
+    assert g["value_tiles"] == 1_109_386
     assert g["pool_tiles"] * 4096 * 8 <= 15e9, g["pool_tiles"]
 
 
