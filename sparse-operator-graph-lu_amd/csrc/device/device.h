@@ -49,7 +49,11 @@ struct DevTask {
     // waits for the tasks that read that value; their flags are war[war_begin ..
     // war_begin + war_count) (value ids)
     int32_t war_begin = 0, war_count = 0;
-    int32_t pad[3] = {0, 0, 0};
+    // persistent executor: the task the same workgroup runs after this one has published
+    // (a GEMM on the critical path that reads an output of this one; it waits for its
+    // other operands like a queued task and is not handed out by the queue), or -1
+    int32_t after = -1;
+    int32_t pad[2] = {0, 0};
 };
 
 // One check entry of a task's chain candidates: candidate `task` (index `cand` in the
@@ -97,6 +101,9 @@ struct Factor {
     DevBuf pool, tasks_d, pairs_d, deps_d, ready_d, ctrl_d, op_tasks_d, op_pairs_d, inputs_d, queue_d, chk_d;
     std::vector<ChainChk> chk;             // chain candidates' check entries (see DevTask)
     int64_t n_chain_cands = 0;             // tasks that are some task's chain candidate
+    bool after_links = false;              // build after-links (set from the context at load)
+    int64_t n_after = 0;                   // after-links (tasks run by their producer's workgroup)
+    std::vector<int32_t> chain_head;       // per task: the queued task whose workgroup runs it
     int32_t chain_claims = 0;              // persistent executor: claims per finished task (0: off)
     std::vector<int32_t> queue;            // persistent executor: task indices handed out, in order
     DevBuf mask_d;                         // per slot: 16x16 block mask (uint32)
@@ -145,6 +152,7 @@ struct Context {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ex0 = nullptr, ex1 = nullptr;
     int num_cu = 0;
     int32_t pool_reuse = SOGL_REUSE_AUTO;  // for the factors loaded from now on (sogl_ctx_set_pool_reuse)
+    bool after_links = false;              // for the factors loaded from now on (sogl_ctx_set_after_links)
     size_t free_mem = 0, total_mem = 0;    // device HBM when the context was made
     std::string err;
     std::map<int32_t, Factor*> factors;

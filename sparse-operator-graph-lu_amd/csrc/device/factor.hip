@@ -527,13 +527,14 @@ __global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
         // its only source, and the barrier below makes it visible to the workgroup);
         // then the tasks it claimed
         int ti = a.queue[qi];
-        int mode = 0;  // 0: from the queue, 1: the static chain's next task, 2: claimed
+        int mode = 0;  // 0: from the queue, 1: the static chain's next task, 2: claimed,
+                       // 3: the after-link of the previous task (waits as a queued task)
         for (;;) {
         const DevTask t = a.tasks[ti];
         const int nxt = __builtin_amdgcn_readfirstlane(t.next);
         long long t_taken = 0, t_ready = 0, t_wait = 0;
         if (a.trace && tid == 0) t_taken = wall_clock64();
-        if (t.kind != TK_GEMM && mode == 0) {
+        if (t.kind != TK_GEMM && (mode == 0 || mode == 3)) {
             if (tid == 0) {
                 for (int d = 0; d < t.dep_count; d++) spin_ready(a, a.deps[t.dep_begin + d]);
             }
@@ -624,10 +625,14 @@ __global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
             n_ran++;
             // the next task of this chain, decided here and read by every wave after the
             // barrier below (nothing writes these words again before the next such
-            // barrier): the static chain's next task, else the ring's head with its masks
+            // barrier): the static chain's next task, else the after-link, else the
+            // ring's head with its masks
             if (nxt >= 0) {
                 s_next_ti = nxt;
                 s_next_mode = 1;
+            } else if (t.after >= 0) {
+                s_next_ti = t.after;
+                s_next_mode = 3;
             } else if (s_ring_n > 0) {
                 const int h = s_ring_head;
                 s_next_ti = s_ring_task[h];
@@ -751,6 +756,18 @@ struct HostGraph {
 // of the longest path - the longest path through t.
 constexpr double CHAIN_SLACK_US = 1.0;
 
+// The cost model of the critical path (microseconds, measured on the C2 / C5 factors):
+// task work, and a hand-off per edge that no chain link removes.
+double task_cost_us(const DevTask& k) {
+    switch (k.kind) {
+        case TK_LU: case TK_LLT: return 10.0;
+        case TK_LINV: case TK_UINV: return 5.0;
+        case TK_GEMM: return 0.8 + 0.5 * k.b + (k.add >= 0 ? 0.2 : 0.0);
+        default: return 1.0;
+    }
+}
+constexpr double HANDOFF_US = 2.5;  // publish + hand-off + operand loads from memory
+
 void build_chains(Factor* f, HostGraph& hg) {
     const size_t T = f->tasks.size();
     f->chk.clear();
@@ -764,16 +781,8 @@ void build_chains(Factor* f, HostGraph& hg) {
     for (size_t t = 0; t < T; t++)
         for (int32_t o : {f->tasks[t].out0, f->tasks[t].out1, f->tasks[t].out2, f->tasks[t].out3})
             if (o >= 0) slot_task[(size_t)o] = (int32_t)t;
-    auto cost = [&](const DevTask& k) -> double {
-        switch (k.kind) {
-            case TK_LU: case TK_LLT: return 10.0;
-            case TK_LINV: case TK_UINV: return 5.0;
-            case TK_GEMM: return 0.8 + 0.5 * k.b + (k.add >= 0 ? 0.2 : 0.0);
-            default: return 1.0;
-        }
-    };
-    constexpr double HANDOFF = 2.5;  // publish + hand-off + operand loads from memory
-    auto edge = [&](size_t p, size_t t) { return f->tasks[p].next == (int32_t)t ? 0.0 : HANDOFF; };
+    auto cost = [&](const DevTask& k) { return task_cost_us(k); };
+    auto edge = [&](size_t p, size_t t) { return f->tasks[p].next == (int32_t)t ? 0.0 : HANDOFF_US; };
     std::vector<double> tl(T, 0.0), bl(T, 0.0);
     std::vector<std::vector<int32_t>> succ(T);
     for (size_t t = 0; t < T; t++) {  // tasks are in dependency-level order
@@ -807,9 +816,14 @@ void build_chains(Factor* f, HostGraph& hg) {
     std::vector<std::vector<int32_t>> cand(T);
     std::vector<int32_t> nchk(T, 0);
     std::vector<int32_t> order;
+    // after-links keep their chains in their heads' workgroups: neither end is claimed
+    std::vector<char> linked(T, 0);
+    for (size_t t = 0; t < T; t++)
+        if (f->tasks[t].after >= 0) linked[(size_t)f->tasks[t].after] = linked[t] = 1;
     for (size_t u = 0; u < T; u++) {
         const DevTask& k = f->tasks[u];
-        if (k.kind == TK_GEMM && k.b >= 1 && k.b <= CHAIN_MAX_PAIRS && cp - (tl[u] + bl[u]) <= CHAIN_SLACK_US)
+        if (k.kind == TK_GEMM && k.b >= 1 && k.b <= CHAIN_MAX_PAIRS && !linked[u] &&
+            cp - (tl[u] + bl[u]) <= CHAIN_SLACK_US)
             order.push_back((int32_t)u);
     }
     std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
@@ -852,6 +866,99 @@ void build_chains(Factor* f, HostGraph& hg) {
         }
         kp.chk_count = (int32_t)f->chk.size() - kp.chk_begin;
     }
+}
+
+// After-links of the persistent executor (off by default: measured slower).  The
+// workgroup that ran task t runs t.after right after publishing t's outputs: a
+// near-critical GEMM of at most AFTER_MAX_PAIRS pairs whose last operand (by the cost
+// model) is t's, so the critical path skips the hand-off to another workgroup and reads
+// t's tiles from this XCD's L2.  The linked task waits for its operands as a queued task
+// does (its descriptor load and its producer's flag poll then come after the publish,
+// where a queued consumer has both done already), and is not queued.
+// Deadlock freedom: tasks are taken in index (dependency-level) order, each chain
+// belongs to its head (the queued task that starts it), and a link t -> u is made only
+// when every other producer of u's operands is an earlier member of t's chain or
+// belongs to a chain whose head comes before t's head in the queue.  With every
+// workgroup resident, the chain with the first unfinished head then waits only for
+// finished work.  AFTER_MAX_LEN bounds a chain, and with it how far its head lies
+// behind its last task (slot reuse takes over a slot only from values whose readers
+// are REUSE_LEVEL_GAP levels back; the slot check verifies their heads as well).
+constexpr int AFTER_MAX_PAIRS = 1;
+constexpr int AFTER_MAX_LEN = 24;
+
+void build_after_links(Factor* f, const HostGraph& hg, std::vector<char>& linked) {
+    const size_t T = f->tasks.size();
+    f->n_after = 0;
+    f->chain_head.assign(T, -1);
+    std::vector<int32_t> slot_task((size_t)f->n_slots, -1);
+    for (size_t t = 0; t < T; t++)
+        for (int32_t o : {f->tasks[t].out0, f->tasks[t].out1, f->tasks[t].out2, f->tasks[t].out3})
+            if (o >= 0) slot_task[(size_t)o] = (int32_t)t;
+    auto edge = [&](size_t p, size_t t) { return f->tasks[p].next == (int32_t)t ? 0.0 : HANDOFF_US; };
+    std::vector<double> fin(T, 0.0), bl(T, 0.0);  // earliest finish, longest path to the end
+    std::vector<int32_t> crit(T, -1);              // the predecessor that finishes last
+    std::vector<std::vector<int32_t>> succ(T);
+    for (size_t t = 0; t < T; t++) {
+        const DevTask& k = f->tasks[t];
+        double st = 0;
+        for (int32_t q = k.dep_begin; q < k.dep_begin + k.dep_count; q++) {
+            const int32_t p = slot_task[(size_t)hg.deps[(size_t)q]];
+            if (p < 0) continue;
+            const double e = fin[(size_t)p] + edge((size_t)p, t);
+            if (crit[t] < 0 || e > st) {
+                st = e;
+                crit[t] = p;
+            }
+            if (succ[(size_t)p].empty() || succ[(size_t)p].back() != (int32_t)t) succ[(size_t)p].push_back((int32_t)t);
+        }
+        fin[t] = st + task_cost_us(k);
+    }
+    double cp = 0;
+    for (size_t t = T; t-- > 0;) {
+        double b = 0;
+        for (int32_t u : succ[t]) b = std::max(b, edge(t, (size_t)u) + bl[(size_t)u]);
+        bl[t] = task_cost_us(f->tasks[t]) + b;
+        cp = std::max(cp, fin[t] - task_cost_us(f->tasks[t]) + bl[t]);
+    }
+    // slack of task t: the longest path minus the longest path through t
+    auto slack = [&](size_t t) { return cp - (fin[t] - task_cost_us(f->tasks[t]) + bl[t]); };
+    std::vector<int32_t> head(T, -1), len(T, 0), pred(T, -1);
+    for (size_t t = 0; t < T; t++) {
+        DevTask& k = f->tasks[t];
+        if (head[t] < 0) head[t] = (int32_t)t;  // queued (or the static chain's head, below)
+        if (k.next >= 0) {  // the lu of this GEMM's sum: same workgroup, same chain
+            head[(size_t)k.next] = head[t];
+            len[(size_t)k.next] = len[t] + 1;
+            pred[(size_t)k.next] = (int32_t)t;
+            continue;
+        }
+        if (len[t] + 1 >= AFTER_MAX_LEN) continue;
+        int32_t best = -1;
+        for (int32_t u : succ[t]) {
+            const DevTask& ku = f->tasks[(size_t)u];
+            if (ku.kind != TK_GEMM || ku.b > AFTER_MAX_PAIRS || linked[(size_t)u] || crit[(size_t)u] != (int32_t)t ||
+                slack((size_t)u) > CHAIN_SLACK_US)
+                continue;
+            bool ok = true;
+            for (int32_t q = ku.dep_begin; q < ku.dep_begin + ku.dep_count && ok; q++) {
+                const int32_t p = slot_task[(size_t)hg.deps[(size_t)q]];
+                if (p < 0 || p == (int32_t)t) continue;
+                if ((size_t)p > t) { ok = false; break; }  // not placed yet
+                bool member = false;  // an earlier task of t's own chain
+                for (int32_t x = pred[t]; x >= 0 && !member; x = pred[(size_t)x]) member = x == p;
+                ok = member || head[(size_t)p] < head[t];
+            }
+            if (ok && (best < 0 || bl[(size_t)u] > bl[(size_t)best])) best = u;
+        }
+        if (best < 0) continue;
+        k.after = best;
+        linked[(size_t)best] = 1;
+        head[(size_t)best] = head[t];
+        len[(size_t)best] = len[t] + 1;
+        pred[(size_t)best] = (int32_t)t;
+        f->n_after++;
+    }
+    f->chain_head = head;
 }
 
 // The task graph of a factor description, on the host only (no device calls).
@@ -1334,6 +1441,9 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
             chained[u] = 1;
             f->n_chained++;
         }
+        for (DevTask& k : f->tasks) k.after = -1;
+        if (f->after_links) build_after_links(f, hg, chained);
+        else f->chain_head.clear();
         f->queue.clear();
         for (size_t u = 0; u < f->tasks.size(); u++)
             if (!chained[u]) f->queue.push_back((int32_t)u);
@@ -1460,25 +1570,62 @@ int validate_graph(Context* c, const sogl_factor_desc* d, const Factor* f, const
         for (int32_t q = k.dep_begin; q < k.dep_begin + k.dep_count; q++)
             if (!ok_slot(hg.deps[(size_t)q])) return bad("dependency slot of task " + std::to_string(t));
     }
-    // chains and the queue: each link goes from a GEMM to the lu / llt of its sum, at a
-    // later level; every task is handed out by the queue or reached by one link, once
+    // chains and the queue: a static link goes from a GEMM to the lu / llt of its sum, an
+    // after-link from a task that publishes to a GEMM reading one of its outputs, both
+    // at a later level; every task is handed out by the queue or reached by one link,
+    // once.  Every task a linked GEMM waits for is an earlier task of its own chain or
+    // belongs to a chain whose head is queued before its own (see build_after_links).
     {
         std::vector<char> seen(T, 0);
+        std::vector<int32_t> head(T, -1), pred(T, -1);
         for (int32_t q : f->queue) {
             if (q < 0 || (size_t)q >= T || seen[(size_t)q]) return bad("queue entry " + std::to_string(q));
             seen[(size_t)q] = 1;
         }
         for (size_t t = 0; t < T; t++) {
             const DevTask& k = f->tasks[t];
-            if (k.next < 0) continue;
-            if ((size_t)k.next >= T || k.kind != TK_GEMM) return bad("chain of task " + std::to_string(t));
-            const DevTask& u = f->tasks[(size_t)k.next];
-            if ((u.kind != TK_LU && u.kind != TK_LLT) || u.a != k.out0 || u.level <= k.level || seen[(size_t)k.next])
-                return bad("chain of task " + std::to_string(t));
-            seen[(size_t)k.next] = 1;
+            if (k.next >= 0 && k.after >= 0) return bad("two links from task " + std::to_string(t));
+            if (k.next >= 0) {
+                if ((size_t)k.next >= T || k.kind != TK_GEMM) return bad("chain of task " + std::to_string(t));
+                const DevTask& u = f->tasks[(size_t)k.next];
+                if ((u.kind != TK_LU && u.kind != TK_LLT) || u.a != k.out0 || u.level <= k.level ||
+                    seen[(size_t)k.next])
+                    return bad("chain of task " + std::to_string(t));
+                seen[(size_t)k.next] = 1;
+                pred[(size_t)k.next] = (int32_t)t;
+            }
+            if (k.after >= 0) {
+                if ((size_t)k.after >= T) return bad("after-link of task " + std::to_string(t));
+                const DevTask& u = f->tasks[(size_t)k.after];
+                if (u.kind != TK_GEMM || u.level <= k.level || seen[(size_t)k.after])
+                    return bad("after-link of task " + std::to_string(t));
+                seen[(size_t)k.after] = 1;
+                pred[(size_t)k.after] = (int32_t)t;
+            }
         }
         for (size_t t = 0; t < T; t++)
             if (!seen[t]) return bad("task " + std::to_string(t) + " is neither queued nor chained");
+        // heads: links go to later levels, so a task's predecessor has a lower index
+        for (size_t t = 0; t < T; t++) {
+            if (pred[t] >= 0 && (size_t)pred[t] >= t) return bad("link order at task " + std::to_string(t));
+            head[t] = pred[t] < 0 ? (int32_t)t : head[(size_t)pred[t]];
+        }
+        std::vector<int32_t> producer((size_t)n, -1);
+        for (size_t t = 0; t < T; t++)
+            for (int32_t o : {f->tasks[t].out0, f->tasks[t].out1, f->tasks[t].out2, f->tasks[t].out3})
+                if (o >= 0) producer[(size_t)o] = (int32_t)t;
+        for (size_t t = 0; t < T; t++) {
+            const DevTask& k = f->tasks[t];
+            if (pred[t] < 0 || k.kind != TK_GEMM || f->tasks[(size_t)pred[t]].next == (int32_t)t) continue;
+            for (int32_t q = k.dep_begin; q < k.dep_begin + k.dep_count; q++) {
+                const int32_t p = producer[(size_t)hg.deps[(size_t)q]];
+                if (p < 0) continue;
+                bool member = false;
+                for (int32_t x = pred[t]; x >= 0 && !member; x = pred[(size_t)x]) member = x == p;
+                if (!member && head[(size_t)p] >= head[t])
+                    return bad("after-linked task " + std::to_string(t) + " waits for a later chain");
+            }
+        }
     }
     // chain candidates: each check entry names a queued GEMM of at most CHAIN_MAX_PAIRS
     // pairs, an operand position of it and that operand's slot (-1: an output of the
@@ -1840,6 +1987,10 @@ int validate_slots(Context* c, const Factor* f, const HostGraph& hg) {
                     return bad("slot " + std::to_string(l) + " is reused at the level of a reader");
                 if (!std::binary_search(wf.begin(), wf.end(), done_flag(f, r)))
                     return bad("task " + std::to_string(pw) + " does not wait for a reader of its slot");
+                // an after-linked task waits only for chains with earlier heads
+                if (!f->chain_head.empty() && f->chain_head[(size_t)pw] != pw &&
+                    f->chain_head[(size_t)r] >= f->chain_head[(size_t)pw])
+                    return bad("after-linked task " + std::to_string(pw) + " waits for a reader in a later chain");
             }
         }
     }
@@ -2002,6 +2153,7 @@ int upload_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells,
 int build_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells, Factor* f) {
     HostGraph hg;
     f->keep_ops = c->pool_reuse != SOGL_REUSE_ON;  // a factor with reuse cannot run per op
+    f->after_links = c->after_links;
     int rc = build_host_graph(c, d, f, hg);
     if (rc == SOGL_OK) rc = validate_graph(c, d, f, hg);
     if (rc == SOGL_OK) {
@@ -2258,6 +2410,7 @@ int graph_check(const sogl_factor_desc* d, bool cells, int64_t* info) {
     f.precision = d->precision;
     f.elem = d->precision == SOGL_FP64 ? 8 : 4;
     f.n_inputs = d->n_inputs;
+    f.after_links = true;  // checked with the after-links (their head order included)
     HostGraph hg;
     int rc = build_host_graph(nullptr, d, &f, hg);
     if (rc == SOGL_OK) rc = validate_graph(nullptr, d, &f, hg);
@@ -2277,6 +2430,7 @@ int graph_check(const sogl_factor_desc* d, bool cells, int64_t* info) {
         info[7] = f.n_chained;
         info[8] = f.n_locs;
         info[9] = f.n_pair_tasks;
+        info[10] = f.n_after;
     }
     return rc;
 }
@@ -2288,6 +2442,13 @@ extern "C" int sogl_graph_check_plan(const sogl_plan* plan, int32_t precision, i
     sogl_factor_desc d;
     if (!sogl::plan_factor_desc(plan, precision, &d)) return set_error(nullptr, SOGL_ERR_ARG, "null plan");
     return graph_check(&d, true, info);
+}
+
+extern "C" int sogl_ctx_set_after_links(sogl_ctx* x, int32_t on) {
+    if (!x) return set_error(nullptr, SOGL_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> g(x->c.mu);
+    x->c.after_links = on != 0;
+    return SOGL_OK;
 }
 
 extern "C" int sogl_ctx_set_pool_reuse(sogl_ctx* x, int32_t mode) {

@@ -128,6 +128,7 @@ def load(name: str = "libsogl.so") -> C.CDLL:
         "sogl_factor_set_masks": (C.c_int, [vp, i32, i32]),
             "sogl_factor_set_chaining": (C.c_int, [vp, i32, i32]),
             "sogl_ctx_set_pool_reuse": (C.c_int, [vp, i32]),
+            "sogl_ctx_set_after_links": (C.c_int, [vp, i32]),
             "sogl_factor_get_uinv_check": (C.c_int, [vp, i32, P(i32), P(dbl)]),
             "sogl_refine_attach": (C.c_int, [vp, i32, vp]),
             "sogl_solve_refined": (C.c_int, [vp, i32, P(dbl), P(dbl), i32, dbl, i32, P(i32),
@@ -303,7 +304,7 @@ def make_desc(n_ids: int, ops, input_ids, input_tiles, ltiles, utiles, block_row
 
 
 GRAPH_INFO = ("tasks", "value_tiles", "levels", "pairs", "deps", "folded", "inv_aliased", "chained", "pool_tiles",
-              "pair_tasks")
+              "pair_tasks", "after_links")
 
 
 def graph_check_ops(n_ids: int, ops, input_ids, input_tiles, ltiles, utiles, block_rows: int,
@@ -395,6 +396,10 @@ class Device:
         finished task (0 / False: off; True: the default, 2)."""
         n = 2 if claims is True else int(claims)
         self._check(self.lib.sogl_factor_set_chaining(self.h, fid, n))
+
+    def set_after_links(self, on: bool):
+        """After-links for the factors loaded from now on (sogl_ctx_set_after_links)."""
+        self._check(self.lib.sogl_ctx_set_after_links(self.h, 1 if on else 0))
 
     def set_pool_reuse(self, mode):
         """Pool slot reuse for the factors loaded from now on: True / REUSE_ON, False /
