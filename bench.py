@@ -285,10 +285,9 @@ def main():
                     help="1: factors reuse pool slots of values no longer read (smaller pool, coherent "
                          "tile loads); 0: one slot per value; 2 (default): reuse when the values would "
                          "take over a quarter of the device's HBM")
-    ap.add_argument("--after-links", type=int, default=0, choices=(0, 1),
-                    help="1: the workgroup that made a tile on the critical path runs the one-pair GEMM "
-                         "that reads it next (sogl_ctx_set_after_links); 0 (default, faster as measured): "
-                         "every task but the static GEMM -> lu chains is queued")
+    ap.add_argument("--task-links", type=int, default=sogl.LINK_PANEL, choices=(0, 1, 2, 3),
+                    help="mask of sogl_ctx_set_task_links: 2 (default) = the lu task runs its most "
+                         "critical panel GEMM itself; 1 = after-links (measured slower); 0 = none")
     ap.add_argument("--no-reuse-leg", action="store_true",
                     help="skip the FP64 leg that factors with pool slot reuse forced on")
     ap.add_argument("--no-chain", action="store_true",
@@ -330,7 +329,7 @@ def main():
                    for k, s in (("fp64", 8), ("fp32", 4))}
     dev = sogl.Device(0)
     dev.set_pool_reuse(args.pool_reuse)
-    dev.set_after_links(bool(args.after_links))
+    dev.set_task_links(args.task_links)
     paths = {}
     sweep = () if args.no_ir_sweep or nrhs != 1 else tuple(range(0, 7))
     ch = 0 if args.no_chain else args.chain_claims
@@ -452,7 +451,7 @@ def main():
                    "n": prob.n, "nnz": prob.nnz, "unit_ops": int(info.n_ops),
                    "gemm_ops": int(info.n_mul + info.n_mulneg + info.n_mult), "stages": int(info.n_stages),
                    "gemm_gflop": round(gemm_flops / 1e9, 3), "host_plan_s": round(t_plan, 3),
-                   "after_links": bool(args.after_links)},
+                   "task_links": args.task_links},
         "roofline": {"bound": "mfma", "achieved": round(head["gemm_tflops"], 4), "peak": PEAK["fp32"],
                      "unit": "TFLOP/s", "frac": round(head["gemm_tflops"] / PEAK["fp32"], 6),
                      "traffic": TRAFFIC_FP32_FACTOR.get(args.config),

@@ -201,28 +201,34 @@ int sogl_factor_load(sogl_ctx* ctx, const sogl_factor_desc* desc, int32_t* facto
 #define SOGL_REUSE_ON 1
 #define SOGL_REUSE_AUTO 2
 int sogl_ctx_set_pool_reuse(sogl_ctx* ctx, int32_t mode);
-/* After-links of the persistent executor, for the factors loaded from now on (default
- * off: measured slower, see the design notes): the workgroup that made a tile on the
- * critical path goes on, right after publishing it, to a one-pair GEMM that reads it,
- * instead of handing it to another workgroup.  The links are chosen on the host by a
- * cost model of the critical path, only where they keep the queue deadlock-free;
- * results are the same bits with and without.  No reference counterpart (the
- * reference runs a stage's ops on a thread pool). */
-int sogl_ctx_set_after_links(sogl_ctx* ctx, int32_t on);
+/* Task links of the persistent executor, for the factors loaded from now on (a mask of
+ * SOGL_LINK_*; default SOGL_LINK_PANEL).  No reference counterpart (the reference runs
+ * a stage's ops on a thread pool); results are the same bits with any mask.
+ *   SOGL_LINK_PANEL: the lu task of an FP32 LU factor runs its most critical panel
+ *     GEMM (A U^-1 or L^-1 A, one pair) itself, from the inverse it keeps in LDS, and
+ *     publishes both tiles at once.
+ *   SOGL_LINK_AFTER: the workgroup that made a tile on the critical path goes on, right
+ *     after publishing it, to a one-pair GEMM that reads it, instead of handing it to
+ *     another workgroup (measured slower: off by default).
+ * The links are chosen on the host by a cost model of the critical path, only where
+ * they keep the queue deadlock-free (checked on every load). */
+#define SOGL_LINK_AFTER 1
+#define SOGL_LINK_PANEL 2
+int sogl_ctx_set_task_links(sogl_ctx* ctx, int32_t links);
 /* Builds the executor's task graph of a description on the host only (no device is
  * touched) and checks it: every slot index inside the pool, one writer per tile,
  * every operand produced at an earlier dependency level and waited for, and the pool
  * slot reuse (see sogl_ctx_set_pool_reuse) handing a slot over only from a value to the
  * task that alone reads it.  sogl_factor_load runs the same checks before any upload.
- * info (may be NULL) receives 11 counts: tasks, value tiles (the pool without slot
+ * info (may be NULL) receives 12 counts: tasks, value tiles (the pool without slot
  * reuse), dependency levels, GEMM pairs, dependencies, folded copy/negate/subtract ops,
  * aliased repeated inverses, lu / llt tasks chained to the GEMM that makes their source
  * (same workgroup, no hand-off), pool tiles with slot reuse (the _plan form's inputs
  * can be scattered again, so their slots are reused too), and the GEMM tasks (one pair
  * each) that make the solve's two-row blocks: per pair of block rows 2p, 2p+1, two for
  * the L block when L_2p+1,2p is present and two for the U block when U_2p,2p+1 is, and
- * the after-links (see sogl_ctx_set_after_links), which the check builds whatever the
- * context's setting. */
+ * the after-links and the absorbed panels (see sogl_ctx_set_task_links), which the check
+ * builds whatever the context's setting. */
 int sogl_graph_check(const sogl_factor_desc* desc, int64_t* info);
 int sogl_graph_check_plan(const sogl_plan* plan, int32_t precision, int64_t* info);
 /* Convenience: sogl_factor_load from a host plan. */

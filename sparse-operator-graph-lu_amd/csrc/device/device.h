@@ -53,7 +53,11 @@ struct DevTask {
     // (a GEMM on the critical path that reads an output of this one; it waits for its
     // other operands like a queued task and is not handed out by the queue), or -1
     int32_t after = -1;
-    int32_t pad[2] = {0, 0};
+    // lu (FP32 LU factors, persistent executor): the panel GEMMs this task runs inside
+    // itself, before publishing, from the inverses it keeps in LDS: pan_l = A U^-1 (its
+    // U^-1 the second operand), pan_u = L^-1 A (its L^-1 the first); -1: none
+    int32_t pan_l = -1, pan_u = -1;
+    int32_t pad[3] = {0, 0, 0};
 };
 
 // One check entry of a task's chain candidates: candidate `task` (index `cand` in the
@@ -102,6 +106,8 @@ struct Factor {
     std::vector<ChainChk> chk;             // chain candidates' check entries (see DevTask)
     int64_t n_chain_cands = 0;             // tasks that are some task's chain candidate
     bool after_links = false;              // build after-links (set from the context at load)
+    bool absorb_panels = false;            // let lu tasks absorb a panel GEMM (set from the context)
+    int64_t n_absorbed = 0;                // panel GEMMs absorbed by their lu task
     int64_t n_after = 0;                   // after-links (tasks run by their producer's workgroup)
     std::vector<int32_t> chain_head;       // per task: the queued task whose workgroup runs it
     int32_t chain_claims = 0;              // persistent executor: claims per finished task (0: off)
@@ -152,7 +158,8 @@ struct Context {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ex0 = nullptr, ex1 = nullptr;
     int num_cu = 0;
     int32_t pool_reuse = SOGL_REUSE_AUTO;  // for the factors loaded from now on (sogl_ctx_set_pool_reuse)
-    bool after_links = false;              // for the factors loaded from now on (sogl_ctx_set_after_links)
+    bool after_links = false;              // for the factors loaded from now on (sogl_ctx_set_task_links)
+    bool absorb_panels = true;             // for the factors loaded from now on (sogl_ctx_set_task_links)
     size_t free_mem = 0, total_mem = 0;    // device HBM when the context was made
     std::string err;
     std::map<int32_t, Factor*> factors;

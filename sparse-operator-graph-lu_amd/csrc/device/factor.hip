@@ -416,10 +416,15 @@ struct PairWait {
 // slot's first value in the run (no cache holds an earlier one), and this CU's own
 // caches have its stores, which coherent loads could pass (the chain does not wait for
 // them).
+// xcopy / zcopy (FP32 LU kernels of the persistent executor): LDS for the copies of the
+// inverses an lu task with absorbed panels keeps.  Returns an lu task's growth (1
+// otherwise).
 template <typename T, bool BF16, bool ACC, bool LLT, bool MASKS, bool ONE = false, bool COH = false, typename Wait>
-__device__ __attribute__((always_inline)) void run_task(const DevTask& t, const Args& a, char* lds, uint32_t* s_mask,
-                                                        const Wait& wait, bool chained = false) {
+__device__ __attribute__((always_inline)) double run_task(const DevTask& t, const Args& a, char* lds, uint32_t* s_mask,
+                                                          const Wait& wait, bool chained = false, T* xcopy = nullptr,
+                                                          T* zcopy = nullptr) {
     T* pool = reinterpret_cast<T*>(a.pool);
+    double g = 1.0;
     switch (t.kind) {
         case TK_GEMM:
             if constexpr (BF16 && !ACC) {
@@ -435,11 +440,16 @@ __device__ __attribute__((always_inline)) void run_task(const DevTask& t, const 
             break;
         case TK_LU:
             if constexpr (!LLT) {
-                double g = lu_task<T>(pool, t.a, t.out0, t.out1, t.out2, t.out3, lds, a.growth_limit, a.ctrl + 2,
-                                      COH && !chained);
-                if (threadIdx.x == 0 && sizeof(T) == 4) {
-                    float gf = (float)g;
-                    atomicMax(a.ctrl + 3, __float_as_int(gf));
+                if constexpr (sizeof(T) == 4) {
+                    g = lu_task<T>(pool, t.a, t.out0, t.out1, t.out2, t.out3, lds, a.growth_limit, a.ctrl + 2,
+                                   COH && !chained, xcopy, zcopy);
+                    if (threadIdx.x == 0) {
+                        float gf = (float)g;
+                        atomicMax(a.ctrl + 3, __float_as_int(gf));
+                    }
+                } else {
+                    g = lu_task<T>(pool, t.a, t.out0, t.out1, t.out2, t.out3, lds, a.growth_limit, a.ctrl + 2,
+                                   COH && !chained);
                 }
             }
             break;
@@ -457,6 +467,63 @@ __device__ __attribute__((always_inline)) void run_task(const DevTask& t, const 
             break;
         default:
             break;
+    }
+    return g;
+}
+
+// A panel GEMM an lu task runs inside itself (DevTask::pan_l / pan_u, FP32 LU factors):
+// the other operand (made earlier: its flag is waited for as a pair's) into LDS where
+// the GEMM task puts it, the lu's inverse (lp: U^-1 as the second operand, else L^-1 as
+// the first) from the LDS copy made beside its stores (from the stored tile after an
+// FP64 refactorisation: copy is then reloaded), then the GEMM task's own product and
+// epilogue, so the tile and its masks are the bits the GEMM task makes.  The panel's
+// block masks are left in out_mask[w].  The lu's stores may still be in flight.
+template <typename T, bool MASKS, bool COH>
+__device__ __attribute__((always_inline)) void absorbed_panel(const Args& a, const DevTask& lu, const DevTask& pd, bool lp,
+                                                              char* lds, T* copy, bool refactored, uint32_t* out_mask,
+                                                              uint32_t* s_word) {
+    typedef typename MfmaOp<T>::acc_t acc_t;
+    T* pool = reinterpret_cast<T*>(a.pool);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32, li = lane & 15, lk = lane >> 4;
+    const Pair pr = a.pairs[pd.a];
+    T* icopy = copy;
+    const int oslot = lp ? pair_a(pr) : pair_b(pr);
+    if (tid == 0) *s_word = (uint32_t)spin_ready(a, lp ? pr.fa : pr.fb) & MASK_ALL;
+    if (refactored) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the inverse is read back below
+    __syncthreads();
+    const uint32_t om = *s_word;
+    T* la = reinterpret_cast<T*>(lds);
+    T* lb = la + TS * LDA;
+    tile_to_lds_masked<COH>(pool + (size_t)oslot * TE, lp ? la : lb, lp ? LDA : LDB, tid, om);
+    if (refactored) tile_to_lds<true>(pool + (size_t)(lp ? lu.out3 : lu.out2) * TE, icopy, lp ? LDB : LDA, tid);
+    __syncthreads();
+    acc_t acc[2][2];
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int n = 0; n < 2; n++) acc[m][n] = acc_t{0, 0, 0, 0};
+    const uint32_t ma = lp ? om : MASK_ALL, mb = lp ? MASK_ALL : om;
+    if (!MASKS || (ma != 0 && mb != 0)) {  // a zero product is skipped, as in the GEMM task
+        uint32_t kuse = 0xFu;
+        if constexpr (MASKS) kuse = k_blocks_used(ma, mb, wr >> 4, wc >> 4);
+        gemm_lds<T, false>(acc, lp ? la : icopy, lp ? icopy : lb, pr.b < 0 ? T(-1) : T(1), wr, wc, li, lk, kuse);
+    }
+    T* C = pool + (size_t)pd.out0 * TE;
+    bool nz[2][2] = {{false, false}, {false, false}};
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int n = 0; n < 2; n++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const T v = acc[m][n][r];
+                gst(&C[(wr + m * 16 + MfmaOp<T>::row(lk, r)) * TS + wc + n * 16 + li], v);
+                if constexpr (MASKS) nz[m][n] |= fabs(v) > T(MASK_TINY);
+            }
+    if constexpr (MASKS) {
+        const uint32_t qm = quad_mask(nz, wr >> 4, wc >> 4);
+        if (lane == 0) out_mask[w] = qm;
     }
 }
 
@@ -491,7 +558,19 @@ constexpr int CHAIN_MAX_CANDS = 16;    // claim bits 16..31 of a ready flag
 constexpr int CHAIN_MAX_CHECKS = 64;   // check entries per task: one lane of wave 0 each
 template <typename T, bool BF16, bool LLT, bool MASKS, bool ONE, bool COH>
 __global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
-    __shared__ __attribute__((aligned(16))) char lds[lds_bytes<T>()];
+    // FP32 LU kernels built for one workgroup per CU: lu tasks run their panel GEMMs
+    // (DevTask::pan_l / pan_u) inside themselves, keeping copies of the inverses in LDS
+    // beyond what any task kind uses.  The kernels built for two (no registers to spare)
+    // run them as after-links instead: right after the lu, in the same workgroup.
+    constexpr bool ABSORB = sizeof(T) == 4 && !BF16 && !LLT && ONE;
+    constexpr int XCOPY_BYTES = ABSORB ? TS * LDA * (int)sizeof(T) : 0;
+    constexpr int ZCOPY_BYTES = ABSORB ? TS * LDB * (int)sizeof(T) : 0;
+    __shared__ __attribute__((aligned(16))) char lds[lds_bytes<T>() + XCOPY_BYTES + ZCOPY_BYTES];
+    T* const xcopy = ABSORB ? reinterpret_cast<T*>(lds + lds_bytes<T>()) : nullptr;
+    T* const zcopy = ABSORB ? reinterpret_cast<T*>(lds + lds_bytes<T>() + XCOPY_BYTES) : nullptr;
+    __shared__ uint32_t s_pmask[2][NT / 64];
+    __shared__ uint32_t s_word;
+    __shared__ int s_pend;  // an absorbed panel still to run as an after-link (kernels without the copies)
     __shared__ int s_task;
     __shared__ int s_abort;
     __shared__ int s_lost;
@@ -512,6 +591,7 @@ __global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
             s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             s_task = s_abort ? a.nqueue : atomicAdd(a.ctrl, a.chunk);
             s_ring_head = s_ring_n = 0;
+            s_pend = -1;
         }
         __syncthreads();
         // wave-uniform loop control: the exit branch must not depend on the lane
@@ -558,11 +638,25 @@ __global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
             const int32_t cs = a.chk[t.chk_begin + tid].slot;
             if (cs >= 0) cv = __hip_atomic_load(a.ready + cs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        run_task<T, BF16, false, LLT, MASKS, ONE, COH>(
+        // an lu with absorbed panels (wave-uniform task fields): the panels' descriptors
+        const bool absorbs = ABSORB && t.kind == TK_LU && (t.pan_l >= 0 || t.pan_u >= 0);
+        const bool has_l = absorbs && t.pan_l >= 0, has_u = absorbs && t.pan_u >= 0;
+        const DevTask pdl = has_l ? a.tasks[t.pan_l] : t;
+        const DevTask pdu = has_u ? a.tasks[t.pan_u] : t;
+        const double growth = run_task<T, BF16, false, LLT, MASKS, ONE, COH>(
             t, a, lds, s_mask,
             PairWait{&a, &t_wait, s_pm, mode == 2 ? s_pre : nullptr, mode == 0 ? t.ctag : 0, &s_lost, t.war_begin,
                      t.war_count},
-            mode == 1);
+            mode == 1, has_u ? xcopy : nullptr, has_l ? zcopy : nullptr);
+        if constexpr (ABSORB) {
+            if (absorbs) {
+                __syncthreads();  // the lu's LDS (the inverses' copies included) is complete
+                const bool refactored = !(growth <= a.growth_limit);
+                if (has_l) absorbed_panel<T, MASKS, COH>(a, t, pdl, true, lds, zcopy, refactored, s_pmask[0], &s_word);
+                if (has_l && has_u) __syncthreads();  // the first panel's operands leave LDS
+                if (has_u) absorbed_panel<T, MASKS, COH>(a, t, pdu, false, lds, xcopy, refactored, s_pmask[1], &s_word);
+            }
+        }
         if (__builtin_amdgcn_readfirstlane(s_lost)) {  // claimed by another workgroup: dropped
             n_dropped++;
             break;
@@ -611,6 +705,22 @@ __global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
                 if (t.f3 >= 0)
                     __hip_atomic_store(a.ready + t.f3, fv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
+            for (int q = 0; q < 2; q++) {  // the absorbed panels, published with the lu
+                if (!(q == 0 ? has_l : has_u)) continue;
+                const DevTask& pd = q == 0 ? pdl : pdu;
+                const uint32_t* pm = s_pmask[q];
+                const uint32_t mp = MASKS ? (pm[0] | pm[1] | pm[2] | pm[3]) : MASK_ALL;
+                __hip_atomic_store(a.ready + pd.f0, want | mp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                write_masks(pd, a, mp);
+                if (a.trace) {
+                    long long* tr = a.trace + (size_t)(q == 0 ? t.pan_l : t.pan_u) * 4;
+                    gst(&tr[0], t_taken);
+                    gst(&tr[1], t_ready);
+                    gst(&tr[2], (long long)wall_clock64());
+                    gst(&tr[3], t_wait | (3ll << 60));
+                }
+                n_ran++;
+            }
             // consumers in this run read the masks from the flags; the mask array serves
             // the solve and the other executors, after the kernel
             write_masks(t, a, m);
@@ -627,11 +737,22 @@ __global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
             // barrier below (nothing writes these words again before the next such
             // barrier): the static chain's next task, else the after-link, else the
             // ring's head with its masks
+            // (kernels without the inverses' copies run an lu's panels as after-links)
+            const int pl = !absorbs && t.kind == TK_LU ? t.pan_l : -1;
+            const int pu = !absorbs && t.kind == TK_LU ? t.pan_u : -1;
             if (nxt >= 0) {
                 s_next_ti = nxt;
                 s_next_mode = 1;
+            } else if (pl >= 0 || pu >= 0) {
+                s_next_ti = pl >= 0 ? pl : pu;
+                s_pend = pl >= 0 ? pu : -1;
+                s_next_mode = 3;
             } else if (t.after >= 0) {
                 s_next_ti = t.after;
+                s_next_mode = 3;
+            } else if (s_pend >= 0) {
+                s_next_ti = s_pend;
+                s_pend = -1;
                 s_next_mode = 3;
             } else if (s_ring_n > 0) {
                 const int h = s_ring_head;
@@ -819,7 +940,8 @@ void build_chains(Factor* f, HostGraph& hg) {
     // after-links keep their chains in their heads' workgroups: neither end is claimed
     std::vector<char> linked(T, 0);
     for (size_t t = 0; t < T; t++)
-        if (f->tasks[t].after >= 0) linked[(size_t)f->tasks[t].after] = linked[t] = 1;
+        for (int32_t u : {f->tasks[t].after, f->tasks[t].pan_l, f->tasks[t].pan_u})
+            if (u >= 0) linked[(size_t)u] = linked[t] = 1;
     for (size_t u = 0; u < T; u++) {
         const DevTask& k = f->tasks[u];
         if (k.kind == TK_GEMM && k.b >= 1 && k.b <= CHAIN_MAX_PAIRS && !linked[u] &&
@@ -886,9 +1008,19 @@ void build_chains(Factor* f, HostGraph& hg) {
 constexpr int AFTER_MAX_PAIRS = 1;
 constexpr int AFTER_MAX_LEN = 24;
 
-void build_after_links(Factor* f, const HostGraph& hg, std::vector<char>& linked) {
+// Absorbed panels (absorb_on: FP32 LU factors).  The lu task of a diagonal tile runs
+// the one-pair panel GEMMs that multiply one of its inverses by a tile made earlier
+// (L_IK = A_IK U_KK^-1 and U_KJ = L_KK^-1 A_KJ, BlockPlanner.cpp:865-939) inside itself,
+// from the copies of the inverses it keeps in LDS, and publishes them with its own
+// tiles: the hop from the lu to its panels (publish, hand-off, the inverse's load)
+// leaves the critical path, for both panels the next Schur update waits for.  At most
+// one of each kind, near-critical (by the cost model) with the lu as their last
+// operand, under the same head order as after-links.
+void build_after_links(Factor* f, const HostGraph& hg, std::vector<char>& linked, bool after_on,
+                       bool absorb_on, const std::vector<int64_t>& rank) {
     const size_t T = f->tasks.size();
     f->n_after = 0;
+    f->n_absorbed = 0;
     f->chain_head.assign(T, -1);
     std::vector<int32_t> slot_task((size_t)f->n_slots, -1);
     for (size_t t = 0; t < T; t++)
@@ -923,6 +1055,27 @@ void build_after_links(Factor* f, const HostGraph& hg, std::vector<char>& linked
     // slack of task t: the longest path minus the longest path through t
     auto slack = [&](size_t t) { return cp - (fin[t] - task_cost_us(f->tasks[t]) + bl[t]); };
     std::vector<int32_t> head(T, -1), len(T, 0), pred(T, -1);
+    std::vector<char> absorbed(T, 0);
+    // every producer of u's operands other than t is an earlier task of t's chain or
+    // belongs to a chain whose head comes first
+    auto heads_before = [&](size_t t, const DevTask& ku) {
+        for (int32_t q = ku.dep_begin; q < ku.dep_begin + ku.dep_count; q++) {
+            const int32_t p = slot_task[(size_t)hg.deps[(size_t)q]];
+            if (p < 0 || p == (int32_t)t) continue;
+            if ((size_t)p > t) return false;  // not placed yet
+            bool member = false;
+            for (int32_t x = pred[t]; x >= 0 && !member; x = pred[(size_t)x]) member = x == p;
+            if (!member && rank[(size_t)head[(size_t)p]] >= rank[(size_t)head[t]]) return false;
+        }
+        return true;
+    };
+    auto link = [&](size_t t, int32_t u) {
+        f->tasks[t].after = u;
+        linked[(size_t)u] = 1;
+        head[(size_t)u] = head[t];
+        len[(size_t)u] = len[t] + 1;
+        pred[(size_t)u] = (int32_t)t;
+    };
     for (size_t t = 0; t < T; t++) {
         DevTask& k = f->tasks[t];
         if (head[t] < 0) head[t] = (int32_t)t;  // queued (or the static chain's head, below)
@@ -932,30 +1085,46 @@ void build_after_links(Factor* f, const HostGraph& hg, std::vector<char>& linked
             pred[(size_t)k.next] = (int32_t)t;
             continue;
         }
+        if (absorbed[t]) continue;  // runs inside its lu: no link of its own
         if (len[t] + 1 >= AFTER_MAX_LEN) continue;
+        if (absorb_on && k.kind == TK_LU) {
+            auto own = [&](int32_t v) { return v == k.out0 || v == k.out1 || v == k.out2 || v == k.out3; };
+            int32_t best[2] = {-1, -1};  // A U^-1, L^-1 A
+            for (int32_t u : succ[t]) {
+                const DevTask& ku = f->tasks[(size_t)u];
+                if (ku.kind != TK_GEMM || ku.b != 1 || ku.add >= 0 || ku.next >= 0 || linked[(size_t)u] ||
+                    crit[(size_t)u] != (int32_t)t || slack((size_t)u) > CHAIN_SLACK_US || !heads_before(t, ku))
+                    continue;
+                const Pair& pr = hg.pairs[(size_t)ku.a];
+                int which = -1;
+                if (pr.a >= 0 && k.out3 >= 0 && pair_b(pr) == k.out3 && !own(pair_a(pr))) which = 0;
+                else if (pr.a >= 0 && k.out2 >= 0 && pair_a(pr) == k.out2 && !own(pair_b(pr))) which = 1;
+                if (which >= 0 && (best[which] < 0 || bl[(size_t)u] > bl[(size_t)best[which]])) best[which] = u;
+            }
+            for (int which = 0; which < 2; which++) {
+                const int32_t u = best[which];
+                if (u < 0) continue;
+                (which == 0 ? k.pan_l : k.pan_u) = u;
+                linked[(size_t)u] = 1;
+                head[(size_t)u] = head[t];
+                len[(size_t)u] = len[t] + 1;
+                pred[(size_t)u] = (int32_t)t;
+                absorbed[(size_t)u] = 1;
+                f->n_absorbed++;
+            }
+            if (best[0] >= 0 || best[1] >= 0) continue;
+        }
+        if (!after_on) continue;
         int32_t best = -1;
         for (int32_t u : succ[t]) {
             const DevTask& ku = f->tasks[(size_t)u];
             if (ku.kind != TK_GEMM || ku.b > AFTER_MAX_PAIRS || linked[(size_t)u] || crit[(size_t)u] != (int32_t)t ||
                 slack((size_t)u) > CHAIN_SLACK_US)
                 continue;
-            bool ok = true;
-            for (int32_t q = ku.dep_begin; q < ku.dep_begin + ku.dep_count && ok; q++) {
-                const int32_t p = slot_task[(size_t)hg.deps[(size_t)q]];
-                if (p < 0 || p == (int32_t)t) continue;
-                if ((size_t)p > t) { ok = false; break; }  // not placed yet
-                bool member = false;  // an earlier task of t's own chain
-                for (int32_t x = pred[t]; x >= 0 && !member; x = pred[(size_t)x]) member = x == p;
-                ok = member || head[(size_t)p] < head[t];
-            }
-            if (ok && (best < 0 || bl[(size_t)u] > bl[(size_t)best])) best = u;
+            if (heads_before(t, ku) && (best < 0 || bl[(size_t)u] > bl[(size_t)best])) best = u;
         }
         if (best < 0) continue;
-        k.after = best;
-        linked[(size_t)best] = 1;
-        head[(size_t)best] = head[t];
-        len[(size_t)best] = len[t] + 1;
-        pred[(size_t)best] = (int32_t)t;
+        link(t, best);
         f->n_after++;
     }
     f->chain_head = head;
@@ -1441,12 +1610,21 @@ int build_host_graph(Context* c, const sogl_factor_desc* d, Factor* f, HostGraph
             chained[u] = 1;
             f->n_chained++;
         }
-        for (DevTask& k : f->tasks) k.after = -1;
-        if (f->after_links) build_after_links(f, hg, chained);
+        for (DevTask& k : f->tasks) k.after = k.pan_l = k.pan_u = -1;
+        // queue order: the task order (by dependency level).  Handing the GEMMs whose lu
+        // follows them out last in their level lets more panels pass the head order
+        // below (C5: 23k instead of 8k) but delays the diagonal chain (C5 FP32 factor
+        // 375 -> 382 ms with the panels, FP64 485 -> 496 ms without), so the queue keeps
+        // the task order; the checks compare queue positions either way.
+        std::vector<int64_t> rank(f->tasks.size());
+        for (size_t u = 0; u < f->tasks.size(); u++) rank[u] = (int64_t)u;
+        const bool absorb = f->absorb_panels && f->precision == SOGL_FP32 && !llt;
+        if (f->after_links || absorb) build_after_links(f, hg, chained, f->after_links, absorb, rank);
         else f->chain_head.clear();
         f->queue.clear();
         for (size_t u = 0; u < f->tasks.size(); u++)
             if (!chained[u]) f->queue.push_back((int32_t)u);
+        std::sort(f->queue.begin(), f->queue.end(), [&](int32_t x, int32_t y) { return rank[(size_t)x] < rank[(size_t)y]; });
     }
     build_chains(f, hg);
 
@@ -1578,13 +1756,19 @@ int validate_graph(Context* c, const sogl_factor_desc* d, const Factor* f, const
     {
         std::vector<char> seen(T, 0);
         std::vector<int32_t> head(T, -1), pred(T, -1);
-        for (int32_t q : f->queue) {
+        std::vector<int64_t> qpos(T, -1);
+        for (size_t i = 0; i < f->queue.size(); i++) {
+            const int32_t q = f->queue[i];
             if (q < 0 || (size_t)q >= T || seen[(size_t)q]) return bad("queue entry " + std::to_string(q));
             seen[(size_t)q] = 1;
+            qpos[(size_t)q] = (int64_t)i;
+            if (i > 0 && f->tasks[(size_t)q].level < f->tasks[(size_t)f->queue[i - 1]].level)
+                return bad("queue order at entry " + std::to_string(i));
         }
         for (size_t t = 0; t < T; t++) {
             const DevTask& k = f->tasks[t];
-            if (k.next >= 0 && k.after >= 0) return bad("two links from task " + std::to_string(t));
+            if ((k.next >= 0) + (k.after >= 0) + (k.pan_l >= 0 || k.pan_u >= 0) > 1)
+                return bad("two kinds of link from task " + std::to_string(t));
             if (k.next >= 0) {
                 if ((size_t)k.next >= T || k.kind != TK_GEMM) return bad("chain of task " + std::to_string(t));
                 const DevTask& u = f->tasks[(size_t)k.next];
@@ -1601,6 +1785,21 @@ int validate_graph(Context* c, const sogl_factor_desc* d, const Factor* f, const
                     return bad("after-link of task " + std::to_string(t));
                 seen[(size_t)k.after] = 1;
                 pred[(size_t)k.after] = (int32_t)t;
+            }
+            for (int which = 0; which < 2; which++) {  // an lu running one-pair panel GEMMs on its inverses
+                const int32_t p = which == 0 ? k.pan_l : k.pan_u;
+                if (p < 0) continue;
+                if (k.kind != TK_LU || f->precision != SOGL_FP32 || (size_t)p >= T || seen[(size_t)p])
+                    return bad("absorbed panel of task " + std::to_string(t));
+                const DevTask& u = f->tasks[(size_t)p];
+                if (u.kind != TK_GEMM || u.level <= k.level || u.b != 1 || u.add >= 0 || u.next >= 0 || u.after >= 0 ||
+                    u.pan_l >= 0 || u.pan_u >= 0 || u.a < 0 || (size_t)u.a >= hg.pairs.size())
+                    return bad("absorbed panel of task " + std::to_string(t));
+                const Pair& pr = hg.pairs[(size_t)u.a];
+                if (pr.a < 0 || (which == 0 ? pair_b(pr) != k.out3 : pair_a(pr) != k.out2))
+                    return bad("absorbed panel of task " + std::to_string(t));
+                seen[(size_t)p] = 1;
+                pred[(size_t)p] = (int32_t)t;
             }
         }
         for (size_t t = 0; t < T; t++)
@@ -1622,7 +1821,7 @@ int validate_graph(Context* c, const sogl_factor_desc* d, const Factor* f, const
                 if (p < 0) continue;
                 bool member = false;
                 for (int32_t x = pred[t]; x >= 0 && !member; x = pred[(size_t)x]) member = x == p;
-                if (!member && head[(size_t)p] >= head[t])
+                if (!member && qpos[(size_t)head[(size_t)p]] >= qpos[(size_t)head[t]])
                     return bad("after-linked task " + std::to_string(t) + " waits for a later chain");
             }
         }
@@ -1738,6 +1937,13 @@ int32_t done_flag(const Factor* f, int32_t t) {
 void assign_slots(Factor* f, HostGraph& hg, bool reuse, bool donate_inputs) {
     const int64_t V = f->n_slots;
     const size_t T = f->tasks.size();
+    // a panel GEMM absorbed by its lu stores without write-after-read waits: no slot
+    // freed by another value's readers (a sole-reader operand's slot is fine: the
+    // operand is in LDS before the stores)
+    std::vector<char> absorbed_gemm(T, 0);
+    for (size_t t = 0; t < T; t++)
+        for (int32_t p : {f->tasks[t].pan_l, f->tasks[t].pan_u})
+            if (p >= 0) absorbed_gemm[(size_t)p] = 1;
     for (auto* pv : {&hg.pairs, &hg.op_pairs})
         for (Pair& p : *pv) {
             p.fa = pair_a(p);
@@ -1833,7 +2039,7 @@ void assign_slots(Factor* f, HostGraph& hg, bool reuse, bool donate_inputs) {
             if (from >= 0) {
                 given[(size_t)from] = 1;
                 loc[(size_t)o] = loc[(size_t)from];
-            } else if (!fresh && k.kind == TK_GEMM && !freed.empty() &&
+            } else if (!fresh && !absorbed_gemm[t] && k.kind == TK_GEMM && !freed.empty() &&
                        freed.front().level <= k.level - REUSE_LEVEL_GAP) {
                 const Freed fr = freed.front();
                 freed.pop_front();
@@ -1892,6 +2098,8 @@ int validate_slots(Context* c, const Factor* f, const HostGraph& hg) {
     if (!f->reuse) return SOGL_OK;
     const int64_t V = f->n_slots;
     const size_t T = f->tasks.size();
+    std::vector<int64_t> qpos(T, 0);  // queue position of each queued task (chain heads)
+    for (size_t i = 0; i < f->queue.size(); i++) qpos[(size_t)f->queue[i]] = (int64_t)i;
     std::vector<int32_t> loc((size_t)V, -1), prod((size_t)V, -1);
     auto put = [&](int32_t v, int32_t l) {
         if (v < 0) return true;
@@ -1987,10 +2195,10 @@ int validate_slots(Context* c, const Factor* f, const HostGraph& hg) {
                     return bad("slot " + std::to_string(l) + " is reused at the level of a reader");
                 if (!std::binary_search(wf.begin(), wf.end(), done_flag(f, r)))
                     return bad("task " + std::to_string(pw) + " does not wait for a reader of its slot");
-                // an after-linked task waits only for chains with earlier heads
+                // a linked task waits only for chains with earlier heads in the queue
                 if (!f->chain_head.empty() && f->chain_head[(size_t)pw] != pw &&
-                    f->chain_head[(size_t)r] >= f->chain_head[(size_t)pw])
-                    return bad("after-linked task " + std::to_string(pw) + " waits for a reader in a later chain");
+                    qpos[(size_t)f->chain_head[(size_t)r]] >= qpos[(size_t)f->chain_head[(size_t)pw]])
+                    return bad("linked task " + std::to_string(pw) + " waits for a reader in a later chain");
             }
         }
     }
@@ -2154,6 +2362,7 @@ int build_graph(Context* c, const sogl_factor_desc* d, const InputCells* cells, 
     HostGraph hg;
     f->keep_ops = c->pool_reuse != SOGL_REUSE_ON;  // a factor with reuse cannot run per op
     f->after_links = c->after_links;
+    f->absorb_panels = c->absorb_panels;
     int rc = build_host_graph(c, d, f, hg);
     if (rc == SOGL_OK) rc = validate_graph(c, d, f, hg);
     if (rc == SOGL_OK) {
@@ -2410,7 +2619,8 @@ int graph_check(const sogl_factor_desc* d, bool cells, int64_t* info) {
     f.precision = d->precision;
     f.elem = d->precision == SOGL_FP64 ? 8 : 4;
     f.n_inputs = d->n_inputs;
-    f.after_links = true;  // checked with the after-links (their head order included)
+    f.after_links = true;  // checked with the after-links and absorbed panels (head order included)
+    f.absorb_panels = true;
     HostGraph hg;
     int rc = build_host_graph(nullptr, d, &f, hg);
     if (rc == SOGL_OK) rc = validate_graph(nullptr, d, &f, hg);
@@ -2431,6 +2641,7 @@ int graph_check(const sogl_factor_desc* d, bool cells, int64_t* info) {
         info[8] = f.n_locs;
         info[9] = f.n_pair_tasks;
         info[10] = f.n_after;
+        info[11] = f.n_absorbed;
     }
     return rc;
 }
@@ -2444,10 +2655,12 @@ extern "C" int sogl_graph_check_plan(const sogl_plan* plan, int32_t precision, i
     return graph_check(&d, true, info);
 }
 
-extern "C" int sogl_ctx_set_after_links(sogl_ctx* x, int32_t on) {
+extern "C" int sogl_ctx_set_task_links(sogl_ctx* x, int32_t links) {
     if (!x) return set_error(nullptr, SOGL_ERR_ARG, "null context");
+    if (links & ~(SOGL_LINK_AFTER | SOGL_LINK_PANEL)) return set_error(&x->c, SOGL_ERR_ARG, "unknown task links");
     std::lock_guard<std::mutex> g(x->c.mu);
-    x->c.after_links = on != 0;
+    x->c.after_links = (links & SOGL_LINK_AFTER) != 0;
+    x->c.absorb_panels = (links & SOGL_LINK_PANEL) != 0;
     return SOGL_OK;
 }
 

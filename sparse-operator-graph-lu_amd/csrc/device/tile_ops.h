@@ -1067,7 +1067,8 @@ __device__ __forceinline__ void u_blk_to_global(const C* a, T* Ug, int I, int J,
 template <typename T, typename C, bool CHOL = false>
 __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ pool, int src, int outl, int outu,
                                                               int outli, int outui, char* lds_raw,
-                                                              float* amax_out, float* umax_out, bool coh = false) {
+                                                              float* amax_out, float* umax_out, bool coh = false,
+                                                              C* xcopy = nullptr, C* zcopy = nullptr) {
     typedef typename MfmaOp<C>::acc_t acc_t;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
@@ -1109,18 +1110,31 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
         st16(acc, slot(s), BS, bli, blk);
     };
     // X_IJ = -X_II N_IJ and Z_IJ = -M_IJ Z_JJ to HBM, and in place over N / M (keep)
+    // xcopy / zcopy (not CHOL; nullptr: none): copies of X (row stride LDA) and Z (row
+    // stride LDB) in LDS beside the stores, for panel products the caller runs next
+    const bool cpx = !CHOL && xcopy, cpz = !CHOL && zcopy;
     auto x_fin = [&](int s, int I, int J, bool keep) {
         acc_t acc = acc_t{0, 0, 0, 0};
         mm16<C, 4>(acc, xd + I * BE, BS, slot(s), BS, bli, blk);
         if (keep) st16(acc, slot(s), BS, bli, blk, C(-1));
         if constexpr (CHOL) st16_global_rs(acc, Xg, I, J, bli, blk, C(-1), (const C*)(dsc + TS));
         else st16_global(acc, Xg, I, J, bli, blk, C(-1));
+        if (cpx) st16(acc, xcopy + 16 * I * LDA + 16 * J, LDA, bli, blk, C(-1));
     };
     auto z_fin = [&](int s, int I, int J, bool keep) {
         acc_t acc = acc_t{0, 0, 0, 0};
         mm16<C, 4>(acc, slot(s), BS, zd + J * BE, BS, bli, blk);
         if (keep) st16(acc, slot(s), BS, bli, blk, C(-1));
         st16_global(acc, Zg, I, J, bli, blk, C(-1));
+        if (cpz) st16(acc, zcopy + 16 * I * LDB + 16 * J, LDB, bli, blk, C(-1));
+    };
+    // 16x16 block (I, J) of a copy: from an LDS block of stride BS, or zeros
+    auto cp_blk = [&](C* cp, const C* pb, int I, int J, int ld) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int r = 4 * q + (bl >> 4), c = bl & 15;
+            cp[(16 * I + r) * ld + 16 * J + c] = pb ? pb[r * BS + c] : C(0);
+        }
     };
     // block column K of L, block row K of U, the diagonal blocks of X and Z, and the
     // zero halves of each
@@ -1138,10 +1152,18 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
             if constexpr (CHOL) blk_to_global_rs(xd + K * BE, BS, Xg, K, K, bl, (const C*)(dsc + TS));
             else blk_to_global(xd + K * BE, BS, Xg, K, K, bl);
             for (int J = K + 1; J < 4; J++) zero_blk_global(Xg, K, J, bl);
+            if (cpx) {
+                cp_blk(xcopy, xd + K * BE, K, K, LDA);
+                for (int J = K + 1; J < 4; J++) cp_blk(xcopy, nullptr, K, J, LDA);
+            }
         }
         if (Zg) {
             blk_to_global(zd + K * BE, BS, Zg, K, K, bl);
             for (int I = K + 1; I < 4; I++) zero_blk_global(Zg, I, K, bl);
+            if (cpz) {
+                cp_blk(zcopy, zd + K * BE, K, K, LDB);
+                for (int I = K + 1; I < 4; I++) cp_blk(zcopy, nullptr, I, K, LDB);
+            }
         }
     };
     {
@@ -1411,15 +1433,18 @@ __device__ inline void lu_inv_fp64_lds(T* __restrict__ pool, int src, int outl, 
 // The lu task: FP64 tiles factor in FP64; FP32 tiles factor in FP32 and a tile whose
 // pivot-free growth max|U| / max|A| exceeds growth_limit is refactored from its input
 // in FP64 (counted in n_fp64).  Returns the growth seen.
+// xcopy / zcopy (FP32 tiles): copies of the inverses in LDS (see lu_inv_blocked); after
+// an FP64 refactorisation they are stale (the growth returned exceeds growth_limit).
 template <typename T>
 __device__ __attribute__((always_inline)) double lu_task(T* __restrict__ pool, int src, int outl, int outu, int outli, int outui,
-                          char* lds_raw, double growth_limit, int* n_fp64, bool coh = false) {
+                          char* lds_raw, double growth_limit, int* n_fp64, bool coh = false,
+                          float* xcopy = nullptr, float* zcopy = nullptr) {
     if constexpr (sizeof(T) == 8) {
         lu_inv_blocked<T, double>(pool, src, outl, outu, outli, outui, lds_raw, nullptr, nullptr, coh);
         return 1.0;
     } else {
         float amax = 0, umax = 0;
-        lu_inv_blocked<T, float>(pool, src, outl, outu, outli, outui, lds_raw, &amax, &umax, coh);
+        lu_inv_blocked<T, float>(pool, src, outl, outu, outli, outui, lds_raw, &amax, &umax, coh, xcopy, zcopy);
         const double growth = amax > 0 ? (double)umax / (double)amax : 1.0;
         if (!(growth <= growth_limit)) {
             __syncthreads();

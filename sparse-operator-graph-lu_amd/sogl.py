@@ -19,6 +19,7 @@ OK = 0
 FP64, FP32, BF16 = 0, 1, 2
 EXEC_PERSISTENT, EXEC_LEVELS, EXEC_PER_OP, EXEC_GRAPH, EXEC_AUTO = 0, 1, 2, 3, 4
 REUSE_OFF, REUSE_ON, REUSE_AUTO = 0, 1, 2
+LINK_AFTER, LINK_PANEL = 1, 2  # sogl_ctx_set_task_links bits
 MASKS_OFF, MASKS_ON, MASKS_AUTO = 0, 1, 2
 TUNE_CANDIDATES = ("persistent chunk 1", "persistent chunk 2", "persistent chunk 4",
                    "persistent chunk 1, 1 workgroup/CU", "levels", "level graph")
@@ -128,7 +129,7 @@ def load(name: str = "libsogl.so") -> C.CDLL:
         "sogl_factor_set_masks": (C.c_int, [vp, i32, i32]),
             "sogl_factor_set_chaining": (C.c_int, [vp, i32, i32]),
             "sogl_ctx_set_pool_reuse": (C.c_int, [vp, i32]),
-            "sogl_ctx_set_after_links": (C.c_int, [vp, i32]),
+            "sogl_ctx_set_task_links": (C.c_int, [vp, i32]),
             "sogl_factor_get_uinv_check": (C.c_int, [vp, i32, P(i32), P(dbl)]),
             "sogl_refine_attach": (C.c_int, [vp, i32, vp]),
             "sogl_solve_refined": (C.c_int, [vp, i32, P(dbl), P(dbl), i32, dbl, i32, P(i32),
@@ -304,7 +305,7 @@ def make_desc(n_ids: int, ops, input_ids, input_tiles, ltiles, utiles, block_row
 
 
 GRAPH_INFO = ("tasks", "value_tiles", "levels", "pairs", "deps", "folded", "inv_aliased", "chained", "pool_tiles",
-              "pair_tasks", "after_links")
+              "pair_tasks", "after_links", "absorbed_panels")
 
 
 def graph_check_ops(n_ids: int, ops, input_ids, input_tiles, ltiles, utiles, block_rows: int,
@@ -397,9 +398,9 @@ class Device:
         n = 2 if claims is True else int(claims)
         self._check(self.lib.sogl_factor_set_chaining(self.h, fid, n))
 
-    def set_after_links(self, on: bool):
-        """After-links for the factors loaded from now on (sogl_ctx_set_after_links)."""
-        self._check(self.lib.sogl_ctx_set_after_links(self.h, 1 if on else 0))
+    def set_task_links(self, links: int):
+        """Task links (LINK_* mask) for the factors loaded from now on (sogl_ctx_set_task_links)."""
+        self._check(self.lib.sogl_ctx_set_task_links(self.h, int(links)))
 
     def set_pool_reuse(self, mode):
         """Pool slot reuse for the factors loaded from now on: True / REUSE_ON, False /

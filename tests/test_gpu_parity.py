@@ -669,33 +669,37 @@ def test_one_workgroup_per_cu_kernel(dev, precision, case):
 
 @pytest.mark.parametrize("precision", [sogl.FP64, sogl.FP32])
 @pytest.mark.parametrize("case", sorted(PERSISTENT_CASES))
-def test_after_links_keep_bits(dev, precision, case):
-    """After-links (the workgroup that made a critical tile runs the one-pair GEMM that
-    reads it next, not queued): every task runs once, and the factor bits and x equal
-    the levels executor's, at one and two workgroups per CU."""
+def test_task_links_keep_bits(dev, precision, case):
+    """Task links of the persistent executor: an lu task that runs its panel GEMM itself
+    from the inverse it keeps in LDS (FP32 LU factors), after-links (the workgroup that
+    made a critical tile runs the one-pair GEMM that reads it next, not queued), both
+    and none: every task runs once, and the factor bits and x equal the levels
+    executor's, at one and two workgroups per CU."""
     p = PERSISTENT_CASES[case]()
     pl = sogl.Plan(p.n, p.row, p.col, p.val, p.b, symmetric=p.symmetric)
     n_u = 0 if p.symmetric else pl.info.n_u_tiles
     bp = pl.permute_rhs(p.b)
+    runs = [(0, sogl.EXEC_LEVELS, 0)] + [(links, sogl.EXEC_PERSISTENT, wg)
+                                         for links in (sogl.LINK_PANEL, sogl.LINK_AFTER, 3) for wg in (1, 0)]
     out = []
     try:
-        for links, mode, wg in ((False, sogl.EXEC_LEVELS, 0), (True, sogl.EXEC_PERSISTENT, 1),
-                                (True, sogl.EXEC_PERSISTENT, 0)):
-            dev.set_after_links(links)
+        for links, mode, wg in runs:
+            dev.set_task_links(links)
             fid = dev.load(pl, precision)
-            dev.factor_config(fid, mode, 1, wg)
-            st = dev.stats(fid)
-            assert st.tasks_run == st.n_tasks or mode == sogl.EXEC_LEVELS, (st.tasks_run, st.n_tasks)
+            for _ in range(2):
+                dev.factor_config(fid, mode, 1, wg)
+                st = dev.stats(fid)
+                assert st.tasks_run == st.n_tasks or mode == sogl.EXEC_LEVELS, (links, wg, st.tasks_run, st.n_tasks)
             out.append((dev.factor_tiles(fid, 0, pl.info.n_l_tiles), dev.factor_tiles(fid, 1, n_u) if n_u else None,
                         dev.solve(fid, bp)))
             dev.release(fid)
     finally:
-        dev.set_after_links(False)
-    for k in (1, 2):
-        assert np.array_equal(out[0][0], out[k][0])
+        dev.set_task_links(sogl.LINK_PANEL)
+    for k in range(1, len(runs)):
+        assert np.array_equal(out[0][0], out[k][0]), runs[k]
         if n_u:
-            assert np.array_equal(out[0][1], out[k][1])
-        assert np.array_equal(out[0][2], out[k][2])
+            assert np.array_equal(out[0][1], out[k][1]), runs[k]
+        assert np.array_equal(out[0][2], out[k][2]), runs[k]
 
 
 def test_resident_factor_repeat_solves(dev):

@@ -74,9 +74,10 @@ def test_c2_graph_passes():
     assert g["levels"] < pl.info.n_stages  # fused tasks: fewer dependency levels than stages
     # most lu tasks follow the Schur-complement GEMM of their tile in the same workgroup
     assert 0 < g["chained"] <= pl.info.n_lu
-    # after-links (built by the check, their head order validated): near-critical
-    # one-pair GEMMs run by the workgroup that made their last operand
-    assert g["after_links"] == 2188
+    # task links (built by the check, their head order validated): panel GEMMs run by
+    # their lu task, and near-critical one-pair GEMMs run by the workgroup that made
+    # their last operand
+    assert g["absorbed_panels"] == 892 and g["after_links"] == 442
     # slot reuse: 224,341 values in 90,316 tiles (the factor tiles, their inverses, the
     # two-row blocks and the zero tile are 80,689; a slot freed fewer than 1024 levels
     # back is not reused)
