@@ -227,21 +227,9 @@ __global__ __launch_bounds__(NTP) void k_trsv_persistent(SolveArgs a) {
                 }
                 uint64_t rem = __ballot(mv != 0u);  // entries with a nonzero tile, in order
                 if (!rem) continue;
-                int j = __builtin_ctzll(rem);
-                rem &= rem - 1;
-                T t0[16], t1[16];  // a tile's elements are loaded only where its block is nonzero
-                uint32_t m0 = __builtin_amdgcn_readlane(mv, j);
-                if ((m0 >> bsh) & 1u) seg16(pool + (size_t)__builtin_amdgcn_readlane(sv, j) * TE + rowoff, t0);
-                for (;;) {
-                    const int J = __builtin_amdgcn_readlane(Jv, j);
-                    const bool has1 = rem != 0;
-                    const int j1 = has1 ? __builtin_ctzll(rem) : 0;
-                    const int J1 = __builtin_amdgcn_readlane(Jv, j1);
-                    const uint32_t m1 = has1 ? __builtin_amdgcn_readlane(mv, j1) : 0u;
-                    // the next tile, loaded while J's column is polled
-                    if ((m1 >> bsh) & 1u) seg16(pool + (size_t)__builtin_amdgcn_readlane(sv, j1) * TE + rowoff, t1);
-                    const bool both = has1 && (J1 >> 1) == (J >> 1);  // stored by one pair task
-                    const bool b0 = (m0 >> bsh) & 1u, b1 = both && ((m1 >> bsh) & 1u);
+                // one step: entry J (tile t0), or J and J1 stored by one pair task (t1),
+                // their columns polled in one spin, then the tiles' products
+                auto consume = [&](int J, int J1, bool both, bool b0, bool b1, const T* u0, const T* u1) {
                     const double* xj = a.out + (int64_t)J * TS;
 #pragma unroll
                     for (int c = 0; c < CW; c++) {
@@ -257,33 +245,92 @@ __global__ __launch_bounds__(NTP) void k_trsv_persistent(SolveArgs a) {
                             for (int k = 0; k < 16; k++) xc[k] = ls[kp * 16 + k];
                             if (b0) {
 #pragma unroll
-                                for (int k = 0; k < 16; k++) s += (double)t0[k] * xc[k];
+                                for (int k = 0; k < 16; k++) s += (double)u0[k] * xc[k];
                             }
                             if (both) {
 #pragma unroll
                                 for (int k = 0; k < 16; k++) xc[k] = ls[TS + kp * 16 + k];
                                 if (b1) {
 #pragma unroll
-                                    for (int k = 0; k < 16; k++) s += (double)t1[k] * xc[k];
+                                    for (int k = 0; k < 16; k++) s += (double)u1[k] * xc[k];
                                 }
                             }
                             __builtin_amdgcn_wave_barrier();  // ls is rewritten for the next column
                             acc[c] -= s;
                         }
                     }
-                    if (!has1) break;
-                    rem &= rem - 1;
-                    if (both) {  // both consumed: the one after is loaded now
-                        if (!rem) break;
-                        j = __builtin_ctzll(rem);
-                        rem &= rem - 1;
-                        m0 = __builtin_amdgcn_readlane(mv, j);
-                        if ((m0 >> bsh) & 1u) seg16(pool + (size_t)__builtin_amdgcn_readlane(sv, j) * TE + rowoff, t0);
-                    } else {
-                        j = j1;
-                        m0 = m1;
+                };
+                auto on = [&](int j) { return (__builtin_amdgcn_readlane(mv, j) >> bsh) & 1u; };
+                auto tile = [&](int j) { return pool + (size_t)__builtin_amdgcn_readlane(sv, j) * TE + rowoff; };
+                if constexpr (sizeof(T) == 4) {
+                    // FP32 tiles: whole steps, the next step's tiles loaded while the
+                    // current step's columns are polled (a step that takes two entries
+                    // leaves no tile of its own to load during its wait)
+                    auto next_step = [&](uint64_t& r, int& ja, int& jb) {
+                        if (!r) return false;
+                        ja = __builtin_ctzll(r);
+                        r &= r - 1;
+                        jb = -1;
+                        if (r) {
+                            const int c = __builtin_ctzll(r);
+                            if ((__builtin_amdgcn_readlane(Jv, c) >> 1) == (__builtin_amdgcn_readlane(Jv, ja) >> 1)) {
+                                jb = c;
+                                r &= r - 1;
+                            }
+                        }
+                        return true;
+                    };
+                    T t0[16], t1[16], n0[16], n1[16];  // loaded only where the block is nonzero
+                    int ja, jb;
+                    next_step(rem, ja, jb);
+                    if (on(ja)) seg16(tile(ja), t0);
+                    if (jb >= 0 && on(jb)) seg16(tile(jb), t1);
+                    for (;;) {
+                        int na = 0, nb = -1;
+                        const bool more = next_step(rem, na, nb);
+                        if (more && on(na)) seg16(tile(na), n0);
+                        if (more && nb >= 0 && on(nb)) seg16(tile(nb), n1);
+                        const bool both = jb >= 0;
+                        const int J = __builtin_amdgcn_readlane(Jv, ja);
+                        const int J1 = __builtin_amdgcn_readlane(Jv, both ? jb : ja);
+                        consume(J, J1, both, on(ja), both && on(jb), t0, t1);
+                        if (!more) break;
+                        ja = na;
+                        jb = nb;
 #pragma unroll
-                        for (int k = 0; k < 16; k++) t0[k] = t1[k];
+                        for (int k = 0; k < 16; k++) {
+                            t0[k] = n0[k];
+                            t1[k] = n1[k];
+                        }
+                    }
+                } else {
+                    // FP64 tiles (no registers for a second step): the next entry's tile
+                    // is loaded while the current one's column is polled
+                    int j = __builtin_ctzll(rem);
+                    rem &= rem - 1;
+                    T t0[16], t1[16];
+                    if (on(j)) seg16(tile(j), t0);
+                    for (;;) {
+                        const int J = __builtin_amdgcn_readlane(Jv, j);
+                        const bool has1 = rem != 0;
+                        const int j1 = has1 ? __builtin_ctzll(rem) : 0;
+                        const int J1 = __builtin_amdgcn_readlane(Jv, j1);
+                        const bool on1 = has1 && on(j1);
+                        if (on1) seg16(tile(j1), t1);  // the next tile, loaded while J's column is polled
+                        const bool both = has1 && (J1 >> 1) == (J >> 1);  // stored by one pair task
+                        consume(J, J1, both, on(j), both && on1, t0, t1);
+                        if (!has1) break;
+                        rem &= rem - 1;
+                        if (both) {  // both consumed: the one after is loaded now
+                            if (!rem) break;
+                            j = __builtin_ctzll(rem);
+                            rem &= rem - 1;
+                            if (on(j)) seg16(tile(j), t0);
+                        } else {
+                            j = j1;
+#pragma unroll
+                            for (int k = 0; k < 16; k++) t0[k] = t1[k];
+                        }
                     }
                 }
             }
