@@ -908,9 +908,10 @@ __device__ __forceinline__ void zero_blk_global(T* tile, int I, int J, int lane)
 // One wave factors the 16x16 diagonal block at (r0, r0) of the LDS tile a in place
 // (L strict lower with unit diagonal, U on and above the diagonal, |u_kk| < 1e-9
 // clamped to +-1e-9 as in MatrixStdDouble.cpp:2751-2761) and writes L^-1 to xd and
-// U^-1 to zd.  Lanes 0-15 hold the block's rows, lanes 16-31 its columns (lanes
-// 32-63 repeat the rows); a row_newbcast of lane k hands every lane of a 16-lane
-// group row k (or column k) of the block.  In both layouts step k updates
+// U^-1 to zd.  Lanes 0-15 and 32-47 hold the block's rows, lanes 16-31 and 48-63 its
+// columns, lanes 0-31 the even entries and lanes 32-63 the odd ones; a row_newbcast of
+// lane k hands every lane of a 16-lane group row k (or column k) of the block.  In
+// both layouts step k updates
 // x[c] -= (own_k / p) * bc[c] for lanes past k, with own_k the lane's own entry of
 // column (row) k: the row layout keeps own_k / p as the multiplier l_ik, the column
 // layout keeps own_k = u_kj.  The same factor f = own_k / p is the step-k multiplier
@@ -931,36 +932,45 @@ __device__ __forceinline__ C clamp_pivot(C p) {
 
 // The 16 elimination steps on the registers of diag16 (FP32), one scheduled asm
 // statement written by gen_diag16.py: the pivot clamp, the multipliers, both
-// inverses' updates, x[k] = l_ik for li > k, and pd = the clamped pivot of step li.
+// inverses' updates, x[k] = l_ik for li > k, and pd = the clamped pivot of step li
+// (in the half of the wave that holds entry li).  x[j], y[j]: entry 2j + (lane >> 5);
+// lg0 / lg1: li in half 0 / 1 and 0 in the other, le0 / le1: li or 99.
 template <bool CHOL>
-__device__ __forceinline__ void diag16_elim_f32(float (&x)[16], float (&y)[16], float& pd, int li);
+__device__ __forceinline__ void diag16_elim_f32(float (&x)[8], float (&y)[8], float& pd, int lg0, int lg1, int le0,
+                                                int le1);
 #include "diag16_asm.h"
 
 template <typename C, bool CHOL = false>
 __device__ inline void diag16(C* a, int r0, C* xd, C* zd, int lane) {
     static_assert(sizeof(C) == 4, "FP32 only; FP64 tiles use diag16_split");
-    const int i = lane & 15, g = lane >> 4;
-    const bool colw = g == 1;
+    const int i = lane & 15, h = lane >> 5;
+    const bool colw = (lane >> 4) & 1;
     const int rs = colw ? 1 : LDA, cs = colw ? LDA : 1;
-    float x[16], y[16];
+    float x[8], y[8];
 #pragma unroll
-    for (int c = 0; c < 16; c++) {
-        x[c] = a[r0 * LDA + r0 + i * rs + c * cs];
-        y[c] = c == i ? 1.f : 0.f;
+    for (int j = 0; j < 8; j++) {
+        const int c = 2 * j + h;
+        x[j] = a[r0 * LDA + r0 + i * rs + c * cs];
+        y[j] = c == i ? 1.f : 0.f;
     }
     float pd = 1.f;
-    diag16_elim_f32<CHOL>(x, y, pd, i);
-    if (g == 0) {
+    diag16_elim_f32<CHOL>(x, y, pd, h == 0 ? i : 0, h == 1 ? i : 0, h == 0 ? i : 99, h == 1 ? i : 99);
+    // pd of row / column i was set in half i & 1: lanes 32-63 of the swap's first result
+    // take lanes 0-31 of its second and the other way round, so the first holds half 0's
+    // pd in both halves and the second half 1's
+    const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(pd), __float_as_uint(pd), false, false);
+    pd = __uint_as_float((i & 1) ? sw[1] : sw[0]);
+    if (!colw) {
 #pragma unroll
-        for (int c = 0; c < 16; c++) {
-            a[(r0 + i) * LDA + r0 + c] = x[c];
-            xd[i * BS + c] = y[c];
+        for (int j = 0; j < 8; j++) {
+            a[(r0 + i) * LDA + r0 + 2 * j + h] = x[j];
+            xd[i * BS + 2 * j + h] = y[j];
         }
-        a[(r0 + i) * LDA + r0 + i] = pd;  // the clamped pivot over u_ii (same lane, after)
-    } else if (colw) {
+        if ((i & 1) == h) a[(r0 + i) * LDA + r0 + i] = pd;  // the clamped pivot over u_ii (same lane, after)
+    } else {
         const float dq = recip(pd);  // 1 / u_ii as taken at step i
 #pragma unroll
-        for (int c = 0; c < 16; c++) zd[c * BS + i] = y[c] * dq;
+        for (int j = 0; j < 8; j++) zd[(2 * j + h) * BS + i] = y[j] * dq;
     }
 }
 
@@ -1200,7 +1210,7 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
         if constexpr (sizeof(C) == 8) diag16_split<C, CHOL>(a, r0, xd + K * BE, zd + K * BE, lane);
         else diag16<C, CHOL>(a, r0, xd + K * BE, zd + K * BE, lane);
         if constexpr (CHOL) {
-            if (lane < 16) {  // this lane stored the pivot of row r0 + lane
+            if (lane < 16) {  // this wave stored the pivot of row r0 + lane (FP32: lane or lane + 32)
                 const C d = sqrt(a[(r0 + lane) * LDA + r0 + lane]);
                 dsc[r0 + lane] = d;
                 dsc[TS + r0 + lane] = C(1) / d;

@@ -2,10 +2,11 @@
 the device-scope releases that publish tiles and rows.
 
 The diagonal-block elimination issues its row-broadcast FMAs as asm statements
-(v_fmac_*_dpp row_newbcast), which the compiler's hazard recognizer does not see
-through.  A DPP source VGPR must not be written by a VALU instruction in the two
-wait states before the DPP instruction, and EXEC must not be written by a VALU
-instruction in the five before.  This test compiles the factor kernels to gfx950
+(v_fmac_*_dpp row_newbcast) and hands values between the wave's halves with
+v_permlane32_swap, in asm statements the compiler's hazard recognizer does not see
+through.  A DPP source VGPR, or either operand of a v_permlane32_swap, must not be
+written by a VALU instruction in the two wait states before, and EXEC must not be
+written by a VALU instruction in the five before a DPP instruction.  This test compiles the factor kernels to gfx950
 assembly and checks every DPP instruction in it against those two rules, counting
 one wait state per instruction and N + 1 per s_nop N, and treating a label inside
 the window as a violation (the branch into it could come from anywhere).
@@ -74,10 +75,12 @@ def dpp_violations(asm, vgpr_states=2, exec_states=5):
         if kind != "inst":
             continue
         op = s.split()[0]
-        if "_dpp" not in op and "row_newbcast" not in s:
+        swap = op.startswith("v_permlane")  # reads (and writes) both operands
+        if "_dpp" not in op and "row_newbcast" not in s and not swap:
             continue
         ndpp += 1
-        src0 = vregs(s.split(None, 1)[1].split(",")[1])
+        opnds = s.split(None, 1)[1].split(",")
+        src0 = vregs(opnds[1]) | (vregs(opnds[0]) if swap else set())
         waits, j = 0, idx - 1
         while j >= 0 and waits < exec_states:
             k2, t = insts[j]
@@ -114,11 +117,15 @@ def test_checker_finds_planted_hazards():
         "\tv_fmac_f32_dpp v8, v8, v53 row_newbcast:4 row_mask:0xf bank_mask:0xf",  # 2 states: fine
         "\tv_cmpx_lt_f32_e32 vcc, v1, v2", "\ts_nop 1",
         "\tv_mov_b32_dpp v50, v7 row_newbcast:1 row_mask:0xf bank_mask:0xf",  # EXEC 2 states before
+        "\tv_cndmask_b32_e64 v60, v1, v2, s[4:5]", "\ts_nop 0",
+        "\tv_permlane32_swap_b32_e32 v61, v62",  # v60 written 1 state before: not an operand
+        "\tv_cndmask_b32_e64 v63, v1, v2, s[4:5]",
+        "\tv_permlane32_swap_b32_e32 v63, v64",  # its first operand written just before
     ])
     n, bad = dpp_violations(asm)
-    assert n == 5
-    assert len(bad) == 3
-    assert "v27" in bad[0][1] and "v[4:5]" in bad[1][1] and "EXEC" in bad[2][1]
+    assert n == 7
+    assert len(bad) == 4
+    assert "v27" in bad[0][1] and "v[4:5]" in bad[1][1] and "EXEC" in bad[2][1] and "v63" in bad[3][1]
 
 
 def release_violations(asm):
