@@ -1148,20 +1148,19 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
     };
     // block column K of L, block row K of U, the diagonal blocks of X and Z, and the
     // zero halves of each
+    // the zero blocks of the outputs (L and L^-1 above the diagonal blocks, U and U^-1
+    // below) are stored by waves 1-3 while wave 0 factors diagonal block 0
     auto l_col = [&](int K) {
         for (int I = K; I < 4; I++) l_blk_to_global<T, C, CHOL>(a, Lg, I, K, bl, dsc);
-        for (int J = K + 1; J < 4; J++) zero_blk_global(Lg, K, J, bl);
     };
     C um = 0;
     auto u_row = [&](int K) {
         for (int J = K; J < 4; J++) u_blk_to_global<T, C>(a, Ug, K, J, bl, um);
-        for (int I = K + 1; I < 4; I++) zero_blk_global(Ug, I, K, bl);
     };
     auto inv_diag = [&](int K) {
         if (Xg) {
             if constexpr (CHOL) blk_to_global_rs(xd + K * BE, BS, Xg, K, K, bl, (const C*)(dsc + TS));
             else blk_to_global(xd + K * BE, BS, Xg, K, K, bl);
-            for (int J = K + 1; J < 4; J++) zero_blk_global(Xg, K, J, bl);
             if (cpx) {
                 cp_blk(xcopy, xd + K * BE, K, K, LDA);
                 for (int J = K + 1; J < 4; J++) cp_blk(xcopy, nullptr, K, J, LDA);
@@ -1169,7 +1168,6 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
         }
         if (Zg) {
             blk_to_global(zd + K * BE, BS, Zg, K, K, bl);
-            for (int I = K + 1; I < 4; I++) zero_blk_global(Zg, I, K, bl);
             if (cpz) {
                 cp_blk(zcopy, zd + K * BE, K, K, LDB);
                 for (int I = K + 1; I < 4; I++) cp_blk(zcopy, nullptr, I, K, LDB);
@@ -1256,7 +1254,18 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     };
-    if (w == 0) diag_step(0);
+    if (w == 0) {
+        diag_step(0);
+    } else {
+        // the 24 zero blocks (6 per output: block (K, J), J > K, of L and L^-1 and its
+        // transpose of U and U^-1), 8 per wave; the tile is in LDS, so an output slot that
+        // is the source's as well is not read again
+        for (int e = w - 1; e < 24; e += 3) {
+            T* const o = e < 6 ? Lg : (e < 12 ? Xg : (e < 18 ? Ug : Zg));
+            const int q = e % 6, K = q < 3 ? 0 : (q < 5 ? 1 : 2), J = q < 3 ? q + 1 : (q < 5 ? q - 1 : 3);
+            if (o) zero_blk_global(o, e < 12 ? K : J, e < 12 ? J : K, lane);
+        }
+    }
     __syncthreads();
 #pragma unroll 1
     for (int K = 0; K < 3; K++) {
@@ -1271,6 +1280,9 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
             trail(K + 1, K + 1, K, li, lk);
             __builtin_amdgcn_wave_barrier();
             diag_step(K + 1);
+            // FP32: the elimination ends before waves 1 and 2 finish their share of step
+            // 0, so wave 0 forms N10 = L10 X00 from its own two blocks (and wave 3 M01)
+            if (sizeof(C) == 4 && K == 0 && Xg) n_add(N10, true, 1, 0, xd);
         } else {
             asm volatile("" : "+v"(bl));
             bli = bl & 15;
@@ -1289,19 +1301,20 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
                 if (w == 1) {
                     l_col(0);
                     if (Xg) {
-                        n_add(N10, true, 1, 0, xd);
+                        if (sizeof(C) != 4) n_add(N10, true, 1, 0, xd);
                         n_add(N20, true, 2, 0, xd);
-                        n_add(N30, true, 3, 0, xd);
                     }
                 } else if (w == 2) {
                     if (Ug) u_row(0);
                     if (Zg) {
-                        m_add(M01, true, zd, 0, 1);
+                        if (sizeof(C) != 4) m_add(M01, true, zd, 0, 1);
                         m_add(M02, true, zd, 0, 2);
-                        m_add(M03, true, zd, 0, 3);
                     }
-                } else {
+                } else {  // the diagonal inverses' stores are the shortest share
                     inv_diag(0);
+                    if (sizeof(C) == 4 && Zg) m_add(M01, true, zd, 0, 1);
+                    if (Xg) n_add(N30, true, 3, 0, xd);
+                    if (Zg) m_add(M03, true, zd, 0, 3);
                 }
             } else if (K == 1) {
                 if (w == 1) {
