@@ -1146,10 +1146,10 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
             cp[(16 * I + r) * ld + 16 * J + c] = pb ? pb[r * BS + c] : C(0);
         }
     };
-    // block column K of L, block row K of U, the diagonal blocks of X and Z, and the
-    // zero halves of each
-    // the zero blocks of the outputs (L and L^-1 above the diagonal blocks, U and U^-1
-    // below) are stored by waves 1-3 while wave 0 factors diagonal block 0
+    // block column K of L, block row K of U, the diagonal blocks of X and Z, and the zero
+    // halves of the LDS copies of X and Z (the outputs' zero blocks, L and L^-1 above the
+    // diagonal blocks and U and U^-1 below, are stored by waves 1-3 while wave 0 factors
+    // diagonal block 0)
     auto l_col = [&](int K) {
         for (int I = K; I < 4; I++) l_blk_to_global<T, C, CHOL>(a, Lg, I, K, bl, dsc);
     };
