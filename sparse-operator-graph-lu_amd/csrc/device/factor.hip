@@ -422,9 +422,12 @@ struct PairWait {
 template <typename T, bool BF16, bool ACC, bool LLT, bool MASKS, bool ONE = false, bool COH = false, typename Wait>
 __device__ __attribute__((always_inline)) double run_task(const DevTask& t, const Args& a, char* lds, uint32_t* s_mask,
                                                           const Wait& wait, bool chained = false, T* xcopy = nullptr,
-                                                          T* zcopy = nullptr) {
+                                                          T* zcopy = nullptr, bool handoff = false) {
     T* pool = reinterpret_cast<T*>(a.pool);
     double g = 1.0;
+    // handoff (persistent executor): this GEMM's lu runs next on this workgroup and takes
+    // the sum from LDS; chained: this lu is that task (the L L^T and bf16 kernels load it)
+    constexpr bool LDS_HANDOFF = !LLT && !BF16;
     switch (t.kind) {
         case TK_GEMM:
             if constexpr (BF16 && !ACC) {
@@ -432,7 +435,7 @@ __device__ __attribute__((always_inline)) double run_task(const DevTask& t, cons
                                                       lds, s_mask, wait, t.add, t.add_neg != 0, t.fadd);
             } else {
                 gemm_task<T, ACC, LLT, MASKS, Wait, ONE, COH>(pool, a.pairs + t.a, t.b, t.out0, lds, s_mask, wait,
-                                                              t.add, t.add_neg != 0, t.fadd);
+                                                              t.add, t.add_neg != 0, t.fadd, LDS_HANDOFF && handoff);
             }
             break;
         case TK_SUB:
@@ -442,14 +445,14 @@ __device__ __attribute__((always_inline)) double run_task(const DevTask& t, cons
             if constexpr (!LLT) {
                 if constexpr (sizeof(T) == 4) {
                     g = lu_task<T>(pool, t.a, t.out0, t.out1, t.out2, t.out3, lds, a.growth_limit, a.ctrl + 2,
-                                   COH && !chained, xcopy, zcopy);
+                                   COH && !chained, xcopy, zcopy, LDS_HANDOFF && chained);
                     if (threadIdx.x == 0) {
                         float gf = (float)g;
                         atomicMax(a.ctrl + 3, __float_as_int(gf));
                     }
                 } else {
                     g = lu_task<T>(pool, t.a, t.out0, t.out1, t.out2, t.out3, lds, a.growth_limit, a.ctrl + 2,
-                                   COH && !chained);
+                                   COH && !chained, nullptr, nullptr, LDS_HANDOFF && chained);
                 }
             }
             break;
@@ -647,7 +650,7 @@ __global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
             t, a, lds, s_mask,
             PairWait{&a, &t_wait, s_pm, mode == 2 ? s_pre : nullptr, mode == 0 ? t.ctag : 0, &s_lost, t.war_begin,
                      t.war_count},
-            mode == 1, has_u ? xcopy : nullptr, has_l ? zcopy : nullptr);
+            mode == 1, has_u ? xcopy : nullptr, has_l ? zcopy : nullptr, nxt >= 0);
         if constexpr (ABSORB) {
             if (absorbs) {
                 __syncthreads();  // the lu's LDS (the inverses' copies included) is complete

@@ -401,7 +401,7 @@ template <typename T, bool ACC = false, bool TBOK = false, bool MASKS = true, ty
           bool STAGE64 = false, bool COH = false>
 __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, const Pair* __restrict__ pairs, int npairs,
                           int out, char* lds_raw, uint32_t* out_mask, const Wait& wait, int add = -1,
-                          bool add_neg = false, int fadd = -1) {
+                          bool add_neg = false, int fadd = -1, bool to_lds = false) {
     typedef typename MfmaOp<T>::acc_t acc_t;
     T* la = reinterpret_cast<T*>(lds_raw);
     T* lb = la + TS * LDA;
@@ -528,6 +528,15 @@ __device__ __attribute__((always_inline)) void gemm_task(T* __restrict__ pool, c
     if constexpr (MASKS) {
         const uint32_t qm = quad_mask(nz, wr >> 4, wc >> 4);
         if (lane == 0) out_mask[w] = qm;
+    }
+    if (to_lds) {  // the lu this workgroup runs next takes the sum from LDS (row stride LDA)
+        __syncthreads();  // every wave is done reading the staged operands
+#pragma unroll
+        for (int m = 0; m < 2; m++)
+#pragma unroll
+            for (int n = 0; n < 2; n++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) la[(wr + m * 16 + MfmaOp<T>::row(lk, r)) * LDA + wc + n * 16 + li] = acc[m][n][r];
     }
 }
 
@@ -1078,7 +1087,7 @@ template <typename T, typename C, bool CHOL = false>
 __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ pool, int src, int outl, int outu,
                                                               int outli, int outui, char* lds_raw,
                                                               float* amax_out, float* umax_out, bool coh = false,
-                                                              C* xcopy = nullptr, C* zcopy = nullptr) {
+                                                              C* xcopy = nullptr, C* zcopy = nullptr, bool in_lds = false) {
     typedef typename MfmaOp<C>::acc_t acc_t;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
@@ -1179,14 +1188,22 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
         const int r = tid >> 2, c0 = (tid & 3) * 16;
         if (tid == 0) *pcnt = 0;
         C m = 0;
-        C av[16];
-        load16(A, r * TS + c0, coh, av);
+        if (!CHOL && in_lds) {  // the GEMM that made the source left it in a
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int c = c0 + q;
-            const C x = av[q];
-            a[r * LDA + c] = x;
-            m = fabs(x) > m ? fabs(x) : m;
+            for (int q = 0; q < 16; q++) {
+                const C x = a[r * LDA + c0 + q];
+                m = fabs(x) > m ? fabs(x) : m;
+            }
+        } else {
+            C av[16];
+            load16(A, r * TS + c0, coh, av);
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int c = c0 + q;
+                const C x = av[q];
+                a[r * LDA + c] = x;
+                m = fabs(x) > m ? fabs(x) : m;
+            }
         }
         if constexpr (CHOL) {  // only the lower triangle is read: mirror it
             __syncthreads();
@@ -1458,16 +1475,19 @@ __device__ inline void lu_inv_fp64_lds(T* __restrict__ pool, int src, int outl, 
 // in FP64 (counted in n_fp64).  Returns the growth seen.
 // xcopy / zcopy (FP32 tiles): copies of the inverses in LDS (see lu_inv_blocked); after
 // an FP64 refactorisation they are stale (the growth returned exceeds growth_limit).
+// in_lds: the source is already in LDS (row stride LDA), left there by the GEMM that made
+// it (its global copy is still read by an FP64 refactorisation).
 template <typename T>
 __device__ __attribute__((always_inline)) double lu_task(T* __restrict__ pool, int src, int outl, int outu, int outli, int outui,
                           char* lds_raw, double growth_limit, int* n_fp64, bool coh = false,
-                          float* xcopy = nullptr, float* zcopy = nullptr) {
+                          float* xcopy = nullptr, float* zcopy = nullptr, bool in_lds = false) {
     if constexpr (sizeof(T) == 8) {
-        lu_inv_blocked<T, double>(pool, src, outl, outu, outli, outui, lds_raw, nullptr, nullptr, coh);
+        lu_inv_blocked<T, double>(pool, src, outl, outu, outli, outui, lds_raw, nullptr, nullptr, coh, nullptr, nullptr,
+                                  in_lds);
         return 1.0;
     } else {
         float amax = 0, umax = 0;
-        lu_inv_blocked<T, float>(pool, src, outl, outu, outli, outui, lds_raw, &amax, &umax, coh, xcopy, zcopy);
+        lu_inv_blocked<T, float>(pool, src, outl, outu, outli, outui, lds_raw, &amax, &umax, coh, xcopy, zcopy, in_lds);
         const double growth = amax > 0 ? (double)umax / (double)amax : 1.0;
         if (!(growth <= growth_limit)) {
             __syncthreads();
