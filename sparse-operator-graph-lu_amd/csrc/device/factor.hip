@@ -425,9 +425,9 @@ __device__ __attribute__((always_inline)) double run_task(const DevTask& t, cons
                                                           T* zcopy = nullptr, bool handoff = false) {
     T* pool = reinterpret_cast<T*>(a.pool);
     double g = 1.0;
-    // handoff (persistent executor): this GEMM's lu runs next on this workgroup and takes
-    // the sum from LDS; chained: this lu is that task (the L L^T and bf16 kernels load it)
-    constexpr bool LDS_HANDOFF = !LLT && !BF16;
+    // handoff (persistent executor): this GEMM's lu / llt runs next on this workgroup and
+    // takes the sum from LDS; chained: this lu / llt is that task (the bf16 kernels load it)
+    constexpr bool LDS_HANDOFF = !BF16;
     switch (t.kind) {
         case TK_GEMM:
             if constexpr (BF16 && !ACC) {
@@ -463,7 +463,7 @@ __device__ __attribute__((always_inline)) double run_task(const DevTask& t, cons
             tri_inv_task<T, false, COH>(pool, t.a, t.out0, lds);
             break;
         case TK_LLT:
-            if constexpr (LLT) llt_task<T>(pool, t.a, t.out0, t.out2, lds, COH && !chained);
+            if constexpr (LLT) llt_task<T>(pool, t.a, t.out0, t.out2, lds, COH && !chained, LDS_HANDOFF && chained);
             break;
         case TK_TRANS:
             if constexpr (LLT) trans_task<T, MASKS, COH>(pool, t.a, t.out0, lds, s_mask);

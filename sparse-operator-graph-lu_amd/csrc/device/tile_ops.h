@@ -1188,7 +1188,7 @@ __device__ __attribute__((always_inline)) void lu_inv_blocked(T* __restrict__ po
         const int r = tid >> 2, c0 = (tid & 3) * 16;
         if (tid == 0) *pcnt = 0;
         C m = 0;
-        if (!CHOL && in_lds) {  // the GEMM that made the source left it in a
+        if (in_lds) {  // the GEMM that made the source left it in a (CHOL: mirrored below)
 #pragma unroll
             for (int q = 0; q < 16; q++) {
                 const C x = a[r * LDA + c0 + q];
@@ -1522,9 +1522,9 @@ __device__ __attribute__((always_inline)) void tri_inv_task(T* __restrict__ pool
 // to 1e-20 before the square root as it does (2612).  FP32 tiles factor in FP32.
 template <typename T>
 __device__ __attribute__((always_inline)) void llt_task(T* __restrict__ pool, int src, int out, int outinv,
-                                                        char* lds_raw, bool coh = false) {
+                                                        char* lds_raw, bool coh = false, bool in_lds = false) {
     typedef typename std::conditional<sizeof(T) == 8, double, float>::type C;
-    lu_inv_blocked<T, C, true>(pool, src, out, -1, outinv, -1, lds_raw, nullptr, nullptr, coh);
+    lu_inv_blocked<T, C, true>(pool, src, out, -1, outinv, -1, lds_raw, nullptr, nullptr, coh, nullptr, nullptr, in_lds);
 }
 
 // R = A^T through LDS (odd row stride: the column reads hit distinct banks); wave w
