@@ -624,14 +624,18 @@ __global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
             __syncthreads();
         }
         if (a.trace && tid == 0) t_wait = wall_clock64();
-        if (tid == 0) {
-            s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_lost = 0;
-        }
-        __syncthreads();
-        if (__builtin_amdgcn_readfirstlane(s_abort)) {
-            aborted = true;
-            break;
+        if (mode != 1) {  // a chained task waits for nothing: the next queued task looks
+            if (tid == 0) {
+                s_abort = __hip_atomic_load(a.ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s_lost = 0;
+            }
+            __syncthreads();
+            if (__builtin_amdgcn_readfirstlane(s_abort)) {
+                aborted = true;
+                break;
+            }
+        } else if (tid == 0) {
+            s_lost = 0;  // read after the task's own barriers
         }
         // the claim candidates' operand flags, read now (wave 0, one lane per entry) and
         // looked at when the task is done
@@ -646,11 +650,22 @@ __global__ __launch_bounds__(NT, ONE ? 1 : 2) void k_factor_persistent(Args a) {
         const bool has_l = absorbs && t.pan_l >= 0, has_u = absorbs && t.pan_u >= 0;
         const DevTask pdl = has_l ? a.tasks[t.pan_l] : t;
         const DevTask pdu = has_u ? a.tasks[t.pan_u] : t;
+        // the task this workgroup runs next (the static chain): touch its descriptor now,
+        // so that its fetch at the top of the loop hits the cache (the loads are consumed
+        // after this task; a GEMM's first wait for its operands covers their latency)
+        int pf0 = 0, pf1 = 0, pf2 = 0;
+        if (nxt >= 0) {
+            const int32_t* pd = reinterpret_cast<const int32_t*>(a.tasks + nxt);
+            pf0 = pd[0];
+            pf1 = pd[sizeof(DevTask) / 8];
+            pf2 = pd[sizeof(DevTask) / 4 - 1];
+        }
         const double growth = run_task<T, BF16, false, LLT, MASKS, ONE, COH>(
             t, a, lds, s_mask,
             PairWait{&a, &t_wait, s_pm, mode == 2 ? s_pre : nullptr, mode == 0 ? t.ctag : 0, &s_lost, t.war_begin,
                      t.war_count},
             mode == 1, has_u ? xcopy : nullptr, has_l ? zcopy : nullptr, nxt >= 0);
+        if (nxt >= 0) asm volatile("" ::"s"(pf0), "s"(pf1), "s"(pf2));
         if constexpr (ABSORB) {
             if (absorbs) {
                 __syncthreads();  // the lu's LDS (the inverses' copies included) is complete
