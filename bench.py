@@ -47,11 +47,11 @@ HBM_PEAK = 8000.0  # GB/s
 # --pmc runs (FETCH_SIZE and WRITE_SIZE, scaled by what they read for a plain 1 GiB
 # copy): profiles/k_factor_persistent_c2_pmc_traffic.md (the timed run: dense tiles)
 # and profiles/k_factor_persistent_c5_pmc_traffic.md (16x16 block masks on)
-TRAFFIC_FP32_FACTOR = {"c2": 22.55e9, "c5": 24.42e9}
+TRAFFIC_FP32_FACTOR = {"c2": 22.46e9, "c5": 24.43e9}
 # MFMA busy share of the FP32 factor kernel (SQ_VALU_MFMA_BUSY_CYCLES over the kernel's
 # SIMD cycles) and the MFMA flops it executes, from profiles/k_factor_persistent_*_mfma_busy.md:
 # at C5 the block masks skip 97% of the nominal products (84% of the 16x16 blocks are zero)
-MFMA_FP32_FACTOR = {"c2": {"busy_frac": 0.0582, "executed_gflop": 527.3},
+MFMA_FP32_FACTOR = {"c2": {"busy_frac": 0.0586, "executed_gflop": 527.3},
                     "c5": {"busy_frac": 0.0008, "executed_gflop": 46.7}}
 
 WORKLOADS = {
